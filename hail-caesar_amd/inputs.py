@@ -1,0 +1,356 @@
+"""Seeded synthetic inputs in the reference's own file formats.
+
+Writes ASCII DEMs (``ncols/nrows/xllcorner/yllcorner/cellsize/NODATA_value``
+header as read by ``LSDRaster::read_ascii_raster``, src/topotools/LSDRaster.cpp:797-852),
+rainfall files (one row per ``rain_data_time_step``, one column per rain cell,
+read by ``LSDCatchmentModel::read_rainfalldata``, LSDCatchmentModel.cpp:694-717),
+grain data files (``x y idx grain[0..10] strata[10][9]`` per line,
+LSDCatchmentModel.cpp:611-683) and ``key: value`` parameter files
+(src/catchmentmodel/LSDUtils.cpp:12-77).
+
+Elevations are written with ``%.17g`` so the reference and this build read
+bit-identical FP64 values.  NODATA is the literal -9999 that every nodata test
+in the reference compares against (LSDCatchmentModel.cpp:2853, 3086).
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+NODATA = -9999.0
+
+# Grain sizes hard-coded in the reference (LSDCatchmentModel.hpp:557-565).
+REF_GRAIN_D = [0.000065, 0.001, 0.002, 0.004, 0.008, 0.016, 0.032, 0.064, 0.128]
+# Nine-fraction set of the flow+erosion workload (d1 overridden to 0.0005 m).
+EROSION_GRAIN_D = [0.0005, 0.001, 0.002, 0.004, 0.008, 0.016, 0.032, 0.064, 0.128]
+
+
+# ---------------------------------------------------------------------------
+# terrain
+# ---------------------------------------------------------------------------
+def diamond_square(p: int, roughness: float, rng: np.random.Generator) -> np.ndarray:
+    """Diamond-square fractal on a (2**p+1)^2 grid; the displacement scale is
+    multiplied by ``roughness`` at every level."""
+    n = 2 ** p + 1
+    a = np.zeros((n, n), dtype=np.float64)
+    a[0, 0], a[0, -1], a[-1, 0], a[-1, -1] = rng.uniform(-1.0, 1.0, 4)
+    step = n - 1
+    scale = 1.0
+    while step > 1:
+        half = step // 2
+        # diamond step: centres of squares
+        c = (a[0:-1:step, 0:-1:step] + a[0:-1:step, step::step]
+             + a[step::step, 0:-1:step] + a[step::step, step::step]) * 0.25
+        a[half:-1:step, half:-1:step] = c + rng.uniform(-scale, scale, c.shape)
+        # square step: edge midpoints.  Rows with even offset (midpoints of
+        # horizontal edges) and odd offset (vertical edges).
+        for (r0, c0) in ((0, half), (half, 0)):
+            rows = np.arange(r0, n, step)
+            cols = np.arange(c0, n, step)
+            rr, cc = np.meshgrid(rows, cols, indexing="ij")
+            tot = np.zeros(rr.shape)
+            cnt = np.zeros(rr.shape)
+            for dr, dc in ((-half, 0), (half, 0), (0, -half), (0, half)):
+                r2, c2 = rr + dr, cc + dc
+                ok = (r2 >= 0) & (r2 < n) & (c2 >= 0) & (c2 < n)
+                tot[ok] += a[r2[ok], c2[ok]]
+                cnt[ok] += 1
+            a[rr, cc] = tot / cnt + rng.uniform(-scale, scale, rr.shape)
+        step = half
+        scale *= roughness
+    return a
+
+
+def fractal_noise(nrows: int, ncols: int, amplitude: float, roughness: float,
+                  rng: np.random.Generator) -> np.ndarray:
+    """Zero-mean fractal field with peak-to-peak range ``amplitude``."""
+    p = max(1, int(math.ceil(math.log2(max(nrows, ncols) - 1))) if max(nrows, ncols) > 2 else 1)
+    ds = diamond_square(p, roughness, rng)[:nrows, :ncols]
+    ds = ds - ds.mean()
+    rng_pp = ds.max() - ds.min()
+    if rng_pp > 0:
+        ds *= amplitude / rng_pp
+    return ds
+
+
+def carve_channel(z: np.ndarray, depth: float, width: int = 3) -> np.ndarray:
+    """Carve a ``width``-cell channel ``depth`` metres deep down the central
+    column to the south edge (row index increases southward)."""
+    z = z.copy()
+    nrows, ncols = z.shape
+    c0 = ncols // 2 - width // 2
+    z[:, c0:c0 + width] -= depth
+    return z
+
+
+def planar_dem(nrows: int, ncols: int, dx: float, slope: float, amplitude: float,
+               roughness: float, seed: int, channel_depth: float = 2.0,
+               south_only: bool = True, base: float = 10.0) -> np.ndarray:
+    """Planar slope toward the south edge plus fractal noise and a carved
+    channel.  With ``south_only`` the north/east/west edge ring is NODATA so
+    that only the south edge can drain (the reference drains all four edges,
+    LSDCatchmentModel.cpp:3005-3047)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    i = np.arange(nrows, dtype=np.float64)[:, None]
+    z = base + slope * dx * (nrows - i) + fractal_noise(nrows, ncols, amplitude, roughness, rng)
+    z = carve_channel(z, channel_depth)
+    if south_only:
+        z[0, :] = NODATA
+        z[:, 0] = NODATA
+        z[:, -1] = NODATA
+    return z
+
+
+# ---------------------------------------------------------------------------
+# file writers
+# ---------------------------------------------------------------------------
+def write_asc(path: str, arr: np.ndarray, dx: float, nodata: float = NODATA,
+              fmt: str = "%.17g") -> None:
+    nrows, ncols = arr.shape
+    with open(path, "w") as f:
+        f.write(f"ncols {ncols}\nnrows {nrows}\nxllcorner 0\nyllcorner 0\n")
+        f.write(f"cellsize {dx:.17g}\nNODATA_value {nodata:.17g}\n")
+        np.savetxt(f, arr, fmt=fmt, delimiter=" ")
+
+
+def write_asc_int(path: str, arr: np.ndarray, dx: float, nodata: int = -9999) -> None:
+    nrows, ncols = arr.shape
+    with open(path, "w") as f:
+        f.write(f"ncols {ncols}\nnrows {nrows}\nxllcorner 0\nyllcorner 0\n")
+        f.write(f"cellsize {dx:.17g}\nNODATA_value {nodata}\n")
+        np.savetxt(f, arr.astype(np.int64), fmt="%d", delimiter=" ")
+
+
+def read_asc(path: str) -> np.ndarray:
+    with open(path) as f:
+        hdr = [f.readline() for _ in range(6)]
+    ncols = int(hdr[0].split()[1])
+    nrows = int(hdr[1].split()[1])
+    a = np.loadtxt(path, skiprows=6, ndmin=2)
+    return a.reshape(nrows, ncols)
+
+
+def write_rain(path: str, rates_mm_h: np.ndarray, pad_rows: int = 200) -> None:
+    """One row per rain data step, one column per rain cell (mm/h).  Zero rows
+    are appended because the reference indexes past the end of the storm
+    (LSDCatchmentModel.cpp:3676)."""
+    # values are made float32-exact so any reader (the reference parses them
+    # as float) recovers identical numbers from the 9-digit text
+    r = np.asarray(rates_mm_h, dtype=np.float32).astype(np.float64)
+    if r.ndim == 1:
+        r = r[:, None]
+    r = np.vstack([r, np.zeros((pad_rows, r.shape[1]))])
+    np.savetxt(path, r, fmt="%.9g", delimiter=" ")
+
+
+def write_graindata(path: str, props: np.ndarray, active: float,
+                    strata_props: Optional[np.ndarray] = None) -> None:
+    """Grain data file: one line per cell, ``x y idx grain[0..10]
+    strata[z=0..9][n=0..8]`` with 1-based padded coordinates x=row, y=col
+    (LSDCatchmentModel.cpp:611-683).  ``props`` is (nrows, ncols, 9) surface
+    proportions; grain[n] = active*props[n-1] for n=1..9, grain[0]=grain[10]=0.
+    Strata layers get the same proportions times ``active``."""
+    nrows, ncols, nf = props.shape
+    assert nf == 9
+    if strata_props is None:
+        strata_props = props
+    lines: List[str] = []
+    idx = 0
+    for i in range(nrows):
+        for j in range(ncols):
+            idx += 1
+            g = [0.0] + list(active * props[i, j]) + [0.0]
+            st = list(np.tile(active * strata_props[i, j], 10))
+            vals = [str(i + 1), str(j + 1), str(idx)] + ["%.17g" % v for v in g + st]
+            lines.append(" ".join(vals))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+# Parameter defaults shared by every synthetic workload.  Keys and meanings
+# follow LSDCatchmentModel::initialise_variables (LSDCatchmentModel.cpp:842-1610).
+BASE_PARAMS: Dict[str, str] = {
+    "dem_read_extension": "asc",
+    "dem_write_extension": "asc",
+    "timeseries_save_interval": "5",
+    "min_time_step": "0",
+    "max_time_step": "300",
+    "run_time_start": "0",
+    "memory_limit": "1",
+    "transport_law": "wilcock",
+    "max_tau_velocity": "5",
+    "active_layer_thickness": "0.1",
+    "chann_lateral_erosion": "20",
+    "erosion_limit": "0.02",
+    "suspended_sediment_on": "yes",
+    "read_in_graindata_from_file": "no",
+    "bedrock_layer_on": "no",
+    "lateral_erosion_on": "no",
+    "hydro_model_only": "yes",
+    "topmodel_m_value": "0.0005",
+    "in_out_difference": "0",
+    "min_q_for_depth_calc": "0.00001",
+    "max_q_for_depth_calc": "1000.0",
+    "slope_on_edge_cell": "0.01",
+    "evaporation_rate": "0.0",
+    "courant_number": "0.7",
+    "froude_num_limit": "0.8",
+    "mannings_n": "0.04",
+    "hflow_threshold": "0.001",
+    "rainfall_data_on": "yes",
+    "rain_data_time_step": "60",
+    "spatial_var_rain": "no",
+    "num_unique_rain_cells": "1",
+    "spatially_complex_rainfall_on": "no",
+    "vegetation_on": "no",
+    "grass_grow_rate": "0.0",
+    "vegetation_crit_shear": "0.0",
+    "veg_erosion_prop": "0.0",
+    "raster_output_interval": "100000",
+    "write_waterdepth_file": "no",
+    "write_elev_file": "no",
+    "write_grainsize_file": "no",
+    "write_elevdiff_file": "no",
+    "debug_print_cycle": "no",
+}
+
+
+def write_params(path: str, params: Dict[str, str]) -> None:
+    with open(path, "w") as f:
+        f.write("# synthetic workload parameter file\n")
+        for k, v in params.items():
+            f.write(f"{k}: {v}\n")
+
+
+@dataclass
+class Workload:
+    """A complete on-disk input set: directory, parameter file name, and the
+    arrays that were written (unpadded, row 0 = north)."""
+    path: str
+    params_file: str
+    params: Dict[str, str]
+    dem: np.ndarray
+    dx: float
+    mannings: Optional[np.ndarray] = None
+    water_init: Optional[np.ndarray] = None
+    rain: Optional[np.ndarray] = None
+    extra: Dict[str, object] = field(default_factory=dict)
+
+
+def make_workload(path: str, dem: np.ndarray, dx: float, rain_mm_h: np.ndarray,
+                  rain_step_min: float, run_hours: int, overrides: Dict[str, str] = None,
+                  mannings: Optional[np.ndarray] = None,
+                  water_init: Optional[np.ndarray] = None,
+                  hydroindex: Optional[np.ndarray] = None,
+                  graindata: Optional[np.ndarray] = None,
+                  name: str = "run") -> Workload:
+    os.makedirs(path, exist_ok=True)
+    p = dict(BASE_PARAMS)
+    p["read_path"] = "./"
+    p["write_path"] = "./out/"
+    p["read_fname"] = "dem"
+    p["write_fname"] = f"{name}.dat"
+    p["rainfall_data_file"] = "rain.txt"
+    p["rain_data_time_step"] = "%g" % rain_step_min
+    p["max_run_duration"] = str(int(run_hours))
+    os.makedirs(os.path.join(path, "out"), exist_ok=True)
+    write_asc(os.path.join(path, "dem.asc"), dem, dx)
+    write_rain(os.path.join(path, "rain.txt"), rain_mm_h)
+    if mannings is not None:
+        write_asc(os.path.join(path, "mannings.asc"), mannings, dx)
+        p["spatially_variable_mannings_on"] = "yes"
+        p["spatial_mannings_dem_file"] = "mannings.asc"
+    if water_init is not None:
+        write_asc(os.path.join(path, "water_init.asc"), water_init, dx)
+        p["water_init_from_raster_on"] = "yes"
+        p["water_init_raster_file"] = "water_init.asc"
+    if hydroindex is not None:
+        write_asc_int(os.path.join(path, "hydroindex.asc"), hydroindex, dx)
+        p["spatial_var_rain"] = "yes"
+        p["hydroindex_file"] = "hydroindex.asc"
+        p["num_unique_rain_cells"] = str(int(hydroindex.max()))
+    if graindata is not None:
+        active = float((overrides or {}).get("active_layer_thickness", p["active_layer_thickness"]))
+        write_graindata(os.path.join(path, "grain.dat"), graindata, active)
+        p["read_in_graindata_from_file"] = "yes"
+        p["grain_data_file"] = "grain.dat"
+    if overrides:
+        p.update(overrides)
+    pfile = f"{name}.params"
+    write_params(os.path.join(path, pfile), p)
+    return Workload(path=path, params_file=pfile, params=p, dem=dem, dx=dx,
+                    mannings=mannings, water_init=water_init,
+                    rain=np.asarray(rain_mm_h))
+
+
+# ---------------------------------------------------------------------------
+# the five workloads (seeds 1000+k; ensemble members 4000+m)
+# ---------------------------------------------------------------------------
+def flow_only_256(path: str, size: int = 256, hours: int = 2, seed: int = 1001) -> Workload:
+    """Plain method, flow only: size^2 DEM at 5 m, 0.01 planar slope toward
+    the south, diamond-square noise (amplitude 5 m, roughness 0.6), a 3-cell
+    channel carved 2 m deep, 10 mm/h rain for 1 h then 0, n=0.04,
+    hflow_threshold 0.001 m, courant 0.7, south edge open only."""
+    dem = planar_dem(size, size, 5.0, 0.01, 5.0, 0.6, seed)
+    rain = np.array([10.0, 0.0])
+    return make_workload(path, dem, 5.0, rain, 60, hours, name="flow256")
+
+
+def triangular_storm(hours: float, peak: float, step_min: float) -> np.ndarray:
+    n = int(round(hours * 60 / step_min))
+    t = (np.arange(n) + 0.5) / n
+    return peak * (1.0 - np.abs(2.0 * t - 1.0))
+
+
+def dirichlet_props(nrows: int, ncols: int, rng: np.random.Generator,
+                    alpha: float = 2.0) -> np.ndarray:
+    return rng.dirichlet(np.full(9, alpha), size=(nrows, ncols))
+
+
+def erosion_1024(path: str, size: int = 1024, hours: int = 24, seed: int = 1002) -> Workload:
+    """Flow + erosion: size^2 at 10 m, noise amplitude 20 m, 9 grain fractions
+    with seeded Dirichlet surface proportions, Wilcock-Crowe, 1 m active layer,
+    6 h triangular storm peaking at 40 mm/h."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dem = planar_dem(size, size, 10.0, 0.01, 20.0, 0.6, seed)
+    props = dirichlet_props(size, size, rng)
+    rain = triangular_storm(6, 40.0, 15)
+    w = make_workload(path, dem, 10.0, rain, 15, hours, name="erode1024",
+                      overrides={"hydro_model_only": "no",
+                                 "active_layer_thickness": "1"},
+                      graindata=props)
+    w.extra["grain_d"] = EROSION_GRAIN_D
+    return w
+
+
+def gamma_rain(hours: float, step_min: float, shape: float, mean: float,
+               rng: np.random.Generator, ncells: int = 1) -> np.ndarray:
+    n = int(round(hours * 60 / step_min))
+    return rng.gamma(shape, mean / shape, size=(n, ncells))
+
+
+def lognormal_mannings(nrows: int, ncols: int, median: float, sigma: float,
+                       rng: np.random.Generator) -> np.ndarray:
+    return median * np.exp(sigma * rng.standard_normal((nrows, ncols)))
+
+
+def large_4096(path: str, size: int = 4096, hours: int = 12, seed: int = 1003,
+               wet_depth: Optional[float] = None) -> Workload:
+    """Large single grid: size^2 at 2 m, fractal amplitude 30 m with carved
+    drainage, log-normal Manning n (median 0.035, sigma 0.3), 12 h of
+    Gamma(0.8, mean 8 mm/h) 15-minute rain.  ``wet_depth`` adds a uniform
+    initial depth (fully wet measurement state)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dem = planar_dem(size, size, 2.0, 0.002, 30.0, 0.6, seed, channel_depth=3.0)
+    n = lognormal_mannings(size, size, 0.035, 0.3, rng)
+    rain = gamma_rain(hours, 15, 0.8, 8.0, rng)
+    wi = None
+    if wet_depth is not None:
+        wi = np.full((size, size), wet_depth)
+        wi[dem == NODATA] = 0.0
+    return make_workload(path, dem, 2.0, rain, 15, hours, mannings=n,
+                         water_init=wi, name="grid4096")
