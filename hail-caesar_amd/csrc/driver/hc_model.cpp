@@ -1,0 +1,909 @@
+// C++ run-loop driver: the host side of the drop-in.
+//
+// CatchmentModel keeps the method surface of the reference's
+// LSDCatchmentModel (include/catchmentmodel/LSDCatchmentModel.hpp:73-423) and
+// run() is the loop of src/main.cpp:139-204.  Host-side scalar logic
+// (clocks, Courant timestep, TOPMODEL rainfall-runoff, drainage area and
+// input points, interval time series) stays here in FP64 exactly as the
+// reference computes it; every raster-wide method goes through the device
+// entry points of include/hc.h.
+#include <algorithm>
+#include <cctype>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/hc.h"
+#include "../../../include/hc_model.h"
+
+namespace hcdrv {
+
+constexpr double kGravity = 9.81;  // LSDCatchmentModel.hpp:437
+constexpr int kGMax = 10;          // LSDCatchmentModel.hpp:444
+// D8 offsets of LSDCatchmentModel.hpp:445-446 (index 0 unused).
+constexpr int kDX8[9] = {0, 0, 1, 1, 1, 0, -1, -1, -1};
+constexpr int kDY8[9] = {0, -1, -1, 0, 1, 1, 1, 0, -1};
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// x86-64 behaviour of static_cast<int>(double) on out-of-range values
+// (reached at LSDCatchmentModel.cpp:2283 when erode returns 0).
+int trunc_int(double v) {
+  if (!(v > -2147483649.0 && v < 2147483648.0)) return INT_MIN;
+  return static_cast<int>(v);
+}
+
+// One `key: value # comment` line, as LSDUtils::parse_line splits it
+// (src/catchmentmodel/LSDUtils.cpp:12-77).
+bool parse_line(std::istream& in, std::string& key, std::string& value) {
+  char c;
+  std::string buf;
+  int word = 0;
+  bool any = false;
+  key = "NULL";
+  value = "NULL";
+  while (in.get(c)) {
+    any = true;
+    if (c == '#') {
+      if (word == 0) { key = "NULL"; value = "NULL"; }
+      if (word == 1) value = "NULL";
+      word = 2;
+    }
+    if ((c == ' ' || c == '\t') && buf.empty() && word < 2) continue;
+    if ((c == ':' && word == 0) || ((c == ' ' || c == '\n' || c == '\t') && word == 1)) {
+      while (!buf.empty() && (buf.back() == ' ' || buf.back() == '\t')) buf.pop_back();
+      if (word == 0) key = buf; else value = buf;
+      ++word;
+      buf.clear();
+    } else if (c == '\n' && word == 0) {
+      key = "NULL";
+      value = buf;
+      ++word;
+    } else if (word < 2) {
+      buf.push_back(c);
+    }
+    if (c == '\n') break;
+  }
+  return any;
+}
+
+std::string strip_ctrl(std::string s) {
+  while (!s.empty() && std::iscntrl(static_cast<unsigned char>(s.back()))) s.pop_back();
+  return s;
+}
+
+struct Raster {
+  int ncols = 0, nrows = 0;
+  double xll = 0, yll = 0, cell = 0, nodata = -9999;
+  std::vector<double> v;  // nrows x ncols
+};
+
+Raster read_asc(const std::string& path) {
+  std::ifstream in(path);
+  if (!in) throw Error("cannot open raster " + path);
+  Raster r;
+  std::string s;
+  in >> s >> r.ncols >> s >> r.nrows >> s >> r.xll >> s >> r.yll >> s >> r.cell >> s >> r.nodata;
+  if (!in || r.ncols <= 0 || r.nrows <= 0) throw Error("bad ASCII raster header in " + path);
+  r.v.assign(static_cast<size_t>(r.ncols) * r.nrows, r.nodata);
+  for (auto& x : r.v) in >> x;
+  return r;
+}
+
+class CatchmentModel {
+ public:
+  CatchmentModel(const std::string& pname, const std::string& pfname, const hcm_options& opt)
+      : opt_(opt) {
+    initialise_variables(pname + pfname);
+  }
+  ~CatchmentModel() {
+    if (ctx_) hc_destroy(ctx_);
+  }
+
+  // ------------------------------------------------ set-up (main.cpp:103-120)
+  void initialise_model_domain_extents() {
+    const std::string f = read_path_ + "/" + read_fname_ + "." + dem_read_extension_;
+    std::ifstream in(f);
+    if (!in) throw Error("No terrain DEM found by name of: " + f);
+    std::string s;
+    in >> s >> jmax_ >> s >> imax_ >> s >> xll_ >> s >> yll_ >> s >> DX_ >> s >> no_data_value_;
+    if (!in || imax_ <= 0 || jmax_ <= 0) throw Error("bad DEM header in " + f);
+  }
+
+  void initialise_arrays() {
+    NR_ = imax_ + 2;
+    NC_ = jmax_ + 2;
+    const size_t n = static_cast<size_t>(NR_) * NC_;
+    elev_.assign(n, -9999);
+    rfarea_.assign(n, 1);
+    area_.assign(n, 0.0);
+    area_depth_.assign(n, 0.0);
+    if (rain_data_time_step_ < 1) rain_data_time_step_ = 1;  // LCM.cpp:1680-1683
+    if (max_time_step_ / 60 > rain_data_time_step_)
+      max_time_step_ = static_cast<int>(rain_data_time_step_) * 60;
+    const int rows = static_cast<int>(maxcycle_ * (60 / rain_data_time_step_)) + 100;
+    rain_.assign(rows, std::vector<float>(rfnum_ + 1, 0.0f));
+    j_.assign(rfnum_ + 1, 0.000000001);
+    jo_.assign(rfnum_ + 1, 0.000000001);
+    j_mean_.assign(rfnum_ + 1, 0.0);
+    old_j_mean_.assign(rfnum_ + 1, 0.0);
+    new_j_mean_.assign(rfnum_ + 1, 0.0);
+    nActual_.assign(rfnum_ + 1, 0);
+    input_counter_.assign(rfnum_ + 1, 0);
+    in_x_.assign(static_cast<size_t>(imax_) * jmax_ + 2, 0);
+    in_y_.assign(static_cast<size_t>(imax_) * jmax_ + 2, 0);
+    sum_grain_.assign(kGMax + 1, 0.0);
+    old_sum_grain_.assign(kGMax + 1, 0.0);
+    Qg_step_.assign(kGMax + 1, 0.0);
+    Qg_hour_.assign(kGMax + 1, 0.0);
+    Qg_over_.assign(kGMax + 1, 0.0);
+    Qg_last_.assign(kGMax + 1, 0.0);
+
+    hc_grid g{imax_, jmax_, DX_, opt_.precision, hydro_only_ ? 0 : 1};
+    hc_params p{};
+    p.courant_number = courant_number_;
+    p.froude_limit = froude_limit_;
+    p.hflow_threshold = hflow_threshold_;
+    p.edgeslope = edgeslope_;
+    p.mannings = mannings_;
+    p.spatially_var_mannings = spatially_var_mannings_ ? 1 : 0;
+    p.water_depth_erosion_threshold = water_depth_erosion_threshold_;
+    p.ERODEFACTOR = ERODEFACTOR_;
+    p.active = active_;
+    p.Csuspmax = Csuspmax_;
+    p.max_vel = max_vel_;
+    p.transport_law = einstein_ ? HC_EINSTEIN : HC_WILCOCK;
+    for (int k = 0; k < 10; ++k) p.d[k] = d_[k];
+    for (int k = 0; k < 11; ++k) {
+      p.dprop[k] = dprop_[k];
+      p.fallVelocity[k] = fall_[k];
+    }
+    p.bedrock_erosion_threshold = bedrock_erosion_threshold_;
+    p.bedrock_erosion_rate = bedrock_erosion_rate_;
+    p.lateral_constant = lateral_constant_;
+    p.chann_lateral_erosion = chann_lateral_erosion_;
+    p.veg_lat_restriction = veg_lat_restriction_;
+    p.flat_dz = opt_.flat_dz;
+    check(hc_create(&ctx_, &g, &p), "hc_create");
+  }
+
+  void load_data() {
+    const std::string demf = read_path_ + "/" + read_fname_ + "." + dem_read_extension_;
+    Raster dem = read_asc(demf);
+    if (dem.nrows != imax_ || dem.ncols != jmax_) throw Error("DEM size changed between reads");
+    for (int i = 0; i < imax_; ++i)
+      for (int j = 0; j < jmax_; ++j) elev_[at(i + 1, j + 1)] = dem.v[static_cast<size_t>(i) * jmax_ + j];
+    check_DEM_edge_condition();
+    upload(HC_FIELD_ELEV, elev_.data());
+    if (spatially_var_rainfall_) {
+      Raster r = read_asc(read_path_ + "/" + hydroindex_fname_);
+      for (int i = 0; i < imax_; ++i)
+        for (int j = 0; j < jmax_; ++j)
+          rfarea_[at(i + 1, j + 1)] = static_cast<int>(r.v[static_cast<size_t>(i) * jmax_ + j]);
+    }
+    if (spatially_var_mannings_) {
+      Raster r = read_asc(read_path_ + "/" + mannings_fname_);
+      std::vector<double> nm(elev_.size(), 0.0);
+      for (int i = 0; i < imax_; ++i)
+        for (int j = 0; j < jmax_; ++j) nm[at(i + 1, j + 1)] = r.v[static_cast<size_t>(i) * jmax_ + j];
+      upload(HC_FIELD_MANNINGS, nm.data());
+    }
+    count_catchment_gridcells();
+    if (bedrock_layer_on_) {
+      // the bedrock raster is copied without the +1 padding offset
+      // (LSDCatchmentModel.cpp:316: bedrock = bedrockR.get_RasterData_dbl())
+      Raster r = read_asc(read_path_ + "/" + bedrock_data_file_);
+      std::vector<double> b(elev_.size(), -9999.0);
+      for (int i = 0; i < std::min(r.nrows, NR_); ++i)
+        for (int j = 0; j < std::min(r.ncols, NC_); ++j) b[at(i, j)] = r.v[static_cast<size_t>(i) * r.ncols + j];
+      if (!hydro_only_) upload(HC_FIELD_BEDROCK, b.data());
+    }
+    if (water_init_from_raster_) {
+      Raster r = read_asc(read_path_ + "/" + water_init_raster_file_);
+      std::vector<double> h(elev_.size(), 0.0);
+      for (int i = 0; i < imax_; ++i)
+        for (int j = 0; j < jmax_; ++j) h[at(i + 1, j + 1)] = r.v[static_cast<size_t>(i) * jmax_ + j];
+      upload(HC_FIELD_WATER_DEPTH, h.data());
+    }
+    if (rainfall_data_on_) read_rainfall(read_path_ + "/" + rainfall_data_file_);
+    if (graindata_from_file_ && !hydro_only_) ingest_graindata(read_path_ + "/" + grain_data_file_);
+  }
+
+  void set_time_counters() {  // LCM.cpp:1862-1871
+    save_time_ = cycle_;
+    time_1_ = cycle_;
+  }
+  void initialise_rainfall_runoff() { topmodel_runoff(1.0); }  // LCM.cpp:2041-2053
+  void initialise_drainage_area() {                           // LCM.cpp:2055-2069
+    zero_and_calc_drainage_area();
+    get_catchment_input_points();
+  }
+
+  // ------------------------------------------ loop methods (main.cpp:141-202)
+  void set_loop_cycle() {  // LCM.cpp:1873-1877
+    previous_ = cycle_;
+    old_cycle_ = std::fmod(cycle_, output_file_save_interval_);
+  }
+  void set_inputoutput_diff() { iod_ = std::fabs(waterinput_ - waterOut_); }
+  double courant_friedrichs_lewy_condition() const {  // LCM.cpp:1884-1887
+    return courant_number_ * (DX_ / std::sqrt(kGravity * (maxdepth_)));
+  }
+  void set_maximum_timestep() {  // LCM.cpp:1890-1910
+    if (maxdepth_ <= 0.1) maxdepth_ = 0.1;
+    if (time_step_ < courant_friedrichs_lewy_condition()) time_step_ = courant_friedrichs_lewy_condition();
+    if (iod_ > in_out_difference_allowed_ && time_step_ > courant_friedrichs_lewy_condition())
+      time_step_ = courant_friedrichs_lewy_condition();
+  }
+  double get_flow_timestep() const {  // LCM.cpp:1913-1923
+    double f = time_step_;
+    if (f > courant_friedrichs_lewy_condition()) f = courant_friedrichs_lewy_condition();
+    return f;
+  }
+  void increment_counters() {  // LCM.cpp:1925-1933
+    counter_++;
+    cycle_ += time_step_ / 60;
+    new_cycle_ = std::fmod(cycle_, output_file_save_interval_);
+  }
+  void reach_water_and_sediment_input() {}  // reach mode is rejected at parse time
+  void catchment_waterinputs() {             // LCM.cpp:2027-2039
+    waterinput_ = 0;
+    catchment_water_input_and_hydrology(get_flow_timestep());
+  }
+  void flow_route() {  // LCM.cpp:3063-3247
+    last_dt_ = get_flow_timestep();
+    if (opt_.fused && hydro_only_) return;  // performed together with depth_update
+    check(hc_flow_route(ctx_, last_dt_), "flow_route");
+  }
+  void depth_update() {  // LCM.cpp:3259-3299
+    const double dt = get_flow_timestep();
+    if (opt_.fused && hydro_only_)
+      check(hc_flow_depth(ctx_, dt, &maxdepth_), "flow_depth");
+    else
+      check(hc_depth_update(ctx_, dt, &maxdepth_), "depth_update");
+  }
+  void check_wetted_area(int interval) {  // LCM.cpp:2019-2025
+    if ((counter_ % interval) == 0) check(hc_scan_area(ctx_), "scan_area");
+  }
+  void call_erosion() {  // LCM.cpp:2277-2294
+    if (counter_ >= erode_call_) {
+      erode_mult_ = trunc_int(ERODEFACTOR_ / erode(erode_mult_));
+      if (erode_mult_ < 1) erode_mult_ = 1;
+      if (erode_mult_ > 5) erode_mult_ = 5;
+      erode_call_ = counter_ + erode_mult_;
+    }
+  }
+  double erode(double mult) {  // LCM.cpp:4211-4845 (device), 4829-4842 (host sums)
+    double tb = 0;
+    float g[10] = {};
+    check(hc_erode(ctx_, mult, &time_step_, &tb, g), "erode");
+    sediQ_ = 0;
+    for (int n = 1; n <= kGMax; ++n) {
+      const double gv = (n < 10) ? static_cast<double>(g[n]) : 0.0;
+      sediQ_ += gv * DX_ * DX_;
+      globalsediq_ += gv * DX_ * DX_;
+      sum_grain_[n] += gv * DX_ * DX_;
+    }
+    return tb;
+  }
+  void water_flux_out() {  // LCM.cpp:3005-3047
+    check(hc_water_flux_out(ctx_, time_step_, &temptot_), "water_flux_out");
+    waterOut_ = temptot_;
+  }
+  void call_channel_landsliding(int interval) {
+    if (!hydro_only_ && (counter_ % interval) == 0) warn_once_landslide();
+  }
+  void write_output_timeseries() {  // LCM.cpp:1952-1966
+    temptotal_ = temptot_;
+    output_data();
+  }
+  void print_cycle() {
+    if (debug_print_cycle_ && !opt_.quiet) std::cout << "Cycle: " << cycle_ << "                  \r" << std::flush;
+  }
+  void save_raster_output() {  // LCM.cpp:1935-1942
+    if (cycle_ >= save_time_) {
+      save_raster_data(std::fabs(cycle_));
+      save_time_ += saveinterval_;
+    }
+  }
+  double get_cycle() const { return cycle_; }
+  int get_maxcycle() const { return maxcycle_; }
+  bool is_hydro_only() const { return hydro_only_; }
+
+  // One loop body of main.cpp:139-202; returns true when main.cpp:204 ends
+  // the run.
+  bool iterate() {
+    set_loop_cycle();
+    set_inputoutput_diff();
+    set_maximum_timestep();
+    increment_counters();
+    reach_water_and_sediment_input();
+    catchment_waterinputs();
+    flow_route();
+    depth_update();
+    check_wetted_area(5);
+    if (!is_hydro_only()) call_erosion();
+    water_flux_out();
+    call_channel_landsliding(10);
+    write_output_timeseries();
+    print_cycle();
+    save_raster_output();
+    return !(get_cycle() / 60 < get_maxcycle());
+  }
+
+  double last_dt() const { return last_dt_; }
+  double temptot() const { return temptot_; }
+  hc_ctx* ctx() { return ctx_; }
+  const std::vector<double>& series() const { return series_; }
+  int NR() const { return NR_; }
+  int NC() const { return NC_; }
+
+  bool scalar(const std::string& n, double* v) const {
+    if (n == "time_step") *v = time_step_;
+    else if (n == "cycle") *v = cycle_;
+    else if (n == "counter") *v = counter_;
+    else if (n == "maxdepth") *v = maxdepth_;
+    else if (n == "temptot") *v = temptot_;
+    else if (n == "waterinput") *v = waterinput_;
+    else if (n == "waterOut") *v = waterOut_;
+    else if (n == "baseflow") *v = baseflow_;
+    else if (n == "totalinputpoints") *v = totalinputpoints_;
+    else if (n == "imax") *v = imax_;
+    else if (n == "jmax") *v = jmax_;
+    else if (n == "DX") *v = DX_;
+    else if (n == "erode_mult") *v = erode_mult_;
+    else if (n == "globalsediq") *v = globalsediq_;
+    else return false;
+    return true;
+  }
+
+ private:
+  size_t at(int x, int y) const { return static_cast<size_t>(x) * NC_ + y; }
+  void check(int rc, const char* what) {
+    if (rc != HC_OK) throw Error(std::string(what) + ": " + hc_last_error(ctx_));
+  }
+  void upload(int field, const void* host) {
+    check(hc_upload(ctx_, field, host, hc_field_count(ctx_, field)), "upload");
+  }
+  void warn_once_landslide() {
+    if (!warned_landslide_ && !opt_.quiet)
+      std::cerr << "note: in_channel_landsliding/creep/global_landsliding are not part of this "
+                   "build; erosion runs omit them\n";
+    warned_landslide_ = true;
+  }
+
+  // ---------------------------------------------------------- parameters
+  // LSDCatchmentModel::initialise_variables (LCM.cpp:842-1610) for the keys
+  // of the hot path and its inputs; options this build does not provide are
+  // rejected instead of silently ignored.
+  void initialise_variables(const std::string& full) {
+    std::ifstream in(full);
+    if (!in) throw Error("cannot open parameter file " + full);
+    std::string key, value;
+    auto yes = [&]() { return value == "yes"; };
+    while (in.good()) {
+      if (!parse_line(in, key, value)) break;
+      if (key == "NULL") continue;
+      std::string k = key;
+      for (auto& ch : k) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+      value = strip_ctrl(value);
+      const char* v = value.c_str();
+      if (k == "dem_read_extension") dem_read_extension_ = value;
+      else if (k == "dem_write_extension") dem_write_extension_ = value;
+      else if (k == "write_path") write_path_ = value;
+      else if (k == "write_fname") write_fname_ = value;
+      else if (k == "read_path") read_path_ = value;
+      else if (k == "read_fname") read_fname_ = value;
+      else if (k == "hydroindex_file") hydroindex_fname_ = value;
+      else if (k == "rainfall_data_file") rainfall_data_file_ = value;
+      else if (k == "grain_data_file") grain_data_file_ = value;
+      else if (k == "bedrock_data_file") bedrock_data_file_ = value;
+      else if (k == "water_init_raster_file") water_init_raster_file_ = value;
+      else if (k == "spatial_mannings_dem_file") mannings_fname_ = value;
+      else if (k == "max_time_step") max_time_step_ = std::atoi(v);
+      else if (k == "max_run_duration") maxcycle_ = std::atoi(v);
+      else if (k == "timeseries_save_interval") output_file_save_interval_ = std::atoi(v);
+      else if (k == "raster_output_interval") saveinterval_ = std::atof(v);
+      else if (k == "write_elevation_file") elev_fname_ = value;
+      else if (k == "write_elev_file") write_elev_file_ = yes();
+      else if (k == "waterdepth_outfile_name") waterdepth_fname_ = value;
+      else if (k == "write_waterdepth_file") write_waterd_file_ = yes();
+      else if (k == "read_in_graindata_from_file") graindata_from_file_ = yes();
+      else if (k == "bedrock_layer_on") bedrock_layer_on_ = yes();
+      else if (k == "transport_law") {
+        if (value == "wilcock") { wilcock_ = true; einstein_ = false; }
+        else if (value == "einstein") { einstein_ = true; wilcock_ = false; }
+        else throw Error("transport_law must be wilcock or einstein");  // LCM.cpp:1134-1142
+      }
+      else if (k == "max_tau_velocity") max_vel_ = std::atof(v);
+      else if (k == "active_layer_thickness") active_ = std::atof(v);
+      else if (k == "chann_lateral_erosion") chann_lateral_erosion_ = std::atof(v);
+      else if (k == "erosion_limit") ERODEFACTOR_ = std::atof(v);
+      else if (k == "lateral_erosion_const") lateral_constant_ = std::atof(v);
+      else if (k == "bedrock_erosion_threshold") bedrock_erosion_threshold_ = std::atof(v);
+      else if (k == "hydro_model_only") hydro_only_ = yes();
+      else if (k == "rainfall_data_on") rainfall_data_on_ = yes();
+      else if (k == "water_init_from_raster_on") water_init_from_raster_ = yes();
+      else if (k == "topmodel_m_value") M_ = std::atof(v);
+      else if (k == "num_unique_rain_cells") rfnum_ = std::atoi(v);
+      else if (k == "rain_data_time_step") rain_data_time_step_ = std::atof(v);
+      else if (k == "spatial_var_rain") spatially_var_rainfall_ = yes();
+      else if (k == "in_out_difference") in_out_difference_allowed_ = std::atof(v);
+      else if (k == "min_q_for_depth_calc") MIN_Q_ = std::atof(v);
+      else if (k == "max_q_for_depth_calc") MIN_Q_MAXVAL_ = std::atof(v);
+      else if (k == "hflow_threshold") hflow_threshold_ = std::atof(v);
+      // the reference's own quirk: this key sets hflow_threshold (LCM.cpp:1318-1323)
+      else if (k == "water_depth_erosion_threshold") hflow_threshold_ = std::atof(v);
+      else if (k == "slope_on_edge_cell") edgeslope_ = std::atof(v);
+      else if (k == "courant_number") courant_number_ = std::atof(v);
+      else if (k == "froude_num_limit") froude_limit_ = std::atof(v);
+      else if (k == "mannings_n") mannings_ = std::atof(v);
+      else if (k == "spatially_variable_mannings_on") spatially_var_mannings_ = yes();
+      else if (k == "veg_erosion_prop") veg_lat_restriction_ = std::atof(v);
+      else if (k == "debug_print_cycle") debug_print_cycle_ = yes();
+      else if (k == "hc_grain_sizes") {  // extension: d1..d9, comma separated
+        std::stringstream ss(value);
+        std::string tok;
+        int idx = 1;
+        while (std::getline(ss, tok, ',') && idx <= 9) d_[idx++] = std::atof(tok.c_str());
+      }
+      else if ((k == "spatially_complex_rainfall_on" || k == "vegetation_on" || k == "groundwater_on" ||
+                k == "reach_mode" || k == "lateral_erosion_on") && yes())
+        throw Error("option '" + k + "' is not provided by this build (hot path only)");
+    }
+    tx_ = output_file_save_interval_;                // LCM.cpp:1599
+    if (!spatially_var_rainfall_) rfnum_ = 1;        // LCM.cpp:1602-1608
+  }
+
+  void check_DEM_edge_condition() {  // LCM.cpp:2845-2889
+    double temp = -9999;
+    for (int n = 1; n <= jmax_; ++n) {
+      if (elev_[at(imax_, n)] > -9999) temp = elev_[at(imax_, n)];
+      if (elev_[at(1, n)] > -9999) temp = elev_[at(1, n)];
+    }
+    for (int n = 1; n <= imax_; ++n) {
+      if (elev_[at(n, 1)] > -9999) temp = elev_[at(n, 1)];
+      if (elev_[at(n, jmax_)] > -9999) temp = elev_[at(n, jmax_)];
+    }
+    if (temp < -10) throw Error("DEM EDGE CONDITION ERROR: all DEM edges are nodata");
+  }
+
+  void count_catchment_gridcells() {  // LCM.cpp:2813-2843
+    for (int i = 1; i < imax_; ++i)
+      for (int j = 1; j < jmax_; ++j)
+        if (elev_[at(i, j)] > -9999) nActual_[rfarea_[at(i, j)]]++;
+  }
+
+  void read_rainfall(const std::string& f) {  // LCM.cpp:694-717 (values parsed as float)
+    std::ifstream in(f);
+    if (!in) throw Error("No rainfall data file found by name of: " + f);
+    rain_.clear();
+    std::string line;
+    while (std::getline(in, line)) {
+      std::stringstream ss(line);
+      std::vector<float> row;
+      float val;
+      while (ss >> val) row.push_back(val);
+      rain_.push_back(row);
+    }
+  }
+
+  void ingest_graindata(const std::string& f) {  // LCM.cpp:611-683
+    std::ifstream in(f);
+    if (!in) throw Error("No grain data file found by name of: " + f);
+    const size_t n = elev_.size();
+    std::vector<double> grain(n * 11, 0.0), strata(n * 90, 0.0);
+    std::vector<int32_t> has(n, 0);
+    std::string line;
+    while (std::getline(in, line)) {
+      std::stringstream ss(line);
+      std::vector<double> cols;
+      double val;
+      while (ss >> val) cols.push_back(val);
+      if (cols.size() < 3) continue;
+      int x1 = static_cast<int>(cols[0]), y1 = static_cast<int>(cols[1]);
+      if (x1 > imax_) x1 = imax_;
+      if (y1 > jmax_) y1 = jmax_;
+      const size_t c = at(x1, y1);
+      has[c] = 1;
+      for (int k = 0; k <= kGMax && 3 + k < static_cast<int>(cols.size()); ++k) grain[c * 11 + k] = cols[3 + k];
+      for (int z = 0; z < 10; ++z)
+        for (int k = 0; k < 9; ++k) {
+          const size_t col = 14 + z * 9 + k;
+          if (col < cols.size()) strata[c * 90 + z * 9 + k] = cols[col];
+        }
+    }
+    upload(HC_FIELD_GRAIN, grain.data());
+    upload(HC_FIELD_STRATA, strata.data());
+    upload(HC_FIELD_HASGRAIN, has.data());
+  }
+
+  // ------------------------------------------------ rainfall-runoff (host)
+  float rain_at(int row, int col) const {
+    if (row < 0 || row >= static_cast<int>(rain_.size())) return 0.0f;
+    if (col < 0 || col >= static_cast<int>(rain_[row].size())) return 0.0f;
+    return rain_[row][col];
+  }
+  void topmodel_runoff(double cyc) {  // LCM.cpp:3649-3715
+    const int ts = static_cast<int>(cyc / rain_data_time_step_);
+    for (int n = 1; n <= rfnum_; ++n) {
+      double local = 0;
+      const double ft = 60;
+      old_j_mean_[n] = new_j_mean_[n];
+      jo_[n] = j_[n];
+      const float r = rain_at(ts, n - 1);
+      if (r > 0) local = rain_factor_ * ((r / 1000) / 3600);
+      if (local == 0) {
+        j_[n] = jo_[n] / (1 + ((jo_[n] * ft) / M_));
+        new_j_mean_[n] = M_ / ft * std::log(1 + ((jo_[n] * ft) / M_));
+      }
+      if (local > 0) {
+        j_[n] = local / (((local - jo_[n]) / jo_[n]) * std::exp((0 - local) * ft / M_) + 1);
+        new_j_mean_[n] = (M_ / ft) * std::log(((local - jo_[n]) + jo_[n] * std::exp((local * ft) / M_)) / local);
+      }
+      if (new_j_mean_[n] < 0) new_j_mean_[n] = 0;
+    }
+  }
+  void calchydrograph(double t) {  // LCM.cpp:3717-3724
+    for (int n = 1; n <= rfnum_; ++n)
+      j_mean_[n] = old_j_mean_[n] + (((new_j_mean_[n] - old_j_mean_[n]) / 2) * (2 - t));
+  }
+  void catchment_water_input_and_hydrology(double fts) {  // LCM.cpp:3455-3541
+    for (int i = 1; i <= rfnum_; ++i) waterinput_ += j_mean_[i] * nActual_[i] * DX_ * DX_;
+    pts_x_.clear();
+    pts_y_.clear();
+    pts_a_.clear();
+    for (int z = 1; z <= totalinputpoints_; ++z) {
+      const int i = in_x_[z], j = in_y_[z];
+      const int rf = rfarea_[at(i, j)];
+      double amt = (j_mean_[rf] * nActual_[rf]) / (input_counter_[rf]) * fts;
+      if (amt > ERODEFACTOR_) amt = ERODEFACTOR_;
+      pts_x_.push_back(i);
+      pts_y_.push_back(j);
+      pts_a_.push_back(amt);
+    }
+    check(hc_add_point_inputs(ctx_, pts_x_.data(), pts_y_.data(), pts_a_.data(),
+                              static_cast<int>(pts_x_.size())),
+          "add_point_inputs");
+    if (cycle_ >= time_1_) {
+      do {
+        time_1_++;
+        topmodel_runoff(time_1_);
+        if (time_step_ > max_time_step_) {
+          double jmt = 0;
+          for (int n = 1; n <= rfnum_; ++n)
+            if (new_j_mean_[n] > jmt) jmt = new_j_mean_[n];
+          const unsigned cells = static_cast<unsigned>(imax_) * static_cast<unsigned>(jmax_);
+          if (jmt > (0.2 / (cells * DX_ * DX_))) {
+            cycle_ = time_1_ + (max_time_step_ / 60);
+            time_step_ = max_time_step_;
+          }
+        }
+      } while (time_1_ < cycle_);
+    }
+    calchydrograph(time_1_ - cycle_);
+    double jmeanmax = 0;
+    for (int n = 1; n <= rfnum_; ++n)
+      if (j_mean_[n] > jmeanmax) jmeanmax = j_mean_[n];
+    if (jmeanmax >= baseflow_) {
+      baseflow_ = baseflow_ * 3;
+      zero_and_calc_drainage_area();
+      get_catchment_input_points();
+    }
+    if (baseflow_ > (jmeanmax * 3) && baseflow_ > 0.000001) {
+      baseflow_ = jmeanmax * 1.25;
+      zero_and_calc_drainage_area();
+      get_catchment_input_points();
+    }
+  }
+  void get_catchment_input_points() {  // LCM.cpp:3741-3767
+    totalinputpoints_ = 0;
+    for (int n = 1; n <= rfnum_; ++n) input_counter_[n] = 0;
+    for (int i = 1; i <= imax_; ++i)
+      for (int j = 1; j <= jmax_; ++j) {
+        const double v = area_[at(i, j)] * baseflow_ * 3 * DX_ * DX_;
+        if (v > MIN_Q_ && v < MIN_Q_MAXVAL_) {
+          ++totalinputpoints_;
+          in_x_[totalinputpoints_] = i;
+          in_y_[totalinputpoints_] = j;
+          input_counter_[rfarea_[at(i, j)]]++;
+        }
+      }
+    if (totalinputpoints_ == 0) totalinputpoints_ = 1;
+  }
+  // LCM.cpp:2071-2270.  The elevation sort is std::sort over
+  // (imax+2)(jmax+2) pairs exactly as the reference builds them, so ties
+  // keep the reference's order.
+  void zero_and_calc_drainage_area() {
+    if (!hydro_only_) refresh_elev();
+    for (int i = 1; i <= imax_; ++i)
+      for (int j = 1; j <= jmax_; ++j) {
+        area_depth_[at(i, j)] = 1;
+        area_[at(i, j)] = 0;
+        if (elev_[at(i, j)] == -9999) area_depth_[at(i, j)] = 0.0;
+      }
+    std::vector<std::pair<size_t, double>> pr(elev_.size(), std::make_pair(size_t(0), 0.0));
+    size_t inc = 1;
+    for (int i = 1; i <= imax_; ++i)
+      for (int j = 1; j <= jmax_; ++j) pr[inc++] = std::make_pair(at(i, j), elev_[at(i, j)]);
+    std::sort(pr.begin(), pr.end(), [](const std::pair<size_t, double>& a, const std::pair<size_t, double>& b) {
+      return a.second < b.second;
+    });
+    const size_t ncell = static_cast<size_t>(imax_) * jmax_;
+    for (size_t k = ncell; k >= 1; --k) {
+      const size_t c = pr[k].first;
+      const int i = static_cast<int>(c / NC_), j = static_cast<int>(c % NC_);
+      if (!(area_depth_[c] > 0)) continue;
+      if (area_depth_[c] > area_[c]) area_[c] = area_depth_[c];
+      double difftot = 0;
+      for (int dir = 1; dir <= 8; ++dir) {
+        int j2 = std::min(std::max(j + kDY8[dir], 1), jmax_);
+        int i2 = std::min(std::max(i + kDX8[dir], 1), imax_);
+        const double e2 = elev_[at(i2, j2)], e = elev_[c];
+        if (e2 < e) difftot += (dir % 2 != 0) ? (e - e2) : (e - e2) / 1.414;
+      }
+      if (difftot > 0) {
+        for (int dir = 1; dir <= 8; ++dir) {
+          int i2 = std::min(std::max(i + kDX8[dir], 1), imax_);
+          int j2 = std::min(std::max(j + kDY8[dir], 1), jmax_);
+          const size_t c2 = at(i2, j2);
+          const double e2 = elev_[c2], e = elev_[c];
+          if (e2 < e) {
+            if (dir % 2 != 0) area_depth_[c2] += area_depth_[c] * ((e - e2) / difftot);
+            else area_depth_[c2] += area_depth_[c] * (((e - e2) / 1.414) / difftot);
+          }
+        }
+      }
+      area_depth_[c] = 0;
+    }
+  }
+  void refresh_elev() {
+    check(hc_download(ctx_, HC_FIELD_ELEV, elev_.data(), elev_.size()), "download elev");
+  }
+
+  // --------------------------------------------------- time series output
+  void output_data() {  // LCM.cpp:2310-2478
+    Qw_newvol_ += temptotal_ * ((cycle_ - previous_) * 60);
+    for (int nn = 1; nn <= rfnum_; ++nn)
+      Jw_newvol_ += (j_mean_[nn] * DX_ * DX_ * nActual_[nn]) * ((cycle_ - previous_) * 60);
+    const double osi = output_file_save_interval_;
+    if ((new_cycle_ < old_cycle_) || (cycle_ - previous_ >= osi)) {
+      while ((tx_ > previous_) && (cycle_ >= tx_)) {
+        hours_++;
+        Qs_step_ = globalsediq_ - old_sediq_;
+        Qs_over_ = Qs_step_ * ((cycle_ - tx_) / (cycle_ - tlastcalc_));
+        Qs_hour_ = Qs_step_ - Qs_over_ + Qs_last_;
+        if (cycle_ >= tx_ + osi) { Qs_last_ = 0; old_sediq_ = globalsediq_ - Qs_over_; }
+        if (cycle_ < tx_ + osi) { Qs_last_ = Qs_over_; old_sediq_ = globalsediq_; }
+        for (int n = 1; n <= kGMax - 1; ++n) {
+          Qg_step_[n] = sum_grain_[n] - old_sum_grain_[n];
+          Qg_over_[n] = Qg_step_[n] * ((cycle_ - tx_) / (cycle_ - tlastcalc_));
+          Qg_hour_[n] = Qg_step_[n] - Qg_over_[n] + Qg_last_[n];
+          if (cycle_ >= tx_ + osi) { Qg_last_[n] = 0; old_sum_grain_[n] = sum_grain_[n] - Qg_over_[n]; }
+          if (cycle_ < tx_ + osi) { Qg_last_[n] = Qg_over_[n]; old_sum_grain_[n] = sum_grain_[n]; }
+        }
+        Qw_overvol_ = temptotal_ * ((cycle_ - tx_) * 60);
+        Qw_stepvol_ = Qw_newvol_ - Qw_oldvol_;
+        Qw_hourvol_ = Qw_stepvol_ - Qw_overvol_ + Qw_lastvol_;
+        Qw_hour_ = Qw_hourvol_ / (60 * osi);
+        for (int nn = 1; nn <= rfnum_; ++nn)
+          Jw_overvol_ += (j_mean_[nn] * DX_ * DX_ * nActual_[nn]) * ((cycle_ - tx_) * 60);
+        Jw_stepvol_ = Jw_newvol_ - Jw_oldvol_;
+        Jw_hourvol_ = Jw_stepvol_ - Jw_overvol_ + Jw_lastvol_;
+        Jw_hour_ = Jw_hourvol_ / (60 * osi);
+        if (cycle_ >= tx_ + osi) {
+          Qw_lastvol_ = 0; Qw_oldvol_ = Qw_newvol_ - Qw_overvol_;
+          Jw_lastvol_ = 0; Jw_oldvol_ = Jw_newvol_ - Jw_overvol_;
+        }
+        if (cycle_ < tx_ + osi) {
+          Qw_lastvol_ = Qw_overvol_; Qw_oldvol_ = Qw_newvol_;
+          Jw_lastvol_ = Jw_overvol_; Jw_oldvol_ = Jw_newvol_;
+        }
+        const double Tx = tx_;
+        tx_ = Tx + osi;
+        const double row[5] = {static_cast<double>(hours_), Qw_hour_, Jw_hour_, sand_out_, Qs_hour_};
+        series_.insert(series_.end(), row, row + 5);
+        for (int n = 1; n <= kGMax - 1; ++n) series_.push_back(Qg_hour_[n]);
+        if (opt_.write_files) append_series_row();
+        sand_out_ = 0;
+      }
+      tlastcalc_ = cycle_;
+    }
+  }
+  void append_series_row() {  // column formats of LCM.cpp:2427-2474
+    std::ostringstream os;
+    const double* r = &series_[series_.size() - 14];
+    os << std::fixed << std::setprecision(0) << hours_;
+    os << " " << std::fixed << std::setprecision(6) << r[1];
+    os << " " << std::fixed << std::setprecision(6) << r[2];
+    os << " " << std::fixed << std::setprecision(6) << r[3];
+    os << " " << std::fixed << std::setprecision(10) << r[4];
+    for (int n = 0; n < 9; ++n) os << " " << std::fixed << std::setprecision(10) << r[5 + n];
+    std::ofstream f(write_path_ + "/" + write_fname_, std::ios_base::app | std::ios_base::out);
+    f << os.str() << std::endl;
+  }
+
+  // ASCII raster without the padding ring, 6 significant digits
+  // (LSDRaster::write_double_asc_raster, src/topotools/LSDRaster.cpp:1194-1221).
+  void write_asc(const std::string& fname, const std::vector<double>& a) const {
+    std::ofstream out(fname);
+    if (!out) throw Error("unable to write to " + fname);
+    out << "ncols\t" << jmax_ << "\nnrows\t" << imax_ << "\nxllcorner\t" << std::setprecision(14)
+        << xll_ << "\nyllcorner\t" << std::setprecision(14) << yll_ << "\ncellsize\t" << DX_
+        << "\nNODATA_value\t" << no_data_value_ << std::endl;
+    for (int i = 1; i <= imax_; ++i) {
+      for (int j = 1; j <= jmax_; ++j) out << std::setprecision(6) << a[at(i, j)] << " ";
+      if (i != imax_) out << std::endl;
+    }
+  }
+  void save_raster_data(double tempcycle) {  // LCM.cpp:2672-2808
+    if (!opt_.write_files) return;
+    const std::string suffix = std::to_string(static_cast<int>(tempcycle)) + "." + dem_write_extension_;
+    if (write_waterd_file_) {
+      std::vector<double> h(elev_.size());
+      check(hc_download(ctx_, HC_FIELD_WATER_DEPTH, h.data(), h.size()), "download h");
+      write_asc(write_path_ + "/" + waterdepth_fname_ + suffix, h);
+    }
+    if (write_elev_file_) {
+      std::vector<double> e(elev_.size());
+      check(hc_download(ctx_, HC_FIELD_ELEV, e.data(), e.size()), "download elev");
+      write_asc(write_path_ + "/" + elev_fname_ + suffix, e);
+    }
+  }
+
+  hcm_options opt_;
+  hc_ctx* ctx_ = nullptr;
+  // inputs
+  std::string dem_read_extension_ = "asc", dem_write_extension_ = "asc", write_path_, read_path_;
+  std::string write_fname_ = "catchment.dat", read_fname_, hydroindex_fname_, rainfall_data_file_,
+              grain_data_file_, bedrock_data_file_, water_init_raster_file_, mannings_fname_,
+              elev_fname_, waterdepth_fname_;
+  int imax_ = 0, jmax_ = 0, NR_ = 0, NC_ = 0;
+  double xll_ = 0, yll_ = 0, DX_ = 5.0, no_data_value_ = -9999.9;
+  // parameters (LSDCatchmentModel.hpp:437-568 defaults)
+  double water_depth_erosion_threshold_ = 0.01, in_out_difference_allowed_ = 0, mannings_ = 0.04;
+  int rfnum_ = 1, maxcycle_ = 1000, max_time_step_ = 0;
+  double ERODEFACTOR_ = 0.05, MIN_Q_ = 0.01, MIN_Q_MAXVAL_ = 1000.0, active_ = 0.2;
+  double M_ = 0.005, baseflow_ = 0.00000005, rain_factor_ = 1;
+  double output_file_save_interval_ = 60, max_vel_ = 5, courant_number_ = 0.7;
+  double edgeslope_ = 0.001, chann_lateral_erosion_ = 20.0, veg_lat_restriction_ = 0.1;
+  double froude_limit_ = 0.8, Csuspmax_ = 0.05, hflow_threshold_ = 0.00001;
+  double lateral_constant_ = 0.0000002, bedrock_erosion_threshold_ = 0, bedrock_erosion_rate_ = 0;
+  double saveinterval_ = 1000, rain_data_time_step_ = 60;
+  double d_[10] = {0.0, 0.000065, 0.001, 0.002, 0.004, 0.008, 0.016, 0.032, 0.064, 0.128};
+  double dprop_[11] = {0.0, 0.05, 0.05, 0.15, 0.225, 0.25, 0.1, 0.075, 0.05, 0.05, 0.0};
+  double fall_[11] = {0.0, 0.033, 0.109, 0.164, 0.237, 0.338, 0.479, 0.678, 0.959, 1.357, 0.0};
+  bool hydro_only_ = false, rainfall_data_on_ = false, water_init_from_raster_ = false,
+       spatially_var_rainfall_ = false, spatially_var_mannings_ = false,
+       graindata_from_file_ = false, bedrock_layer_on_ = false, wilcock_ = false,
+       einstein_ = false, write_elev_file_ = false, write_waterd_file_ = false,
+       debug_print_cycle_ = false, warned_landslide_ = false;
+  // state
+  std::vector<double> elev_, area_, area_depth_;
+  std::vector<int> rfarea_;
+  std::vector<std::vector<float>> rain_;
+  std::vector<double> j_, jo_, j_mean_, old_j_mean_, new_j_mean_;
+  std::vector<int> nActual_, input_counter_, in_x_, in_y_;
+  std::vector<int> pts_x_, pts_y_;
+  std::vector<double> pts_a_;
+  int totalinputpoints_ = 0, counter_ = 0, hours_ = 0;
+  double time_step_ = 1, cycle_ = 0, maxdepth_ = 10, temptot_ = 0, waterinput_ = 0,
+         waterOut_ = 0, iod_ = 0, time_1_ = 1, save_time_ = 0, previous_ = 0, new_cycle_ = 0,
+         old_cycle_ = 0, erode_mult_ = 1, erode_call_ = 0, globalsediq_ = 0, sediQ_ = 0,
+         last_dt_ = 0;
+  double tx_ = 60, tlastcalc_ = 0, Qs_step_ = 0, Qs_hour_ = 0, Qs_over_ = 0, Qs_last_ = 0,
+         Qw_newvol_ = 0, Qw_oldvol_ = 0, Qw_lastvol_ = 0, Qw_stepvol_ = 0, Qw_hourvol_ = 0,
+         Qw_hour_ = 0, Qw_overvol_ = 0, temptotal_ = 0, old_sediq_ = 0, Jw_newvol_ = 0,
+         Jw_oldvol_ = 0, Jw_lastvol_ = 0, Jw_stepvol_ = 0, Jw_hourvol_ = 0, Jw_hour_ = 0,
+         Jw_overvol_ = 0, sand_out_ = 0;
+  std::vector<double> sum_grain_, old_sum_grain_, Qg_step_, Qg_hour_, Qg_over_, Qg_last_;
+  std::vector<double> series_;
+};
+
+}  // namespace hcdrv
+
+struct hcm_model {
+  hcdrv::CatchmentModel* m = nullptr;
+  std::string err;
+};
+
+extern "C" {
+
+int hcm_create(hcm_model** out, const char* path, const char* pfile, const hcm_options* opt) {
+  if (!out || !path || !pfile) return HC_ERR_ARG;
+  *out = nullptr;
+  hcm_model* h = new hcm_model;
+  hcm_options o{};
+  o.precision = HC_FP64;
+  o.fused = 1;
+  o.quiet = 1;
+  o.write_files = 1;
+  o.flat_dz = 0.0;
+  if (opt) o = *opt;
+  try {
+    h->m = new hcdrv::CatchmentModel(path, pfile, o);
+    h->m->initialise_model_domain_extents();
+    h->m->initialise_arrays();
+    h->m->load_data();
+    h->m->set_time_counters();
+    h->m->initialise_rainfall_runoff();
+    h->m->initialise_drainage_area();
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    // report through a model that holds only the message
+    delete h->m;
+    h->m = nullptr;
+    *out = h;
+    return HC_ERR_STATE;
+  }
+  *out = h;
+  return HC_OK;
+}
+
+int hcm_destroy(hcm_model* h) {
+  if (!h) return HC_OK;
+  delete h->m;
+  delete h;
+  return HC_OK;
+}
+
+const char* hcm_last_error(const hcm_model* h) { return h ? h->err.c_str() : "null model"; }
+
+int hcm_iterate(hcm_model* h, int n, double* tt, double* cy, double* dtf, int* done, int* finished) {
+  if (!h || !h->m) return HC_ERR_ARG;
+  int k = 0;
+  bool fin = false;
+  try {
+    for (; k < n && !fin; ++k) {
+      fin = h->m->iterate();
+      if (tt) tt[k] = h->m->temptot();
+      if (cy) cy[k] = h->m->get_cycle();
+      if (dtf) dtf[k] = h->m->last_dt();
+    }
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    if (done) *done = k;
+    return HC_ERR_HIP;
+  }
+  if (done) *done = k;
+  if (finished) *finished = fin ? 1 : 0;
+  return HC_OK;
+}
+
+size_t hcm_field_count(const hcm_model* h, int field) {
+  if (!h || !h->m) return 0;
+  return hc_field_count(const_cast<hcdrv::CatchmentModel*>(h->m)->ctx(), field);
+}
+
+int hcm_download(hcm_model* h, int field, void* host, size_t count) {
+  if (!h || !h->m) return HC_ERR_ARG;
+  const int rc = hc_download(h->m->ctx(), field, host, count);
+  if (rc) h->err = hc_last_error(h->m->ctx());
+  return rc;
+}
+
+int hcm_scalar(hcm_model* h, const char* name, double* v) {
+  if (!h || !h->m || !name || !v) return HC_ERR_ARG;
+  return h->m->scalar(name, v) ? HC_OK : HC_ERR_ARG;
+}
+
+int hcm_series(hcm_model* h, double* out, int max_rows) {
+  if (!h || !h->m) return 0;
+  const auto& s = h->m->series();
+  const int rows = static_cast<int>(s.size() / 14);
+  const int nc = std::min(rows, max_rows);
+  if (out && nc > 0) std::memcpy(out, s.data(), sizeof(double) * 14 * nc);
+  return rows;
+}
+
+}  // extern "C"

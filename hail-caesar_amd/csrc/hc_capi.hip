@@ -1,0 +1,464 @@
+// C-ABI entry points of libhc.so (include/hc.h): context lifetime, host <->
+// device layout conversion, and the per-method calls of the hot path.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "hc_internal.hpp"
+
+int hc_fail(hc_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+namespace {
+
+int planes_of(int field) {
+  switch (field) {
+    case HC_FIELD_VEL_DIR: return 9;
+    case HC_FIELD_GRAIN: return 11;
+    case HC_FIELD_STRATA: return 90;
+    case HC_FIELD_SR: case HC_FIELD_SL: case HC_FIELD_SU: case HC_FIELD_SD: return 10;
+    default: return 1;
+  }
+}
+
+bool is_sediment_field(int f) {
+  return f == HC_FIELD_VSUSP || f == HC_FIELD_QXS || f == HC_FIELD_QYS || f == HC_FIELD_VEL_DIR ||
+         f == HC_FIELD_GRAIN || f == HC_FIELD_STRATA || f == HC_FIELD_HASGRAIN ||
+         f == HC_FIELD_BEDROCK || f == HC_FIELD_TAU || f == HC_FIELD_VEL || f == HC_FIELD_SS ||
+         f == HC_FIELD_SR || f == HC_FIELD_SL || f == HC_FIELD_SU || f == HC_FIELD_SD;
+}
+
+template <typename T>
+int dalloc(hc_ctx* c, T** p, size_t n) {
+  HC_HIP(c, hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
+  HC_HIP(c, hipMemsetAsync(*p, 0, n * sizeof(T), c->stream));
+  return HC_OK;
+}
+
+// Copy one plane between a host (NR x NC, stride `hstride` elements between
+// consecutive cells, offset `hoff`) array of doubles and a device plane.
+int put_plane(hc_ctx* c, void* dev, bool dev_float, const double* host, int hstride, int hoff) {
+  const size_t n = c->ncell;
+  if (dev_float) {
+    std::vector<float> tmp(n, 0.0f);
+    for (int x = 0; x < c->NR; ++x)
+      for (int y = 0; y < c->NC; ++y)
+        tmp[static_cast<size_t>(x) * c->P + y] =
+            static_cast<float>(host[(static_cast<size_t>(x) * c->NC + y) * hstride + hoff]);
+    HC_HIP(c, hipMemcpy(dev, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    if (hstride == 1) {
+      HC_HIP(c, hipMemcpy2D(dev, c->P * sizeof(double), host, c->NC * sizeof(double),
+                            c->NC * sizeof(double), c->NR, hipMemcpyHostToDevice));
+    } else {
+      std::vector<double> tmp(n, 0.0);
+      for (int x = 0; x < c->NR; ++x)
+        for (int y = 0; y < c->NC; ++y)
+          tmp[static_cast<size_t>(x) * c->P + y] =
+              host[(static_cast<size_t>(x) * c->NC + y) * hstride + hoff];
+      HC_HIP(c, hipMemcpy(dev, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  return HC_OK;
+}
+
+int get_plane(hc_ctx* c, const void* dev, bool dev_float, double* host, int hstride, int hoff) {
+  const size_t n = c->ncell;
+  if (dev_float) {
+    std::vector<float> tmp(n);
+    HC_HIP(c, hipMemcpy(tmp.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (int x = 0; x < c->NR; ++x)
+      for (int y = 0; y < c->NC; ++y)
+        host[(static_cast<size_t>(x) * c->NC + y) * hstride + hoff] =
+            tmp[static_cast<size_t>(x) * c->P + y];
+  } else {
+    if (hstride == 1) {
+      HC_HIP(c, hipMemcpy2D(host, c->NC * sizeof(double), dev, c->P * sizeof(double),
+                            c->NC * sizeof(double), c->NR, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<double> tmp(n);
+      HC_HIP(c, hipMemcpy(tmp.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int x = 0; x < c->NR; ++x)
+        for (int y = 0; y < c->NC; ++y)
+          host[(static_cast<size_t>(x) * c->NC + y) * hstride + hoff] =
+              tmp[static_cast<size_t>(x) * c->P + y];
+    }
+  }
+  return HC_OK;
+}
+
+int put_int_plane(hc_ctx* c, int32_t* dev, const int32_t* host) {
+  std::vector<int32_t> tmp(c->ncell, 0);
+  for (int x = 0; x < c->NR; ++x)
+    for (int y = 0; y < c->NC; ++y)
+      tmp[static_cast<size_t>(x) * c->P + y] = host[static_cast<size_t>(x) * c->NC + y];
+  HC_HIP(c, hipMemcpy(dev, tmp.data(), c->ncell * sizeof(int32_t), hipMemcpyHostToDevice));
+  return HC_OK;
+}
+
+int get_int_plane(hc_ctx* c, const int32_t* dev, int32_t* host) {
+  std::vector<int32_t> tmp(c->ncell);
+  HC_HIP(c, hipMemcpy(tmp.data(), dev, c->ncell * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int x = 0; x < c->NR; ++x)
+    for (int y = 0; y < c->NC; ++y)
+      host[static_cast<size_t>(x) * c->NC + y] = tmp[static_cast<size_t>(x) * c->P + y];
+  return HC_OK;
+}
+
+int read_scalar(hc_ctx* c, const void* dev, double* out, bool as_bits) {
+  HC_HIP(c, hipMemcpyAsync(c->h_pinned, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  if (as_bits) {
+    unsigned long long b;
+    std::memcpy(&b, c->h_pinned, sizeof(b));
+    double v;
+    std::memcpy(&v, &b, sizeof(v));
+    *out = v;
+  } else {
+    *out = c->h_pinned[0];
+  }
+  return HC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
+  if (!out || !grid || !params) return HC_ERR_ARG;
+  *out = nullptr;
+  if (grid->imax < 1 || grid->jmax < 1 || !(grid->DX > 0)) return HC_ERR_ARG;
+  if (grid->precision != HC_FP64 && grid->precision != HC_FP32) return HC_ERR_ARG;
+  if (grid->sediment && grid->precision != HC_FP64) return HC_ERR_ARG;
+  hc_ctx* c = new (std::nothrow) hc_ctx;
+  if (!c) return HC_ERR_ARG;
+  c->g = *grid;
+  c->prm = *params;
+  c->prec = grid->precision;
+  c->esz = (c->prec == HC_FP64) ? 8 : 4;
+  c->NR = grid->imax + 2;
+  c->NC = grid->jmax + 2;
+  c->P = ((c->NC + 63) / 64) * 64;
+  c->W = c->P / 64;
+  c->ncell = static_cast<size_t>(c->NR) * c->P;
+  int rc = HC_OK;
+  auto fail = [&](int r) {
+    hc_destroy(c);
+    return r;
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return HC_ERR_HIP;
+  }
+  const size_t n = c->ncell;
+  for (int k = 0; k < 2; ++k) {
+    if (c->prec == HC_FP64) {
+      if ((rc = dalloc(c, reinterpret_cast<double**>(&c->h[k]), n)) ||
+          (rc = dalloc(c, reinterpret_cast<double**>(&c->qx[k]), n)) ||
+          (rc = dalloc(c, reinterpret_cast<double**>(&c->qy[k]), n)))
+        return fail(rc);
+    } else {
+      if ((rc = dalloc(c, reinterpret_cast<float**>(&c->h[k]), n)) ||
+          (rc = dalloc(c, reinterpret_cast<float**>(&c->qx[k]), n)) ||
+          (rc = dalloc(c, reinterpret_cast<float**>(&c->qy[k]), n)))
+        return fail(rc);
+    }
+  }
+  if (c->prec == HC_FP64) {
+    if ((rc = dalloc(c, reinterpret_cast<double**>(&c->z), n))) return fail(rc);
+    if (params->spatially_var_mannings &&
+        (rc = dalloc(c, reinterpret_cast<double**>(&c->nman), n)))
+      return fail(rc);
+    if ((rc = hc_launch_fill(c, c->z, n, -9999.0, false))) return fail(rc);
+  } else {
+    if ((rc = dalloc(c, reinterpret_cast<float**>(&c->z), n))) return fail(rc);
+    if ((rc = dalloc(c, &c->z64, n))) return fail(rc);
+    if (params->spatially_var_mannings &&
+        (rc = dalloc(c, reinterpret_cast<float**>(&c->nman), n)))
+      return fail(rc);
+    if ((rc = hc_launch_fill(c, c->z, n, -9999.0, true))) return fail(rc);
+    if ((rc = hc_launch_fill(c, c->z64, n, -9999.0, false))) return fail(rc);
+  }
+  if ((rc = dalloc(c, &c->mask, static_cast<size_t>(c->NR) * c->W))) return fail(rc);
+  if (grid->sediment) {
+    if ((rc = dalloc(c, &c->V, n)) || (rc = dalloc(c, &c->qxs, n)) ||
+        (rc = dalloc(c, &c->qys, n)) || (rc = dalloc(c, &c->veldir, 4 * n)) ||
+        (rc = dalloc(c, &c->grain, 11 * n)) || (rc = dalloc(c, &c->strata, 90 * n)) ||
+        (rc = dalloc(c, &c->hasgrain, n)) || (rc = dalloc(c, &c->needgrain, n)) ||
+        (rc = dalloc(c, &c->bedrock, n)) || (rc = dalloc(c, &c->Tau, n)) ||
+        (rc = dalloc(c, &c->Vel, n)) || (rc = dalloc(c, &c->ss, n)) ||
+        (rc = dalloc(c, &c->sdir, 40 * n)) || (rc = dalloc(c, &c->etot, n)) ||
+        (rc = dalloc(c, &c->etot3, n)) || (rc = dalloc(c, &c->gtot, 20)))
+      return fail(rc);
+    if ((rc = hc_launch_fill(c, c->bedrock, n, -9999.0, false))) return fail(rc);
+  }
+  if ((rc = dalloc(c, &c->d_red, 8)) || (rc = dalloc(c, &c->d_partials, 1024)) ||
+      (rc = dalloc(c, &c->d_count, 16)) || (rc = dalloc(c, &c->d_out, 8)))
+    return fail(rc);
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 8 * sizeof(double)) != hipSuccess)
+    return fail(HC_ERR_HIP);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(HC_ERR_HIP);
+  *out = c;
+  return HC_OK;
+}
+
+int hc_destroy(hc_ctx* c) {
+  if (!c) return HC_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->h[0], c->h[1], c->qx[0], c->qx[1], c->qy[0], c->qy[1], c->z, c->z64,
+                  c->nman, c->mask, c->V, c->qxs, c->qys, c->veldir, c->grain, c->strata,
+                  c->hasgrain, c->needgrain, c->bedrock, c->Tau, c->Vel, c->ss, c->sdir,
+                  c->etot, c->etot3, c->gtot, c->d_red, c->d_partials, c->d_count, c->d_out,
+                  c->d_pts, c->d_amt};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return HC_OK;
+}
+
+const char* hc_last_error(const hc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+size_t hc_field_count(const hc_ctx* c, int field) {
+  if (!c) return 0;
+  return static_cast<size_t>(c->NR) * c->NC * planes_of(field);
+}
+
+int hc_upload(hc_ctx* c, int field, const void* host, size_t count) {
+  if (!c || !host) return HC_ERR_ARG;
+  if (count != hc_field_count(c, field))
+    return hc_fail(c, HC_ERR_ARG, "upload: element count does not match the grid");
+  if (is_sediment_field(field) && !c->g.sediment)
+    return hc_fail(c, HC_ERR_STATE, "upload: sediment field on a hydro-only context");
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  const double* d = static_cast<const double*>(host);
+  const bool f32 = (c->prec == HC_FP32);
+  switch (field) {
+    case HC_FIELD_WATER_DEPTH: return put_plane(c, c->h[c->cur], f32, d, 1, 0);
+    case HC_FIELD_QX: return put_plane(c, c->qx[c->cur], f32, d, 1, 0);
+    case HC_FIELD_QY: return put_plane(c, c->qy[c->cur], f32, d, 1, 0);
+    case HC_FIELD_ELEV: {
+      int rc = put_plane(c, c->z, f32, d, 1, 0);
+      if (rc == HC_OK && f32) rc = put_plane(c, c->z64, false, d, 1, 0);
+      return rc;
+    }
+    case HC_FIELD_MANNINGS:
+      if (!c->nman) return hc_fail(c, HC_ERR_STATE, "upload: context has uniform Manning n");
+      return put_plane(c, c->nman, f32, d, 1, 0);
+    case HC_FIELD_VSUSP: return put_plane(c, c->V, false, d, 1, 0);
+    case HC_FIELD_QXS: return put_plane(c, c->qxs, false, d, 1, 0);
+    case HC_FIELD_QYS: return put_plane(c, c->qys, false, d, 1, 0);
+    case HC_FIELD_BEDROCK: return put_plane(c, c->bedrock, false, d, 1, 0);
+    case HC_FIELD_TAU: return put_plane(c, c->Tau, false, d, 1, 0);
+    case HC_FIELD_VEL: return put_plane(c, c->Vel, false, d, 1, 0);
+    case HC_FIELD_SS: return put_plane(c, c->ss, false, d, 1, 0);
+    case HC_FIELD_VEL_DIR: {
+      const int dirs[4] = {1, 3, 5, 7};
+      for (int k = 0; k < 4; ++k) {
+        int rc = put_plane(c, c->veldir + k * c->ncell, false, d, 9, dirs[k]);
+        if (rc) return rc;
+      }
+      return HC_OK;
+    }
+    case HC_FIELD_GRAIN:
+      for (int k = 0; k < 11; ++k) {
+        int rc = put_plane(c, c->grain + k * c->ncell, false, d, 11, k);
+        if (rc) return rc;
+      }
+      return HC_OK;
+    case HC_FIELD_STRATA:
+      for (int k = 0; k < 90; ++k) {
+        int rc = put_plane(c, c->strata + k * c->ncell, false, d, 90, k);
+        if (rc) return rc;
+      }
+      return HC_OK;
+    case HC_FIELD_SU: case HC_FIELD_SR: case HC_FIELD_SD: case HC_FIELD_SL: {
+      const int dir = (field == HC_FIELD_SU) ? 0 : (field == HC_FIELD_SR) ? 1 : (field == HC_FIELD_SD) ? 2 : 3;
+      for (int k = 0; k < 10; ++k) {
+        int rc = put_plane(c, c->sdir + (dir * 10 + k) * c->ncell, false, d, 10, k);
+        if (rc) return rc;
+      }
+      return HC_OK;
+    }
+    case HC_FIELD_HASGRAIN:
+      return put_int_plane(c, c->hasgrain, static_cast<const int32_t*>(host));
+    case HC_FIELD_MASK: {
+      const int32_t* m = static_cast<const int32_t*>(host);
+      std::vector<uint64_t> bits(static_cast<size_t>(c->NR) * c->W, 0);
+      for (int x = 1; x <= c->g.imax; ++x)
+        for (int y = 1; y <= c->g.jmax; ++y)
+          if (m[static_cast<size_t>(x) * c->NC + y])
+            bits[static_cast<size_t>(x) * c->W + (y >> 6)] |= 1ull << (y & 63);
+      HC_HIP(c, hipMemcpy(c->mask, bits.data(), bits.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+      return HC_OK;
+    }
+    default:
+      return hc_fail(c, HC_ERR_ARG, "upload: unknown field");
+  }
+}
+
+int hc_download(hc_ctx* c, int field, void* host, size_t count) {
+  if (!c || !host) return HC_ERR_ARG;
+  if (count != hc_field_count(c, field))
+    return hc_fail(c, HC_ERR_ARG, "download: element count does not match the grid");
+  if (is_sediment_field(field) && !c->g.sediment)
+    return hc_fail(c, HC_ERR_STATE, "download: sediment field on a hydro-only context");
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  double* d = static_cast<double*>(host);
+  const bool f32 = (c->prec == HC_FP32);
+  switch (field) {
+    case HC_FIELD_WATER_DEPTH: return get_plane(c, c->h[c->cur], f32, d, 1, 0);
+    case HC_FIELD_QX: return get_plane(c, c->qx[c->cur], f32, d, 1, 0);
+    case HC_FIELD_QY: return get_plane(c, c->qy[c->cur], f32, d, 1, 0);
+    case HC_FIELD_ELEV: return get_plane(c, f32 ? static_cast<void*>(c->z64) : c->z, false, d, 1, 0);
+    case HC_FIELD_MANNINGS:
+      if (!c->nman) return hc_fail(c, HC_ERR_STATE, "download: context has uniform Manning n");
+      return get_plane(c, c->nman, f32, d, 1, 0);
+    case HC_FIELD_VSUSP: return get_plane(c, c->V, false, d, 1, 0);
+    case HC_FIELD_QXS: return get_plane(c, c->qxs, false, d, 1, 0);
+    case HC_FIELD_QYS: return get_plane(c, c->qys, false, d, 1, 0);
+    case HC_FIELD_BEDROCK: return get_plane(c, c->bedrock, false, d, 1, 0);
+    case HC_FIELD_TAU: return get_plane(c, c->Tau, false, d, 1, 0);
+    case HC_FIELD_VEL: return get_plane(c, c->Vel, false, d, 1, 0);
+    case HC_FIELD_SS: return get_plane(c, c->ss, false, d, 1, 0);
+    case HC_FIELD_VEL_DIR: {
+      std::fill(d, d + count, 0.0);
+      const int dirs[4] = {1, 3, 5, 7};
+      for (int k = 0; k < 4; ++k) {
+        int rc = get_plane(c, c->veldir + k * c->ncell, false, d, 9, dirs[k]);
+        if (rc) return rc;
+      }
+      return HC_OK;
+    }
+    case HC_FIELD_GRAIN:
+      for (int k = 0; k < 11; ++k) {
+        int rc = get_plane(c, c->grain + k * c->ncell, false, d, 11, k);
+        if (rc) return rc;
+      }
+      return HC_OK;
+    case HC_FIELD_STRATA:
+      for (int k = 0; k < 90; ++k) {
+        int rc = get_plane(c, c->strata + k * c->ncell, false, d, 90, k);
+        if (rc) return rc;
+      }
+      return HC_OK;
+    case HC_FIELD_SU: case HC_FIELD_SR: case HC_FIELD_SD: case HC_FIELD_SL: {
+      const int dir = (field == HC_FIELD_SU) ? 0 : (field == HC_FIELD_SR) ? 1 : (field == HC_FIELD_SD) ? 2 : 3;
+      for (int k = 0; k < 10; ++k) {
+        int rc = get_plane(c, c->sdir + (dir * 10 + k) * c->ncell, false, d, 10, k);
+        if (rc) return rc;
+      }
+      return HC_OK;
+    }
+    case HC_FIELD_HASGRAIN:
+      return get_int_plane(c, c->hasgrain, static_cast<int32_t*>(host));
+    case HC_FIELD_MASK: {
+      int32_t* m = static_cast<int32_t*>(host);
+      std::vector<uint64_t> bits(static_cast<size_t>(c->NR) * c->W, 0);
+      HC_HIP(c, hipMemcpy(bits.data(), c->mask, bits.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      for (int x = 0; x < c->NR; ++x)
+        for (int y = 0; y < c->NC; ++y)
+          m[static_cast<size_t>(x) * c->NC + y] =
+              static_cast<int32_t>((bits[static_cast<size_t>(x) * c->W + (y >> 6)] >> (y & 63)) & 1ull);
+      return HC_OK;
+    }
+    default:
+      return hc_fail(c, HC_ERR_ARG, "download: unknown field");
+  }
+}
+
+int hc_flow_route(hc_ctx* c, double dt) {
+  if (!c) return HC_ERR_ARG;
+  return hc_launch_flow_route(c, dt);
+}
+
+int hc_depth_update(hc_ctx* c, double dt, double* maxdepth) {
+  if (!c) return HC_ERR_ARG;
+  int rc = hc_launch_depth_update(c, dt);
+  if (rc) return rc;
+  if (maxdepth) return read_scalar(c, c->d_red, maxdepth, true);
+  return HC_OK;
+}
+
+int hc_flow_depth(hc_ctx* c, double dt, double* maxdepth) {
+  if (!c) return HC_ERR_ARG;
+  int rc = hc_launch_flow_depth(c, dt);
+  if (rc) return rc;
+  if (maxdepth) return read_scalar(c, c->d_red, maxdepth, true);
+  return HC_OK;
+}
+
+int hc_scan_area(hc_ctx* c) {
+  if (!c) return HC_ERR_ARG;
+  return hc_launch_scan(c);
+}
+
+int hc_water_flux_out(hc_ctx* c, double time_step, double* temptot) {
+  if (!c) return HC_ERR_ARG;
+  int rc = hc_launch_flux_out(c, time_step);
+  if (rc) return rc;
+  if (temptot) return read_scalar(c, c->d_out, temptot, false);
+  return HC_OK;
+}
+
+int hc_add_point_inputs(hc_ctx* c, const int* x, const int* y, const double* amt, int n) {
+  if (!c || n < 0 || (n > 0 && (!x || !y || !amt))) return HC_ERR_ARG;
+  if (n == 0) return HC_OK;
+  if (n > c->pts_cap) {
+    HC_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->d_pts) (void)hipFree(c->d_pts);
+    if (c->d_amt) (void)hipFree(c->d_amt);
+    c->d_pts = nullptr;
+    c->d_amt = nullptr;
+    const int cap = std::max(n, 1024);
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_pts), cap * sizeof(int)));
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_amt), cap * sizeof(double)));
+    c->pts_cap = cap;
+  }
+  std::vector<int> idx(n);
+  for (int k = 0; k < n; ++k) {
+    if (x[k] < 0 || x[k] >= c->NR || y[k] < 0 || y[k] >= c->NC)
+      return hc_fail(c, HC_ERR_ARG, "add_point_inputs: point outside the padded grid");
+    idx[k] = x[k] * c->P + y[k];
+  }
+  HC_HIP(c, hipMemcpyAsync(c->d_pts, idx.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HC_HIP(c, hipMemcpyAsync(c->d_amt, amt, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int rc = hc_launch_points(c, n);
+  if (rc) return rc;
+  // the host vectors die at return: the copies above are from pageable
+  // memory and complete before hipMemcpyAsync returns
+  return HC_OK;
+}
+
+int hc_erode(hc_ctx* c, double mult, double* time_step, double* tempbmax, float gtot[10]) {
+  if (!c || !time_step) return HC_ERR_ARG;
+  if (!c->g.sediment) return hc_fail(c, HC_ERR_STATE, "erode: context has no sediment state");
+  return hc_erode_impl(c, mult, time_step, tempbmax, gtot);
+}
+
+int hc_synchronize(hc_ctx* c) {
+  if (!c) return HC_ERR_ARG;
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  return HC_OK;
+}
+
+int hc_check_health(hc_ctx* c, long long* bad) {
+  if (!c) return HC_ERR_ARG;
+  int rc = hc_launch_health(c);
+  if (rc) return rc;
+  HC_HIP(c, hipMemcpyAsync(c->h_pinned, c->d_red + 2, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  unsigned long long n;
+  std::memcpy(&n, c->h_pinned, sizeof(n));
+  if (bad) *bad = static_cast<long long>(n);
+  if (n) return hc_fail(c, HC_ERR_UNSTABLE, "health: non-finite or negative water depth");
+  return HC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Internal declarations shared by the libhc translation units.
+//
+// Device layout: every 2-D field is stored row-major with the reference's
+// padding ring, (imax+2) rows by `pitch` columns, pitch = jmax+2 rounded up
+// to a multiple of 64 so each 64-lane wavefront reads one aligned run of a
+// row.  Columns jmax+2..pitch-1 are inert (elev = -9999, everything else 0).
+// The active mask is one bit per cell, one 64-bit word per wavefront-wide
+// run: word (x, w) holds columns 64w..64w+63 of row x.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hc.h"
+
+#define HC_WAVE 64
+
+namespace hc {
+
+constexpr double kGravity = 9.81;
+constexpr double kNoData = -9999.0;
+
+// Hydro kernel arguments (all passed by value).
+struct HydroArgs {
+  int imax, jmax;
+  int pitch;      // elements per row
+  int mwords;     // mask words per row
+  double DX;
+  double hthr;    // hflow_threshold
+  double edgeslope;
+  double froude;
+  double mannings;
+  int spatial_n;
+  double flat_dz; // FP32 form only
+};
+
+// Erosion arguments.
+struct ErodeArgs {
+  int imax, jmax, pitch, mwords;
+  double DX, wdet, ERODEFACTOR, active, Csuspmax, max_vel, mannings;
+  int spatial_n, wilcock, einstein;
+  double d[10], logd[10], dprop[11], fall[11];
+  double bedrock_thr, bedrock_rate, lateral_constant, chann_lateral, veg_lat;
+};
+
+}  // namespace hc
+
+struct hc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hc_grid g{};
+  hc_params prm{};
+  int NR = 0, NC = 0;   // padded rows/cols of the reference layout
+  int P = 0;            // device pitch (elements)
+  int W = 0;            // mask words per row
+  size_t ncell = 0;     // NR * P
+  int prec = HC_FP64;
+  size_t esz = 8;       // bytes per state element
+
+  // hydro state (element type double or float); h/qx/qy are ping-pong pairs
+  void* h[2] = {nullptr, nullptr};
+  void* qx[2] = {nullptr, nullptr};
+  void* qy[2] = {nullptr, nullptr};
+  int cur = 0;
+  void* z = nullptr;          // elevation (state type)
+  double* z64 = nullptr;      // FP32 form: FP64 elevation for near-flat faces
+  void* nman = nullptr;       // Manning n (state type), spatial only
+  uint64_t* mask = nullptr;
+
+  // sediment state (FP64)
+  double* V = nullptr;        // Vsusptot
+  double* qxs = nullptr;
+  double* qys = nullptr;
+  double* veldir = nullptr;   // 4 planes: directions 1,3,5,7
+  double* grain = nullptr;    // 11 planes
+  double* strata = nullptr;   // 90 planes (layer-major: z*9+n)
+  int32_t* hasgrain = nullptr;
+  int32_t* needgrain = nullptr;
+  double* bedrock = nullptr;
+  double* Tau = nullptr;
+  double* Vel = nullptr;
+  double* ss = nullptr;
+  double* sdir = nullptr;     // 4 directions x 10 planes: su, sr, sd, sl
+  double* etot = nullptr;     // erodetot
+  double* etot3 = nullptr;    // erodetot3
+  float* gtot = nullptr;      // 20 floats
+
+  // reductions and small device scalars
+  unsigned long long* d_red = nullptr;  // [0] maxdepth bits, [1] tempbmax bits, [2] health count
+  double* d_partials = nullptr;         // per-block partial sums
+  unsigned int* d_count = nullptr;      // last-block counters
+  double* d_out = nullptr;              // [0] temptot
+  double* h_pinned = nullptr;           // pinned host mirror (8 doubles)
+  int* d_pts = nullptr;                 // point input cells (x*P+y)
+  double* d_amt = nullptr;              // point input amounts
+  int pts_cap = 0;
+
+  std::string err;
+};
+
+// helpers implemented in hc_capi.hip
+int hc_fail(hc_ctx* c, int code, const std::string& msg);
+#define HC_HIP(ctx, call)                                                     \
+  do {                                                                        \
+    hipError_t _e = (call);                                                   \
+    if (_e != hipSuccess)                                                     \
+      return hc_fail((ctx), HC_ERR_HIP,                                       \
+                     std::string(#call) + ": " + hipGetErrorString(_e));      \
+  } while (0)
+
+// launchers (hc_hydro.hip)
+int hc_launch_flow_route(hc_ctx* c, double dt);
+int hc_launch_depth_update(hc_ctx* c, double dt);
+int hc_launch_flow_depth(hc_ctx* c, double dt);
+int hc_launch_scan(hc_ctx* c);
+int hc_launch_flux_out(hc_ctx* c, double ts);
+int hc_launch_points(hc_ctx* c, int n);
+int hc_launch_health(hc_ctx* c);
+int hc_launch_fill(hc_ctx* c, void* p, size_t n, double v, bool is_float);
+// erosion (hc_erode.hip)
+int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot);

@@ -1,0 +1,225 @@
+"""ctypes binding of the C-ABI in include/hc.h (libhc.so).
+
+Host arrays use the reference's padded layout ((imax+2) x (jmax+2), x=row,
+y=column contiguous; multi-plane fields planes-last).  The library must have
+been built (``python hail-caesar_amd/build.py``); importing this module on a
+machine without it raises, and nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhc.so")
+
+HC_OK, HC_ERR_ARG, HC_ERR_HIP, HC_ERR_STATE, HC_ERR_UNSTABLE, HC_ERR_HALTED = 0, -1, -2, -3, -4, -5
+HC_FP64, HC_FP32 = 0, 1
+HC_WILCOCK, HC_EINSTEIN = 1, 2
+
+FIELDS = {
+    "water_depth": 0, "elev": 1, "qx": 2, "qy": 3, "mannings": 4, "Vsusptot": 5,
+    "qxs": 6, "qys": 7, "vel_dir": 8, "mask": 9, "grain": 10, "strata": 11,
+    "hasgrain": 12, "bedrock": 13, "Tau": 14, "Vel": 15, "ss": 16, "sr": 17,
+    "sl": 18, "su": 19, "sd": 20,
+}
+PLANES = {"vel_dir": 9, "grain": 11, "strata": 90, "sr": 10, "sl": 10, "su": 10, "sd": 10}
+INT_FIELDS = {"mask", "hasgrain"}
+
+# exported symbols (must match include/hc.h)
+SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upload",
+           "hc_download", "hc_flow_route", "hc_depth_update", "hc_flow_depth",
+           "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
+           "hc_synchronize", "hc_check_health"]
+
+
+class hc_grid(ctypes.Structure):
+    _fields_ = [("imax", ctypes.c_int), ("jmax", ctypes.c_int), ("DX", ctypes.c_double),
+                ("precision", ctypes.c_int), ("sediment", ctypes.c_int)]
+
+
+class hc_params(ctypes.Structure):
+    _fields_ = [
+        ("courant_number", ctypes.c_double), ("froude_limit", ctypes.c_double),
+        ("hflow_threshold", ctypes.c_double), ("edgeslope", ctypes.c_double),
+        ("mannings", ctypes.c_double), ("spatially_var_mannings", ctypes.c_int),
+        ("water_depth_erosion_threshold", ctypes.c_double), ("ERODEFACTOR", ctypes.c_double),
+        ("active", ctypes.c_double), ("Csuspmax", ctypes.c_double), ("max_vel", ctypes.c_double),
+        ("transport_law", ctypes.c_int), ("d", ctypes.c_double * 10),
+        ("dprop", ctypes.c_double * 11), ("fallVelocity", ctypes.c_double * 11),
+        ("bedrock_erosion_threshold", ctypes.c_double), ("bedrock_erosion_rate", ctypes.c_double),
+        ("lateral_constant", ctypes.c_double), ("chann_lateral_erosion", ctypes.c_double),
+        ("veg_lat_restriction", ctypes.c_double), ("flat_dz", ctypes.c_double),
+    ]
+
+
+class HCError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libhc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with hail-caesar_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+        L.hc_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(hc_grid), ctypes.POINTER(hc_params)]
+        L.hc_destroy.argtypes = [vp]
+        L.hc_last_error.argtypes = [vp]
+        L.hc_last_error.restype = ctypes.c_char_p
+        L.hc_field_count.argtypes = [vp, i]
+        L.hc_field_count.restype = sz
+        L.hc_upload.argtypes = [vp, i, vp, sz]
+        L.hc_download.argtypes = [vp, i, vp, sz]
+        L.hc_flow_route.argtypes = [vp, d]
+        L.hc_depth_update.argtypes = [vp, d, ctypes.POINTER(d)]
+        L.hc_flow_depth.argtypes = [vp, d, ctypes.POINTER(d)]
+        L.hc_scan_area.argtypes = [vp]
+        L.hc_water_flux_out.argtypes = [vp, d, ctypes.POINTER(d)]
+        L.hc_add_point_inputs.argtypes = [vp, vp, vp, vp, i]
+        L.hc_erode.argtypes = [vp, d, ctypes.POINTER(d), ctypes.POINTER(d), vp]
+        L.hc_synchronize.argtypes = [vp]
+        L.hc_check_health.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+        _lib = L
+    return _lib
+
+
+def default_params(**kw) -> hc_params:
+    """Reference defaults (LSDCatchmentModel.hpp:437-568) overridden by kw."""
+    p = hc_params()
+    p.courant_number = 0.7
+    p.froude_limit = 0.8
+    p.hflow_threshold = 0.00001
+    p.edgeslope = 0.001
+    p.mannings = 0.04
+    p.spatially_var_mannings = 0
+    p.water_depth_erosion_threshold = 0.01
+    p.ERODEFACTOR = 0.05
+    p.active = 0.2
+    p.Csuspmax = 0.05
+    p.max_vel = 5.0
+    p.transport_law = HC_WILCOCK
+    d = [0.0, 0.000065, 0.001, 0.002, 0.004, 0.008, 0.016, 0.032, 0.064, 0.128]
+    dprop = [0.0, 0.05, 0.05, 0.15, 0.225, 0.25, 0.1, 0.075, 0.05, 0.05, 0.0]
+    fall = [0.0, 0.033, 0.109, 0.164, 0.237, 0.338, 0.479, 0.678, 0.959, 1.357, 0.0]
+    for k in range(10):
+        p.d[k] = d[k]
+    for k in range(11):
+        p.dprop[k] = dprop[k]
+        p.fallVelocity[k] = fall[k]
+    p.bedrock_erosion_threshold = 0.0
+    p.bedrock_erosion_rate = 0.0
+    p.lateral_constant = 0.0000002
+    p.chann_lateral_erosion = 20.0
+    p.veg_lat_restriction = 0.1
+    p.flat_dz = 0.0
+    for k, v in kw.items():
+        if k in ("d", "dprop", "fallVelocity"):
+            arr = getattr(p, k)
+            for j, x in enumerate(v):
+                arr[j] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Context:
+    """One device context (hc_ctx*)."""
+
+    def __init__(self, imax: int, jmax: int, DX: float, params: hc_params,
+                 precision: int = HC_FP64, sediment: bool = False):
+        self.L = lib()
+        g = hc_grid(imax, jmax, DX, precision, 1 if sediment else 0)
+        h = ctypes.c_void_p()
+        rc = self.L.hc_create(ctypes.byref(h), ctypes.byref(g), ctypes.byref(params))
+        if rc != HC_OK:
+            raise HCError(rc, "hc_create failed")
+        self.h = h
+        self.imax, self.jmax, self.DX = imax, jmax, DX
+        self.shape = (imax + 2, jmax + 2)
+        self.precision = precision
+        self.sediment = sediment
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.hc_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != HC_OK:
+            raise HCError(rc, self.L.hc_last_error(self.h).decode())
+
+    def upload(self, name: str, arr: np.ndarray) -> None:
+        fid = FIELDS[name]
+        if name in INT_FIELDS:
+            a = np.ascontiguousarray(arr, dtype=np.int32)
+        else:
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._check(self.L.hc_upload(self.h, fid, a.ctypes.data, a.size))
+
+    def download(self, name: str) -> np.ndarray:
+        fid = FIELDS[name]
+        k = PLANES.get(name, 1)
+        shape = self.shape if k == 1 else self.shape + (k,)
+        a = np.zeros(shape, dtype=np.int32 if name in INT_FIELDS else np.float64)
+        self._check(self.L.hc_download(self.h, fid, a.ctypes.data, a.size))
+        return a
+
+    def flow_route(self, dt: float) -> None:
+        self._check(self.L.hc_flow_route(self.h, dt))
+
+    def depth_update(self, dt: float) -> float:
+        md = ctypes.c_double()
+        self._check(self.L.hc_depth_update(self.h, dt, ctypes.byref(md)))
+        return md.value
+
+    def flow_depth(self, dt: float) -> float:
+        md = ctypes.c_double()
+        self._check(self.L.hc_flow_depth(self.h, dt, ctypes.byref(md)))
+        return md.value
+
+    def scan_area(self) -> None:
+        self._check(self.L.hc_scan_area(self.h))
+
+    def water_flux_out(self, time_step: float) -> float:
+        t = ctypes.c_double()
+        self._check(self.L.hc_water_flux_out(self.h, time_step, ctypes.byref(t)))
+        return t.value
+
+    def add_point_inputs(self, xs: Sequence[int], ys: Sequence[int], amts: Sequence[float]) -> None:
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        y = np.ascontiguousarray(ys, dtype=np.int32)
+        a = np.ascontiguousarray(amts, dtype=np.float64)
+        self._check(self.L.hc_add_point_inputs(self.h, x.ctypes.data, y.ctypes.data, a.ctypes.data, len(x)))
+
+    def erode(self, mult: float, time_step: float):
+        ts = ctypes.c_double(time_step)
+        tb = ctypes.c_double()
+        g = np.zeros(10, dtype=np.float32)
+        self._check(self.L.hc_erode(self.h, mult, ctypes.byref(ts), ctypes.byref(tb), g.ctypes.data))
+        return ts.value, tb.value, g
+
+    def synchronize(self) -> None:
+        self._check(self.L.hc_synchronize(self.h))
+
+    def check_health(self) -> int:
+        n = ctypes.c_longlong()
+        rc = self.L.hc_check_health(self.h, ctypes.byref(n))
+        if rc not in (HC_OK, HC_ERR_UNSTABLE):
+            self._check(rc)
+        return n.value

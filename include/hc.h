@@ -1,0 +1,182 @@
+/* hc.h - C-ABI of the MI355X-native HAIL-CAESAR hot path (libhc.so).
+ *
+ * The reference has no plugin or FFI layer: the hot path is the private
+ * member functions of LSDCatchmentModel acting on TNT arrays
+ * (include/catchmentmodel/LSDCatchmentModel.hpp:175-396), called in a fixed
+ * order by main() (src/main.cpp:139-204).  Each entry point below replaces one
+ * of those methods; the comment on each names the reference method (file:line,
+ * paths relative to the reference root, LCM.cpp =
+ * src/catchmentmodel/LSDCatchmentModel.cpp).
+ *
+ * Conventions
+ *   - Every function returns 0 (HC_OK) or a negative hc_status; the message
+ *     is available from hc_last_error(ctx).  Nothing throws across the ABI.
+ *   - Host arrays use the reference's padded layout: (imax+2) x (jmax+2)
+ *     row-major, x = row 1..imax (first index), y = column 1..jmax
+ *     (contiguous), a one-cell padding ring (LCM.cpp:1622-1705).  Multi-plane
+ *     fields are cell-major with planes last, exactly like the reference's
+ *     TNT::Array3D (vel_dir: 9 planes, grain: 11, strata: 10x9, sr/sl/su/sd: 10).
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - A context owns its device buffers and one HIP stream and is used by one
+ *     host thread at a time.  Contexts are independent of each other.
+ *   - Entry points that return a host scalar (maxdepth, temptot, tempbmax)
+ *     synchronise the context's stream; the others only enqueue work.
+ */
+#ifndef HC_H
+#define HC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hc_ctx hc_ctx;
+
+typedef enum hc_status {
+  HC_OK = 0,
+  HC_ERR_ARG = -1,      /* bad argument (size, field id, null pointer) */
+  HC_ERR_HIP = -2,      /* a HIP runtime call failed */
+  HC_ERR_STATE = -3,    /* call not valid in this context's configuration */
+  HC_ERR_UNSTABLE = -4, /* health check tripped (NaN/Inf/negative depth) */
+  HC_ERR_HALTED = -5    /* device-resident stepping stopped early */
+} hc_status;
+
+typedef enum hc_precision {
+  HC_FP64 = 0, /* plain form: FP64 state, reference operation order */
+  HC_FP32 = 1  /* FP32 state, FP64 mass-balance accumulators */
+} hc_precision;
+
+typedef enum hc_transport_law { HC_WILCOCK = 1, HC_EINSTEIN = 2 } hc_transport_law;
+
+/* Grid shape and storage options (LCM.cpp:92-138 reads imax/jmax/DX from the
+ * DEM header). */
+typedef struct hc_grid {
+  int imax;       /* rows (nrows) */
+  int jmax;       /* columns (ncols) */
+  double DX;      /* cell size, metres */
+  int precision;  /* hc_precision */
+  int sediment;   /* 1: allocate erosion state (hydro_model_only: no) */
+} hc_grid;
+
+/* Scalar model parameters.  Names and meanings follow the reference's
+ * members (LCM.hpp:437-568) as set by initialise_variables (LCM.cpp:842-1610). */
+typedef struct hc_params {
+  double courant_number;        /* courant_number */
+  double froude_limit;          /* froude_limit */
+  double hflow_threshold;       /* hflow_threshold */
+  double edgeslope;             /* edgeslope (slope_on_edge_cell) */
+  double mannings;              /* uniform Manning n (mannings_n) */
+  int spatially_var_mannings;   /* 1: per-cell n uploaded as HC_FIELD_MANNINGS */
+  double water_depth_erosion_threshold; /* 0.01 in the reference (LCM.hpp:448) */
+  double ERODEFACTOR;           /* erosion_limit */
+  double active;                /* active_layer_thickness */
+  double Csuspmax;              /* 0.05 (LCM.hpp:548) */
+  double max_vel;               /* max_tau_velocity */
+  int transport_law;            /* hc_transport_law */
+  double d[10];                 /* grain sizes d1..d9 in d[1..9] (LCM.hpp:557-565) */
+  double dprop[11];             /* default proportions (LCM.hpp:568) */
+  double fallVelocity[11];      /* LCM.cpp:1848-1860 */
+  double bedrock_erosion_threshold;
+  double bedrock_erosion_rate;
+  double lateral_constant;
+  double chann_lateral_erosion;
+  double veg_lat_restriction;
+  /* FP32 form only: faces whose single-precision elevation difference is
+   * smaller than this (metres) take the difference from the FP64 DEM. */
+  double flat_dz;
+} hc_params;
+
+/* Field identifiers for hc_upload / hc_download. */
+typedef enum hc_field {
+  HC_FIELD_WATER_DEPTH = 0, /* water_depth           double */
+  HC_FIELD_ELEV = 1,        /* elev                  double */
+  HC_FIELD_QX = 2,          /* qx                    double */
+  HC_FIELD_QY = 3,          /* qy                    double */
+  HC_FIELD_MANNINGS = 4,    /* spat_var_mannings     double */
+  HC_FIELD_VSUSP = 5,       /* Vsusptot              double */
+  HC_FIELD_QXS = 6,         /* qxs                   double */
+  HC_FIELD_QYS = 7,         /* qys                   double */
+  HC_FIELD_VEL_DIR = 8,     /* vel_dir (x,y,9)       double */
+  HC_FIELD_MASK = 9,        /* down_scan membership  int32 0/1 */
+  HC_FIELD_GRAIN = 10,      /* grain (x,y,11)        double, dense per cell */
+  HC_FIELD_STRATA = 11,     /* strata (x,y,10,9)     double, dense per cell */
+  HC_FIELD_HASGRAIN = 12,   /* index[x][y] != -9999  int32 0/1 */
+  HC_FIELD_BEDROCK = 13,    /* bedrock               double */
+  HC_FIELD_TAU = 14,        /* Tau                   double */
+  HC_FIELD_VEL = 15,        /* Vel                   double */
+  HC_FIELD_SS = 16,         /* ss                    double */
+  HC_FIELD_SR = 17,         /* sr (x,y,10)           double */
+  HC_FIELD_SL = 18,         /* sl (x,y,10)           double */
+  HC_FIELD_SU = 19,         /* su (x,y,10)           double */
+  HC_FIELD_SD = 20          /* sd (x,y,10)           double */
+} hc_field;
+
+/* ---- lifetime -------------------------------------------------------- */
+
+/* Replaces LSDCatchmentModel construction + initialise_arrays/zero_values
+ * (LCM.cpp:1617-1843, 2891-2982): allocates zeroed state with elev = -9999
+ * on device 0 (or the device current on the calling thread). */
+int hc_create(hc_ctx** ctx, const hc_grid* grid, const hc_params* params);
+int hc_destroy(hc_ctx* ctx);
+const char* hc_last_error(const hc_ctx* ctx);
+/* Element count of one host array of `field` for this grid. */
+size_t hc_field_count(const hc_ctx* ctx, int field);
+
+/* Copy a whole field between a host array in the reference layout and the
+ * device (load_data's array fills, LCM.cpp:140-606).  `count` must equal
+ * hc_field_count(ctx, field).  FP32 contexts convert on the way. */
+int hc_upload(hc_ctx* ctx, int field, const void* host, size_t count);
+int hc_download(hc_ctx* ctx, int field, void* host, size_t count);
+
+/* ---- the per-timestep hot path ---------------------------------------- */
+
+/* LSDCatchmentModel::flow_route (LCM.cpp:3063-3247): local-inertial qx/qy
+ * update on the cells of the active mask; with sediment state also
+ * qxs/qys and vel_dir.  `dt_flow` is get_flow_timestep() (LCM.cpp:1913). */
+int hc_flow_route(hc_ctx* ctx, double dt_flow);
+
+/* LSDCatchmentModel::depth_update (LCM.cpp:3259-3299): h (and Vsusptot) from
+ * the flux divergence; returns maxdepth (max h over updated cells, h > 0). */
+int hc_depth_update(hc_ctx* ctx, double dt_flow, double* maxdepth);
+
+/* flow_route followed by depth_update in one fused launch (hydro-only
+ * contexts).  Results are identical to the two calls above. */
+int hc_flow_depth(hc_ctx* ctx, double dt_flow, double* maxdepth);
+
+/* LSDCatchmentModel::scan_area (LCM.cpp:3821-3849): rebuild the active mask
+ * (3x3 wet dilation).  check_wetted_area (LCM.cpp:2019) calls it every 5th
+ * iteration. */
+int hc_scan_area(hc_ctx* ctx);
+
+/* LSDCatchmentModel::water_flux_out (LCM.cpp:3005-3047): drain the four
+ * edges to water_depth_erosion_threshold; returns temptot (m^3/s), summed
+ * in FP64. */
+int hc_water_flux_out(hc_ctx* ctx, double time_step, double* temptot);
+
+/* Input-point injection of catchment_water_input_and_hydrology
+ * (LCM.cpp:3463-3477): water_depth[x[k]][y[k]] += amt[k].  Points must be
+ * distinct cells. */
+int hc_add_point_inputs(hc_ctx* ctx, const int* x, const int* y,
+                        const double* amt, int n);
+
+/* LSDCatchmentModel::erode (LCM.cpp:4211-4845) with its grain helpers
+ * (LCM.cpp:3874-4171).  *time_step is read and updated exactly as the
+ * reference updates its member; returns tempbmax and the per-fraction edge
+ * outputs gtot2[0..9] (accumulated in float, as the reference's float
+ * gtot2[20]). */
+int hc_erode(hc_ctx* ctx, double mult_factor, double* time_step,
+             double* tempbmax, float gtot[10]);
+
+/* Block until every call enqueued on the context has finished. */
+int hc_synchronize(hc_ctx* ctx);
+
+/* Health check: counts non-finite or negative water depths on interior
+ * cells.  Returns HC_ERR_UNSTABLE when any are found (count in *bad). */
+int hc_check_health(hc_ctx* ctx, long long* bad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HC_H */

@@ -1,0 +1,63 @@
+/* hc_model.h - C-ABI of the C++ run-loop driver (libhc.so).
+ *
+ * The driver (hail-caesar_amd/csrc/driver/hc_model.cpp) is the host side of
+ * the drop-in: a C++ class with the method surface of LSDCatchmentModel
+ * (include/catchmentmodel/LSDCatchmentModel.hpp:73-423) whose loop is the one
+ * of src/main.cpp:139-204, with every hot-path method delegated to the
+ * device entry points of hc.h.  It reads the reference's parameter files,
+ * ASCII DEMs, rainfall and grain-data files and writes the reference's
+ * time-series and ASCII raster outputs.  These functions expose it to
+ * non-C++ callers (the Python tests and benchmark).
+ */
+#ifndef HC_MODEL_H
+#define HC_MODEL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hcm_model hcm_model;
+
+typedef struct hcm_options {
+  int precision;  /* hc_precision: HC_FP64 (plain form) or HC_FP32 */
+  int fused;      /* 1: flow_route+depth_update as one fused launch */
+  int quiet;      /* 1: no progress output on stdout */
+  int write_files;/* 1: write the time series and rasters like the reference */
+  double flat_dz; /* FP32 form: see hc_params.flat_dz */
+} hcm_options;
+
+/* Constructor + initialise_model_domain_extents + initialise_arrays +
+ * load_data + set_time_counters + initialise_rainfall_runoff +
+ * initialise_drainage_area (src/main.cpp:103-120).  `path` is prepended to
+ * `pfile` exactly as the reference does (pname + pfname). */
+int hcm_create(hcm_model** m, const char* path, const char* pfile, const hcm_options* opt);
+int hcm_destroy(hcm_model* m);
+const char* hcm_last_error(const hcm_model* m);
+
+/* Run up to n loop bodies (src/main.cpp:139-202), stopping when the loop
+ * condition (main.cpp:204) ends the run.  Optional per-iteration records:
+ * temptot after water_flux_out, cycle (minutes) and the flow timestep.
+ * *done receives the number of iterations performed; *finished 1 when the
+ * run is over. */
+int hcm_iterate(hcm_model* m, int n, double* temptot, double* cycle, double* dtflow,
+                int* done, int* finished);
+
+/* Download a field (hc_field ids of hc.h) in the reference layout. */
+int hcm_download(hcm_model* m, int field, void* host, size_t count);
+size_t hcm_field_count(const hcm_model* m, int field);
+
+/* Scalars by member name: time_step, cycle, counter, maxdepth, temptot,
+ * waterinput, waterOut, baseflow, totalinputpoints, imax, jmax, DX,
+ * erode_mult, globalsediq. */
+int hcm_scalar(hcm_model* m, const char* name, double* value);
+
+/* Rows written by output_data so far (14 columns: hours, Qw, Jw, sand_out,
+ * Qs, Qg[1..9]); copies at most max_rows; returns the row count. */
+int hcm_series(hcm_model* m, double* out, int max_rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HC_MODEL_H */

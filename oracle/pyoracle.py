@@ -362,6 +362,7 @@ class RefModel:
             L.ref_vec_int.restype = ctypes.POINTER(ctypes.c_int)
             L.ref_vec_int.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
             L.ref_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+            L.ref_set_threads.argtypes = [ctypes.c_int]
             cls._lib = L
         return cls._lib
 
@@ -371,8 +372,9 @@ class RefModel:
              "write_output_timeseries": 10, "drainage_area": 11, "input_points": 12,
              "erode": 13, "get_flow_timestep": 14, "topmodel_runoff": 15}
 
-    def __init__(self, path: str, params_file: str):
+    def __init__(self, path: str, params_file: str, threads: int = 1):
         self.L = self.lib()
+        self.L.ref_set_threads(threads)
         cwd = os.getcwd()
         os.chdir(path)
         try:

@@ -67,6 +67,12 @@ void* ref_create(const char* path, const char* pfile) {
   return r;
 }
 
+// OpenMP thread count for the reference's parallel loops.  The shared
+// `mannings` member written inside flow_route (LSDCatchmentModel.cpp:3091)
+// and the neighbour writes of erode (4684-4747) race with more than one
+// thread, so fixtures are made with 1.
+void ref_set_threads(int n) { omp_set_num_threads(n); }
+
 void ref_destroy(void* h) {
   RefRun* r = static_cast<RefRun*>(h);
   delete r->runoff;

@@ -1,0 +1,291 @@
+"""Pins for the oracle (oracle/hc_oracle.cpp), CPU only.
+
+What pins what:
+  * the compiled reference (oracle/_ref, built from the reference's sources
+    here) -> bitwise equality on whole runs (skipped where it is not built);
+  * the reference's own known-good answers for Boscastle
+    (test/known_good_answers/boscastle50m_72hr_u, copied to tests/golden) ->
+    hydrograph and final depths at the reference test's printed precision;
+  * committed golden vectors written by tests/golden/make_golden.py from the
+    compiled reference -> per-call flow_route / depth_update / scan_area /
+    water_flux_out results, bitwise;
+  * closed forms the mathematics fixes -> lake at rest, the steady
+    uniform-flow fixed point |q| = h sqrt(S)/n of the h^3 friction law,
+    Froude and depth caps, the CFL timestep, and mass conservation.
+"""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import hail_caesar_amd.inputs as inputs
+from pyoracle import FileWorkload, OracleModel, RefModel, pad, params_from_dict
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def bos_workload(tmp):
+    src = os.path.join(GOLDEN, "boscastle")
+    dst = os.path.join(str(tmp), "bos")
+    os.makedirs(os.path.join(dst, "out"), exist_ok=True)
+    for f in ("boscastle_square_50m.asc", "boscastle_72hr_rain_u.txt"):
+        shutil.copy(os.path.join(src, f), dst)
+    lines = open(os.path.join(src, "boscastle_test_72hr_50m_u.params")).read().splitlines()
+    out = []
+    for ln in lines:
+        if ln.startswith("read_path:"):
+            ln = "read_path: ./"
+        elif ln.startswith("write_path:"):
+            ln = "write_path: ./out/"
+        out.append(ln)
+    open(os.path.join(dst, "bos.params"), "w").write("\n".join(out) + "\n")
+    return dst, "bos.params"
+
+
+# ----------------------------------------------------------- reference pins
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("size", [48, 96])
+def test_oracle_bitwise_vs_compiled_reference(tmp_path, size):
+    w = inputs.flow_only_256(str(tmp_path / f"f{size}"), size=size, hours=2)
+    r = RefModel(w.path, w.params_file)
+    o = OracleModel.from_workload(w, nthreads=2)
+    assert np.array_equal(r.field("area"), o.field("area"))
+    rt, rc, rd = r.iterate(10 ** 7)
+    ot, oc, od = o.iterate(10 ** 7)
+    assert len(rt) == len(ot)
+    assert np.array_equal(rt, ot) and np.array_equal(rc, oc) and np.array_equal(rd, od)
+    for f in ("water_depth", "qx", "qy"):
+        assert np.array_equal(r.field(f), o.field(f)), f
+    assert np.array_equal(r.mask(), o.field_int("mask"))
+
+
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+def test_oracle_spatial_n_vs_compiled_reference(tmp_path):
+    rng = np.random.Generator(np.random.PCG64(5))
+    dem = inputs.planar_dem(40, 40, 5.0, 0.01, 5.0, 0.6, 1001)
+    n = inputs.lognormal_mannings(40, 40, 0.035, 0.3, rng)
+    w = inputs.make_workload(str(tmp_path / "n"), dem, 5.0, np.array([20.0, 5.0, 0.0]), 30, 1,
+                             mannings=n, water_init=np.where(dem > -9999, 0.02, 0.0),
+                             name="spn")
+    r = RefModel(w.path, w.params_file)
+    o = OracleModel.from_workload(w)
+    rt, _, _ = r.iterate(10 ** 7)
+    ot, _, _ = o.iterate(10 ** 7)
+    assert np.array_equal(rt, ot)
+    for f in ("water_depth", "qx", "qy"):
+        assert np.array_equal(r.field(f), o.field(f)), f
+
+
+def test_oracle_boscastle_known_good_answers(tmp_path):
+    """The reference's own regression case (test/run_tests.sh +
+    basic_kga_tests.py): 71 simulated hours, 88450 iterations."""
+    path, pf = bos_workload(tmp_path)
+    w = FileWorkload(path, pf)
+    o = OracleModel.from_workload(w, nthreads=4)
+    tt, cy, dt = o.iterate(10 ** 7)
+    assert len(tt) == 88450
+    kga = np.loadtxt(os.path.join(GOLDEN, "boscastle", "boscastle_50m_72hr_u.dat"))
+    s = o.series()
+    assert s.shape == kga.shape
+    # printed with 6 decimals (LSDCatchmentModel.cpp:2436): identical text
+    assert np.array_equal(np.round(s[:, 1], 6), kga[:, 1])
+    np.testing.assert_allclose(s[:, 1], kga[:, 1], rtol=1e-3, atol=5e-7)
+    ref_h = inputs.read_asc(os.path.join(GOLDEN, "boscastle", "WaterDepths3360.asc"))
+    # the KGA raster is the state at cycle>=3360 min; compare the final state
+    # of a run stopped there
+    o2 = OracleModel.from_workload(w, nthreads=4)
+    while True:
+        _, c2, _ = o2.iterate(1)
+        if c2[-1] >= 3360:
+            break
+    h = o2.field("water_depth")[1:-1, 1:-1]
+    np.testing.assert_allclose(h, ref_h, rtol=1e-5, atol=1e-6)
+
+
+# --------------------------------------------------------- golden vectors
+def _golden(name):
+    p = os.path.join(GOLDEN, name)
+    if not os.path.exists(p):
+        pytest.skip(f"{name} not generated")
+    return np.load(p, allow_pickle=False)
+
+
+def _oracle_from_golden(g):
+    import json
+    params = json.loads(str(g["params_json"]))
+    imax, jmax = int(g["imax"]), int(g["jmax"])
+    o = OracleModel(params_from_dict(params, imax, jmax, float(g["DX"])))
+    o.field("elev")[...] = g["elev"]
+    o.field("water_depth")[...] = g["h0"]
+    o.field("qx")[...] = g["qx0"]
+    o.field("qy")[...] = g["qy0"]
+    o.field_int("mask")[...] = g["mask0"]
+    if "n" in g.files:
+        o.field("mannings")[...] = g["n"]
+    o.scalar("maxdepth", float(g["maxdepth0"]))
+    o.scalar("time_step", float(g["time_step0"]))
+    return o
+
+
+@pytest.mark.parametrize("name", ["flow_step_64.npz", "flow_step_spatial_n_48.npz"])
+def test_oracle_per_call_golden(name):
+    g = _golden(name)
+    o = _oracle_from_golden(g)
+    dt = o.call("get_flow_timestep")
+    assert dt == float(g["dt"])
+    o.call("flow_route")
+    assert np.array_equal(o.field("qx"), g["qx1"]) and np.array_equal(o.field("qy"), g["qy1"])
+    md = o.call("depth_update")
+    assert np.array_equal(o.field("water_depth"), g["h1"])
+    assert md == float(g["maxdepth1"])
+    o.call("scan_area")
+    assert np.array_equal(o.field_int("mask"), g["mask1"])
+    o.scalar("time_step", float(g["time_step0"]))
+    t = o.call("water_flux_out")
+    assert t == float(g["temptot1"])
+    assert np.array_equal(o.field("water_depth"), g["h2"])
+
+
+def test_oracle_full_run_golden():
+    """Config-1 shape (64^2, 2 h) run by the compiled reference: per-step
+    outlet discharge, clock and flow timestep, final fields."""
+    import json
+    g = _golden("flow_run_64.npz")
+    params = json.loads(str(g["params_json"]))
+
+    class W:  # in-memory workload equivalent to the one the reference read
+        pass
+    w = W()
+    w.params = params
+    w.dem = g["dem"]
+    w.dx = float(g["DX"])
+    w.mannings = None
+    w.water_init = None
+    w.extra = {}
+    w.path = None
+    o = OracleModel(params_from_dict(params, w.dem.shape[0], w.dem.shape[1], w.dx))
+    o.field("elev")[...] = pad(w.dem, -9999.0)
+    o.set_rain(g["rain"])
+    o.L.orc_initialise(o.h)
+    tt, cy, dt = o.iterate(10 ** 7)
+    assert np.array_equal(tt, g["temptot"]) and np.array_equal(cy, g["cycle"])
+    assert np.array_equal(dt, g["dtflow"])
+    for f in ("water_depth", "qx", "qy"):
+        assert np.array_equal(o.field(f), g[f]), f
+
+
+# ------------------------------------------------------------ closed forms
+def _flat_oracle(nrows, ncols, dx=5.0, **over):
+    p = dict(inputs.BASE_PARAMS)
+    p.update(over)
+    o = OracleModel(params_from_dict(p, nrows, ncols, dx))
+    return o
+
+
+def test_lake_at_rest():
+    """Equal water-surface elevation and q = 0: every interior face away from
+    the edge-slope overrides keeps q = 0 (LSDCatchmentModel.cpp:3098-3115).
+    Elevations are multiples of 2^-10 so eta = z + (2 - z) is exact."""
+    o = _flat_oracle(12, 12)
+    z = np.random.Generator(np.random.PCG64(3)).integers(0, 512, (12, 12)) / 1024.0
+    o.field("elev")[1:-1, 1:-1] = z
+    o.field("water_depth")[1:-1, 1:-1] = 2.0 - z
+    o.field_int("mask")[1:-1, 1:-1] = 1
+    o.scalar("maxdepth", 2.0)
+    o.scalar("time_step", 0.5)
+    o.call("flow_route")
+    # faces of rows/cols 3..imax-1 (x<=2 and x==imax take the edge slope)
+    assert np.all(o.field("qx")[3:-2, 1:-1] == 0)
+    assert np.all(o.field("qy")[1:-1, 3:-2] == 0)
+
+
+def test_uniform_flow_fixed_point():
+    """Steady state of q <- (q - g h dt S) / (1 + g h dt n^2 |q| / h^3) at
+    fixed depth: q|q| = -S h^3 / n^2, i.e. |q| = h^1.5 sqrt(S) / n.  The
+    friction denominator is h^3 because 10/3 is integer division in
+    pow(hflow, 10/3) (LSDCatchmentModel.cpp:3115); Manning's intended
+    h^(10/3) would give h^(5/3) sqrt(S) / n instead."""
+    S, n, h, dx, N = 1e-4, 0.05, 0.5, 5.0, 8
+    o = _flat_oracle(N, N, dx, mannings_n=str(n), froude_num_limit="10")
+    x = np.arange(1, N + 1)[:, None] * np.ones((1, N))
+    o.field("elev")[1:-1, 1:-1] = 100.0 - S * dx * x  # drops southward
+    o.field_int("mask")[1:-1, 1:-1] = 1
+    o.scalar("maxdepth", h)
+    o.scalar("time_step", 50.0)
+    for _ in range(4000):
+        o.field("water_depth")[1:-1, 1:-1] = h  # hold the depth: momentum only
+        o.call("flow_route")
+    q = o.field("qx")[4, 4]
+    # hflow is the upstream depth as rounded through eta = z + h
+    zc, zn = 100.0 - S * dx * 4, 100.0 - S * dx * 3
+    hflow = max(zc + h, zn + h) - max(zn, zc)
+    eta_slope = ((zn + h) - (zc + h)) / dx
+    expect = -np.sqrt(eta_slope * hflow ** 3) / n
+    assert abs(q - expect) <= 1e-9 * abs(expect)
+    assert abs(expect + h ** 1.5 * np.sqrt(S) / n) <= 1e-6 * abs(expect)
+
+
+def test_froude_cap():
+    """q > Fr_lim sqrt(g h) h is replaced by exactly that value
+    (LSDCatchmentModel.cpp:3124-3134)."""
+    o = _flat_oracle(6, 6, 5.0, froude_num_limit="0.8")
+    o.field("elev")[1:-1, 1:-1] = 10.0
+    o.field("elev")[4, 1:-1] = 9.0  # 1 m drop from row 3 to row 4
+    o.field("water_depth")[1:-1, 1:-1] = 0.3
+    o.field_int("mask")[1:-1, 1:-1] = 1
+    o.scalar("maxdepth", 0.3)
+    o.scalar("time_step", 0.8)
+    assert o.call("get_flow_timestep") == 0.8
+    o.call("flow_route")
+    hflow = max(9.0 + 0.3, 10.0 + 0.3) - max(10.0, 9.0)
+    qF = 0 - (hflow * (np.sqrt(9.81 * hflow) * 0.8))
+    assert o.field("qx")[4, 3] == qF
+
+
+def test_depth_cap():
+    """|q| dt / DX > h_source / 4 sets q = h_source DX / 5 / dt
+    (LSDCatchmentModel.cpp:3138-3148)."""
+    o = _flat_oracle(6, 6, 5.0, froude_num_limit="100")
+    o.field("elev")[1:-1, 1:-1] = 10.0
+    o.field("elev")[4, 1:-1] = 9.0
+    o.field("water_depth")[1:-1, 1:-1] = 0.3
+    o.field_int("mask")[1:-1, 1:-1] = 1
+    o.scalar("maxdepth", 0.001)
+    o.scalar("time_step", 50.0)
+    dt = o.call("get_flow_timestep")
+    o.call("flow_route")
+    assert o.field("qx")[4, 3] == 0 - ((0.3 * 5.0) / 5) / dt
+
+
+def test_cfl_timestep():
+    o = _flat_oracle(4, 4, 7.0, courant_number="0.6")
+    for md in (0.05, 0.1, 2.5):
+        o.scalar("maxdepth", md)
+        o.scalar("time_step", 1e6)
+        dt = o.call("get_flow_timestep")
+        assert dt == 0.6 * (7.0 / np.sqrt(9.81 * md))
+        o.call("set_maximum_timestep")
+        assert o.scalar("maxdepth") == max(md, 0.1)
+
+
+def test_mass_conservation():
+    """Each face flux leaves one cell and enters its neighbour
+    (LSDCatchmentModel.cpp:3276); with every interior cell on the active mask
+    and no nodata, the faces on the padding ring carry nothing and the
+    interior volume is conserved to rounding."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    N = 24
+    o = _flat_oracle(N, N, 5.0)
+    o.field("elev")[1:-1, 1:-1] = rng.uniform(0, 2, (N, N))
+    o.field("water_depth")[1:-1, 1:-1] = rng.uniform(0.01, 0.5, (N, N))
+    o.field_int("mask")[1:-1, 1:-1] = 1
+    o.scalar("maxdepth", 1.0)
+    o.scalar("time_step", 0.3)
+    v0 = o.field("water_depth").sum()
+    for _ in range(50):
+        o.call("flow_route")
+        o.call("depth_update")
+    v1 = o.field("water_depth").sum()
+    assert abs(v1 - v0) <= 1e-12 * v0
+    assert np.abs(o.field("qx")).max() > 0
