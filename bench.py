@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-timestep hot path on one MI355X.
+
+Workload: the large single grid (4096^2 cells at 2 m, fractal relief,
+log-normal Manning n, Gamma rainfall, flow only) from a fully wet state (a
+uniform 0.2 m initial depth).  One step is one iteration of the reference's
+loop body (src/main.cpp:139-204): Courant timestep, clocks, rainfall input,
+flow_route + depth_update (one fused kernel), scan_area every 5th step and
+water_flux_out, all device-resident and replayed from a captured HIP graph.
+
+The headline value is FP32-storage cell-steps/s; the FP64 plain form runs on
+the same inputs and is reported beside it, with the FP32/FP64 ratio.  The
+dominant kernel's duration is measured with HIP events on the context stream
+in a second pass of the same steps.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "hail-caesar_amd")
+HBM_PEAK_GBS = 8000.0
+BASELINE = os.path.join(ROOT, "BASELINE.json")
+TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def load_package():
+    if "hail_caesar_amd" in sys.modules:
+        return sys.modules["hail_caesar_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "hail_caesar_amd", os.path.join(PKG, "__init__.py"), submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["hail_caesar_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def bytes_per_cell_step(precision: str, spatial_n: bool = True) -> int:
+    """Algorithmic bytes of one fused flux+depth step per cell: reads h, z,
+    qx, qy (and n), writes h, qx, qy."""
+    esz = 4 if precision == "fp32" else 8
+    return esz * ((5 if spatial_n else 4) + 3)
+
+
+def run_gpu(precision: str, arrays, params, steps: int, warmup: int):
+    from hail_caesar_amd import hc
+    from hail_caesar_amd.model import Model
+    prec = hc.HC_FP32 if precision == "fp32" else hc.HC_FP64
+    t0 = time.time()
+    m = Model(None, None, precision=prec, arrays=arrays, params=params, device_loop=True,
+              use_graph=True, flat_dz=0.01 if precision == "fp32" else 0.0)
+    setup_s = time.time() - t0
+    m.iterate(warmup)
+    # the project's tile tuner picks rows per wavefront on the warmed state
+    rows, tune_us = m.context().tune_tiles(0.05)
+    m.iterate(2)
+    # timed region: K loop bodies, synchronised on both sides
+    m.synchronize()
+    t1 = time.perf_counter()
+    m.iterate(steps)
+    m.synchronize()
+    wall = time.perf_counter() - t1
+    # kernel timing pass: the same loop without graphs, HIP events around
+    # every fused flow_route+depth_update launch on the context stream
+    ctx = m.context()
+    ctx.loop_profile(True)
+    m.set_graph(False)
+    m.iterate(steps)
+    ktot, kn = ctx.loop_kernel_time()
+    ctx.loop_profile(False)
+    m.set_graph(True)
+    mask = m.download("mask")
+    active = float(mask[1:-1, 1:-1].mean())
+    h = m.download("water_depth")
+    return {
+        "wall_s": wall, "setup_s": setup_s, "kernel_ms": ktot / max(kn, 1), "kernel_launches": kn,
+        "active_fraction": active, "h": h, "counter": m.scalar("counter"), "cycle": m.scalar("cycle"),
+        "model": m, "tile_rows": rows, "tune_us": tune_us,
+    }
+
+
+def cpu_baseline(arrays, params, budget_s: float = 20.0):
+    """The oracle (FP64 CPU restatement, OpenMP) on the same 4096^2 state:
+    timed over whole loop bodies after the active mask is built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import OracleModel, pad, params_from_dict
+    ncpu = os.cpu_count() or 1
+    threads = max(1, min(16, ncpu))
+    dem = arrays["dem"]
+    p = params_from_dict(params, dem.shape[0], dem.shape[1], float(arrays["DX"]), nthreads=threads)
+    o = OracleModel(p)
+    o.field("elev")[...] = pad(dem, -9999.0)
+    o.field("mannings")[1:-1, 1:-1] = arrays["mannings"]
+    o.field("water_depth")[1:-1, 1:-1] = arrays["water_init"]
+    o.set_rain(np.asarray(arrays["rain"], dtype=np.float64))
+    o.L.orc_initialise(o.h)
+    o.iterate(5)  # the active mask is first built at counter 5
+    t = time.perf_counter()
+    o.iterate(1)
+    one = time.perf_counter() - t
+    n = max(1, min(50, int(budget_s / max(one, 1e-3))))
+    t = time.perf_counter()
+    o.iterate(n)
+    el = time.perf_counter() - t
+    cells = dem.shape[0] * dem.shape[1]
+    return {"value": cells * n / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (FP64 OpenMP restatement) on the same fully wet {dem.shape[0]}x{dem.shape[1]} "
+                      f"state: {n} loop bodies after 6 warm-up bodies, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU row-strip decomposition is a design, not a build).", file=sys.stderr)
+        sys.exit(2)
+    load_package()
+    from hail_caesar_amd import inputs
+    with open(BASELINE) as f:
+        base = json.load(f)
+    arrays, params = inputs.grid4096_arrays(size=args.size)
+    cells = args.size * args.size
+    res = {}
+    for prec in (["fp32", "fp64"] if args.only is None else [args.only]):
+        r = run_gpu(prec, arrays, params, args.steps, args.warmup)
+        bpc = bytes_per_cell_step(prec)
+        r["cell_steps_per_s"] = cells * args.steps / r["wall_s"]
+        r["ms_per_step"] = 1000.0 * r["wall_s"] / args.steps
+        r["kernel_gbs"] = bpc * cells / (r["kernel_ms"] * 1e-3) / 1e9
+        r["bytes_per_cell_step"] = bpc
+        res[prec] = r
+        print(f"[bench] {prec}: {r['cell_steps_per_s']:.4e} cell-steps/s, {r['ms_per_step']:.4f} ms/step, "
+              f"fused kernel {r['kernel_ms']*1e3:.1f} us = {r['kernel_gbs']:.0f} GB/s, "
+              f"active {r['active_fraction']:.3f}, tile rows {r['tile_rows']}", file=sys.stderr, flush=True)
+    head = res.get("fp32") or res.get("fp64")
+    prec = "fp32" if "fp32" in res else "fp64"
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+        traffic = t.get(f"{prec}_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(head["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(head["kernel_gbs"] / HBM_PEAK_GBS, 4),
+                "traffic": traffic,
+                "kernel": "k_flow_depth32 (fused flow_route+depth_update)" if prec == "fp32"
+                else "k_flow_depth64 (fused flow_route+depth_update)",
+                "bytes_per_cell_step": head["bytes_per_cell_step"], "cells_per_launch": cells,
+                "kernel_us": round(head["kernel_ms"] * 1e3, 2), "tile_rows": head["tile_rows"]}
+    out = {
+        "metric": base["metric"], "value": round(head["cell_steps_per_s"], 1), "unit": "cell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": f"{args.size}^2 grid at 2 m, fractal DEM (30 m) + carved channel, "
+                               "log-normal Manning n, 15-min Gamma rain, flow only, fully wet "
+                               "(0.2 m initial depth); one step = one loop body, device-resident",
+                   "cells": cells, "active_fraction": round(head["active_fraction"], 4)},
+        "roofline": roofline,
+    }
+    if "fp32" in res and "fp64" in res:
+        f32, f64 = res["fp32"], res["fp64"]
+        out["fp64"] = {"value": round(f64["cell_steps_per_s"], 1), "ms_per_step": round(f64["ms_per_step"], 5),
+                       "kernel_us": round(f64["kernel_ms"] * 1e3, 2),
+                       "kernel_gbs": round(f64["kernel_gbs"], 1),
+                       "frac": round(f64["kernel_gbs"] / HBM_PEAK_GBS, 4)}
+        out["fp32_over_fp64"] = round(f32["cell_steps_per_s"] / f64["cell_steps_per_s"], 3)
+        # FP32 vs FP64 depth after the same number of loop bodies
+        h32, h64 = f32["h"][1:-1, 1:-1], f64["h"][1:-1, 1:-1]  # interior cells
+        dh = np.abs(h32 - h64)
+        out["fp32_depth_error"] = {"max_abs_m": float(dh.max()),
+                                   "max_rel_to_max_depth": float(dh.max() / np.abs(h64).max()),
+                                   "l2_rel": float(np.sqrt((dh ** 2).sum() / (h64 ** 2).sum())),
+                                   "steps": int(f64["counter"]),
+                                   "sim_minutes_fp64": round(f64["cycle"], 3),
+                                   "sim_minutes_fp32": round(f32["cycle"], 3)}
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(arrays, params)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -11,7 +11,8 @@
 
 int main(int argc, char* argv[]) {
   if (argc < 3) {
-    std::cout << "usage: " << argv[0] << " <path> <parameter file> [--fp32] [--separate]\n";
+    std::cout << "usage: " << argv[0]
+              << " <path> <parameter file> [--fp32] [--separate] [--device-loop] [--graph]\n";
     return 0;
   }
   hcm_options opt{};
@@ -20,8 +21,12 @@ int main(int argc, char* argv[]) {
   opt.quiet = 0;
   opt.write_files = 1;
   opt.flat_dz = 0.0;
+  opt.device_loop = 0;
+  opt.use_graph = 0;
   for (int a = 3; a < argc; ++a) {
     if (!std::strcmp(argv[a], "--fp32")) opt.precision = HC_FP32;
+    else if (!std::strcmp(argv[a], "--device-loop")) opt.device_loop = 1;
+    else if (!std::strcmp(argv[a], "--graph")) { opt.device_loop = 1; opt.use_graph = 1; }
     else if (!std::strcmp(argv[a], "--separate")) opt.fused = 0;
   }
   const auto t0 = std::chrono::steady_clock::now();

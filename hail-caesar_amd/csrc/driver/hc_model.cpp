@@ -105,7 +105,15 @@ class CatchmentModel {
  public:
   CatchmentModel(const std::string& pname, const std::string& pfname, const hcm_options& opt)
       : opt_(opt) {
-    initialise_variables(pname + pfname);
+    std::ifstream in(pname + pfname);
+    if (!in) throw Error("cannot open parameter file " + pname + pfname);
+    initialise_variables(in);
+  }
+  // inputs held in memory (hcm_create_arrays)
+  CatchmentModel(const std::string& params_text, const hcm_arrays& arr, const hcm_options& opt)
+      : opt_(opt), arrays_(&arr) {
+    std::istringstream in(params_text);
+    initialise_variables(in);
   }
   ~CatchmentModel() {
     if (ctx_) hc_destroy(ctx_);
@@ -113,6 +121,16 @@ class CatchmentModel {
 
   // ------------------------------------------------ set-up (main.cpp:103-120)
   void initialise_model_domain_extents() {
+    if (arrays_) {
+      imax_ = arrays_->nrows;
+      jmax_ = arrays_->ncols;
+      DX_ = arrays_->DX;
+      xll_ = arrays_->xll;
+      yll_ = arrays_->yll;
+      no_data_value_ = -9999;
+      if (imax_ <= 0 || jmax_ <= 0 || !arrays_->dem) throw Error("bad in-memory DEM");
+      return;
+    }
     const std::string f = read_path_ + "/" + read_fname_ + "." + dem_read_extension_;
     std::ifstream in(f);
     if (!in) throw Error("No terrain DEM found by name of: " + f);
@@ -178,22 +196,40 @@ class CatchmentModel {
     check(hc_create(&ctx_, &g, &p), "hc_create");
   }
 
+  // An input raster: from memory when the model was built from arrays.
+  Raster input_raster(const std::string& file, const double* mem) const {
+    if (!arrays_) return read_asc(read_path_ + "/" + file);
+    if (!mem) throw Error("in-memory input missing for " + file);
+    Raster r;
+    r.nrows = arrays_->nrows;
+    r.ncols = arrays_->ncols;
+    r.cell = arrays_->DX;
+    r.v.assign(mem, mem + static_cast<size_t>(r.nrows) * r.ncols);
+    return r;
+  }
+
   void load_data() {
-    const std::string demf = read_path_ + "/" + read_fname_ + "." + dem_read_extension_;
-    Raster dem = read_asc(demf);
+    Raster dem = input_raster(read_fname_ + "." + dem_read_extension_, arrays_ ? arrays_->dem : nullptr);
     if (dem.nrows != imax_ || dem.ncols != jmax_) throw Error("DEM size changed between reads");
     for (int i = 0; i < imax_; ++i)
       for (int j = 0; j < jmax_; ++j) elev_[at(i + 1, j + 1)] = dem.v[static_cast<size_t>(i) * jmax_ + j];
     check_DEM_edge_condition();
     upload(HC_FIELD_ELEV, elev_.data());
     if (spatially_var_rainfall_) {
-      Raster r = read_asc(read_path_ + "/" + hydroindex_fname_);
-      for (int i = 0; i < imax_; ++i)
-        for (int j = 0; j < jmax_; ++j)
-          rfarea_[at(i + 1, j + 1)] = static_cast<int>(r.v[static_cast<size_t>(i) * jmax_ + j]);
+      if (arrays_) {
+        if (!arrays_->hydroindex) throw Error("in-memory hydroindex missing");
+        for (int i = 0; i < imax_; ++i)
+          for (int j = 0; j < jmax_; ++j)
+            rfarea_[at(i + 1, j + 1)] = arrays_->hydroindex[static_cast<size_t>(i) * jmax_ + j];
+      } else {
+        Raster r = read_asc(read_path_ + "/" + hydroindex_fname_);
+        for (int i = 0; i < imax_; ++i)
+          for (int j = 0; j < jmax_; ++j)
+            rfarea_[at(i + 1, j + 1)] = static_cast<int>(r.v[static_cast<size_t>(i) * jmax_ + j]);
+      }
     }
     if (spatially_var_mannings_) {
-      Raster r = read_asc(read_path_ + "/" + mannings_fname_);
+      Raster r = input_raster(mannings_fname_, arrays_ ? arrays_->mannings : nullptr);
       std::vector<double> nm(elev_.size(), 0.0);
       for (int i = 0; i < imax_; ++i)
         for (int j = 0; j < jmax_; ++j) nm[at(i + 1, j + 1)] = r.v[static_cast<size_t>(i) * jmax_ + j];
@@ -210,13 +246,23 @@ class CatchmentModel {
       if (!hydro_only_) upload(HC_FIELD_BEDROCK, b.data());
     }
     if (water_init_from_raster_) {
-      Raster r = read_asc(read_path_ + "/" + water_init_raster_file_);
+      Raster r = input_raster(water_init_raster_file_, arrays_ ? arrays_->water_init : nullptr);
       std::vector<double> h(elev_.size(), 0.0);
       for (int i = 0; i < imax_; ++i)
         for (int j = 0; j < jmax_; ++j) h[at(i + 1, j + 1)] = r.v[static_cast<size_t>(i) * jmax_ + j];
       upload(HC_FIELD_WATER_DEPTH, h.data());
     }
-    if (rainfall_data_on_) read_rainfall(read_path_ + "/" + rainfall_data_file_);
+    if (rainfall_data_on_) {
+      if (arrays_) {
+        if (!arrays_->rain) throw Error("in-memory rainfall missing");
+        rain_.assign(arrays_->rain_rows, std::vector<float>(arrays_->rain_cols));
+        for (int r = 0; r < arrays_->rain_rows; ++r)
+          for (int c = 0; c < arrays_->rain_cols; ++c)
+            rain_[r][c] = arrays_->rain[static_cast<size_t>(r) * arrays_->rain_cols + c];
+      } else {
+        read_rainfall(read_path_ + "/" + rainfall_data_file_);
+      }
+    }
     if (graindata_from_file_ && !hydro_only_) ingest_graindata(read_path_ + "/" + grain_data_file_);
   }
 
@@ -341,6 +387,106 @@ class CatchmentModel {
     return !(get_cycle() / 60 < get_maxcycle());
   }
 
+  bool device_loop_enabled() const { return opt_.device_loop && hydro_only_; }
+  void set_graph(bool g) { opt_.use_graph = g ? 1 : 0; }
+
+  // Device-resident stepping (hc_loop_*): n loop bodies with the scalar
+  // state on the device; output_data is replayed here from the per-iteration
+  // records so the interval series is the one the host loop would write.
+  int iterate_device(int n, double* tt, double* cy, double* dtf, bool* fin) {
+    if (!loop_started_) begin_device_loop();
+    check(hc_loop_run(ctx_, n, opt_.use_graph), "loop_run");
+    std::vector<hc_loop_record> rec(static_cast<size_t>(std::max(n, 1)));
+    hc_loop_state st{};
+    int nrec = 0;
+    check(hc_loop_sync(ctx_, &st, rec.data(), n, &nrec), "loop_sync");
+    const int k = std::min(nrec, n);
+    for (int i = 0; i < k; ++i) {
+      const hc_loop_record& r = rec[i];
+      previous_ = r.previous;
+      old_cycle_ = r.old_cycle;
+      cycle_ = r.cycle;
+      new_cycle_ = r.new_cycle;
+      temptot_ = r.temptot;
+      for (int m = 0; m <= rfnum_; ++m) j_mean_[m] = r.j_mean[m];
+      write_output_timeseries();
+      if (tt) tt[i] = r.temptot;
+      if (cy) cy[i] = r.cycle;
+      if (dtf) dtf[i] = r.dt_flow;
+    }
+    time_step_ = st.time_step;
+    cycle_ = st.cycle;
+    counter_ = st.counter;
+    maxdepth_ = st.maxdepth;
+    temptot_ = st.temptot;
+    waterinput_ = st.waterinput;
+    waterOut_ = st.waterOut;
+    time_1_ = st.time_1;
+    baseflow_ = st.baseflow;
+    for (int m = 0; m <= rfnum_; ++m) {
+      j_mean_[m] = st.j_mean[m];
+      old_j_mean_[m] = st.old_j_mean[m];
+      new_j_mean_[m] = st.new_j_mean[m];
+    }
+    *fin = st.finished != 0;
+    return k;
+  }
+
+  void begin_device_loop() {
+    if (rfnum_ >= HC_RF_MAX) throw Error("device loop supports at most 16 rain cells");
+    hc_loop_init in{};
+    in.time_step = time_step_;
+    in.cycle = cycle_;
+    in.maxdepth = maxdepth_;
+    in.waterinput = waterinput_;
+    in.waterOut = waterOut_;
+    in.time_1 = time_1_;
+    in.baseflow = baseflow_;
+    in.counter = counter_;
+    in.courant_number = courant_number_;
+    in.in_out_difference_allowed = in_out_difference_allowed_;
+    in.output_file_save_interval = output_file_save_interval_;
+    in.ERODEFACTOR = ERODEFACTOR_;
+    in.MIN_Q = MIN_Q_;
+    in.MIN_Q_MAXVAL = MIN_Q_MAXVAL_;
+    in.max_time_step = max_time_step_;
+    in.maxcycle = maxcycle_;
+    in.rfnum = rfnum_;
+    for (int m = 0; m <= rfnum_; ++m) {
+      in.j_mean[m] = j_mean_[m];
+      in.old_j_mean[m] = old_j_mean_[m];
+      in.new_j_mean[m] = new_j_mean_[m];
+      in.nActual[m] = nActual_[m];
+      in.input_counter[m] = input_counter_[m];
+    }
+    // TOPMODEL results for every later value of time_1, from the current state
+    const int t0 = static_cast<int>(time_1_);
+    const int ntop = std::max(t0 + 2, maxcycle_ * 60 + 2 * std::max(1, max_time_step_ / 60) + 64);
+    topmodel_table_.assign(static_cast<size_t>(ntop) * HC_RF_MAX, 0.0);
+    std::vector<double> j = j_, jo = jo_, oj = old_j_mean_, nj = new_j_mean_;
+    for (int m = 0; m <= rfnum_; ++m) topmodel_table_[static_cast<size_t>(t0) * HC_RF_MAX + m] = nj[m];
+    for (int t = t0 + 1; t < ntop; ++t) {
+      topmodel_step(t, j, jo, oj, nj);
+      for (int m = 0; m <= rfnum_; ++m) topmodel_table_[static_cast<size_t>(t) * HC_RF_MAX + m] = nj[m];
+    }
+    in.topmodel = topmodel_table_.data();
+    in.ntop = ntop;
+    in.area = area_.data();
+    in.rfarea = rfarea_.data();
+    std::vector<int> px, py;
+    for (int z = 1; z <= totalinputpoints_; ++z) {
+      px.push_back(in_x_[z]);
+      py.push_back(in_y_[z]);
+    }
+    in.in_x = px.data();
+    in.in_y = py.data();
+    in.npoints = totalinputpoints_;
+    in.first_x = in_x_[1];
+    in.first_y = in_y_[1];
+    check(hc_loop_begin(ctx_, &in), "loop_begin");
+    loop_started_ = true;
+  }
+
   double last_dt() const { return last_dt_; }
   double temptot() const { return temptot_; }
   hc_ctx* ctx() { return ctx_; }
@@ -386,9 +532,7 @@ class CatchmentModel {
   // LSDCatchmentModel::initialise_variables (LCM.cpp:842-1610) for the keys
   // of the hot path and its inputs; options this build does not provide are
   // rejected instead of silently ignored.
-  void initialise_variables(const std::string& full) {
-    std::ifstream in(full);
-    if (!in) throw Error("cannot open parameter file " + full);
+  void initialise_variables(std::istream& in) {
     std::string key, value;
     auto yes = [&]() { return value == "yes"; };
     while (in.good()) {
@@ -534,24 +678,30 @@ class CatchmentModel {
     if (col < 0 || col >= static_cast<int>(rain_[row].size())) return 0.0f;
     return rain_[row][col];
   }
-  void topmodel_runoff(double cyc) {  // LCM.cpp:3649-3715
+  void topmodel_runoff(double cyc) {
+    topmodel_step(cyc, j_, jo_, old_j_mean_, new_j_mean_);
+  }
+  // LCM.cpp:3649-3715 on explicit state (the device loop tabulates the
+  // sequence ahead of time with this same function)
+  void topmodel_step(double cyc, std::vector<double>& j, std::vector<double>& jo,
+                     std::vector<double>& old_jm, std::vector<double>& new_jm) const {
     const int ts = static_cast<int>(cyc / rain_data_time_step_);
     for (int n = 1; n <= rfnum_; ++n) {
       double local = 0;
       const double ft = 60;
-      old_j_mean_[n] = new_j_mean_[n];
-      jo_[n] = j_[n];
+      old_jm[n] = new_jm[n];
+      jo[n] = j[n];
       const float r = rain_at(ts, n - 1);
       if (r > 0) local = rain_factor_ * ((r / 1000) / 3600);
       if (local == 0) {
-        j_[n] = jo_[n] / (1 + ((jo_[n] * ft) / M_));
-        new_j_mean_[n] = M_ / ft * std::log(1 + ((jo_[n] * ft) / M_));
+        j[n] = jo[n] / (1 + ((jo[n] * ft) / M_));
+        new_jm[n] = M_ / ft * std::log(1 + ((jo[n] * ft) / M_));
       }
       if (local > 0) {
-        j_[n] = local / (((local - jo_[n]) / jo_[n]) * std::exp((0 - local) * ft / M_) + 1);
-        new_j_mean_[n] = (M_ / ft) * std::log(((local - jo_[n]) + jo_[n] * std::exp((local * ft) / M_)) / local);
+        j[n] = local / (((local - jo[n]) / jo[n]) * std::exp((0 - local) * ft / M_) + 1);
+        new_jm[n] = (M_ / ft) * std::log(((local - jo[n]) + jo[n] * std::exp((local * ft) / M_)) / local);
       }
-      if (new_j_mean_[n] < 0) new_j_mean_[n] = 0;
+      if (new_jm[n] < 0) new_jm[n] = 0;
     }
   }
   void calchydrograph(double t) {  // LCM.cpp:3717-3724
@@ -762,6 +912,7 @@ class CatchmentModel {
   }
 
   hcm_options opt_;
+  const hcm_arrays* arrays_ = nullptr;
   hc_ctx* ctx_ = nullptr;
   // inputs
   std::string dem_read_extension_ = "asc", dem_write_extension_ = "asc", write_path_, read_path_;
@@ -808,30 +959,67 @@ class CatchmentModel {
          Jw_overvol_ = 0, sand_out_ = 0;
   std::vector<double> sum_grain_, old_sum_grain_, Qg_step_, Qg_hour_, Qg_over_, Qg_last_;
   std::vector<double> series_;
+  std::vector<double> topmodel_table_;
+  bool loop_started_ = false;
 };
 
 }  // namespace hcdrv
 
 struct hcm_model {
   hcdrv::CatchmentModel* m = nullptr;
+  hcm_arrays arrays{};  // borrowed pointers, valid during hcm_create_arrays only
   std::string err;
 };
 
 extern "C" {
 
-int hcm_create(hcm_model** out, const char* path, const char* pfile, const hcm_options* opt) {
-  if (!out || !path || !pfile) return HC_ERR_ARG;
-  *out = nullptr;
-  hcm_model* h = new hcm_model;
+static hcm_options default_options(const hcm_options* opt) {
   hcm_options o{};
   o.precision = HC_FP64;
   o.fused = 1;
   o.quiet = 1;
   o.write_files = 1;
   o.flat_dz = 0.0;
+  o.device_loop = 0;
+  o.use_graph = 0;
   if (opt) o = *opt;
+  return o;
+}
+
+static int finish_create(hcm_model* h, hcm_model** out);
+
+int hcm_create(hcm_model** out, const char* path, const char* pfile, const hcm_options* opt) {
+  if (!out || !path || !pfile) return HC_ERR_ARG;
+  *out = nullptr;
+  hcm_model* h = new hcm_model;
   try {
-    h->m = new hcdrv::CatchmentModel(path, pfile, o);
+    h->m = new hcdrv::CatchmentModel(path, pfile, default_options(opt));
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    *out = h;
+    return HC_ERR_STATE;
+  }
+  return finish_create(h, out);
+}
+
+int hcm_create_arrays(hcm_model** out, const char* params_text, const hcm_arrays* arrays,
+                      const hcm_options* opt) {
+  if (!out || !params_text || !arrays) return HC_ERR_ARG;
+  *out = nullptr;
+  hcm_model* h = new hcm_model;
+  try {
+    h->arrays = *arrays;
+    h->m = new hcdrv::CatchmentModel(params_text, h->arrays, default_options(opt));
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    *out = h;
+    return HC_ERR_STATE;
+  }
+  return finish_create(h, out);
+}
+
+static int finish_create(hcm_model* h, hcm_model** out) {
+  try {
     h->m->initialise_model_domain_extents();
     h->m->initialise_arrays();
     h->m->load_data();
@@ -864,11 +1052,15 @@ int hcm_iterate(hcm_model* h, int n, double* tt, double* cy, double* dtf, int* d
   int k = 0;
   bool fin = false;
   try {
-    for (; k < n && !fin; ++k) {
-      fin = h->m->iterate();
-      if (tt) tt[k] = h->m->temptot();
-      if (cy) cy[k] = h->m->get_cycle();
-      if (dtf) dtf[k] = h->m->last_dt();
+    if (h->m->device_loop_enabled()) {
+      k = h->m->iterate_device(n, tt, cy, dtf, &fin);
+    } else {
+      for (; k < n && !fin; ++k) {
+        fin = h->m->iterate();
+        if (tt) tt[k] = h->m->temptot();
+        if (cy) cy[k] = h->m->get_cycle();
+        if (dtf) dtf[k] = h->m->last_dt();
+      }
     }
   } catch (const std::exception& e) {
     h->err = e.what();
@@ -896,6 +1088,14 @@ int hcm_scalar(hcm_model* h, const char* name, double* v) {
   if (!h || !h->m || !name || !v) return HC_ERR_ARG;
   return h->m->scalar(name, v) ? HC_OK : HC_ERR_ARG;
 }
+
+int hcm_set_graph(hcm_model* h, int use_graph) {
+  if (!h || !h->m) return HC_ERR_ARG;
+  h->m->set_graph(use_graph != 0);
+  return HC_OK;
+}
+
+hc_ctx* hcm_context(hcm_model* h) { return (h && h->m) ? h->m->ctx() : nullptr; }
 
 int hcm_series(hcm_model* h, double* out, int max_rows) {
   if (!h || !h->m) return 0;

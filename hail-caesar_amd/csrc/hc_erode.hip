@@ -1,6 +1,578 @@
-// Erosion kernels (LSDCatchmentModel::erode and grain helpers).
+// Erosion and deposition ("erodepo"): LSDCatchmentModel::erode
+// (LSDCatchmentModel.cpp:4211-4845) with its grain helpers sort_active,
+// addGS, sand_fraction, d50 and slide_GS (3874-4171), FP64.
+//
+// Layout: dense per-cell grain (11 planes), strata (10 layers x 9 fractions
+// = 90 planes, plane z*9+n), a has-grain flag standing for index != -9999,
+// and the directional bedload su/sr/sd/sl as 4 x 10 planes.
+//
+// Phases and their parallel structure:
+//   entrainment (4236-4565)   per-cell map over the active mask, y < jmax;
+//                             grain rows a cell allocates for a neighbour are
+//                             flagged and initialised by a second pass
+//                             (addGS is deterministic, so order is moot);
+//                             tempbmax is a max-reduction; repeated on the
+//                             host while tempbmax > ERODEFACTOR.
+//   deposition + x slide      one thread per column y walks x in the
+//   (4570-4682)               reference's order: every write stays in the
+//                             column, and each cell sees its upper
+//                             neighbour's slide, exactly as the serial loop.
+//   y slide (4684-4747)       one thread per row x walks y in order (writes
+//                             stay in the row).
+//   edge outputs (4750-4827)  nine lanes, one per fraction, accumulate in
+//                             float in the reference order: a zeroed private
+//                             sum per loop, added to the total afterwards, as
+//                             GCC's OpenMP array reduction does.
+// Transcendentals (pow, exp, log, sqrt) are the device library's, so values
+// agree with the libm reference to a few ulp rather than bitwise.
+#include <algorithm>
+#include <cstring>
+
+#include "hc_device.hpp"
 #include "hc_internal.hpp"
 
-int hc_erode_impl(hc_ctx* c, double, double*, double*, float*) {
-  return hc_fail(c, HC_ERR_STATE, "erode: not built yet");
+using hc::ErodeArgs;
+using hc::kGravity;
+using namespace hcdev;
+
+namespace {
+
+constexpr int kG = 10;  // G_MAX
+
+struct Grain {
+  double* g;       // 11 planes
+  double* s;       // 90 planes
+  int32_t* has;
+  size_t nc;       // plane stride (cells)
+  __device__ double& G(size_t c, int n) const { return g[n * nc + c]; }
+  __device__ double& S(size_t c, int z, int n) const { return s[(z * 9 + n) * nc + c]; }
+};
+
+// LCM.cpp:3874-3947
+__device__ void sort_active(const Grain& gr, size_t c, const ErodeArgs& a);
+
+// LCM.cpp:3949-3983 (sort_active afterwards, as addGS does)
+__device__ void add_gs(const Grain& gr, size_t c, double elev, double bedrock, const ErodeArgs& a) {
+  gr.has[c] = 1;
+  gr.G(c, 0) = 0;
+  for (int n = 1; n <= kG - 1; ++n) gr.G(c, n) = a.active * a.dprop[n];
+  gr.G(c, kG) = 0;
+  for (int z = 0; z <= 9; ++z) {
+    const bool zero = elev - (a.active * (z + 1)) < (bedrock - a.active);
+    for (int n2 = 0; n2 <= kG - 2; ++n2) gr.S(c, z, n2) = zero ? 0.0 : (a.active) * a.dprop[n2 + 1];
+  }
+  sort_active(gr, c, a);
+}
+
+__device__ void sort_active(const Grain& gr, size_t c, const ErodeArgs& a) {
+  double total = 0.0;
+  for (int n = 0; n <= kG; ++n) total += gr.G(c, n);
+  if (total > (a.active * 1.5)) {
+    for (int z = 9; z >= 1; --z)
+      for (int n = 0; n <= kG - 2; ++n) gr.S(c, z, n) = gr.S(c, z - 1, n);
+    const double coeff = a.active / total;
+    for (int n = 1; n <= kG - 1; ++n) {
+      const double gv = gr.G(c, n);
+      if (gv > 0.0) {
+        const double amount = coeff * (gv);
+        gr.S(c, 0, n - 1) = amount;
+        gr.G(c, n) = gv - amount;
+      }
+    }
+  }
+  if (total < (a.active / 4)) {
+    for (int n = 1; n <= kG - 1; ++n) gr.G(c, n) += gr.S(c, 0, n - 1);
+    for (int z = 0; z <= 8; ++z)
+      for (int n = 0; n <= kG - 2; ++n) gr.S(c, z, n) = gr.S(c, z + 1, n);
+    const double amount = a.active;
+    for (int n = 1; n <= kG - 1; ++n) gr.S(c, 9, n - 1) = amount * a.dprop[n];
+  }
+}
+
+// sort_active as called from outside addGS: allocates first if needed
+__device__ void sort_active_alloc(const Grain& gr, size_t c, const double* elev,
+                                  const double* bedrock, const ErodeArgs& a) {
+  if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
+  sort_active(gr, c, a);
+}
+
+// LCM.cpp:3985-4009
+__device__ double sand_fraction(const Grain& gr, size_t c) {
+  double at = 0, st = 0;
+  for (int n = 1; n <= kG; ++n) at += (gr.G(c, n));
+  for (int n = 1; n <= 2; ++n) st += (gr.G(c, n));
+  if (at < 0.0001) return 0.0;
+  return st / at;
+}
+
+// LCM.cpp:4038-4075
+__device__ double d50(const Grain& gr, size_t c, const ErodeArgs& a) {
+  double at = 0, mx = 0, mn = 0;
+  double cum[12];
+  cum[0] = 0;
+  for (int n = 1; n <= kG; ++n) {
+    at += (gr.G(c, n));
+    cum[n] = at;
+  }
+  cum[11] = 0;
+  int i = 1;
+  while (cum[i] < (at * 0.5) && i <= 9) i++;
+  if (i >= 1 && i <= 9) {
+    mn = a.logd[(i == 1) ? 1 : i - 1];
+    mx = a.logd[i];
+  }
+  double D = exp(mx - ((mx - mn) * ((cum[i] - (at * 0.5)) / (cum[i] - cum[i - 1]))));
+  if (at < 0.0000001) D = 0;
+  return D;
+}
+
+// LCM.cpp:4077-4171, material moves from `src` to `dst`
+__device__ void slide_gs(const Grain& gr, size_t src, size_t dst, double amount, const double* elev,
+                         const double* bedrock, const ErodeArgs& a) {
+  double total = 0;
+  if (gr.has[src] && gr.has[dst]) {
+    for (int n = 1; n <= kG - 1; ++n)
+      if (gr.G(src, n) > 0) total += gr.G(src, n);
+    if (amount > total) {
+      for (int n = 1; n <= kG - 1; ++n) gr.G(dst, n) += (amount - total) * a.dprop[n];
+      amount = total;
+    }
+    if (total > 0) {
+      for (int n = 1; n <= kG - 1; ++n) {
+        const double tr = amount * (gr.G(src, n) / total);
+        gr.G(dst, n) += tr;
+        gr.G(src, n) -= tr;
+        if (gr.G(src, n) < 0) gr.G(src, n) = 0;
+      }
+    }
+    sort_active_alloc(gr, src, elev, bedrock, a);
+    sort_active_alloc(gr, dst, elev, bedrock, a);
+    return;
+  }
+  if (!gr.has[src] && gr.has[dst]) {
+    for (int n = 1; n <= kG - 1; ++n) gr.G(dst, n) += (amount) * a.dprop[n];
+    sort_active_alloc(gr, dst, elev, bedrock, a);
+    return;
+  }
+  if (gr.has[src] && !gr.has[dst]) {
+    add_gs(gr, dst, elev[dst], bedrock[dst], a);
+    if (amount > a.active) {
+      for (int n = 1; n <= kG - 1; ++n) gr.G(dst, n) += amount * a.dprop[n];
+      amount = a.active;
+    }
+    for (int n = 1; n <= kG - 1; ++n)
+      if (gr.G(src, n) > 0) total += gr.G(src, n);
+    for (int n = 1; n <= kG - 1; ++n) {
+      if (total > 0) {
+        gr.G(dst, n) += amount * (gr.G(src, n) / total);
+        if (gr.G(src, n) > 0.0001) gr.G(src, n) -= amount * (gr.G(src, n) / total);
+        if (gr.G(src, n) < 0) gr.G(src, n) = 0;
+      }
+    }
+    sort_active_alloc(gr, src, elev, bedrock, a);
+    sort_active_alloc(gr, dst, elev, bedrock, a);
+    return;
+  }
+}
+
+// direction p (1,3,5,7) -> (dx, dy) of LCM.hpp:445-446 and bedload plane
+__device__ __forceinline__ int dxp(int p) { return p == 3 ? 1 : (p == 7 ? -1 : 0); }
+__device__ __forceinline__ int dyp(int p) { return p == 1 ? -1 : (p == 5 ? 1 : 0); }
+// su (p=1) -> 0, sr (p=3) -> 1, sd (p=5) -> 2, sl (p=7) -> 3
+__device__ __forceinline__ int sdir_of(int p) { return (p - 1) / 2; }
+
+// -------------------------------------------------------- entrainment
+__global__ __launch_bounds__(256) void k_entrain(ErodeArgs a, double time_step, double mult,
+                                                 const double* __restrict__ h,
+                                                 const double* __restrict__ elev,
+                                                 double* __restrict__ bedrock,
+                                                 const double* __restrict__ nman,
+                                                 const uint64_t* __restrict__ mask,
+                                                 const double* __restrict__ V,
+                                                 const double* __restrict__ vd,
+                                                 double* __restrict__ Vel, double* __restrict__ Tau,
+                                                 double* __restrict__ ss, double* __restrict__ sdir,
+                                                 Grain gr, int32_t* __restrict__ need,
+                                                 unsigned long long* __restrict__ red) {
+  const int y = blockIdx.x * 64 + threadIdx.x;
+  const int x = blockIdx.y * blockDim.y + threadIdx.y;
+  double tbmax = 0;
+  if (x >= 1 && x <= a.imax && y >= 1 && y < a.jmax && mask_bit(mask, a.mwords, x, y)) {
+    const size_t P = a.pitch, nc = gr.nc;
+    const size_t c = static_cast<size_t>(x) * P + y;
+    Vel[c] = 0;
+    Tau[c] = 0;
+    for (int d = 0; d < 4; ++d)
+      for (int n = 0; n < kG; ++n) sdir[(d * 10 + n) * nc + c] = 0;
+    ss[c] = 0;
+    const double hc = h[c];
+    if (hc > a.wdet) {
+      const double rho = 1000.0;
+      double veltot = 0, vel = 0, qtot = 0, tau = 0, slopetot = 0;
+      double tempdir[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      double temp_dist[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      const double mann = a.spatial_n ? nman[c] : a.mannings;
+      if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
+      for (int p = 1; p <= 7; p += 2) {
+        const size_t c2 = c + static_cast<long>(dxp(p)) * P + dyp(p);
+        if (h[c2] > a.wdet) {
+          const double vdp = vd[sdir_of(p) * nc + c];
+          if (vdp > 0) {
+            vel = vdp;
+            if (vel > a.max_vel) vel = a.max_vel;
+            tempdir[p] = vel * vel;
+            veltot += tempdir[p];
+            qtot += (vel * vel);
+            slopetot += ((elev[c] - elev[c2]) / a.DX) * vel;
+          }
+        }
+      }
+      if (qtot > 0) {
+        vel = (sqrt(qtot));
+        Vel[c] = vel;
+        if (vel > a.max_vel) vel = a.max_vel;
+        const double ci = kGravity * (mann * mann) * pow(hc, -0.33);
+        if (slopetot > 0) slopetot = 0;
+        tau = 1000 * ci * vel * vel * (1 + (1 * (slopetot / vel)));
+        Tau[c] = tau;
+      }
+      if (tau > 0) {
+        double d_50 = 0, Fs = 0, graintot = 0;
+        if (a.wilcock == 1) {
+          d_50 = d50(gr, c, a);
+          if (d_50 < a.d[1]) d_50 = a.d[1];
+          Fs = sand_fraction(gr, c);
+          for (int n = 1; n <= kG; ++n) graintot += (gr.G(c, n));
+        }
+        double temptot1 = 0;
+        for (int n = 1; n <= kG - 1; ++n) {
+          const double Di = a.d[n];
+          const double gn = gr.G(c, n);
+          if (a.wilcock == 1) {
+            const double tau_ri = (0.021 + (0.015 * exp(-20 * Fs))) * (rho * kGravity * d_50) *
+                                  pow((Di / d_50), (0.67 / (1 + exp(1.5 - (Di / d_50)))));
+            const double U_star = pow(tau / rho, 0.5);
+            const double Fi = gn / graintot;
+            double Wi_star;
+            if ((tau / tau_ri) < 1.35)
+              Wi_star = 0.002 * pow(tau / tau_ri, 7.5);
+            else
+              Wi_star = 14 * pow(1 - (0.894 / pow(tau / tau_ri, 0.5)), 4.5);
+            temp_dist[n] = mult * time_step *
+                           ((Fi * (U_star * U_star * U_star)) / ((2.65 - 1) * kGravity)) * Wi_star / a.DX;
+          }
+          if (a.einstein == 1) {
+            temp_dist[n] = mult * time_step *
+                           (40 * pow((1 / (((2650 - 1000) * Di) / (tau / kGravity))), 3)) /
+                           sqrt(1000 / ((2250 - 1000) * kGravity * (Di * Di * Di))) / a.DX;
+          }
+          if (temp_dist[n] > gn) temp_dist[n] = gn;
+          if (n == 1) {
+            if ((temp_dist[n] + V[c]) / hc > a.Csuspmax) temp_dist[n] = (hc * a.Csuspmax) - V[c];
+          }
+          if (temp_dist[n] < 0) temp_dist[n] = 0;
+          temptot1 += temp_dist[n];
+        }
+        if (elev[c] - temptot1 <= bedrock[c]) {
+          const double elevdiff = elev[c] - bedrock[c];
+          const double temptot3 = temptot1;
+          temptot1 = 0;
+          for (int n = 1; n <= kG - 1; ++n) {
+            if (elev[c] <= bedrock[c]) {
+              temp_dist[n] = 0;
+            } else {
+              temp_dist[n] = elevdiff * (temp_dist[n] / temptot3);
+              if (temp_dist[n] < 0) temp_dist[n] = 0;
+            }
+            temptot1 += temp_dist[n];
+          }
+          if (tau > a.bedrock_thr) {
+            const double amount =
+                pow(a.bedrock_rate * tau, 1.5) * time_step * mult * 0.000000317;
+            bedrock[c] -= amount;
+            for (int n2 = 1; n2 <= kG - 1; ++n2) gr.G(c, n2) += amount * a.dprop[n2];
+          }
+        }
+        if (temptot1 > tbmax) tbmax = temptot1;
+        if (temptot1 > 0) {
+          for (int p = 1; p <= 7; p += 2) {
+            const size_t c2 = c + static_cast<long>(dxp(p)) * P + dyp(p);
+            if (h[c2] > a.wdet) {
+              if (!gr.has[c2]) need[c2] = 1;
+              double factor = 0;
+              if (vd[sdir_of(p) * nc + c] > 0) factor += 0.75 * tempdir[p] / veltot;
+              for (int n = 1; n <= kG - 1; ++n) {
+                if (temp_dist[n] > 0) {
+                  if (n == 1)
+                    ss[c] = temp_dist[n];
+                  else
+                    sdir[(sdir_of(p) * 10 + n) * nc + c] = temp_dist[n] * factor;
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  tbmax = wave_max(tbmax);
+  if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) atomic_max_pos(red, tbmax);
+}
+
+// addGS for the neighbours the entrainment pass flagged
+__global__ __launch_bounds__(256) void k_alloc(ErodeArgs a, const double* __restrict__ elev,
+                                               const double* __restrict__ bedrock, Grain gr,
+                                               int32_t* __restrict__ need, size_t n) {
+  for (size_t c = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; c < n;
+       c += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    if (need[c]) {
+      if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
+      need[c] = 0;
+    }
+  }
+}
+
+// Lateral slide from neighbour `cn` into cell `c` (LCM.cpp:4619-4664 and
+// 4689-4742; veg and edge are zero in this build).  As in the reference, once
+// the first amount is positive the slide is carried out even when the bedrock
+// or domain-edge test has set it back to zero: slide_GS with a zero amount
+// still clamps negative fractions and re-sorts both active layers.
+__device__ __forceinline__ void lateral_slide(const ErodeArgs& a, const Grain& gr, double mult,
+                                              double time_step, const double* h, double* elev,
+                                              const double* bedrock, double tau_c, double e3,
+                                              size_t cn, size_t c, bool at_edge,
+                                              double& elev_update) {
+  double amt = 0;
+  if (h[cn] < a.wdet)
+    amt = mult * a.lateral_constant * tau_c * 0.0 * time_step / a.DX;
+  else
+    amt = a.chann_lateral * e3 * (elev[cn] - elev[c]) / a.DX * 0.1;
+  if (amt > 0) {
+    amt *= 1 - (0.0 * (1 - a.veg_lat));
+    if ((elev[cn] - amt) < bedrock[cn] || at_edge) amt = 0;
+    if (amt > a.ERODEFACTOR * 0.1) amt = a.ERODEFACTOR * 0.1;
+    elev_update += amt;
+    elev[cn] -= amt;
+    slide_gs(gr, cn, c, amt, elev, bedrock, a);
+  }
+}
+
+// ------------------------------------------- deposition + x-lateral slide
+__global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, double mult,
+                                                const double* __restrict__ h,
+                                                double* __restrict__ elev,
+                                                const double* __restrict__ bedrock,
+                                                const uint64_t* __restrict__ mask,
+                                                double* __restrict__ V,
+                                                const double* __restrict__ Tau,
+                                                const double* __restrict__ ss,
+                                                const double* __restrict__ sdir, Grain gr,
+                                                double* __restrict__ etot,
+                                                double* __restrict__ etot3) {
+  const int y = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= a.jmax) return;
+  const size_t P = a.pitch, nc = gr.nc;
+  auto S = [&](int d, int n, size_t cell) { return sdir[(d * 10 + n) * nc + cell]; };
+  for (int x = 2; x < a.imax; ++x) {
+    if (!mask_bit(mask, a.mwords, x, y)) continue;
+    const size_t c = static_cast<size_t>(x) * P + y;
+    if (!(h[c] > a.wdet)) continue;
+    if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
+    double et = 0, et3 = 0;
+    for (int n = 1; n <= kG - 1; ++n) {
+      if (n == 1) {
+        double v = V[c] + ss[c];
+        gr.G(c, n) -= ss[c];
+        et -= ss[c];
+        double coeff = (a.fall[n] * time_step) / h[c];
+        if (coeff > 1) coeff = 1;
+        double drop = coeff * v;
+        if (drop > 0.001) drop = 0.001;
+        gr.G(c, n) += drop;
+        et += drop;
+        V[c] = v - drop;
+      } else {
+        const double val1 = (S(0, n, c) + S(1, n, c) + S(2, n, c) + S(3, n, c));
+        const double val2 = (S(0, n, c + 1) + S(2, n, c - 1) + S(3, n, c + P) + S(1, n, c - P));
+        gr.G(c, n) += val2 - val1;
+        et += val2 - val1;
+        et3 += val1;
+      }
+    }
+    etot[c] = et;
+    etot3[c] = et3;
+    elev[c] += et;
+    if (et < 0) sort_active_alloc(gr, c, elev, bedrock, a);
+    if (et3 > 0) {
+      double elev_update = 0;
+      const size_t cu = c - P, cd = c + P;
+      if (elev[cu] > elev[c] && x > 2)
+        lateral_slide(a, gr, mult, time_step, h, elev, bedrock, Tau[c], et3, cu, c, x - 1 == 1,
+                      elev_update);
+      if (elev[cd] > elev[c] && x < a.imax - 1)
+        lateral_slide(a, gr, mult, time_step, h, elev, bedrock, Tau[c], et3, cd, c,
+                      x + 1 == a.imax, elev_update);
+      elev[c] += elev_update;
+    }
+  }
+}
+
+// --------------------------------------------------------- y-lateral slide
+__global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step, double mult,
+                                                  const double* __restrict__ h,
+                                                  double* __restrict__ elev,
+                                                  const double* __restrict__ bedrock,
+                                                  const uint64_t* __restrict__ mask,
+                                                  const double* __restrict__ Tau, Grain gr,
+                                                  const double* __restrict__ etot3) {
+  const int x = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (x > a.imax) return;
+  const size_t P = a.pitch;
+  for (int y = 2; y < a.jmax; ++y) {
+    const size_t c = static_cast<size_t>(x) * P + y;
+    const double et3 = etot3[c];
+    if (!(et3 > 0) || !mask_bit(mask, a.mwords, x, y)) continue;
+    double elev_update = 0;
+    const size_t cl = c - 1, cr = c + 1;
+    if (elev[cl] > elev[c])
+      lateral_slide(a, gr, mult, time_step, h, elev, bedrock, Tau[c], et3, cl, c, y - 1 == 1,
+                    elev_update);
+    if (elev[cr] > elev[c])
+      lateral_slide(a, gr, mult, time_step, h, elev, bedrock, Tau[c], et3, cr, c, y + 1 == a.jmax,
+                    elev_update);
+    elev[c] += elev_update;
+  }
+}
+
+// ------------------------------------------------------------ edge outputs
+__global__ __launch_bounds__(64) void k_edges(ErodeArgs a, const double* __restrict__ h,
+                                              double* __restrict__ V,
+                                              const double* __restrict__ sdir, size_t nc,
+                                              float* __restrict__ gtot) {
+  const int n = threadIdx.x;  // fraction
+  const bool act = (n >= 1 && n <= kG - 1);
+  const int I = a.imax, J = a.jmax;
+  const size_t P = a.pitch;
+  auto S = [&](int d, size_t cell) { return sdir[(d * 10 + n) * nc + cell]; };
+  float part1 = 0.0f, part2 = 0.0f;
+  for (int y = 2; y < J; ++y) {
+    const size_t cA = static_cast<size_t>(I) * P + y;
+    const double vA = V[cA];
+    const bool condA = (h[cA] > a.wdet || vA > 0);
+    if (act && condA) part1 = static_cast<float>(static_cast<double>(part1) + (n == 1 ? vA : S(1, cA - P)));
+    __syncthreads();
+    if (n == 1 && condA) V[cA] = 0;
+    __syncthreads();
+    const size_t cB = static_cast<size_t>(1) * P + y;
+    const double vB = V[cB];
+    const bool condB = (h[cB] > a.wdet || vB > 0);
+    if (act && condB) part1 = static_cast<float>(static_cast<double>(part1) + (n == 1 ? vB : S(3, cB + P)));
+    __syncthreads();
+    if (n == 1 && condB) V[cB] = 0;
+    __syncthreads();
+  }
+  for (int x = 2; x < I; ++x) {
+    const size_t cA = static_cast<size_t>(x) * P + J;
+    const double vA = V[cA];
+    const bool condA = (h[cA] > a.wdet || vA > 0);
+    if (act && condA) part2 = static_cast<float>(static_cast<double>(part2) + (n == 1 ? vA : S(2, cA - 1)));
+    __syncthreads();
+    if (n == 1 && condA) V[cA] = 0;
+    __syncthreads();
+    const size_t cB = static_cast<size_t>(x) * P + 1;
+    const double vB = V[cB];
+    const bool condB = (h[cB] > a.wdet || vB > 0);
+    if (act && condB) part2 = static_cast<float>(static_cast<double>(part2) + (n == 1 ? vB : S(0, cB + 1)));
+    __syncthreads();
+    if (n == 1 && condB) V[cB] = 0;
+    __syncthreads();
+  }
+  if (n < 20) gtot[n] = act ? (part1 + part2) : 0.0f;
+}
+
+ErodeArgs erode_args(const hc_ctx* c) {
+  ErodeArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.imax = c->g.imax;
+  a.jmax = c->g.jmax;
+  a.pitch = c->P;
+  a.mwords = c->W;
+  a.DX = c->g.DX;
+  a.wdet = c->prm.water_depth_erosion_threshold;
+  a.ERODEFACTOR = c->prm.ERODEFACTOR;
+  a.active = c->prm.active;
+  a.Csuspmax = c->prm.Csuspmax;
+  a.max_vel = c->prm.max_vel;
+  a.mannings = c->prm.mannings;
+  a.spatial_n = c->prm.spatially_var_mannings;
+  a.wilcock = c->prm.transport_law == HC_WILCOCK ? 1 : 0;
+  a.einstein = c->prm.transport_law == HC_EINSTEIN ? 1 : 0;
+  for (int k = 0; k < 10; ++k) {
+    a.d[k] = c->prm.d[k];
+    a.logd[k] = (k >= 1) ? std::log(c->prm.d[k]) : 0.0;  // host libm, as the reference
+  }
+  for (int k = 0; k < 11; ++k) {
+    a.dprop[k] = c->prm.dprop[k];
+    a.fall[k] = c->prm.fallVelocity[k];
+  }
+  a.bedrock_thr = c->prm.bedrock_erosion_threshold;
+  a.bedrock_rate = c->prm.bedrock_erosion_rate;
+  a.lateral_constant = c->prm.lateral_constant;
+  a.chann_lateral = c->prm.chann_lateral_erosion;
+  a.veg_lat = c->prm.veg_lat_restriction;
+  return a;
+}
+
+}  // namespace
+
+int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot_out) {
+  if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "erode: FP64 contexts only");
+  const ErodeArgs a = erode_args(c);
+  const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
+  const double* h = static_cast<const double*>(c->h[c->cur]);
+  double* elev = static_cast<double*>(c->z);
+  const double* nman = static_cast<const double*>(c->nman);
+  const dim3 blk(64, 4);
+  const dim3 cells(static_cast<unsigned>(c->P / 64), static_cast<unsigned>((c->NR + 3) / 4));
+  double ts = *time_step * 1.5;  // LCM.cpp:4219
+  double tb = 0;
+  for (int pass = 0;; ++pass) {
+    if (pass > 10000) return hc_fail(c, HC_ERR_UNSTABLE, "erode: timestep loop did not converge");
+    HC_HIP(c, hipMemsetAsync(c->d_red + 1, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_entrain, cells, blk, 0, c->stream, a, ts, mult, h, elev, c->bedrock, nman,
+                       c->mask, c->V, c->veldir, c->Vel, c->Tau, c->ss, c->sdir, gr, c->needgrain,
+                       c->d_red + 1);
+    HC_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_alloc, dim3(1024), dim3(256), 0, c->stream, a, elev, c->bedrock, gr,
+                       c->needgrain, c->ncell);
+    HC_HIP(c, hipGetLastError());
+    HC_HIP(c, hipMemcpyAsync(c->h_pinned, c->d_red + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HC_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(&tb, c->h_pinned, sizeof(double));
+    if (tb > a.ERODEFACTOR) ts *= (a.ERODEFACTOR / tb) * 0.5;  // LCM.cpp:4550-4553
+    if (!(tb > a.ERODEFACTOR)) break;
+  }
+  HC_HIP(c, hipMemsetAsync(c->etot, 0, c->ncell * sizeof(double), c->stream));
+  HC_HIP(c, hipMemsetAsync(c->etot3, 0, c->ncell * sizeof(double), c->stream));
+  const int ncol = std::max(0, c->g.jmax - 2);
+  if (ncol > 0) {
+    hipLaunchKernelGGL(k_deposit, dim3((ncol + 63) / 64), dim3(64), 0, c->stream, a, ts, mult, h,
+                       elev, c->bedrock, c->mask, c->V, c->Tau, c->ss, c->sdir, gr, c->etot,
+                       c->etot3);
+    HC_HIP(c, hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_lateral_y, dim3((c->g.imax + 63) / 64), dim3(64), 0, c->stream, a, ts, mult,
+                     h, elev, c->bedrock, c->mask, c->Tau, gr, c->etot3);
+  HC_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_edges, dim3(1), dim3(64), 0, c->stream, a, h, c->V, c->sdir, c->ncell,
+                     c->gtot);
+  HC_HIP(c, hipGetLastError());
+  float g[20];
+  HC_HIP(c, hipMemcpyAsync(g, c->gtot, sizeof(float) * 10, hipMemcpyDeviceToHost, c->stream));
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  *time_step = ts;
+  if (tempbmax) *tempbmax = tb;
+  if (gtot_out)
+    for (int n = 0; n < 10; ++n) gtot_out[n] = g[n];
+  return HC_OK;
 }

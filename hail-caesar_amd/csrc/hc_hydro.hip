@@ -15,47 +15,17 @@
 //    slope = (dz + (h_nb - h_cell)) / DX
 // and faces with |dz| < flat_dz take dz from the FP64 DEM.
 #include <algorithm>
+#include <type_traits>
 
+#include "hc_device.hpp"
 #include "hc_internal.hpp"
 
 using hc::HydroArgs;
 using hc::kGravity;
 
+using namespace hcdev;
+
 namespace {
-
-// ---------------------------------------------------------------- helpers
-__device__ __forceinline__ double dmax_ref(double a, double b) {
-  return (a < b) ? b : a;  // std::max(a, b)
-}
-
-// Correctly rounded a*a*a: a^2 = p + e and p*a = q + f exactly (fma), so
-// a^3 = q + (f + e*a) with the bracket far below half an ulp of q.
-__device__ __forceinline__ double cube_rn(double a) {
-  const double p = a * a;
-  const double e = __builtin_fma(a, a, -p);
-  const double q = p * a;
-  const double f = __builtin_fma(p, a, -q);
-  const double t = __builtin_fma(e, a, f);
-  return q + t;
-}
-
-__device__ __forceinline__ bool mask_bit(const uint64_t* __restrict__ m, int W, int x, int y) {
-  return (m[static_cast<size_t>(x) * W + (y >> 6)] >> (y & 63)) & 1ull;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const double w = __shfl_xor(v, o, 64);
-    v = (v < w) ? w : v;
-  }
-  return v;
-}
-
-__device__ __forceinline__ void atomic_max_pos(unsigned long long* dst, double v) {
-  // v >= 0: the bit patterns of non-negative doubles order like the values
-  if (v > 0) atomicMax(dst, static_cast<unsigned long long>(__double_as_longlong(v)));
-}
 
 // One face of the FP64 plain form (LSDCatchmentModel.cpp:3094-3191 for the
 // x-face, 3195-3242 for the y-face).  Cell values (hc, zc), neighbour across
@@ -212,244 +182,156 @@ __global__ __launch_bounds__(256) void k_depth64(HydroArgs a, double dt, double*
   if ((threadIdx.x & 63) == 0) atomic_max_pos(red, md);
 }
 
-// ----------------------------------------------------- fused, FP64 form
-// Each wavefront owns a 64-column strip and marches down R output rows,
-// keeping the previous row in registers.  Reads generation `0` buffers and
-// writes generation `1` buffers for every cell of its rows (unmasked cells
-// are copied through, which keeps the reference's stale-flux semantics).
-// Column neighbours come from lane shuffles; lane 0 loads the cell left of
-// the strip and lane 63 recomputes the y-face right of the strip, which the
-// depth update of its own cell needs.
-template <int R>
-__global__ __launch_bounds__(256) void k_flow_depth64(HydroArgs a, double dt,
-                                                      const double* __restrict__ h0,
-                                                      const double* __restrict__ z,
-                                                      const double* __restrict__ qx0,
-                                                      const double* __restrict__ qy0,
-                                                      const double* __restrict__ nman,
-                                                      const uint64_t* __restrict__ mask,
-                                                      double* __restrict__ h1,
-                                                      double* __restrict__ qx1,
-                                                      double* __restrict__ qy1,
-                                                      unsigned long long* red) {
+// ------------------------------------------------------------ fused kernel
+// flow_route followed by depth_update in one pass, reading generation-0
+// buffers and writing generation 1 for every cell (unmasked cells are copied,
+// which keeps the reference's stale-flux semantics).
+//
+// Tiling: a wavefront covers 64 consecutive columns starting at 62*w and owns
+// the 62 in the middle; its first and last lanes are halo columns that only
+// feed the y-face neighbours through lane shuffles (the left neighbour's
+// depth and elevation, the right neighbour's new y-face flux), so no lane
+// issues a special load.  Column 0 (padding) is owned by lane 0 of wavefront
+// 0.  Each wavefront takes R output rows and first loads its whole tile
+// (R+2 rows of h and z, R+1 rows of qx, qy, n and the mask) so all of a
+// tile's loads are in flight together; then it computes the rows in order.
+// Blocks are four wavefronts side by side; consecutive block ids go to the
+// eight XCDs in turn, so the block id is remapped to give each XCD a
+// contiguous band of tiles (whose halo rows then hit the same L2).
+//
+// MODE 0: FP64 reference form.  MODE 1: FP32 state with FP32 elevations (and
+// the FP64 fallback for near-flat faces when flat_dz > 0).  MODE 2: FP32
+// state with FP64 elevations (exact face differences).
+template <int MODE>
+struct FusedTypes {
+  using T = typename std::conditional<MODE == 0, double, float>::type;
+  using Z = typename std::conditional<MODE == 1, float, double>::type;
+};
+
+template <int MODE, int R>
+__global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int nbx, int nblocks,
+                                               const typename FusedTypes<MODE>::T* __restrict__ h0,
+                                               const typename FusedTypes<MODE>::Z* __restrict__ zz,
+                                               const double* __restrict__ z64,
+                                               const typename FusedTypes<MODE>::T* __restrict__ qx0,
+                                               const typename FusedTypes<MODE>::T* __restrict__ qy0,
+                                               const typename FusedTypes<MODE>::T* __restrict__ nman,
+                                               const uint64_t* __restrict__ mask,
+                                               typename FusedTypes<MODE>::T* __restrict__ h1,
+                                               typename FusedTypes<MODE>::T* __restrict__ qx1,
+                                               typename FusedTypes<MODE>::T* __restrict__ qy1,
+                                               unsigned long long* red,
+                                               const double* __restrict__ dtp,
+                                               const int* __restrict__ haltp) {
+  using T = typename FusedTypes<MODE>::T;
+  using Z = typename FusedTypes<MODE>::Z;
+  if (haltp && *haltp) return;  // device-resident loop has finished
+  const double dtd = dtp ? *dtp : dt_arg;
+  // XCD-aware block order
+  const int b = blockIdx.x;
+  const int q8 = nblocks / 8, r8 = nblocks % 8;
+  const int xcd = b % 8;
+  const int bb = xcd * q8 + min(xcd, r8) + b / 8;
+  const int bx = bb % nbx, by = bb / nbx;
+
   const int lane = threadIdx.x & 63;
-  const int c0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-  if (c0 >= a.pitch) return;  // uniform per wavefront
-  const int c = c0 + lane;
-  const int NR = a.imax + 2;
-  const int r0 = blockIdx.y * R;
-  const int rend = min(r0 + R, NR);
-  const int xlast = (rend < NR) ? rend : NR - 1;
+  const int w = bx * 4 + (threadIdx.x >> 6);
+  const int NR = a.imax + 2, NC = a.jmax + 2;
+  if (62 * w + 1 > NC - 1) return;  // no owned column (uniform per wavefront)
+  const int c = 62 * w + lane;
+  const bool owned = ((lane >= 1 && lane <= 62) || (w == 0 && lane == 0)) && c <= NC - 1;
   const size_t P = a.pitch;
-  const int W = a.mwords;
-  const int wcol = c0 >> 6;
-  const bool has_right = (c0 + 64) < a.pitch;
+  const int cl = min(c, a.pitch - 1);  // load column (lanes past the row reuse the last)
+  const int r0 = by * R;
 
-  double hP = 0, zP = -9999;
-  if (r0 >= 1) {
-    hP = h0[static_cast<size_t>(r0 - 1) * P + c];
-    zP = z[static_cast<size_t>(r0 - 1) * P + c];
+  T hh[R + 2];
+  Z zt[R + 2];
+  T qxo[R + 2], qyo[R + 2], nn[R + 2];
+  uint32_t mbits = 0;
+  const T nu = static_cast<T>(a.mannings);
+#pragma unroll
+  for (int k = 0; k < R + 2; ++k) {
+    const int x = r0 - 1 + k;
+    const bool v = (x >= 0 && x < NR);
+    const size_t o = static_cast<size_t>(v ? x : 0) * P + cl;
+    hh[k] = v ? h0[o] : T(0);
+    zt[k] = v ? zz[o] : Z(-9999);
+    if (k >= 1) {
+      qxo[k] = v ? qx0[o] : T(0);
+      qyo[k] = v ? qy0[o] : T(0);
+      nn[k] = (a.spatial_n && v) ? nman[o] : nu;
+      const uint64_t word = v ? mask[static_cast<size_t>(x) * a.mwords + (cl >> 6)] : 0ull;
+      mbits |= static_cast<uint32_t>((word >> (cl & 63)) & 1ull) << k;
+    }
   }
-  double qxNp = 0, qyNp = 0, qyRp = 0;
-  bool mP = false;
-  double md = 0;
 
-  for (int x = r0; x <= xlast; ++x) {
-    const size_t row = static_cast<size_t>(x) * P;
-    const double hx = h0[row + c];
-    const double zx = z[row + c];
-    const double qxo = qx0[row + c];
-    const double qyo = qy0[row + c];
-    const uint64_t mw = mask[static_cast<size_t>(x) * W + wcol];
-    const bool mx = (mw >> lane) & 1ull;
-    double hl = __shfl_up(hx, 1, 64);
-    double zl = __shfl_up(zx, 1, 64);
-    if (lane == 0) {
-      if (c0 > 0) {
-        hl = h0[row + c - 1];
-        zl = z[row + c - 1];
+  const T dt = static_cast<T>(dtd);
+  const float invdt = 1.0f / static_cast<float>(dtd);
+  const float fthr = static_cast<float>(a.flat_dz);
+  T qxNp = 0, qyNp = 0, qyRp = 0;
+  T md = 0;
+#pragma unroll
+  for (int k = 1; k < R + 2; ++k) {
+    const int x = r0 - 1 + k;
+    const T hl = __shfl_up(hh[k], 1, 64);
+    const Z zl = __shfl_up(zt[k], 1, 64);
+    T qxN = qxo[k], qyN = qyo[k];
+    const bool m = (mbits >> k) & 1u;
+    if (m && zt[k] > Z(-9999)) {
+      if constexpr (MODE == 0) {
+        double dummy = 0;
+        face64<false>(qxN, dummy, hh[k], hh[k - 1], zt[k], zt[k - 1], nn[k], dtd, a,
+                      edge_code(x, a.imax), true, 0, 0, nullptr, nullptr);
+        face64<false>(qyN, dummy, hh[k], hl, zt[k], zl, nn[k], dtd, a, edge_code(c, a.jmax), false,
+                      0, 0, nullptr, nullptr);
       } else {
-        hl = 0;
-        zl = -9999;
+        auto dz = [&](Z zn, Z zc, size_t cn, size_t cc) -> float {
+          if constexpr (MODE == 2) {
+            return static_cast<float>(zn - zc);
+          } else {
+            float d = zn - zc;
+            if (fthr > 0.0f && fabsf(d) < fthr) d = static_cast<float>(z64[cn] - z64[cc]);
+            return d;
+          }
+        };
+        const size_t oc = static_cast<size_t>(x) * P + cl;
+        if ((hh[k] > 0.0f || hh[k - 1] > 0.0f) && zt[k - 1] > Z(-9999))
+          qxN = face32(qxo[k], hh[k], hh[k - 1], dz(zt[k - 1], zt[k], oc - P, oc), nn[k], dt, invdt,
+                       a, edge_code(x, a.imax));
+        if ((hh[k] > 0.0f || hl > 0.0f) && zl > Z(-9999))
+          qyN = face32(qyo[k], hh[k], hl, dz(zl, zt[k], oc - 1, oc), nn[k], dt, invdt, a,
+                       edge_code(c, a.jmax));
       }
     }
-    double qxN = qxo, qyN = qyo;
-    if (mx && zx > -9999) {
-      const double n = a.spatial_n ? nman[row + c] : a.mannings;
-      double dummy = 0;
-      face64<false>(qxN, dummy, hx, hP, zx, zP, n, dt, a, edge_code(x, a.imax), true, 0, 0,
-                    nullptr, nullptr);
-      face64<false>(qyN, dummy, hx, hl, zx, zl, n, dt, a, edge_code(c, a.jmax), false, 0, 0,
-                    nullptr, nullptr);
-    }
-    double qyR = __shfl_down(qyN, 1, 64);
-    if (lane == 63) {
-      qyR = 0;
-      if (has_right) {
-        const size_t cr = row + c + 1;
-        const uint64_t mwr = mask[static_cast<size_t>(x) * W + wcol + 1];
-        qyR = qy0[cr];
-        const double zr = z[cr];
-        if ((mwr & 1ull) && zr > -9999) {
-          const double n = a.spatial_n ? nman[cr] : a.mannings;
-          double dummy = 0;
-          face64<false>(qyR, dummy, h0[cr], hx, zr, zx, n, dt, a, edge_code(c + 1, a.jmax), false,
-                        0, 0, nullptr, nullptr);
+    const T qyR = __shfl_down(qyN, 1, 64);
+    if (k >= 2) {  // row x-1 has both of its x-faces and y-faces now
+      const int xo = x - 1;
+      T hn = hh[k - 1];
+      if ((mbits >> (k - 1)) & 1u) {
+        if constexpr (MODE == 0) {
+          hn = hh[k - 1] + dtd * (qxN - qxNp + qyRp - qyNp) / a.DX;
+        } else {
+          hn = hh[k - 1] + static_cast<float>(dtd / a.DX) * ((qxN - qxNp) + (qyRp - qyNp));
         }
-      }
-    }
-    if (x > r0) {  // finalise row x-1
-      const size_t o = row - P + c;
-      double hn = hP;
-      if (mP) {
-        hn = hP + dt * (qxN - qxNp + qyRp - qyNp) / a.DX;
-        if (hn > 0) {
-          if (zP == -9999) hn = 0;
+        if (hn > T(0)) {
+          if (zt[k - 1] == Z(-9999)) hn = T(0);
           if (hn > md) md = hn;
         }
       }
-      h1[o] = hn;
-      qx1[o] = qxNp;
-      qy1[o] = qyNp;
+      if (owned && xo < NR) {
+        const size_t o = static_cast<size_t>(xo) * P + c;
+        h1[o] = hn;
+        qx1[o] = qxNp;
+        qy1[o] = qyNp;
+      }
     }
-    hP = hx;
-    zP = zx;
     qxNp = qxN;
     qyNp = qyN;
     qyRp = qyR;
-    mP = mx;
   }
-  if (rend == NR) {  // last padding row: never on the mask, copy through
-    const size_t o = static_cast<size_t>(NR - 1) * P + c;
-    h1[o] = hP;
-    qx1[o] = qxNp;
-    qy1[o] = qyNp;
-  }
-  md = wave_max(md);
-  if (lane == 0) atomic_max_pos(red, md);
-}
-
-// ----------------------------------------------------- fused, FP32 form
-__device__ __forceinline__ float flat_dz(float dz, const double* __restrict__ z64, size_t cn,
-                                         size_t cc, float thr) {
-  if (fabsf(dz) < thr) dz = static_cast<float>(z64[cn] - z64[cc]);
-  return dz;
-}
-
-template <int R>
-__global__ __launch_bounds__(256) void k_flow_depth32(HydroArgs a, float dt,
-                                                      const float* __restrict__ h0,
-                                                      const float* __restrict__ z,
-                                                      const double* __restrict__ z64,
-                                                      const float* __restrict__ qx0,
-                                                      const float* __restrict__ qy0,
-                                                      const float* __restrict__ nman,
-                                                      const uint64_t* __restrict__ mask,
-                                                      float* __restrict__ h1,
-                                                      float* __restrict__ qx1,
-                                                      float* __restrict__ qy1,
-                                                      unsigned long long* red) {
-  const int lane = threadIdx.x & 63;
-  const int c0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-  if (c0 >= a.pitch) return;
-  const int c = c0 + lane;
-  const int NR = a.imax + 2;
-  const int r0 = blockIdx.y * R;
-  const int rend = min(r0 + R, NR);
-  const int xlast = (rend < NR) ? rend : NR - 1;
-  const size_t P = a.pitch;
-  const int W = a.mwords;
-  const int wcol = c0 >> 6;
-  const bool has_right = (c0 + 64) < a.pitch;
-  const float invdt = 1.0f / dt;
-  const float fthr = static_cast<float>(a.flat_dz);
-  const float dtDX = dt / static_cast<float>(a.DX);
-  const float nu = static_cast<float>(a.mannings);
-
-  float hP = 0.0f, zP = -9999.0f;
-  if (r0 >= 1) {
-    hP = h0[static_cast<size_t>(r0 - 1) * P + c];
-    zP = z[static_cast<size_t>(r0 - 1) * P + c];
-  }
-  float qxNp = 0, qyNp = 0, qyRp = 0;
-  bool mP = false;
-  float md = 0;
-
-  for (int x = r0; x <= xlast; ++x) {
-    const size_t row = static_cast<size_t>(x) * P;
-    const float hx = h0[row + c];
-    const float zx = z[row + c];
-    const float qxo = qx0[row + c];
-    const float qyo = qy0[row + c];
-    const float nx = a.spatial_n ? nman[row + c] : nu;
-    const uint64_t mw = mask[static_cast<size_t>(x) * W + wcol];
-    const bool mx = (mw >> lane) & 1ull;
-    float hl = __shfl_up(hx, 1, 64);
-    float zl = __shfl_up(zx, 1, 64);
-    if (lane == 0) {
-      if (c0 > 0) {
-        hl = h0[row + c - 1];
-        zl = z[row + c - 1];
-      } else {
-        hl = 0.0f;
-        zl = -9999.0f;
-      }
-    }
-    float qxN = qxo, qyN = qyo;
-    if (mx && zx > -9999.0f) {
-      if ((hx > 0.0f || hP > 0.0f) && zP > -9999.0f) {
-        const float dz = flat_dz(zP - zx, z64, row - P + c, row + c, fthr);
-        qxN = face32(qxo, hx, hP, dz, nx, dt, invdt, a, edge_code(x, a.imax));
-      }
-      if ((hx > 0.0f || hl > 0.0f) && zl > -9999.0f) {
-        const float dz = flat_dz(zl - zx, z64, row + c - 1, row + c, fthr);
-        qyN = face32(qyo, hx, hl, dz, nx, dt, invdt, a, edge_code(c, a.jmax));
-      }
-    }
-    float qyR = __shfl_down(qyN, 1, 64);
-    if (lane == 63) {
-      qyR = 0.0f;
-      if (has_right) {
-        const size_t cr = row + c + 1;
-        const uint64_t mwr = mask[static_cast<size_t>(x) * W + wcol + 1];
-        qyR = qy0[cr];
-        const float zr = z[cr];
-        const float hr = h0[cr];
-        if ((mwr & 1ull) && zr > -9999.0f && (hr > 0.0f || hx > 0.0f) && zx > -9999.0f) {
-          const float nr = a.spatial_n ? nman[cr] : nu;
-          const float dz = flat_dz(zx - zr, z64, row + c, cr, fthr);
-          qyR = face32(qyR, hr, hx, dz, nr, dt, invdt, a, edge_code(c + 1, a.jmax));
-        }
-      }
-    }
-    if (x > r0) {
-      const size_t o = row - P + c;
-      float hn = hP;
-      if (mP) {
-        hn = hP + dtDX * ((qxN - qxNp) + (qyRp - qyNp));
-        if (hn > 0.0f) {
-          if (zP == -9999.0f) hn = 0.0f;
-          md = fmaxf(md, hn);
-        }
-      }
-      h1[o] = hn;
-      qx1[o] = qxNp;
-      qy1[o] = qyNp;
-    }
-    hP = hx;
-    zP = zx;
-    qxNp = qxN;
-    qyNp = qyN;
-    qyRp = qyR;
-    mP = mx;
-  }
-  if (rend == NR) {
-    const size_t o = static_cast<size_t>(NR - 1) * P + c;
-    h1[o] = hP;
-    qx1[o] = qxNp;
-    qy1[o] = qyNp;
-  }
-  double mdd = wave_max(static_cast<double>(md));
+  // the halo lanes' depths are not this wavefront's results
+  double mdd = owned ? static_cast<double>(md) : 0.0;
+  mdd = wave_max(mdd);
   if (lane == 0) atomic_max_pos(red, mdd);
 }
 
@@ -476,21 +358,6 @@ __global__ __launch_bounds__(256) void k_scan(int imax, int jmax, int P, int W,
 }
 
 // --------------------------------------------------------- water_flux_out
-// Unique edge cells: row 1, row imax, then columns 1 and jmax of rows
-// 2..imax-1.  Every cell drains once, as in the reference where a corner's
-// second visit already finds it at the threshold.
-__device__ __forceinline__ void edge_cell(int k, int imax, int jmax, int& x, int& y) {
-  if (k < jmax) { x = 1; y = k + 1; return; }
-  k -= jmax;
-  if (imax > 1) {
-    if (k < jmax) { x = imax; y = k + 1; return; }
-    k -= jmax;
-  }
-  const int nside = (jmax > 1) ? 2 : 1;
-  x = 2 + k / nside;
-  y = (k % nside == 0) ? 1 : jmax;
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_flux_out(int imax, int jmax, int P, double DX,
                                                   double thr, double ts, T* __restrict__ h,
@@ -589,8 +456,6 @@ dim3 cell_grid(const hc_ctx* c) {
 
 }  // namespace
 
-// Rows per wavefront of the fused kernels (set by the tile tuner).
-int g_fused_rows = 32;
 
 int hc_launch_flow_route(hc_ctx* c, double dt) {
   if (c->prec != HC_FP64)
@@ -633,42 +498,109 @@ int hc_launch_depth_update(hc_ctx* c, double dt) {
   return HC_OK;
 }
 
-int hc_launch_flow_depth(hc_ctx* c, double dt) {
+int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* haltp, bool swap) {
   if (c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
   const HydroArgs a = hydro_args(c);
-  const int R = g_fused_rows;
-  const dim3 grid(static_cast<unsigned>((c->P / 64 + 3) / 4),
-                  static_cast<unsigned>((c->NR + R - 1) / R));
-  const dim3 blk(256);
+  const int R = c->fused_rows;
+  const int nwaves = (c->NC - 1 + 61) / 62;
+  const int nbx = (nwaves + 3) / 4;
+  const int nby = (c->NR + R - 1) / R;
+  const int nblocks = nbx * nby;
   const int k = c->cur, o = c->cur ^ 1;
-  HC_HIP(c, hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream));
+  // in the device-resident loop the control kernel clears the reduction slot
+  if (!dtp) HC_HIP(c, hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream));
+  auto go = [&](auto kern, auto* tag_t, auto* tag_z) {
+    using T = std::remove_pointer_t<decltype(tag_t)>;
+    using Z = std::remove_pointer_t<decltype(tag_z)>;
+    const Z* zp = std::is_same<Z, double>::value && c->prec == HC_FP32
+                      ? reinterpret_cast<const Z*>(c->z64)
+                      : reinterpret_cast<const Z*>(c->z);
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks,
+                       (const T*)c->h[k], zp, (const double*)c->z64, (const T*)c->qx[k],
+                       (const T*)c->qy[k], (const T*)c->nman, (const uint64_t*)c->mask,
+                       (T*)c->h[o], (T*)c->qx[o], (T*)c->qy[o], c->d_red, dtp, haltp);
+  };
+  double* td = nullptr;
+  float* tf = nullptr;
   if (c->prec == HC_FP64) {
-    auto args = [&](auto kern) {
-      hipLaunchKernelGGL(kern, grid, blk, 0, c->stream, a, dt, (const double*)c->h[k],
-                         (const double*)c->z, (const double*)c->qx[k], (const double*)c->qy[k],
-                         (const double*)c->nman, (const uint64_t*)c->mask, (double*)c->h[o],
-                         (double*)c->qx[o], (double*)c->qy[o], c->d_red);
-    };
-    if (R == 16) args(k_flow_depth64<16>);
-    else if (R == 64) args(k_flow_depth64<64>);
-    else if (R == 8) args(k_flow_depth64<8>);
-    else args(k_flow_depth64<32>);
+    if (R == 4) go(k_fused<0, 4>, td, td);
+    else if (R == 16) go(k_fused<0, 16>, td, td);
+    else go(k_fused<0, 8>, td, td);
+  } else if (c->prm.flat_dz < 0) {
+    if (R == 4) go(k_fused<2, 4>, tf, td);
+    else if (R == 16) go(k_fused<2, 16>, tf, td);
+    else go(k_fused<2, 8>, tf, td);
   } else {
-    auto args = [&](auto kern) {
-      hipLaunchKernelGGL(kern, grid, blk, 0, c->stream, a, static_cast<float>(dt),
-                         (const float*)c->h[k], (const float*)c->z, (const double*)c->z64,
-                         (const float*)c->qx[k], (const float*)c->qy[k], (const float*)c->nman,
-                         (const uint64_t*)c->mask, (float*)c->h[o], (float*)c->qx[o],
-                         (float*)c->qy[o], c->d_red);
-    };
-    if (R == 16) args(k_flow_depth32<16>);
-    else if (R == 64) args(k_flow_depth32<64>);
-    else if (R == 8) args(k_flow_depth32<8>);
-    else args(k_flow_depth32<32>);
+    if (R == 4) go(k_fused<1, 4>, tf, tf);
+    else if (R == 16) go(k_fused<1, 16>, tf, tf);
+    else go(k_fused<1, 8>, tf, tf);
   }
   HC_HIP(c, hipGetLastError());
-  c->cur = o;
+  if (swap) c->cur = o;
+  return HC_OK;
+}
+
+// Tile tuner: time the fused kernel for each candidate row count on the
+// context's current state (the kernel reads one generation and writes the
+// other, so repeated launches without swapping leave the state unchanged).
+extern "C" int hc_tune_tiles(hc_ctx* c, double dt, int* best_rows, double* best_us) {
+  if (!c) return HC_ERR_ARG;
+  if (c->g.sediment) return hc_fail(c, HC_ERR_STATE, "tune: hydro-only contexts");
+  const int cand[3] = {4, 8, 16};
+  // the reduction slot carries the last maxdepth into a device-resident loop
+  unsigned long long saved = 0;
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  HC_HIP(c, hipMemcpy(&saved, c->d_red, sizeof(saved), hipMemcpyDeviceToHost));
+  hipEvent_t e0, e1;
+  HC_HIP(c, hipEventCreate(&e0));
+  HC_HIP(c, hipEventCreate(&e1));
+  double best = 1e30;
+  int bestR = c->fused_rows;
+  const int keep = c->fused_rows;
+  for (int R : cand) {
+    c->fused_rows = R;
+    int rc = hc_launch_flow_depth(c, dt, nullptr, nullptr, false);  // warm
+    if (rc) return rc;
+    HC_HIP(c, hipEventRecord(e0, c->stream));
+    const int reps = 10;
+    for (int k = 0; k < reps; ++k) {
+      rc = hc_launch_flow_depth(c, dt, nullptr, nullptr, false);
+      if (rc) return rc;
+    }
+    HC_HIP(c, hipEventRecord(e1, c->stream));
+    HC_HIP(c, hipEventSynchronize(e1));
+    float ms = 0;
+    HC_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    const double us = 1000.0 * ms / reps;
+    if (us < best) {
+      best = us;
+      bestR = R;
+    }
+  }
+  (void)keep;
+  HC_HIP(c, hipMemcpy(c->d_red, &saved, sizeof(saved), hipMemcpyHostToDevice));
+  for (auto& gexec : c->loop_graph) {  // recapture with the chosen tiles
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
+  c->fused_rows = bestR;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (best_rows) *best_rows = bestR;
+  if (best_us) *best_us = best;
+  return HC_OK;
+}
+
+extern "C" int hc_set_tiles(hc_ctx* c, int rows) {
+  if (!c || (rows != 4 && rows != 8 && rows != 16)) return HC_ERR_ARG;
+  if (rows != c->fused_rows) {
+    for (auto& gexec : c->loop_graph) {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+    }
+  }
+  c->fused_rows = rows;
   return HC_OK;
 }
 

@@ -46,6 +46,26 @@ struct ErodeArgs {
   double bedrock_thr, bedrock_rate, lateral_constant, chann_lateral, veg_lat;
 };
 
+// Device-resident loop state (one per context, in device memory).  Mirrors
+// the reference's scalar members that the loop body of src/main.cpp:139-204
+// reads and writes.
+constexpr int kRfMax = 17;  // rain cells 1..16
+struct LoopCtl {
+  double time_step, cycle, maxdepth, temptot, waterinput, waterOut, iod;
+  double previous, old_cycle, new_cycle, time_1, baseflow, dt_flow;
+  double j_mean[kRfMax], old_j_mean[kRfMax], new_j_mean[kRfMax], amt[kRfMax];
+  int input_counter[kRfMax], nActual[kRfMax];
+  int counter, halt, err, npts, pts_dirty, first_new, last_first, iter;
+};
+
+static_assert(kRfMax == HC_RF_MAX, "rain cell capacity");
+using LoopRecord = hc_loop_record;
+
+struct LoopArgs {
+  double courant, DX, in_out_allowed, osi, ERODEFACTOR, MIN_Q, MIN_Q_MAXVAL;
+  int max_time_step, maxcycle, rfnum, imax, jmax, pitch, ntop, scan_interval;
+};
+
 }  // namespace hc
 
 struct hc_ctx {
@@ -65,6 +85,7 @@ struct hc_ctx {
   void* qx[2] = {nullptr, nullptr};
   void* qy[2] = {nullptr, nullptr};
   int cur = 0;
+  int fused_rows = 8;         // output rows per wavefront of the fused kernel (tuned)
   void* z = nullptr;          // elevation (state type)
   double* z64 = nullptr;      // FP32 form: FP64 elevation for near-flat faces
   void* nman = nullptr;       // Manning n (state type), spatial only
@@ -94,6 +115,24 @@ struct hc_ctx {
   unsigned int* d_count = nullptr;      // last-block counters
   double* d_out = nullptr;              // [0] temptot
   double* h_pinned = nullptr;           // pinned host mirror (8 doubles)
+  // device-resident loop
+  hc::LoopCtl* d_ctl = nullptr;
+  hc::LoopRecord* d_rec = nullptr;
+  int rec_cap = 0;
+  double* d_top = nullptr;              // TOPMODEL new_j_mean table [ntop][kRfMax]
+  double* d_area = nullptr;             // drainage area (pitched)
+  int* d_rfarea = nullptr;              // rain cell index (pitched)
+  int* d_lpts = nullptr;                // loop input points (pitched index)
+  int* d_lrf = nullptr;                 // their rain cell
+  hc::LoopArgs largs{};
+  bool loop_ready = false;
+  int loop_cur0 = 0;                    // `cur` when the last run was enqueued
+  int loop_enqueued = 0;                // iterations enqueued by the last run
+  int loop_counter0 = 0;
+  hipGraphExec_t loop_graph[2] = {nullptr, nullptr};  // per starting parity
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev;      // start/stop pairs around fused launches
+  int prof_n = 0;
   int* d_pts = nullptr;                 // point input cells (x*P+y)
   double* d_amt = nullptr;              // point input amounts
   int pts_cap = 0;
@@ -114,7 +153,8 @@ int hc_fail(hc_ctx* c, int code, const std::string& msg);
 // launchers (hc_hydro.hip)
 int hc_launch_flow_route(hc_ctx* c, double dt);
 int hc_launch_depth_update(hc_ctx* c, double dt);
-int hc_launch_flow_depth(hc_ctx* c, double dt);
+int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
+                         const int* haltp = nullptr, bool swap = true);
 int hc_launch_scan(hc_ctx* c);
 int hc_launch_flux_out(hc_ctx* c, double ts);
 int hc_launch_points(hc_ctx* c, int n);

@@ -33,7 +33,13 @@ INT_FIELDS = {"mask", "hasgrain"}
 SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upload",
            "hc_download", "hc_flow_route", "hc_depth_update", "hc_flow_depth",
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
-           "hc_synchronize", "hc_check_health"]
+           "hc_synchronize", "hc_check_health", "hc_loop_begin", "hc_loop_run",
+           "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
+           "hc_set_tiles"]
+MODEL_SYMBOLS = ["hcm_create", "hcm_create_arrays", "hcm_destroy", "hcm_last_error",
+                 "hcm_iterate", "hcm_download", "hcm_field_count", "hcm_scalar",
+                 "hcm_series", "hcm_set_graph", "hcm_context"]
+RF_MAX = 17
 
 
 class hc_grid(ctypes.Structure):
@@ -54,6 +60,43 @@ class hc_params(ctypes.Structure):
         ("lateral_constant", ctypes.c_double), ("chann_lateral_erosion", ctypes.c_double),
         ("veg_lat_restriction", ctypes.c_double), ("flat_dz", ctypes.c_double),
     ]
+
+
+class hc_loop_init(ctypes.Structure):
+    _fields_ = [
+        ("time_step", ctypes.c_double), ("cycle", ctypes.c_double), ("maxdepth", ctypes.c_double),
+        ("waterinput", ctypes.c_double), ("waterOut", ctypes.c_double), ("time_1", ctypes.c_double),
+        ("baseflow", ctypes.c_double), ("counter", ctypes.c_int),
+        ("courant_number", ctypes.c_double), ("in_out_difference_allowed", ctypes.c_double),
+        ("output_file_save_interval", ctypes.c_double), ("ERODEFACTOR", ctypes.c_double),
+        ("MIN_Q", ctypes.c_double), ("MIN_Q_MAXVAL", ctypes.c_double),
+        ("max_time_step", ctypes.c_int), ("maxcycle", ctypes.c_int), ("rfnum", ctypes.c_int),
+        ("j_mean", ctypes.c_double * RF_MAX), ("old_j_mean", ctypes.c_double * RF_MAX),
+        ("new_j_mean", ctypes.c_double * RF_MAX), ("nActual", ctypes.c_int * RF_MAX),
+        ("input_counter", ctypes.c_int * RF_MAX), ("topmodel", ctypes.c_void_p),
+        ("ntop", ctypes.c_int), ("area", ctypes.c_void_p), ("rfarea", ctypes.c_void_p),
+        ("in_x", ctypes.c_void_p), ("in_y", ctypes.c_void_p), ("npoints", ctypes.c_int),
+        ("first_x", ctypes.c_int), ("first_y", ctypes.c_int),
+    ]
+
+
+class hc_loop_record(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("cycle", "previous", "old_cycle", "new_cycle",
+                                                 "temptot", "dt_flow", "time_step", "waterinput")]
+    _fields_ += [("j_mean", ctypes.c_double * RF_MAX)]
+
+
+class hc_loop_state(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("time_step", "cycle", "maxdepth", "temptot",
+                                                 "waterinput", "waterOut", "time_1", "baseflow")]
+    _fields_ += [("counter", ctypes.c_int), ("finished", ctypes.c_int), ("npoints", ctypes.c_int),
+                 ("error", ctypes.c_int), ("j_mean", ctypes.c_double * RF_MAX),
+                 ("old_j_mean", ctypes.c_double * RF_MAX), ("new_j_mean", ctypes.c_double * RF_MAX)]
+
+
+RECORD_DTYPE = np.dtype([(n, np.float64) for n in ("cycle", "previous", "old_cycle", "new_cycle",
+                                                     "temptot", "dt_flow", "time_step", "waterinput")]
+                        + [("j_mean", np.float64, (RF_MAX,))])
 
 
 class HCError(RuntimeError):
@@ -89,6 +132,13 @@ def lib():
         L.hc_erode.argtypes = [vp, d, ctypes.POINTER(d), ctypes.POINTER(d), vp]
         L.hc_synchronize.argtypes = [vp]
         L.hc_check_health.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+        L.hc_loop_begin.argtypes = [vp, ctypes.POINTER(hc_loop_init)]
+        L.hc_loop_run.argtypes = [vp, i, i]
+        L.hc_loop_sync.argtypes = [vp, ctypes.POINTER(hc_loop_state), vp, i, ctypes.POINTER(i)]
+        L.hc_tune_tiles.argtypes = [vp, d, ctypes.POINTER(i), ctypes.POINTER(d)]
+        L.hc_set_tiles.argtypes = [vp, i]
+        L.hc_loop_profile.argtypes = [vp, i]
+        L.hc_loop_kernel_time.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i)]
         _lib = L
     return _lib
 
@@ -135,8 +185,20 @@ def default_params(**kw) -> hc_params:
 class Context:
     """One device context (hc_ctx*)."""
 
+    @classmethod
+    def borrow(cls, handle: int, imax: int, jmax: int):
+        """Wrap a context owned by someone else (e.g. a driver model)."""
+        c = cls.__new__(cls)
+        c.L = lib()
+        c.h = ctypes.c_void_p(handle)
+        c.imax, c.jmax = imax, jmax
+        c.shape = (imax + 2, jmax + 2)
+        c._borrowed = True
+        return c
+
     def __init__(self, imax: int, jmax: int, DX: float, params: hc_params,
                  precision: int = HC_FP64, sediment: bool = False):
+        self._borrowed = False
         self.L = lib()
         g = hc_grid(imax, jmax, DX, precision, 1 if sediment else 0)
         h = ctypes.c_void_p()
@@ -150,6 +212,8 @@ class Context:
         self.sediment = sediment
 
     def close(self):
+        if getattr(self, "_borrowed", False):
+            return
         if getattr(self, "h", None) and self.h.value:
             self.L.hc_destroy(self.h)
             self.h = ctypes.c_void_p()
@@ -223,3 +287,36 @@ class Context:
         if rc not in (HC_OK, HC_ERR_UNSTABLE):
             self._check(rc)
         return n.value
+
+    # ---- device-resident stepping
+    def loop_begin(self, init: hc_loop_init) -> None:
+        self._check(self.L.hc_loop_begin(self.h, ctypes.byref(init)))
+
+    def loop_run(self, n: int, use_graph: bool = True) -> None:
+        self._check(self.L.hc_loop_run(self.h, n, 1 if use_graph else 0))
+
+    def loop_sync(self, max_records: int = 0):
+        st = hc_loop_state()
+        rec = np.zeros(max(1, max_records), dtype=RECORD_DTYPE)
+        n = ctypes.c_int()
+        rc = self.L.hc_loop_sync(self.h, ctypes.byref(st), rec.ctypes.data, max_records, ctypes.byref(n))
+        self._check(rc)
+        return st, rec[:min(n.value, max_records)]
+
+    def loop_profile(self, on: bool) -> None:
+        self._check(self.L.hc_loop_profile(self.h, 1 if on else 0))
+
+    def loop_kernel_time(self):
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        self._check(self.L.hc_loop_kernel_time(self.h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def tune_tiles(self, dt: float):
+        r = ctypes.c_int()
+        us = ctypes.c_double()
+        self._check(self.L.hc_tune_tiles(self.h, dt, ctypes.byref(r), ctypes.byref(us)))
+        return r.value, us.value
+
+    def set_tiles(self, rows: int) -> None:
+        self._check(self.L.hc_set_tiles(self.h, rows))

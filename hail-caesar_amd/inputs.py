@@ -354,3 +354,40 @@ def large_4096(path: str, size: int = 4096, hours: int = 12, seed: int = 1003,
         wi[dem == NODATA] = 0.0
     return make_workload(path, dem, 2.0, rain, 15, hours, mannings=n,
                          water_init=wi, name="grid4096")
+
+
+def params_text_dict(overrides: Dict[str, str] = None, rain_step_min: float = 60,
+                     run_hours: int = 1) -> Dict[str, str]:
+    """Parameter dictionary for in-memory runs (file keys are placeholders)."""
+    p = dict(BASE_PARAMS)
+    p.update({"read_path": "./", "write_path": "./", "read_fname": "dem",
+              "write_fname": "run.dat", "rainfall_data_file": "rain.txt",
+              "rain_data_time_step": "%g" % rain_step_min,
+              "max_run_duration": str(int(run_hours))})
+    if overrides:
+        p.update(overrides)
+    return p
+
+
+def grid4096_arrays(size: int = 4096, seed: int = 1003, wet_depth: float = 0.2,
+                    hours: int = 12):
+    """The large single-grid workload in memory: size^2 cells at 2 m, fractal
+    relief (30 m) on a 0.2 % slope to the south edge with a carved channel,
+    log-normal Manning n (median 0.035, sigma 0.3), 12 h of 15-minute
+    Gamma(0.8, mean 8 mm/h) rain, flow only, and a uniform initial depth
+    (fully wet measurement state).  Input points use the reference's default
+    min_q_for_depth_calc (0.01)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dem = planar_dem(size, size, 2.0, 0.002, 30.0, 0.6, seed, channel_depth=3.0)
+    n = lognormal_mannings(size, size, 0.035, 0.3, rng)
+    rain = gamma_rain(hours, 15, 0.8, 8.0, rng)
+    rain = np.vstack([rain, np.zeros((200, 1))]).astype(np.float32)
+    wi = np.where(dem > NODATA, wet_depth, 0.0)
+    params = params_text_dict({"spatially_variable_mannings_on": "yes",
+                               "spatial_mannings_dem_file": "mannings.asc",
+                               "water_init_from_raster_on": "yes",
+                               "water_init_raster_file": "water_init.asc",
+                               "min_q_for_depth_calc": "0.01"},
+                              rain_step_min=15, run_hours=hours)
+    arrays = {"dem": dem, "mannings": n, "water_init": wi, "rain": rain, "DX": 2.0}
+    return arrays, params

@@ -6,12 +6,39 @@ from typing import Optional
 
 import numpy as np
 
+import contextlib
+import os
+
 from . import hc
+
+
+@contextlib.contextmanager
+def _in_dir(path):
+    """Relative read/write paths in a parameter file are relative to the
+    working directory, as in the reference's CLI; run inside `path`."""
+    if path is None:
+        yield
+        return
+    old = os.getcwd()
+    os.chdir(path)
+    try:
+        yield
+    finally:
+        os.chdir(old)
 
 
 class hcm_options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("fused", ctypes.c_int), ("quiet", ctypes.c_int),
-                ("write_files", ctypes.c_int), ("flat_dz", ctypes.c_double)]
+                ("write_files", ctypes.c_int), ("flat_dz", ctypes.c_double),
+                ("device_loop", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+class hcm_arrays(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int), ("ncols", ctypes.c_int), ("DX", ctypes.c_double),
+                ("xll", ctypes.c_double), ("yll", ctypes.c_double), ("dem", ctypes.c_void_p),
+                ("mannings", ctypes.c_void_p), ("water_init", ctypes.c_void_p),
+                ("hydroindex", ctypes.c_void_p), ("rain", ctypes.c_void_p),
+                ("rain_rows", ctypes.c_int), ("rain_cols", ctypes.c_int)]
 
 
 def _lib():
@@ -20,6 +47,8 @@ def _lib():
         vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         L.hcm_create.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p,
                                  ctypes.POINTER(hcm_options)]
+        L.hcm_create_arrays.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p,
+                                        ctypes.POINTER(hcm_arrays), ctypes.POINTER(hcm_options)]
         L.hcm_destroy.argtypes = [vp]
         L.hcm_last_error.argtypes = [vp]
         L.hcm_last_error.restype = ctypes.c_char_p
@@ -29,6 +58,9 @@ def _lib():
         L.hcm_field_count.restype = ctypes.c_size_t
         L.hcm_scalar.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(d)]
         L.hcm_series.argtypes = [vp, vp, i]
+        L.hcm_set_graph.argtypes = [vp, i]
+        L.hcm_context.argtypes = [vp]
+        L.hcm_context.restype = vp
         L._hcm_ready = True
     return L
 
@@ -36,14 +68,48 @@ def _lib():
 class Model:
     """The reference run loop on the device (src/main.cpp:103-204)."""
 
-    def __init__(self, path: str, params_file: str, precision: int = hc.HC_FP64,
-                 fused: bool = True, write_files: bool = False, flat_dz: float = 0.0):
+    def __init__(self, path: Optional[str], params_file: Optional[str], precision: int = hc.HC_FP64,
+                 fused: bool = True, write_files: bool = False, flat_dz: float = 0.0,
+                 device_loop: bool = False, use_graph: bool = False, arrays: Optional[dict] = None,
+                 params: Optional[dict] = None):
+        """Files (path + params_file, as the reference's CLI) or, with
+        ``arrays`` and ``params``, in-memory inputs: arrays holds 'dem' and
+        optionally 'mannings', 'water_init', 'hydroindex', 'rain' (rows x
+        cells, mm/h) plus 'DX'."""
         self.L = _lib()
-        if not path.endswith("/"):
-            path = path + "/"
-        opt = hcm_options(precision, 1 if fused else 0, 1, 1 if write_files else 0, flat_dz)
+        opt = hcm_options(precision, 1 if fused else 0, 1, 1 if write_files else 0, flat_dz,
+                          1 if device_loop else 0, 1 if use_graph else 0)
         h = ctypes.c_void_p()
-        rc = self.L.hcm_create(ctypes.byref(h), path.encode(), params_file.encode(), ctypes.byref(opt))
+        self.workdir = None
+        if arrays is None:
+            self.workdir = os.path.abspath(path)
+            with _in_dir(self.workdir):
+                rc = self.L.hcm_create(ctypes.byref(h), b"./", params_file.encode(), ctypes.byref(opt))
+        else:
+            keep = {}
+
+            def ptr(name, dtype):
+                if arrays.get(name) is None:
+                    return None
+                a = np.ascontiguousarray(arrays[name], dtype=dtype)
+                keep[name] = a
+                return a.ctypes.data
+            dem = np.ascontiguousarray(arrays["dem"], dtype=np.float64)
+            keep["dem"] = dem
+            rain = arrays.get("rain")
+            rr = rc_ = 0
+            if rain is not None:
+                rain = np.ascontiguousarray(np.atleast_2d(np.asarray(rain, dtype=np.float32).T).T)
+                if rain.ndim == 1:
+                    rain = rain[:, None]
+                keep["rain"] = rain
+                rr, rc_ = rain.shape
+            ar = hcm_arrays(dem.shape[0], dem.shape[1], float(arrays["DX"]), 0.0, 0.0, dem.ctypes.data,
+                            ptr("mannings", np.float64), ptr("water_init", np.float64),
+                            ptr("hydroindex", np.int32), rain.ctypes.data if rain is not None else None,
+                            rr, rc_)
+            text = "".join(f"{k}: {v}\n" for k, v in params.items())
+            rc = self.L.hcm_create_arrays(ctypes.byref(h), text.encode(), ctypes.byref(ar), ctypes.byref(opt))
         self.h = h
         if rc != 0:
             msg = self.L.hcm_last_error(h).decode()
@@ -70,8 +136,9 @@ class Model:
         dt = np.zeros(n)
         done = ctypes.c_int()
         fin = ctypes.c_int()
-        rc = self.L.hcm_iterate(self.h, n, tt.ctypes.data, cy.ctypes.data, dt.ctypes.data,
-                                ctypes.byref(done), ctypes.byref(fin))
+        with _in_dir(self.workdir):
+            rc = self.L.hcm_iterate(self.h, n, tt.ctypes.data, cy.ctypes.data, dt.ctypes.data,
+                                    ctypes.byref(done), ctypes.byref(fin))
         if rc != 0:
             raise hc.HCError(rc, self.L.hcm_last_error(self.h).decode())
         k = done.value
@@ -100,3 +167,12 @@ class Model:
         if rows:
             self.L.hcm_series(self.h, out.ctypes.data, rows)
         return out
+
+    def set_graph(self, on: bool) -> None:
+        self.L.hcm_set_graph(self.h, 1 if on else 0)
+
+    def context(self) -> "hc.Context":
+        return hc.Context.borrow(self.L.hcm_context(self.h), self.imax, self.jmax)
+
+    def synchronize(self) -> None:
+        self.context().synchronize()

@@ -83,8 +83,10 @@ typedef struct hc_params {
   double lateral_constant;
   double chann_lateral_erosion;
   double veg_lat_restriction;
-  /* FP32 form only: faces whose single-precision elevation difference is
-   * smaller than this (metres) take the difference from the FP64 DEM. */
+  /* FP32 form only.  0: single-precision elevations; > 0: faces whose
+   * single-precision elevation difference is smaller than this (metres) take
+   * the difference from the FP64 DEM; < 0: every face takes its elevation
+   * difference from the FP64 DEM (4 more bytes per cell and step). */
   double flat_dz;
 } hc_params;
 
@@ -175,6 +177,78 @@ int hc_synchronize(hc_ctx* ctx);
 /* Health check: counts non-finite or negative water depths on interior
  * cells.  Returns HC_ERR_UNSTABLE when any are found (count in *bad). */
 int hc_check_health(hc_ctx* ctx, long long* bad);
+
+/* ---- device-resident stepping ----------------------------------------- */
+
+/* Rain cells 1..HC_RF_MAX-1 (index 0 unused, as in the reference's vectors). */
+#define HC_RF_MAX 17
+
+/* Everything the loop body of src/main.cpp:139-204 needs besides the raster
+ * state already uploaded: the reference's scalar members at the start of the
+ * run, the rainfall-runoff tables and the input points (hydro-only runs).
+ * `topmodel[t*HC_RF_MAX + n]` is new_j_mean[n] after the TOPMODEL call made
+ * when time_1 reaches t (LCM.cpp:3484-3505; the call sequence does not depend
+ * on the flow state, so the host tabulates it with libm once). */
+typedef struct hc_loop_init {
+  double time_step, cycle, maxdepth, waterinput, waterOut, time_1, baseflow;
+  int counter;
+  double courant_number, in_out_difference_allowed, output_file_save_interval;
+  double ERODEFACTOR, MIN_Q, MIN_Q_MAXVAL;
+  int max_time_step, maxcycle, rfnum;
+  double j_mean[HC_RF_MAX], old_j_mean[HC_RF_MAX], new_j_mean[HC_RF_MAX];
+  int nActual[HC_RF_MAX], input_counter[HC_RF_MAX];
+  const double* topmodel; /* [ntop][HC_RF_MAX] */
+  int ntop;
+  const double* area;     /* drainage area, reference layout */
+  const int* rfarea;      /* rain cell per cell, reference layout */
+  const int* in_x;        /* current input points (LCM.cpp:3741-3767) */
+  const int* in_y;
+  int npoints;            /* totalinputpoints (an empty list is 1 stale point) */
+  int first_x, first_y;   /* in_x[1], in_y[1] kept for that case */
+} hc_loop_init;
+
+/* One record per iteration for output_data's interval series. */
+typedef struct hc_loop_record {
+  double cycle, previous, old_cycle, new_cycle, temptot, dt_flow, time_step, waterinput;
+  double j_mean[HC_RF_MAX];
+} hc_loop_record;
+
+typedef struct hc_loop_state {
+  double time_step, cycle, maxdepth, temptot, waterinput, waterOut, time_1, baseflow;
+  int counter, finished, npoints, error;
+  double j_mean[HC_RF_MAX], old_j_mean[HC_RF_MAX], new_j_mean[HC_RF_MAX];
+} hc_loop_state;
+
+/* Start device-resident stepping (hydro-only contexts): uploads the scalar
+ * state and tables; the flow state is whatever the context holds. */
+int hc_loop_begin(hc_ctx* ctx, const hc_loop_init* init);
+
+/* Enqueue n loop bodies with no host synchronisation: Courant timestep,
+ * clocks, TOPMODEL, input points, fused flow_route+depth_update, scan_area
+ * every 5th iteration and water_flux_out, with every scalar on the device.
+ * Iterations after the end of the run (main.cpp:204) are no-ops.  With
+ * use_graph, blocks of 10 iterations are captured once into a HIP graph and
+ * replayed. */
+int hc_loop_run(hc_ctx* ctx, int n, int use_graph);
+
+/* Wait for the enqueued iterations and read the scalar state and up to
+ * max_records per-iteration records of the last hc_loop_run. */
+int hc_loop_sync(hc_ctx* ctx, hc_loop_state* state, hc_loop_record* records,
+                 int max_records, int* nrecords);
+
+/* Tile tuner: times the fused flow_route+depth_update kernel for each
+ * supported rows-per-wavefront (4, 8, 16) on the context's current state,
+ * which it leaves unchanged, and keeps the fastest for this context.  Call
+ * before hc_loop_run captures a graph.  hc_set_tiles fixes the choice. */
+int hc_tune_tiles(hc_ctx* ctx, double dt_flow, int* best_rows, double* best_us);
+int hc_set_tiles(hc_ctx* ctx, int rows);
+
+/* Kernel timing for the benchmark: with profiling on, hc_loop_run (without
+ * graphs) brackets every fused flow_route+depth_update launch with HIP events
+ * on the context stream; hc_loop_kernel_time returns their summed duration
+ * (ms) and count for the last run. */
+int hc_loop_profile(hc_ctx* ctx, int on);
+int hc_loop_kernel_time(hc_ctx* ctx, double* total_ms, int* launches);
 
 #ifdef __cplusplus
 }

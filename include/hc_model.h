@@ -26,6 +26,9 @@ typedef struct hcm_options {
   int quiet;      /* 1: no progress output on stdout */
   int write_files;/* 1: write the time series and rasters like the reference */
   double flat_dz; /* FP32 form: see hc_params.flat_dz */
+  int device_loop;/* 1: hydro-only runs step with hc_loop_* (scalars on the device;
+                     rasters are not written in this mode) */
+  int use_graph;  /* with device_loop: replay captured HIP graphs */
 } hcm_options;
 
 /* Constructor + initialise_model_domain_extents + initialise_arrays +
@@ -33,6 +36,22 @@ typedef struct hcm_options {
  * initialise_drainage_area (src/main.cpp:103-120).  `path` is prepended to
  * `pfile` exactly as the reference does (pname + pfname). */
 int hcm_create(hcm_model** m, const char* path, const char* pfile, const hcm_options* opt);
+/* Same set-up with the inputs given in memory instead of files: the text
+ * of a parameter file and the rasters (unpadded, row-major nrows x ncols,
+ * row 0 = first row of an ASCII raster).  Optional rasters may be NULL;
+ * the parameter text decides which are used, as the file names would. */
+typedef struct hcm_arrays {
+  int nrows, ncols;
+  double DX, xll, yll;
+  const double* dem;          /* read_fname */
+  const double* mannings;     /* spatial_mannings_dem_file */
+  const double* water_init;   /* water_init_raster_file */
+  const int* hydroindex;      /* hydroindex_file */
+  const float* rain;          /* rainfall_data_file: rain_rows x rain_cols, mm/h */
+  int rain_rows, rain_cols;
+} hcm_arrays;
+int hcm_create_arrays(hcm_model** m, const char* params_text, const hcm_arrays* arrays,
+                      const hcm_options* opt);
 int hcm_destroy(hcm_model* m);
 const char* hcm_last_error(const hcm_model* m);
 
@@ -52,6 +71,12 @@ size_t hcm_field_count(const hcm_model* m, int field);
  * waterinput, waterOut, baseflow, totalinputpoints, imax, jmax, DX,
  * erode_mult, globalsediq. */
 int hcm_scalar(hcm_model* m, const char* name, double* value);
+
+/* Switch graph replay of the device-resident loop on or off between runs. */
+int hcm_set_graph(hcm_model* m, int use_graph);
+
+/* The device context the model drives (for hc.h calls such as profiling). */
+struct hc_ctx* hcm_context(hcm_model* m);
 
 /* Rows written by output_data so far (14 columns: hours, Qw, Jw, sand_out,
  * Qs, Qg[1..9]); copies at most max_rows; returns the row count. */

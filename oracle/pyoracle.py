@@ -305,6 +305,23 @@ class OracleModel:
             st[x1, y1] = row[14:104].reshape(10, 9)
         self.scalar("grain_array_tot", float(tot))
 
+    def load_snapshot(self, snap: dict) -> None:
+        """Inverse of RefModel.snapshot for the oracle's dense layout."""
+        names = {"spat_var_mannings": "mannings"}
+        for k in RefModel.SNAP_2D:
+            if k == "spat_var_mannings" and not self.p.spatially_var_mannings:
+                continue
+            self.field(names.get(k, k))[...] = snap[k]
+        self.field_int("mask")[...] = snap["mask"]
+        self.field("vel_dir")[...] = snap["vel_dir"]
+        for k in ("sr", "sl", "su", "sd"):
+            self.field(k)[...] = snap[k]
+        self.field("grain")[...] = snap["grain"]
+        self.field("strata")[...] = snap["strata"].reshape(self.shape + (90,))
+        self.field_int("hasgrain")[...] = snap["hasgrain"]
+        for k in ("time_step", "maxdepth", "erode_mult", "globalsediq", "counter", "cycle"):
+            self.scalar(k, float(snap[k]))
+
     def iterate(self, n: int):
         tt = np.zeros(n)
         cy = np.zeros(n)
@@ -432,6 +449,35 @@ class RefModel:
             k = np.argmax(row == 0) if np.any(row == 0) else len(row)
             m[row[:k], y] = 1
         return m
+
+    SNAP_2D = ("elev", "water_depth", "qx", "qy", "qxs", "qys", "Vsusptot", "bedrock", "Tau",
+               "Vel", "ss", "spat_var_mannings")
+
+    def snapshot(self) -> dict:
+        """Full erosion-relevant state with the lazily indexed grain rows
+        (index[x][y] -> grain/strata row, LSDCatchmentModel.cpp:3949-3983)
+        expanded to one dense block per cell."""
+        out = {k: self.field(k).copy() for k in self.SNAP_2D}
+        out["mask"] = self.mask()
+        out["vel_dir"] = self.field3("vel_dir").copy()
+        for k in ("sr", "sl", "su", "sd"):
+            out[k] = self.field3(k).copy()
+        idx = self.field_int("index")
+        g = self.field("grain")
+        st = self.field3("strata")
+        has = idx != -9999
+        ni, nj = idx.shape
+        grain = np.zeros((ni, nj, 11))
+        strata = np.zeros((ni, nj, 10, 9))
+        grain[has] = g[idx[has]]
+        strata[has] = st[idx[has]][:, :, :9]
+        out["grain"] = grain
+        out["strata"] = strata
+        out["hasgrain"] = has.astype(np.int32)
+        for k in ("time_step", "maxdepth", "erode_mult", "globalsediq", "counter", "cycle"):
+            out[k] = self.scalar(k)
+        out["sum_grain"] = self.vec("sum_grain").copy()
+        return out
 
     def call(self, name: str, arg: float = 0.0) -> float:
         cwd = os.getcwd()
