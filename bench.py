@@ -45,10 +45,11 @@ def load_package():
 
 
 def bytes_per_cell_step(precision: str, spatial_n: bool = True) -> int:
-    """Algorithmic bytes of one fused flux+depth step per cell: reads h, z,
-    qx, qy (and n), writes h, qx, qy."""
+    """Algorithmic bytes of one fused flux+depth step per cell: reads h, qx,
+    qy, z (and n), writes h, qx, qy.  The FP32 form keeps elevations in FP64
+    (exact elevation differences), so z is 8 bytes in both forms."""
     esz = 4 if precision == "fp32" else 8
-    return esz * ((5 if spatial_n else 4) + 3)
+    return esz * (3 + 3 + (1 if spatial_n else 0)) + 8
 
 
 def run_gpu(precision: str, arrays, params, steps: int, warmup: int):
@@ -57,7 +58,7 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int):
     prec = hc.HC_FP32 if precision == "fp32" else hc.HC_FP64
     t0 = time.time()
     m = Model(None, None, precision=prec, arrays=arrays, params=params, device_loop=True,
-              use_graph=True, flat_dz=0.01 if precision == "fp32" else 0.0)
+              use_graph=True, flat_dz=-1.0 if precision == "fp32" else 0.0)
     setup_s = time.time() - t0
     m.iterate(warmup)
     # the project's tile tuner picks rows per wavefront on the warmed state

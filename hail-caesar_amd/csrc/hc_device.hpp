@@ -36,8 +36,29 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 __device__ __forceinline__ void atomic_max_pos(unsigned long long* dst, double v) {
-  // v >= 0: the bit patterns of non-negative doubles order like the values
-  if (v > 0) atomicMax(dst, static_cast<unsigned long long>(__double_as_longlong(v)));
+  // v >= 0: the bit patterns of non-negative doubles order like the values.
+  // Read first and only contend when this value would raise the maximum:
+  // thousands of blocks hitting one address otherwise serialise.
+  if (v > 0) {
+    const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned long long cur = __atomic_load_n(dst, __ATOMIC_RELAXED);
+    if (bits > cur) atomicMax(dst, bits);
+  }
+}
+
+// Max over a 256-thread block (4 wavefronts) of non-negative values, then one
+// conditional atomic per block.  Every thread of the block must call it.
+__device__ __forceinline__ void block_max_pos(unsigned long long* dst, double v) {
+  __shared__ double s_wave_max[4];
+  v = wave_max(v);
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  if ((t & 63) == 0) s_wave_max[t >> 6] = v;
+  __syncthreads();
+  if (t == 0) {
+    double m = s_wave_max[0];
+    for (int k = 1; k < 4; ++k) m = (m < s_wave_max[k]) ? s_wave_max[k] : m;
+    atomic_max_pos(dst, m);
+  }
 }
 
 // Unique edge cells: row 1, row imax, then columns 1 and jmax of rows
@@ -55,5 +76,102 @@ __device__ __forceinline__ void edge_cell(int k, int imax, int jmax, int& x, int
   y = (k % nside == 0) ? 1 : jmax;
 }
 
+
+// scan_area for one (row, 64-column word): a cell is on the active mask when
+// any cell of its 3x3 neighbourhood holds water (LSDCatchmentModel.cpp:
+// 3821-3849).  Each lane ORs its column over rows x-1..x+1; the row of bits
+// is then dilated sideways by one column, with the columns either side of
+// the word supplied by the edge lanes.  Block: 64 x 4 threads, grid.x = words.
+template <typename T>
+__device__ __forceinline__ void scan_word(int imax, int jmax, int P, int W,
+                                          const T* __restrict__ h,
+                                          uint64_t* __restrict__ mask) {
+  const int lane = threadIdx.x;
+  const int word = blockIdx.x;
+  const int x = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= imax + 2) return;  // uniform per wavefront
+  const int c = word * 64 + lane;
+  const bool row_ok = (x >= 1 && x <= imax);
+  auto col_wet = [&](int cc) -> bool {
+    if (!row_ok || cc < 0 || cc >= P) return false;
+    const size_t b = static_cast<size_t>(x) * P + cc;
+    return (h[b - P] > T(0)) || (h[b] > T(0)) || (h[b + P] > T(0));
+  };
+  const bool v = col_wet(c);
+  bool vside = false;
+  if (lane == 0) vside = col_wet(c - 1);
+  if (lane == 63) vside = col_wet(c + 1);
+  const uint64_t bits = __ballot(v);
+  const uint64_t side = __ballot(vside);
+  uint64_t dil = bits | (bits << 1) | (bits >> 1);
+  if (side & 1ull) dil |= 1ull;             // column word*64-1 is wet
+  if (side & (1ull << 63)) dil |= 1ull << 63;  // column word*64+64 is wet
+  // interior columns 1..jmax only
+  const int lo = word * 64;
+  uint64_t keep = ~0ull;
+  if (lo < 1) keep &= ~1ull;
+  const int hi_excl = jmax + 1 - lo;  // columns >= jmax+1 are outside
+  if (hi_excl <= 0) keep = 0;
+  else if (hi_excl < 64) keep &= (1ull << hi_excl) - 1;
+  if (lane == 0) mask[static_cast<size_t>(x) * W + word] = row_ok ? (dil & keep) : 0ull;
+}
+
+// water_flux_out's edge drain (LSDCatchmentModel.cpp:3005-3047) for the
+// block's share of the unique edge cells; returns the block sum in thread 0.
+template <typename T>
+__device__ __forceinline__ double flux_block(int imax, int jmax, int P, double DX, double thr,
+                                             double ts, T* __restrict__ h) {
+  const int nedge = jmax + (imax > 1 ? jmax : 0) + (imax > 2 ? (imax - 2) * (jmax > 1 ? 2 : 1) : 0);
+  double acc = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nedge; k += gridDim.x * blockDim.x) {
+    int x, y;
+    edge_cell(k, imax, jmax, x, y);
+    T* hp = &h[static_cast<size_t>(x) * P + y];
+    const double hv = static_cast<double>(*hp);
+    if (hv > thr) {
+      acc += (hv - thr) * DX * DX / ts;
+      *hp = static_cast<T>(thr);
+    }
+  }
+  __shared__ double s[256];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  return s[0];
+}
+
+// Deterministic grid sum: every block deposits its partial; the last block to
+// arrive adds them in a fixed tree order.  Returns true (in all threads of
+// that block) with *total set; the counter is re-armed for the next launch.
+__device__ __forceinline__ bool grid_sum_last(double block_sum, double* __restrict__ partials,
+                                              unsigned int* __restrict__ count, double* total) {
+  __shared__ bool s_last;
+  __shared__ double s[256];
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = block_sum;
+    __threadfence();
+    const unsigned prev = atomicAdd(count, 1u);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double v = 0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+    v += __longlong_as_double(
+        __atomic_load_n(reinterpret_cast<const long long*>(&partials[b]), __ATOMIC_RELAXED));
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  *total = s[0];
+  if (threadIdx.x == 0) *count = 0;
+  return true;
+}
 
 }  // namespace hcdev

@@ -315,8 +315,7 @@ __global__ __launch_bounds__(256) void k_entrain(ErodeArgs a, double time_step, 
       }
     }
   }
-  tbmax = wave_max(tbmax);
-  if (((threadIdx.y * 64 + threadIdx.x) & 63) == 0) atomic_max_pos(red, tbmax);
+  block_max_pos(red, tbmax);
 }
 
 // addGS for the neighbours the entrainment pass flagged

@@ -178,8 +178,7 @@ __global__ __launch_bounds__(256) void k_depth64(HydroArgs a, double dt, double*
     }
     h[c] = hv;
   }
-  md = wave_max(md);
-  if ((threadIdx.x & 63) == 0) atomic_max_pos(red, md);
+  block_max_pos(red, md);
 }
 
 // ------------------------------------------------------------ fused kernel
@@ -237,7 +236,7 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   const int lane = threadIdx.x & 63;
   const int w = bx * 4 + (threadIdx.x >> 6);
   const int NR = a.imax + 2, NC = a.jmax + 2;
-  if (62 * w + 1 > NC - 1) return;  // no owned column (uniform per wavefront)
+  const bool active_wave = (62 * w + 1 <= NC - 1);  // has an owned column
   const int c = 62 * w + lane;
   const bool owned = ((lane >= 1 && lane <= 62) || (w == 0 && lane == 0)) && c <= NC - 1;
   const size_t P = a.pitch;
@@ -249,6 +248,8 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   T qxo[R + 2], qyo[R + 2], nn[R + 2];
   uint32_t mbits = 0;
   const T nu = static_cast<T>(a.mannings);
+  T md = 0;
+  if (active_wave) {
 #pragma unroll
   for (int k = 0; k < R + 2; ++k) {
     const int x = r0 - 1 + k;
@@ -269,7 +270,6 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   const float invdt = 1.0f / static_cast<float>(dtd);
   const float fthr = static_cast<float>(a.flat_dz);
   T qxNp = 0, qyNp = 0, qyRp = 0;
-  T md = 0;
 #pragma unroll
   for (int k = 1; k < R + 2; ++k) {
     const int x = r0 - 1 + k;
@@ -329,10 +329,9 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
     qyNp = qyN;
     qyRp = qyR;
   }
+  }  // active_wave
   // the halo lanes' depths are not this wavefront's results
-  double mdd = owned ? static_cast<double>(md) : 0.0;
-  mdd = wave_max(mdd);
-  if (lane == 0) atomic_max_pos(red, mdd);
+  block_max_pos(red, owned && active_wave ? static_cast<double>(md) : 0.0);
 }
 
 // ---------------------------------------------------------------- scan
@@ -340,21 +339,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_scan(int imax, int jmax, int P, int W,
                                               const T* __restrict__ h,
                                               uint64_t* __restrict__ mask) {
-  const int lane = threadIdx.x;
-  const int c = blockIdx.x * 64 + lane;
-  const int x = blockIdx.y * blockDim.y + threadIdx.y;
-  if (x >= imax + 2) return;  // uniform per wavefront
-  bool wet = false;
-  if (x >= 1 && x <= imax && c >= 1 && c <= jmax) {
-    const size_t b = static_cast<size_t>(x) * P + c;
-#pragma unroll
-    for (int dx = -1; dx <= 1; ++dx)
-#pragma unroll
-      for (int dy = -1; dy <= 1; ++dy)
-        wet = wet || (h[b + static_cast<long>(dx) * P + dy] > T(0));
-  }
-  const uint64_t bits = __ballot(wet);
-  if (lane == 0) mask[static_cast<size_t>(x) * W + blockIdx.x] = bits;
+  scan_word<T>(imax, jmax, P, W, h, mask);
 }
 
 // --------------------------------------------------------- water_flux_out
@@ -364,41 +349,9 @@ __global__ __launch_bounds__(256) void k_flux_out(int imax, int jmax, int P, dou
                                                   double* __restrict__ partials,
                                                   unsigned int* __restrict__ count,
                                                   double* __restrict__ out) {
-  const int nedge = jmax + (imax > 1 ? jmax : 0) + (imax > 2 ? (imax - 2) * (jmax > 1 ? 2 : 1) : 0);
-  double acc = 0;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nedge; k += gridDim.x * blockDim.x) {
-    int x, y;
-    edge_cell(k, imax, jmax, x, y);
-    T* hp = &h[static_cast<size_t>(x) * P + y];
-    const double hv = static_cast<double>(*hp);
-    if (hv > thr) {
-      acc += (hv - thr) * DX * DX / ts;
-      *hp = static_cast<T>(thr);
-    }
-  }
-  __shared__ double s[256];
-  __shared__ bool last;
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = 128; o >= 1; o >>= 1) {
-    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = s[0];
-    __threadfence();
-    const unsigned prev = atomicAdd(count, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    double tot = 0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
-      tot += __longlong_as_double(__atomic_load_n(reinterpret_cast<const long long*>(&partials[b]), __ATOMIC_RELAXED));
-    out[0] = tot;
-    *count = 0;
-  }
+  const double bs = flux_block<T>(imax, jmax, P, DX, thr, ts, h);
+  double tot;
+  if (grid_sum_last(bs, partials, count, &tot) && threadIdx.x == 0) out[0] = tot;
 }
 
 // ------------------------------------------------------------- points
@@ -592,6 +545,34 @@ extern "C" int hc_tune_tiles(hc_ctx* c, double dt, int* best_rows, double* best_
   return HC_OK;
 }
 
+// Average duration (us) of `reps` fused launches on the current state,
+// without swapping generations (state unchanged).
+extern "C" int hc_bench_fused(hc_ctx* c, double dt, int reps, double* us) {
+  if (!c || reps < 1) return HC_ERR_ARG;
+  unsigned long long saved = 0;
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  HC_HIP(c, hipMemcpy(&saved, c->d_red, sizeof(saved), hipMemcpyDeviceToHost));
+  hipEvent_t e0, e1;
+  HC_HIP(c, hipEventCreate(&e0));
+  HC_HIP(c, hipEventCreate(&e1));
+  int rc = hc_launch_flow_depth(c, dt, nullptr, nullptr, false);
+  if (rc) return rc;
+  HC_HIP(c, hipEventRecord(e0, c->stream));
+  for (int k = 0; k < reps; ++k) {
+    rc = hc_launch_flow_depth(c, dt, nullptr, nullptr, false);
+    if (rc) return rc;
+  }
+  HC_HIP(c, hipEventRecord(e1, c->stream));
+  HC_HIP(c, hipEventSynchronize(e1));
+  float ms = 0;
+  HC_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HC_HIP(c, hipMemcpy(c->d_red, &saved, sizeof(saved), hipMemcpyHostToDevice));
+  if (us) *us = 1000.0 * ms / reps;
+  return HC_OK;
+}
+
 extern "C" int hc_set_tiles(hc_ctx* c, int rows) {
   if (!c || (rows != 4 && rows != 8 && rows != 16)) return HC_ERR_ARG;
   if (rows != c->fused_rows) {
@@ -621,7 +602,7 @@ int hc_launch_flux_out(hc_ctx* c, double ts) {
   const int imax = c->g.imax, jmax = c->g.jmax;
   const long nedge = jmax + (imax > 1 ? jmax : 0) + (imax > 2 ? (long)(imax - 2) * (jmax > 1 ? 2 : 1) : 0);
   int nb = static_cast<int>((nedge + 255) / 256);
-  if (nb > 64) nb = 64;
+  if (nb > 256) nb = 256;
   if (nb < 1) nb = 1;
   const double thr = c->prm.water_depth_erosion_threshold;
   if (c->prec == HC_FP64)

@@ -195,20 +195,7 @@ __global__ __launch_bounds__(256) void k_loop_scan(const LoopCtl* __restrict__ c
                                                    const T* __restrict__ h,
                                                    uint64_t* __restrict__ mask) {
   if (ctl->halt || ctl->err || (ctl->counter % interval) != 0) return;
-  const int lane = threadIdx.x;
-  const int c = blockIdx.x * 64 + lane;
-  const int x = blockIdx.y * blockDim.y + threadIdx.y;
-  if (x >= imax + 2) return;
-  bool wet = false;
-  if (x >= 1 && x <= imax && c >= 1 && c <= jmax) {
-    const size_t b = static_cast<size_t>(x) * P + c;
-#pragma unroll
-    for (int dx = -1; dx <= 1; ++dx)
-#pragma unroll
-      for (int dy = -1; dy <= 1; ++dy) wet = wet || (h[b + static_cast<long>(dx) * P + dy] > T(0));
-  }
-  const uint64_t bits = __ballot(wet);
-  if (lane == 0) mask[static_cast<size_t>(x) * W + blockIdx.x] = bits;
+  scan_word<T>(imax, jmax, P, W, h, mask);
 }
 
 template <typename T>
@@ -218,43 +205,9 @@ __global__ __launch_bounds__(256) void k_loop_flux(LoopArgs a, LoopCtl* __restri
                                                    unsigned int* __restrict__ count,
                                                    LoopRecord* __restrict__ rec) {
   if (ctl->halt || ctl->err) return;
-  const int imax = a.imax, jmax = a.jmax;
-  const double ts = ctl->time_step;
-  const double DX = a.DX;
-  const int nedge = jmax + (imax > 1 ? jmax : 0) + (imax > 2 ? (imax - 2) * (jmax > 1 ? 2 : 1) : 0);
-  double acc = 0;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nedge; k += gridDim.x * blockDim.x) {
-    int x, y;
-    edge_cell(k, imax, jmax, x, y);
-    T* hp = &h[static_cast<size_t>(x) * a.pitch + y];
-    const double hv = static_cast<double>(*hp);
-    if (hv > thr) {
-      acc += (hv - thr) * DX * DX / ts;
-      *hp = static_cast<T>(thr);
-    }
-  }
-  __shared__ double s[256];
-  __shared__ bool last;
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = 128; o >= 1; o >>= 1) {
-    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = s[0];
-    __threadfence();
-    const unsigned prev = atomicAdd(count, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    double tot = 0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
-      tot += __longlong_as_double(__atomic_load_n(reinterpret_cast<const long long*>(&partials[b]),
-                                                  __ATOMIC_RELAXED));
-    *count = 0;
+  const double bs = flux_block<T>(a.imax, a.jmax, a.pitch, a.DX, thr, ctl->time_step, h);
+  double tot;
+  if (grid_sum_last(bs, partials, count, &tot) && threadIdx.x == 0) {
     LoopCtl& C = *ctl;
     C.temptot = tot;
     C.waterOut = tot;
@@ -271,7 +224,7 @@ int edge_blocks(const hc_ctx* c) {
   const int imax = c->g.imax, jmax = c->g.jmax;
   const long nedge = jmax + (imax > 1 ? jmax : 0) + (imax > 2 ? (long)(imax - 2) * (jmax > 1 ? 2 : 1) : 0);
   int nb = static_cast<int>((nedge + 255) / 256);
-  return std::max(1, std::min(nb, 64));
+  return std::max(1, std::min(nb, 256));
 }
 
 // One iteration's launches on the context stream.

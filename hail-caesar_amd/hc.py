@@ -35,7 +35,7 @@ SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upl
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
            "hc_synchronize", "hc_check_health", "hc_loop_begin", "hc_loop_run",
            "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
-           "hc_set_tiles"]
+           "hc_set_tiles", "hc_bench_fused"]
 MODEL_SYMBOLS = ["hcm_create", "hcm_create_arrays", "hcm_destroy", "hcm_last_error",
                  "hcm_iterate", "hcm_download", "hcm_field_count", "hcm_scalar",
                  "hcm_series", "hcm_set_graph", "hcm_context"]
@@ -137,6 +137,7 @@ def lib():
         L.hc_loop_sync.argtypes = [vp, ctypes.POINTER(hc_loop_state), vp, i, ctypes.POINTER(i)]
         L.hc_tune_tiles.argtypes = [vp, d, ctypes.POINTER(i), ctypes.POINTER(d)]
         L.hc_set_tiles.argtypes = [vp, i]
+        L.hc_bench_fused.argtypes = [vp, d, i, ctypes.POINTER(d)]
         L.hc_loop_profile.argtypes = [vp, i]
         L.hc_loop_kernel_time.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i)]
         _lib = L
@@ -320,3 +321,8 @@ class Context:
 
     def set_tiles(self, rows: int) -> None:
         self._check(self.L.hc_set_tiles(self.h, rows))
+
+    def bench_fused(self, dt: float, reps: int = 20) -> float:
+        us = ctypes.c_double()
+        self._check(self.L.hc_bench_fused(self.h, dt, reps, ctypes.byref(us)))
+        return us.value

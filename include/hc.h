@@ -242,6 +242,9 @@ int hc_loop_sync(hc_ctx* ctx, hc_loop_state* state, hc_loop_record* records,
  * before hc_loop_run captures a graph.  hc_set_tiles fixes the choice. */
 int hc_tune_tiles(hc_ctx* ctx, double dt_flow, int* best_rows, double* best_us);
 int hc_set_tiles(hc_ctx* ctx, int rows);
+/* Average duration in microseconds of `reps` fused launches on the current
+ * state (which is left unchanged); for the tile tuner and benchmarks. */
+int hc_bench_fused(hc_ctx* ctx, double dt_flow, int reps, double* us);
 
 /* Kernel timing for the benchmark: with profiling on, hc_loop_run (without
  * graphs) brackets every fused flow_route+depth_update launch with HIP events

@@ -289,3 +289,52 @@ def test_mass_conservation():
     v1 = o.field("water_depth").sum()
     assert abs(v1 - v0) <= 1e-12 * v0
     assert np.abs(o.field("qx")).max() > 0
+
+
+@pytest.mark.parametrize("name", ["erode_wilcock_32.npz", "erode_einstein_24.npz"])
+def test_oracle_erode_golden(name):
+    """erode() on a wet, grain-bearing state, against the compiled
+    reference's own result for the same state (bitwise)."""
+    import json
+    g = _golden(name)
+    params = json.loads(str(g["params_json"]))
+    n = int(g["imax"])
+    o = OracleModel(params_from_dict(params, n, int(g["jmax"]), float(g["DX"]),
+                                     grain_d=list(g["grain_d"])))
+    o.load_snapshot({k[3:]: g[k] for k in g.files if k.startswith("in_")})
+    mult = float(g["mult"])
+    for c in range(2):
+        assert o.call("erode", mult) == float(g[f"tempbmax{c}"])
+        assert o.scalar("time_step") == float(g[f"time_step{c}"])
+    for f in ("elev", "Vsusptot", "Tau", "Vel", "ss", "bedrock", "grain", "sr", "sl", "su", "sd"):
+        assert np.array_equal(o.field(f), g["out_" + f]), f
+    assert np.array_equal(o.field("strata").reshape(g["out_strata"].shape), g["out_strata"])
+    assert np.array_equal(o.field_int("hasgrain"), g["out_hasgrain"])
+    assert o.scalar("globalsediq") == float(g["out_globalsediq"])
+
+
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+def test_oracle_grain_helpers_vs_reference(tmp_path):
+    """d50, sand_fraction, sort_active, addGS and slide_GS one call at a time
+    on the reference's own grain state (LSDCatchmentModel.cpp:3874-4171)."""
+    rng = np.random.Generator(np.random.PCG64(9))
+    n = 16
+    dem = inputs.planar_dem(n, n, 10.0, 0.02, 6.0, 0.6, 1002, south_only=False)
+    props = inputs.dirichlet_props(n, n, rng)
+    w = inputs.make_workload(str(tmp_path / "g"), dem, 10.0, np.array([10.0, 0.0]), 15, 1,
+                             overrides={"hydro_model_only": "no", "active_layer_thickness": "0.3"},
+                             graindata=props, name="g")
+    r = RefModel(w.path, w.params_file, threads=1)
+    o = OracleModel(params_from_dict(w.params, n, n, 10.0))
+    o.load_snapshot(r.snapshot())
+    idx = r.field_int("index")
+    for (x, y) in [(3, 4), (7, 7), (10, 2)]:
+        assert r.L.ref_grain_call(r.h, 3, int(idx[x, y]), 0, 0.0, 0, 0) == o.call("grain_call", 3, x, y, 0.0, 0, 0)
+        assert r.L.ref_grain_call(r.h, 2, int(idx[x, y]), 0, 0.0, 0, 0) == o.call("grain_call", 2, x, y, 0.0, 0, 0)
+    # slides in both directions, with amounts below and above the source total
+    for (x, y, amt, x2, y2) in [(5, 5, 0.05, 5, 6), (6, 6, 2.0, 7, 6), (2, 9, 0.0, 3, 9)]:
+        r.L.ref_grain_call(r.h, 4, x, y, amt, x2, y2)
+        o.call("grain_call", 4, x, y, amt, x2, y2)
+    s = r.snapshot()
+    assert np.array_equal(o.field("grain"), s["grain"])
+    assert np.array_equal(o.field("strata").reshape(s["strata"].shape), s["strata"])
