@@ -533,10 +533,7 @@ extern "C" int hc_tune_tiles(hc_ctx* c, double dt, int* best_rows, double* best_
   }
   (void)keep;
   HC_HIP(c, hipMemcpy(c->d_red, &saved, sizeof(saved), hipMemcpyHostToDevice));
-  for (auto& gexec : c->loop_graph) {  // recapture with the chosen tiles
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-  }
+  hc_drop_loop_graphs(c);  // recapture with the chosen tiles
   c->fused_rows = bestR;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -575,12 +572,7 @@ extern "C" int hc_bench_fused(hc_ctx* c, double dt, int reps, double* us) {
 
 extern "C" int hc_set_tiles(hc_ctx* c, int rows) {
   if (!c || (rows != 4 && rows != 8 && rows != 16)) return HC_ERR_ARG;
-  if (rows != c->fused_rows) {
-    for (auto& gexec : c->loop_graph) {
-      if (gexec) (void)hipGraphExecDestroy(gexec);
-      gexec = nullptr;
-    }
-  }
+  if (rows != c->fused_rows) hc_drop_loop_graphs(c);
   c->fused_rows = rows;
   return HC_OK;
 }

@@ -139,6 +139,16 @@ struct hc_ctx {
 
   std::string err;
 };
+// Captured loop graphs hold the device addresses of every buffer they touch;
+// any reallocation of such a buffer, or a change of launch shapes, must drop
+// them so the next run captures again.
+inline void hc_drop_loop_graphs(hc_ctx* c) {
+  for (auto& gexec : c->loop_graph) {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
+}
+
 
 // helpers implemented in hc_capi.hip
 int hc_fail(hc_ctx* c, int code, const std::string& msg);

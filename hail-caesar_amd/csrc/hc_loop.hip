@@ -364,10 +364,7 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
   unsigned long long bits;
   std::memcpy(&bits, &in->maxdepth, sizeof(bits));
   HC_HIP(c, hipMemcpy(c->d_red, &bits, sizeof(bits), hipMemcpyHostToDevice));
-  for (auto& gexec : c->loop_graph) {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-  }
+  hc_drop_loop_graphs(c);
   c->loop_ready = true;
   return HC_OK;
 }
@@ -376,11 +373,14 @@ int hc_loop_run(hc_ctx* c, int n, int use_graph) {
   if (!c || n < 0) return HC_ERR_ARG;
   if (!c->loop_ready) return hc_fail(c, HC_ERR_STATE, "loop: hc_loop_begin first");
   if (n > c->rec_cap) {
+    // the captured graphs write records through the old address
     HC_HIP(c, hipStreamSynchronize(c->stream));
+    hc_drop_loop_graphs(c);
     if (c->d_rec) (void)hipFree(c->d_rec);
     c->d_rec = nullptr;
-    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_rec), sizeof(LoopRecord) * (n + 1)));
-    c->rec_cap = n;
+    const int cap = std::max(n, std::max(1024, 2 * c->rec_cap));
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_rec), sizeof(LoopRecord) * (cap + 1)));
+    c->rec_cap = cap;
   }
   c->loop_cur0 = c->cur;
   c->loop_enqueued = n;

@@ -56,15 +56,21 @@ def test_driver_fused_equals_separate(tmp_path):
         assert np.array_equal(a.download(f), b.download(f)), f
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_device_loop_equals_host_loop(tmp_path, use_graph):
+@pytest.mark.parametrize("use_graph,chunks", [(False, (333,)), (True, (333,)),
+                                              (True, (10, 40, 1500, 7, 2600, 20))])
+def test_device_loop_equals_host_loop(tmp_path, use_graph, chunks):
+    """Device-resident stepping replays exactly the host loop; the chunk
+    pattern with growing sizes makes the record buffer grow under captured
+    graphs, which must then be captured again."""
     w = _flow(tmp_path, 64)
     host = Model(w.path, w.params_file)
     th, ch, dh = host.iterate(10 ** 7)
     dev = Model(w.path, w.params_file, device_loop=True, use_graph=use_graph)
     parts = []
+    k = 0
     while True:
-        t, c, d = dev.iterate(333)
+        t, c, d = dev.iterate(chunks[k % len(chunks)])
+        k += 1
         parts.append((t, c, d))
         if dev.finished:
             break
