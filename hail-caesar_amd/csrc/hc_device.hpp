@@ -77,43 +77,72 @@ __device__ __forceinline__ void edge_cell(int k, int imax, int jmax, int& x, int
 }
 
 
-// scan_area for one (row, 64-column word): a cell is on the active mask when
-// any cell of its 3x3 neighbourhood holds water (LSDCatchmentModel.cpp:
-// 3821-3849).  Each lane ORs its column over rows x-1..x+1; the row of bits
-// is then dilated sideways by one column, with the columns either side of
-// the word supplied by the edge lanes.  Block: 64 x 4 threads, grid.x = words.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Rows per scan work item: one wavefront takes one 64-column mask word over
+// a strip of this many rows.
+constexpr int kScanRows = 16;
+
+__host__ __device__ inline int scan_items(int imax, int W) {
+  return W * ((imax + kScanRows - 1) / kScanRows);
+}
+
+// scan_area (LSDCatchmentModel.cpp:3821-3849) for one work item: a cell is
+// on the active mask when any cell of its 3x3 neighbourhood holds water.
+// Each lane reads its column over the strip plus one row either side, ORs
+// vertical triples, and the row of bits is dilated sideways by one column;
+// the edge lanes also read the columns just outside the word.  Columns
+// 1..jmax of rows 1..imax are written; rows 0 and imax+1 are cleared.
 template <typename T>
-__device__ __forceinline__ void scan_word(int imax, int jmax, int P, int W,
+__device__ __forceinline__ void scan_item(int item, int imax, int jmax, int P, int W,
                                           const T* __restrict__ h,
                                           uint64_t* __restrict__ mask) {
-  const int lane = threadIdx.x;
-  const int word = blockIdx.x;
-  const int x = blockIdx.y * blockDim.y + threadIdx.y;
-  if (x >= imax + 2) return;  // uniform per wavefront
-  const int c = word * 64 + lane;
-  const bool row_ok = (x >= 1 && x <= imax);
-  auto col_wet = [&](int cc) -> bool {
-    if (!row_ok || cc < 0 || cc >= P) return false;
-    const size_t b = static_cast<size_t>(x) * P + cc;
-    return (h[b - P] > T(0)) || (h[b] > T(0)) || (h[b + P] > T(0));
-  };
-  const bool v = col_wet(c);
-  bool vside = false;
-  if (lane == 0) vside = col_wet(c - 1);
-  if (lane == 63) vside = col_wet(c + 1);
-  const uint64_t bits = __ballot(v);
-  const uint64_t side = __ballot(vside);
-  uint64_t dil = bits | (bits << 1) | (bits >> 1);
-  if (side & 1ull) dil |= 1ull;             // column word*64-1 is wet
-  if (side & (1ull << 63)) dil |= 1ull << 63;  // column word*64+64 is wet
-  // interior columns 1..jmax only
+  const int lane = threadIdx.x & 63;
+  const int word = item % W;
+  const int strip = item / W;
+  const int x0 = 1 + strip * kScanRows;
+  const int c = word * 64 + lane;  // < P = 64 W
+  const int cs = (lane == 0) ? c - 1 : c + 1;
+  const bool side_lane = (lane == 0 || lane == 63) && cs >= 0 && cs < P;
+  // every lane issues both loads (no divergent branch around a load); the
+  // second is the column outside the word for the edge lanes, a re-read of
+  // its own column for the rest
+  const int c2 = side_lane ? cs : c;
+  bool v[kScanRows + 2], s[kScanRows + 2];
+#pragma unroll
+  for (int k = 0; k < kScanRows + 2; ++k) {
+    const int x = min(x0 - 1 + k, imax + 1);  // rows past the grid repeat the last
+    const size_t b = static_cast<size_t>(x) * P;
+    const T hv = h[b + c];
+    const T hs = h[b + c2];
+    v[k] = hv > T(0);
+    s[k] = side_lane && (hs > T(0));
+  }
   const int lo = word * 64;
   uint64_t keep = ~0ull;
-  if (lo < 1) keep &= ~1ull;
-  const int hi_excl = jmax + 1 - lo;  // columns >= jmax+1 are outside
-  if (hi_excl <= 0) keep = 0;
-  else if (hi_excl < 64) keep &= (1ull << hi_excl) - 1;
-  if (lane == 0) mask[static_cast<size_t>(x) * W + word] = row_ok ? (dil & keep) : 0ull;
+  if (lo == 0) keep &= ~1ull;
+  const int hi = jmax + 1 - lo;  // first column past jmax, relative to the word
+  if (hi <= 0) keep = 0;
+  else if (hi < 64) keep &= (1ull << hi) - 1;
+#pragma unroll
+  for (int k = 1; k <= kScanRows; ++k) {
+    const int x = x0 - 1 + k;
+    if (x > imax) break;  // uniform per wavefront
+    const uint64_t bits = __ballot(v[k - 1] || v[k] || v[k + 1]);
+    const uint64_t side = __ballot(s[k - 1] || s[k] || s[k + 1]);
+    uint64_t dil = bits | (bits << 1) | (bits >> 1);
+    if (side & 1ull) dil |= 1ull;                // column lo-1 is wet
+    if (side & (1ull << 63)) dil |= 1ull << 63;  // column lo+64 is wet
+    if (lane == 0) mask[static_cast<size_t>(x) * W + word] = dil & keep;
+  }
+  if (lane == 0) {
+    if (strip == 0) mask[word] = 0ull;
+    if (x0 + kScanRows > imax) mask[static_cast<size_t>(imax + 1) * W + word] = 0ull;
+  }
 }
 
 // water_flux_out's edge drain (LSDCatchmentModel.cpp:3005-3047) for the

@@ -20,6 +20,8 @@
 #include "hc_device.hpp"
 #include "hc_internal.hpp"
 
+using hc::DrainArgs;
+using hc::fused_edge_tile;
 using hc::HydroArgs;
 using hc::kGravity;
 
@@ -221,7 +223,8 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
                                                typename FusedTypes<MODE>::T* __restrict__ qy1,
                                                unsigned long long* red,
                                                const double* __restrict__ dtp,
-                                               const int* __restrict__ haltp) {
+                                               const int* __restrict__ haltp,
+                                               const DrainArgs drain) {
   using T = typename FusedTypes<MODE>::T;
   using Z = typename FusedTypes<MODE>::Z;
   if (haltp && *haltp) return;  // device-resident loop has finished
@@ -249,6 +252,11 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   uint32_t mbits = 0;
   const T nu = static_cast<T>(a.mannings);
   T md = 0;
+  // water_flux_out (LCM.cpp:3005-3047) on this tile's edge cells, after
+  // depth_update and before the store; maxdepth is taken before the drain
+  const bool drains = drain.ts != nullptr && fused_edge_tile(bx, by, R, a.imax, a.jmax);
+  const double ts = drains ? *drain.ts : 1.0;
+  double dacc = 0;
   if (active_wave) {
 #pragma unroll
   for (int k = 0; k < R + 2; ++k) {
@@ -318,6 +326,14 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
           if (hn > md) md = hn;
         }
       }
+      if (drains && owned && xo >= 1 && xo <= a.imax && c >= 1 && c <= a.jmax &&
+          (xo == 1 || xo == a.imax || c == 1 || c == a.jmax)) {
+        const double hv = static_cast<double>(hn);
+        if (hv > drain.thr) {
+          dacc += (hv - drain.thr) * a.DX * a.DX / ts;
+          hn = static_cast<T>(drain.thr);
+        }
+      }
       if (owned && xo < NR) {
         const size_t o = static_cast<size_t>(xo) * P + c;
         h1[o] = hn;
@@ -332,6 +348,15 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   }  // active_wave
   // the halo lanes' depths are not this wavefront's results
   block_max_pos(red, owned && active_wave ? static_cast<double>(md) : 0.0);
+  if (drains) {  // uniform per block
+    __shared__ double s_dw[4];
+    dacc = wave_sum(dacc);
+    if (lane == 0) s_dw[threadIdx.x >> 6] = dacc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      drain.partials[hc::EdgeSlots(nbx, R, a.imax, a.jmax).slot(bx, by)] =
+          (s_dw[0] + s_dw[1]) + (s_dw[2] + s_dw[3]);
+  }
 }
 
 // ---------------------------------------------------------------- scan
@@ -339,7 +364,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_scan(int imax, int jmax, int P, int W,
                                               const T* __restrict__ h,
                                               uint64_t* __restrict__ mask) {
-  scan_word<T>(imax, jmax, P, W, h, mask);
+  const int items = scan_items(imax, W);
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += gridDim.x * 4)
+    scan_item<T>(it, imax, jmax, P, W, h, mask);
 }
 
 // --------------------------------------------------------- water_flux_out
@@ -451,15 +478,25 @@ int hc_launch_depth_update(hc_ctx* c, double dt) {
   return HC_OK;
 }
 
-int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* haltp, bool swap) {
+void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby) {
+  const int nwaves = (c->NC - 1 + 61) / 62;
+  *nbx = (nwaves + 3) / 4;
+  *nby = (c->NR + R - 1) / R;
+}
+
+int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* haltp, bool swap,
+                         const DrainArgs* drain) {
   if (c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
   const HydroArgs a = hydro_args(c);
   const int R = c->fused_rows;
-  const int nwaves = (c->NC - 1 + 61) / 62;
-  const int nbx = (nwaves + 3) / 4;
-  const int nby = (c->NR + R - 1) / R;
+  int nbx, nby;
+  hc_fused_geometry(c, R, &nbx, &nby);
   const int nblocks = nbx * nby;
+  const DrainArgs dr = drain ? *drain : DrainArgs{};
+  if (dr.ts && static_cast<size_t>(hc::EdgeSlots(nbx, R, c->g.imax, c->g.jmax).count(nby)) >
+                   c->dpart_cap)
+    return hc_fail(c, HC_ERR_STATE, "flow_depth: drain partials buffer too small");
   const int k = c->cur, o = c->cur ^ 1;
   // in the device-resident loop the control kernel clears the reduction slot
   if (!dtp) HC_HIP(c, hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream));
@@ -472,7 +509,7 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks,
                        (const T*)c->h[k], zp, (const double*)c->z64, (const T*)c->qx[k],
                        (const T*)c->qy[k], (const T*)c->nman, (const uint64_t*)c->mask,
-                       (T*)c->h[o], (T*)c->qx[o], (T*)c->qy[o], c->d_red, dtp, haltp);
+                       (T*)c->h[o], (T*)c->qx[o], (T*)c->qy[o], c->d_red, dtp, haltp, dr);
   };
   double* td = nullptr;
   float* tf = nullptr;
@@ -578,13 +615,13 @@ extern "C" int hc_set_tiles(hc_ctx* c, int rows) {
 }
 
 int hc_launch_scan(hc_ctx* c) {
-  const dim3 blk(64, 4);
   const int k = c->cur;
+  const int nb = std::max(1, std::min((scan_items(c->g.imax, c->W) + 3) / 4, 2048));
   if (c->prec == HC_FP64)
-    hipLaunchKernelGGL(k_scan<double>, cell_grid(c), blk, 0, c->stream, c->g.imax, c->g.jmax,
+    hipLaunchKernelGGL(k_scan<double>, dim3(nb), dim3(256), 0, c->stream, c->g.imax, c->g.jmax,
                        c->P, c->W, (const double*)c->h[k], c->mask);
   else
-    hipLaunchKernelGGL(k_scan<float>, cell_grid(c), blk, 0, c->stream, c->g.imax, c->g.jmax,
+    hipLaunchKernelGGL(k_scan<float>, dim3(nb), dim3(256), 0, c->stream, c->g.imax, c->g.jmax,
                        c->P, c->W, (const float*)c->h[k], c->mask);
   HC_HIP(c, hipGetLastError());
   return HC_OK;

@@ -56,6 +56,7 @@ struct LoopCtl {
   double j_mean[kRfMax], old_j_mean[kRfMax], new_j_mean[kRfMax], amt[kRfMax];
   int input_counter[kRfMax], nActual[kRfMax];
   int counter, halt, err, npts, pts_dirty, first_new, last_first, iter;
+  int halt_pending;  // the end-of-run test passed: halt before the next body
 };
 
 static_assert(kRfMax == HC_RF_MAX, "rain cell capacity");
@@ -64,6 +65,46 @@ using LoopRecord = hc_loop_record;
 struct LoopArgs {
   double courant, DX, in_out_allowed, osi, ERODEFACTOR, MIN_Q, MIN_Q_MAXVAL;
   int max_time_step, maxcycle, rfnum, imax, jmax, pitch, ntop, scan_interval;
+};
+
+// water_flux_out folded into the fused kernel of the device-resident loop:
+// the tiles holding edge cells drain them and store one partial sum each;
+// the next kernel on the stream adds the partials in tile order.
+struct DrainArgs {
+  const double* ts = nullptr;       // time_step on the device; null: no drain
+  double thr = 0;                   // water_depth_erosion_threshold
+  double* partials = nullptr;       // one slot per tile of the launch
+};
+
+// Fused-kernel tile (bx, by) holds an edge cell: row 1 or imax, column 1 or
+// jmax.  Tile rows by*R .. by*R+R-1; wavefront w owns columns 62w+1 .. 62w+62
+// (and column 0 for w = 0), four wavefronts per tile column bx.
+__host__ __device__ inline bool fused_edge_tile(int bx, int by, int R, int imax, int jmax) {
+  return by == 0 || by == imax / R || bx == 0 || bx == ((jmax - 1) / 62) / 4;
+}
+
+// Compact slots for the edge tiles' partial sums: the tile rows 0 and imax/R
+// first (nbx slots each), then two slots (one if they coincide) per tile row
+// for tile columns 0 and ((jmax-1)/62)/4.  The latter slots of rows 0 and
+// imax/R are holes that no tile writes and the sum skips.
+struct EdgeSlots {
+  int nbx, r1, c1, nr, nc;
+  __host__ __device__ EdgeSlots(int nbx_, int R, int imax, int jmax)
+      : nbx(nbx_), r1(imax / R), c1(((jmax - 1) / 62) / 4), nr(imax / R == 0 ? 1 : 2),
+        nc(((jmax - 1) / 62) / 4 == 0 ? 1 : 2) {}
+  __host__ __device__ int count(int nby) const { return nr * nbx + nby * nc; }
+  // slot of tile (bx, by), -1 when it holds no edge cell
+  __host__ __device__ int slot(int bx, int by) const {
+    if (by == 0) return bx;
+    if (by == r1) return nbx + bx;
+    const int ci = (bx == 0) ? 0 : (bx == c1 ? 1 : -1);
+    return ci < 0 ? -1 : nr * nbx + by * nc + ci;
+  }
+  __host__ __device__ bool valid(int s) const {
+    if (s < nr * nbx) return true;
+    const int by = (s - nr * nbx) / nc;
+    return by != 0 && by != r1;
+  }
 };
 
 }  // namespace hc
@@ -113,6 +154,8 @@ struct hc_ctx {
   unsigned long long* d_red = nullptr;  // [0] maxdepth bits, [1] tempbmax bits, [2] health count
   double* d_partials = nullptr;         // per-block partial sums
   unsigned int* d_count = nullptr;      // last-block counters
+  double* d_dpart = nullptr;            // fused-kernel edge drain: one slot per tile
+  size_t dpart_cap = 0;
   double* d_out = nullptr;              // [0] temptot
   double* h_pinned = nullptr;           // pinned host mirror (8 doubles)
   // device-resident loop
@@ -164,7 +207,10 @@ int hc_fail(hc_ctx* c, int code, const std::string& msg);
 int hc_launch_flow_route(hc_ctx* c, double dt);
 int hc_launch_depth_update(hc_ctx* c, double dt);
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
-                         const int* haltp = nullptr, bool swap = true);
+                         const int* haltp = nullptr, bool swap = true,
+                         const hc::DrainArgs* drain = nullptr);
+// Geometry of the fused kernel's launch for the context's tile rows.
+void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby);
 int hc_launch_scan(hc_ctx* c);
 int hc_launch_flux_out(hc_ctx* c, double ts);
 int hc_launch_points(hc_ctx* c, int n);

@@ -3,18 +3,26 @@
 // are enqueued (or replayed from a captured HIP graph) without a host
 // round trip per step.
 //
-// Per iteration:
+// Per iteration, three launches:
 //   k_loop_control  (1 block)  set_loop_cycle, set_inputoutput_diff,
 //                              set_maximum_timestep, increment_counters,
 //                              catchment_water_input_and_hydrology (input
 //                              point injection, TOPMODEL catch-up from the
 //                              host-tabulated sequence, calchydrograph,
-//                              baseflow triggers), get_flow_timestep
-//   k_loop_rebuild             input points after a baseflow trigger
-//                              (get_catchment_input_points, LCM.cpp:3741-3767)
-//   fused flow_route + depth_update (dt read from the control block)
-//   k_loop_scan                scan_area when counter % 5 == 0
-//   k_loop_flux                water_flux_out, temptot, end-of-run test
+//                              baseflow triggers), get_flow_timestep, and
+//                              the end-of-run test (main.cpp:204), which
+//                              takes effect before the next body
+//   fused flow_route + depth_update + water_flux_out (dt and time_step read
+//                              from the control block; the edge tiles drain
+//                              and store one partial sum each)
+//   k_loop_scan                temptot from the edge partials (block 0, in
+//                              tile order), scan_area when counter % 5 == 0,
+//                              and the input-point rebuild after a baseflow
+//                              trigger (get_catchment_input_points,
+//                              LCM.cpp:3741-3767)
+// Draining the edges before the scan leaves the mask unchanged: a drained
+// cell keeps the threshold depth, which is still wet.  The rebuilt point list
+// is first used by the next iteration's injection, as in the reference.
 // The scalar arithmetic follows the reference's operation order in FP64.
 #include <algorithm>
 #include <climits>
@@ -51,6 +59,10 @@ __global__ __launch_bounds__(256) void k_loop_control(LoopArgs a, LoopCtl* __res
   if (threadIdx.x == 0) {
     LoopCtl& C = *ctl;
     s_stop = (C.halt || C.err) ? 1 : 0;
+    if (!s_stop && C.halt_pending) {  // the previous body ended the run
+      C.halt = 1;
+      s_stop = 1;
+    }
     if (!s_stop) {
       // a list rebuilt by the previous iteration: an empty one keeps a single
       // stale point, in_x[1]/in_y[1] of the last non-empty list (LCM.cpp:3764)
@@ -152,6 +164,8 @@ __global__ __launch_bounds__(256) void k_loop_control(LoopArgs a, LoopCtl* __res
       r.time_step = C.time_step;
       r.waterinput = C.waterinput;
       for (int n = 0; n < kRfMax; ++n) r.j_mean[n] = C.j_mean[n];
+      // main.cpp:204 on this body's clock (nothing later in the body moves it)
+      if (!(C.cycle / 60 < a.maxcycle)) C.halt_pending = 1;
     }
   }
   __syncthreads();
@@ -164,74 +178,86 @@ __global__ __launch_bounds__(256) void k_loop_control(LoopArgs a, LoopCtl* __res
   }
 }
 
-__global__ __launch_bounds__(256) void k_loop_rebuild(LoopArgs a, LoopCtl* __restrict__ ctl,
-                                                      const double* __restrict__ area,
-                                                      const int* __restrict__ rfarea,
-                                                      int* __restrict__ pts,
-                                                      int* __restrict__ prf) {
-  if (!ctl->pts_dirty || ctl->halt || ctl->err) return;
-  const double b = ctl->baseflow;
-  const long ncell = static_cast<long>(a.imax) * a.jmax;
-  for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < ncell;
-       k += static_cast<long>(gridDim.x) * blockDim.x) {
-    const int x = 1 + static_cast<int>(k / a.jmax);
-    const int y = 1 + static_cast<int>(k % a.jmax);
-    const int idx = x * a.pitch + y;
-    const double v = area[idx] * b * 3 * a.DX * a.DX;
-    if (v > a.MIN_Q && v < a.MIN_Q_MAXVAL) {
-      const int slot = atomicAdd(&ctl->npts, 1);
-      const int rf = rfarea[idx];
-      pts[slot] = idx;
-      prf[slot] = rf;
-      atomicAdd(&ctl->input_counter[rf], 1);
-      atomicMin(&ctl->first_new, idx);
-    }
+// Partial sums of the edge tiles in slot order, then a fixed tree: the
+// water_flux_out total for the iteration.
+__device__ void finish_drain(LoopCtl* __restrict__ ctl, LoopRecord* __restrict__ rec,
+                             const double* __restrict__ partials, int nby, int nbx, int R,
+                             int imax, int jmax) {
+  __shared__ double s_sum[256];
+  const int t = threadIdx.x;
+  const hc::EdgeSlots es(nbx, R, imax, jmax);
+  const int n = es.count(nby);
+  double acc = 0;
+  for (int k = t; k < n; k += 256) {
+    const double v = partials[k];  // holes hold stale values: selected out
+    acc += es.valid(k) ? v : 0.0;
   }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_loop_scan(const LoopCtl* __restrict__ ctl, int imax,
-                                                   int jmax, int P, int W, int interval,
-                                                   const T* __restrict__ h,
-                                                   uint64_t* __restrict__ mask) {
-  if (ctl->halt || ctl->err || (ctl->counter % interval) != 0) return;
-  scan_word<T>(imax, jmax, P, W, h, mask);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_loop_flux(LoopArgs a, LoopCtl* __restrict__ ctl,
-                                                   double thr, T* __restrict__ h,
-                                                   double* __restrict__ partials,
-                                                   unsigned int* __restrict__ count,
-                                                   LoopRecord* __restrict__ rec) {
-  if (ctl->halt || ctl->err) return;
-  const double bs = flux_block<T>(a.imax, a.jmax, a.pitch, a.DX, thr, ctl->time_step, h);
-  double tot;
-  if (grid_sum_last(bs, partials, count, &tot) && threadIdx.x == 0) {
+  s_sum[t] = acc;
+  __syncthreads();
+  for (int o = 128; o >= 1; o >>= 1) {
+    if (t < o) s_sum[t] += s_sum[t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
     LoopCtl& C = *ctl;
+    const double tot = s_sum[0];
     C.temptot = tot;
     C.waterOut = tot;
     rec[C.iter].temptot = tot;
     C.iter++;
-    // main.cpp:204
-    if (!(C.cycle / 60 < a.maxcycle)) C.halt = 1;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restrict__ ctl, int W,
+                                                   const T* __restrict__ h,
+                                                   uint64_t* __restrict__ mask,
+                                                   const double* __restrict__ area,
+                                                   const int* __restrict__ rfarea,
+                                                   int* __restrict__ pts, int* __restrict__ prf,
+                                                   const double* __restrict__ partials,
+                                                   int nby, int nbx, int R,
+                                                   LoopRecord* __restrict__ rec) {
+  // the control kernel alone sets halt, so the fused kernel of this
+  // iteration ran exactly when it is still clear here
+  const bool live = !ctl->halt && !ctl->err;
+  const bool scan = live && (ctl->counter % a.scan_interval) == 0;
+  const bool rebuild = live && ctl->pts_dirty;
+  if (live && blockIdx.x == 0) finish_drain(ctl, rec, partials, nby, nbx, R, a.imax, a.jmax);
+  if (!scan && !rebuild) return;  // uniform over the grid
+  if (scan) {
+    const int items = scan_items(a.imax, W);
+    for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += gridDim.x * 4)
+      scan_item<T>(it, a.imax, a.jmax, a.pitch, W, h, mask);
+  }
+  if (rebuild) {
+    // the slot order of the rebuilt list is arbitrary: every cell appears at
+    // most once and injection adds to distinct cells
+    const double b = ctl->baseflow;
+    const long ncell = static_cast<long>(a.imax) * a.jmax;
+    for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < ncell;
+         k += static_cast<long>(gridDim.x) * blockDim.x) {
+      const int x = 1 + static_cast<int>(k / a.jmax);
+      const int y = 1 + static_cast<int>(k % a.jmax);
+      const int idx = x * a.pitch + y;
+      const double v = area[idx] * b * 3 * a.DX * a.DX;
+      if (v > a.MIN_Q && v < a.MIN_Q_MAXVAL) {
+        const int slot = atomicAdd(&ctl->npts, 1);
+        const int rf = rfarea[idx];
+        pts[slot] = idx;
+        prf[slot] = rf;
+        atomicAdd(&ctl->input_counter[rf], 1);
+        atomicMin(&ctl->first_new, idx);
+      }
+    }
   }
 }
 
 __global__ void k_loop_reset_iter(LoopCtl* ctl) { ctl->iter = 0; }
 
-int edge_blocks(const hc_ctx* c) {
-  const int imax = c->g.imax, jmax = c->g.jmax;
-  const long nedge = jmax + (imax > 1 ? jmax : 0) + (imax > 2 ? (long)(imax - 2) * (jmax > 1 ? 2 : 1) : 0);
-  int nb = static_cast<int>((nedge + 255) / 256);
-  return std::max(1, std::min(nb, 256));
-}
-
 // One iteration's launches on the context stream.
 int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   const LoopArgs& a = c->largs;
-  const dim3 blk(64, 4);
-  const dim3 cells(static_cast<unsigned>(c->P / 64), static_cast<unsigned>((c->NR + 3) / 4));
   const int k = c->cur;
   if (c->prec == HC_FP64)
     hipLaunchKernelGGL(k_loop_control<double>, dim3(1), dim3(256), 0, c->stream, a, c->d_ctl,
@@ -242,32 +268,31 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
                        c->d_top, c->d_lpts, c->d_lrf, c->d_rfarea, (float*)c->h[k], c->d_red,
                        c->d_rec);
   HC_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(k_loop_rebuild, dim3(256), dim3(256), 0, c->stream, a, c->d_ctl, c->d_area,
-                     c->d_rfarea, c->d_lpts, c->d_lrf);
-  HC_HIP(c, hipGetLastError());
   const bool prof = c->prof_on && k_in_run >= 0 && 2 * k_in_run + 1 < static_cast<int>(c->prof_ev.size());
   if (prof) HC_HIP(c, hipEventRecord(c->prof_ev[2 * k_in_run], c->stream));
-  int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt);
+  hc::DrainArgs dr;
+  dr.ts = &c->d_ctl->time_step;
+  dr.thr = c->prm.water_depth_erosion_threshold;
+  dr.partials = c->d_dpart;
+  int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt, true, &dr);
   if (rc) return rc;
   if (prof) {
     HC_HIP(c, hipEventRecord(c->prof_ev[2 * k_in_run + 1], c->stream));
     c->prof_n = k_in_run + 1;
   }
   const int o = c->cur;  // flow_depth flipped the generation
+  const int nb = std::max(1, std::min((hcdev::scan_items(c->g.imax, c->W) + 3) / 4, 1024));
+  const int R = c->fused_rows;
+  int nbx, nby;
+  hc_fused_geometry(c, R, &nbx, &nby);
   if (c->prec == HC_FP64)
-    hipLaunchKernelGGL(k_loop_scan<double>, cells, blk, 0, c->stream, c->d_ctl, c->g.imax,
-                       c->g.jmax, c->P, c->W, a.scan_interval, (const double*)c->h[o], c->mask);
+    hipLaunchKernelGGL(k_loop_scan<double>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
+                       (const double*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_lpts,
+                       c->d_lrf, c->d_dpart, nby, nbx, R, c->d_rec);
   else
-    hipLaunchKernelGGL(k_loop_scan<float>, cells, blk, 0, c->stream, c->d_ctl, c->g.imax,
-                       c->g.jmax, c->P, c->W, a.scan_interval, (const float*)c->h[o], c->mask);
-  HC_HIP(c, hipGetLastError());
-  const double thr = c->prm.water_depth_erosion_threshold;
-  if (c->prec == HC_FP64)
-    hipLaunchKernelGGL(k_loop_flux<double>, dim3(edge_blocks(c)), dim3(256), 0, c->stream, a,
-                       c->d_ctl, thr, (double*)c->h[o], c->d_partials, c->d_count, c->d_rec);
-  else
-    hipLaunchKernelGGL(k_loop_flux<float>, dim3(edge_blocks(c)), dim3(256), 0, c->stream, a,
-                       c->d_ctl, thr, (float*)c->h[o], c->d_partials, c->d_count, c->d_rec);
+    hipLaunchKernelGGL(k_loop_scan<float>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
+                       (const float*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_lpts,
+                       c->d_lrf, c->d_dpart, nby, nbx, R, c->d_rec);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
@@ -314,6 +339,22 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
   a.scan_interval = 5;
   // tables and inputs
   if (!c->d_ctl) HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_ctl), sizeof(LoopCtl)));
+  {
+    // edge-drain partials: the largest slot count over the tile heights
+    size_t need = 1;
+    for (int R : {4, 8, 16}) {
+      int nbx, nby;
+      hc_fused_geometry(c, R, &nbx, &nby);
+      need = std::max(need, static_cast<size_t>(
+                                hc::EdgeSlots(nbx, R, c->g.imax, c->g.jmax).count(nby)));
+    }
+    if (need > c->dpart_cap) {
+      if (c->d_dpart) (void)hipFree(c->d_dpart);
+      c->d_dpart = nullptr;
+      HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_dpart), need * sizeof(double)));
+      c->dpart_cap = need;
+    }
+  }
   if (c->d_top) (void)hipFree(c->d_top);
   HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_top), sizeof(double) * kRfMax * std::max(1, in->ntop)));
   HC_HIP(c, hipMemcpy(c->d_top, in->topmodel, sizeof(double) * kRfMax * in->ntop, hipMemcpyHostToDevice));
@@ -473,7 +514,7 @@ int hc_loop_sync(hc_ctx* c, hc_loop_state* st, hc_loop_record* rec, int max_reco
     st->time_1 = C.time_1;
     st->baseflow = C.baseflow;
     st->counter = C.counter;
-    st->finished = C.halt;
+    st->finished = C.halt || C.halt_pending;
     st->npoints = C.npts;
     st->error = C.err;
     for (int n = 0; n < kRfMax; ++n) {
