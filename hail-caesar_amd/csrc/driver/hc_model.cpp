@@ -396,6 +396,12 @@ class CatchmentModel {
   int iterate_device(int n, double* tt, double* cy, double* dtf, bool* fin) {
     if (!loop_started_) begin_device_loop();
     check(hc_loop_run(ctx_, n, opt_.use_graph), "loop_run");
+    return collect_device(n, tt, cy, dtf, fin);
+  }
+
+  // After n device-resident steps (hc_loop_run here, or an ensemble run that
+  // included this model): read the state and replay output_data per step.
+  int collect_device(int n, double* tt, double* cy, double* dtf, bool* fin) {
     std::vector<hc_loop_record> rec(static_cast<size_t>(std::max(n, 1)));
     hc_loop_state st{};
     int nrec = 0;
@@ -430,6 +436,12 @@ class CatchmentModel {
     }
     *fin = st.finished != 0;
     return k;
+  }
+
+  void ensure_device_loop() {
+    if (!device_loop_enabled())
+      throw std::runtime_error("device loop: model is not hydro-only with device_loop set");
+    if (!loop_started_) begin_device_loop();
   }
 
   void begin_device_loop() {
@@ -1065,6 +1077,33 @@ int hcm_iterate(hcm_model* h, int n, double* tt, double* cy, double* dtf, int* d
   } catch (const std::exception& e) {
     h->err = e.what();
     if (done) *done = k;
+    return HC_ERR_HIP;
+  }
+  if (done) *done = k;
+  if (finished) *finished = fin ? 1 : 0;
+  return HC_OK;
+}
+
+int hcm_loop_begin(hcm_model* h) {
+  if (!h || !h->m) return HC_ERR_ARG;
+  try {
+    h->m->ensure_device_loop();
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return HC_ERR_STATE;
+  }
+  return HC_OK;
+}
+
+int hcm_loop_collect(hcm_model* h, int n, double* tt, double* cy, double* dtf, int* done,
+                     int* finished) {
+  if (!h || !h->m || n < 0) return HC_ERR_ARG;
+  bool fin = false;
+  int k = 0;
+  try {
+    k = h->m->collect_device(n, tt, cy, dtf, &fin);
+  } catch (const std::exception& e) {
+    h->err = e.what();
     return HC_ERR_HIP;
   }
   if (done) *done = k;

@@ -21,6 +21,7 @@
 #include "hc_internal.hpp"
 
 using hc::DrainArgs;
+using hc::EnsMember;
 using hc::fused_edge_tile;
 using hc::HydroArgs;
 using hc::kGravity;
@@ -209,33 +210,42 @@ struct FusedTypes {
   using Z = typename std::conditional<MODE == 1, float, double>::type;
 };
 
-template <int MODE, int R>
-__global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int nbx, int nblocks,
-                                               const typename FusedTypes<MODE>::T* __restrict__ h0,
-                                               const typename FusedTypes<MODE>::Z* __restrict__ zz,
-                                               const double* __restrict__ z64,
-                                               const typename FusedTypes<MODE>::T* __restrict__ qx0,
-                                               const typename FusedTypes<MODE>::T* __restrict__ qy0,
-                                               const typename FusedTypes<MODE>::T* __restrict__ nman,
-                                               const uint64_t* __restrict__ mask,
-                                               typename FusedTypes<MODE>::T* __restrict__ h1,
-                                               typename FusedTypes<MODE>::T* __restrict__ qx1,
-                                               typename FusedTypes<MODE>::T* __restrict__ qy1,
-                                               unsigned long long* red,
-                                               const double* __restrict__ dtp,
-                                               const int* __restrict__ haltp,
-                                               const DrainArgs drain) {
+// Device pointers of one fused launch (one generation in, the other out).
+template <int MODE>
+struct FusedPtrs {
   using T = typename FusedTypes<MODE>::T;
   using Z = typename FusedTypes<MODE>::Z;
-  if (haltp && *haltp) return;  // device-resident loop has finished
-  const double dtd = dtp ? *dtp : dt_arg;
-  // XCD-aware block order
-  const int b = blockIdx.x;
-  const int q8 = nblocks / 8, r8 = nblocks % 8;
-  const int xcd = b % 8;
-  const int bb = xcd * q8 + min(xcd, r8) + b / 8;
-  const int bx = bb % nbx, by = bb / nbx;
+  const T* h0;
+  const Z* zz;
+  const double* z64;
+  const T* qx0;
+  const T* qy0;
+  const T* nman;
+  const uint64_t* mask;
+  T* h1;
+  T* qx1;
+  T* qy1;
+  unsigned long long* red;
+};
 
+// One tile: R output rows x (four wavefronts x 62 owned columns).  Every
+// thread of the block calls it (it ends in block reductions).
+template <int MODE, int R>
+__device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, int nbx, double dtd,
+                                           const FusedPtrs<MODE>& p, const DrainArgs& drain) {
+  using T = typename FusedTypes<MODE>::T;
+  using Z = typename FusedTypes<MODE>::Z;
+  const T* __restrict__ h0 = p.h0;
+  const Z* __restrict__ zz = p.zz;
+  const double* __restrict__ z64 = p.z64;
+  const T* __restrict__ qx0 = p.qx0;
+  const T* __restrict__ qy0 = p.qy0;
+  const T* __restrict__ nman = p.nman;
+  const uint64_t* __restrict__ mask = p.mask;
+  T* __restrict__ h1 = p.h1;
+  T* __restrict__ qx1 = p.qx1;
+  T* __restrict__ qy1 = p.qy1;
+  unsigned long long* red = p.red;
   const int lane = threadIdx.x & 63;
   const int w = bx * 4 + (threadIdx.x >> 6);
   const int NR = a.imax + 2, NC = a.jmax + 2;
@@ -359,6 +369,64 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   }
 }
 
+// One grid: one block per tile, block ids remapped so that each XCD (block
+// b runs on XCD b mod 8) takes a contiguous band of tiles.
+template <int MODE, int R>
+__global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int nbx, int nblocks,
+                                               FusedPtrs<MODE> p, const double* __restrict__ dtp,
+                                               const int* __restrict__ haltp,
+                                               const DrainArgs drain) {
+  if (haltp && *haltp) return;  // device-resident loop has finished
+  const double dtd = dtp ? *dtp : dt_arg;
+  const int b = blockIdx.x;
+  const int q8 = nblocks / 8, r8 = nblocks % 8;
+  const int xcd = b % 8;
+  const int bb = xcd * q8 + min(xcd, r8) + b / 8;
+  fused_tile<MODE, R>(a, bb % nbx, bb / nbx, nbx, dtd, p, drain);
+}
+
+// Ensemble: a persistent grid walks the ensemble's tile list (member-major).
+// Each XCD takes a contiguous eighth of the list and each block a contiguous
+// run of it; a member's own dt, time step and halt flag come from its loop
+// state, and its generation for this step from its parity at run start.
+template <int MODE, int R>
+__global__ __launch_bounds__(256) void k_ens_fused(const EnsMember* __restrict__ mem,
+                                                   const int2* __restrict__ tiles, int ntiles,
+                                                   int step_parity) {
+  using T = typename FusedTypes<MODE>::T;
+  using Z = typename FusedTypes<MODE>::Z;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q8 = nb / 8, r8 = nb % 8;
+  const int xcd = b % 8;
+  const int bb = xcd * q8 + min(xcd, r8) + b / 8;  // rank of this block in list order
+  const long t0 = static_cast<long>(ntiles) * bb / nb;
+  const long t1 = static_cast<long>(ntiles) * (bb + 1) / nb;
+  for (long t = t0; t < t1; ++t) {
+    const int2 tile = tiles[t];
+    const EnsMember& M = mem[tile.x];
+    if (M.ctl->halt) continue;  // uniform per block
+    const int g = M.cur0 ^ step_parity;
+    FusedPtrs<MODE> p;
+    p.h0 = static_cast<const T*>(M.h[g]);
+    p.zz = static_cast<const Z*>(M.z);
+    p.z64 = M.z64;
+    p.qx0 = static_cast<const T*>(M.qx[g]);
+    p.qy0 = static_cast<const T*>(M.qy[g]);
+    p.nman = static_cast<const T*>(M.nman);
+    p.mask = M.mask;
+    p.h1 = static_cast<T*>(M.h[g ^ 1]);
+    p.qx1 = static_cast<T*>(M.qx[g ^ 1]);
+    p.qy1 = static_cast<T*>(M.qy[g ^ 1]);
+    p.red = M.red;
+    DrainArgs d;
+    d.ts = &M.ctl->time_step;
+    d.thr = M.thr;
+    d.partials = M.dpart;
+    __syncthreads();  // the block's shared scratch is reused tile to tile
+    fused_tile<MODE, R>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, M.ctl->dt_flow, p, d);
+  }
+}
+
 // ---------------------------------------------------------------- scan
 template <typename T>
 __global__ __launch_bounds__(256) void k_scan(int imax, int jmax, int P, int W,
@@ -478,6 +546,33 @@ int hc_launch_depth_update(hc_ctx* c, double dt) {
   return HC_OK;
 }
 
+hc::HydroArgs hc_hydro_args(const hc_ctx* c) { return hydro_args(c); }
+
+const void* hc_fused_z(const hc_ctx* c) {
+  return (c->prec == HC_FP32 && c->prm.flat_dz < 0) ? static_cast<const void*>(c->z64) : c->z;
+}
+
+int hc_launch_ens_fused(hipStream_t st, int mode, int R, const EnsMember* mem, const int2* tiles,
+                        int ntiles, int nblocks, int parity) {
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, st, mem, tiles, ntiles, parity);
+  };
+  if (mode == 0) {
+    if (R == 4) go(k_ens_fused<0, 4>);
+    else if (R == 16) go(k_ens_fused<0, 16>);
+    else go(k_ens_fused<0, 8>);
+  } else if (mode == 2) {
+    if (R == 4) go(k_ens_fused<2, 4>);
+    else if (R == 16) go(k_ens_fused<2, 16>);
+    else go(k_ens_fused<2, 8>);
+  } else {
+    if (R == 4) go(k_ens_fused<1, 4>);
+    else if (R == 16) go(k_ens_fused<1, 16>);
+    else go(k_ens_fused<1, 8>);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby) {
   const int nwaves = (c->NC - 1 + 61) / 62;
   *nbx = (nwaves + 3) / 4;
@@ -506,10 +601,21 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     const Z* zp = std::is_same<Z, double>::value && c->prec == HC_FP32
                       ? reinterpret_cast<const Z*>(c->z64)
                       : reinterpret_cast<const Z*>(c->z);
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks,
-                       (const T*)c->h[k], zp, (const double*)c->z64, (const T*)c->qx[k],
-                       (const T*)c->qy[k], (const T*)c->nman, (const uint64_t*)c->mask,
-                       (T*)c->h[o], (T*)c->qx[o], (T*)c->qy[o], c->d_red, dtp, haltp, dr);
+    constexpr int M = std::is_same<T, double>::value ? 0 : (std::is_same<Z, double>::value ? 2 : 1);
+    FusedPtrs<M> p;
+    p.h0 = (const T*)c->h[k];
+    p.zz = zp;
+    p.z64 = (const double*)c->z64;
+    p.qx0 = (const T*)c->qx[k];
+    p.qy0 = (const T*)c->qy[k];
+    p.nman = (const T*)c->nman;
+    p.mask = (const uint64_t*)c->mask;
+    p.h1 = (T*)c->h[o];
+    p.qx1 = (T*)c->qx[o];
+    p.qy1 = (T*)c->qy[o];
+    p.red = c->d_red;
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, p, dtp,
+                       haltp, dr);
   };
   double* td = nullptr;
   float* tf = nullptr;

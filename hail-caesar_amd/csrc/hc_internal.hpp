@@ -76,6 +76,33 @@ struct DrainArgs {
   double* partials = nullptr;       // one slot per tile of the launch
 };
 
+// One member of an ensemble: the device state of an independent catchment
+// context, read by the batched kernels (hc_ens_*).
+struct EnsMember {
+  LoopArgs la;
+  HydroArgs ha;
+  LoopCtl* ctl;
+  LoopRecord* rec;
+  const double* top;
+  int* pts;
+  int* prf;
+  const int* rfarea;
+  const double* area;
+  void* h[2];
+  void* qx[2];
+  void* qy[2];
+  const void* z;        // the elevations the fused kernel reads (FP64, or FP32 copy)
+  const double* z64;    // FP64 elevations (FP32 form)
+  const void* nman;
+  uint64_t* mask;
+  unsigned long long* red;
+  double* dpart;
+  double thr;           // water_depth_erosion_threshold
+  int nbx, nby;         // fused tiles at the ensemble's tile height
+  int cur0;             // generation holding the state when the run starts
+  int W;                // mask words per row
+};
+
 // Fused-kernel tile (bx, by) holds an edge cell: row 1 or imax, column 1 or
 // jmax.  Tile rows by*R .. by*R+R-1; wavefront w owns columns 62w+1 .. 62w+62
 // (and column 0 for w = 0), four wavefronts per tile column bx.
@@ -211,6 +238,12 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
                          const hc::DrainArgs* drain = nullptr);
 // Geometry of the fused kernel's launch for the context's tile rows.
 void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby);
+hc::HydroArgs hc_hydro_args(const hc_ctx* c);
+// Elevation plane the fused kernel reads for this context's mode.
+const void* hc_fused_z(const hc_ctx* c);
+// Ensemble fused step over a tile list (mode: 0 FP64, 1 FP32 z32, 2 FP32 z64).
+int hc_launch_ens_fused(hipStream_t s, int mode, int R, const hc::EnsMember* mem,
+                        const int2* tiles, int ntiles, int nblocks, int parity);
 int hc_launch_scan(hc_ctx* c);
 int hc_launch_flux_out(hc_ctx* c, double ts);
 int hc_launch_points(hc_ctx* c, int n);

@@ -34,6 +34,7 @@
 
 using hc::kGravity;
 using hc::kRfMax;
+using hc::EnsMember;
 using hc::LoopArgs;
 using hc::LoopCtl;
 using hc::LoopRecord;
@@ -45,15 +46,15 @@ __device__ __forceinline__ double cfl(const LoopArgs& a, double maxdepth) {
   return a.courant * (a.DX / sqrt(kGravity * (maxdepth)));  // LCM.cpp:1884-1887
 }
 
+// The scalar part of one loop body for one catchment; one block of 256
+// threads runs it (thread 0 does the scalar chain, the block injects).
 template <typename T>
-__global__ __launch_bounds__(256) void k_loop_control(LoopArgs a, LoopCtl* __restrict__ ctl,
-                                                      const double* __restrict__ top,
-                                                      int* __restrict__ pts,
-                                                      int* __restrict__ prf,
-                                                      const int* __restrict__ rfarea,
-                                                      T* __restrict__ h,
-                                                      unsigned long long* __restrict__ red,
-                                                      LoopRecord* __restrict__ rec) {
+__device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restrict__ ctl,
+                                             const double* __restrict__ top,
+                                             int* __restrict__ pts, int* __restrict__ prf,
+                                             const int* __restrict__ rfarea, T* __restrict__ h,
+                                             unsigned long long* __restrict__ red,
+                                             LoopRecord* __restrict__ rec) {
   __shared__ int s_n, s_stop;
   __shared__ double s_amt[kRfMax];
   if (threadIdx.x == 0) {
@@ -178,6 +179,45 @@ __global__ __launch_bounds__(256) void k_loop_control(LoopArgs a, LoopCtl* __res
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_loop_control(LoopArgs a, LoopCtl* __restrict__ ctl,
+                                                      const double* __restrict__ top,
+                                                      int* __restrict__ pts,
+                                                      int* __restrict__ prf,
+                                                      const int* __restrict__ rfarea,
+                                                      T* __restrict__ h,
+                                                      unsigned long long* __restrict__ red,
+                                                      LoopRecord* __restrict__ rec) {
+  loop_control<T>(a, ctl, top, pts, prf, rfarea, h, red, rec);
+}
+
+// get_catchment_input_points (LCM.cpp:3741-3767) after a baseflow trigger,
+// grid-stride over the catchment's cells.  The slot order of the rebuilt list
+// is arbitrary: every cell appears at most once and injection adds to
+// distinct cells.
+__device__ __forceinline__ void rebuild_points(const LoopArgs& a, LoopCtl* __restrict__ ctl,
+                                               const double* __restrict__ area,
+                                               const int* __restrict__ rfarea,
+                                               int* __restrict__ pts, int* __restrict__ prf) {
+  const double b = ctl->baseflow;
+  const long ncell = static_cast<long>(a.imax) * a.jmax;
+  for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < ncell;
+       k += static_cast<long>(gridDim.x) * blockDim.x) {
+    const int x = 1 + static_cast<int>(k / a.jmax);
+    const int y = 1 + static_cast<int>(k % a.jmax);
+    const int idx = x * a.pitch + y;
+    const double v = area[idx] * b * 3 * a.DX * a.DX;
+    if (v > a.MIN_Q && v < a.MIN_Q_MAXVAL) {
+      const int slot = atomicAdd(&ctl->npts, 1);
+      const int rf = rfarea[idx];
+      pts[slot] = idx;
+      prf[slot] = rf;
+      atomicAdd(&ctl->input_counter[rf], 1);
+      atomicMin(&ctl->first_new, idx);
+    }
+  }
+}
+
 // Partial sums of the edge tiles in slot order, then a fixed tree: the
 // water_flux_out total for the iteration.
 __device__ void finish_drain(LoopCtl* __restrict__ ctl, LoopRecord* __restrict__ rec,
@@ -230,30 +270,57 @@ __global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restri
     for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += gridDim.x * 4)
       scan_item<T>(it, a.imax, a.jmax, a.pitch, W, h, mask);
   }
-  if (rebuild) {
-    // the slot order of the rebuilt list is arbitrary: every cell appears at
-    // most once and injection adds to distinct cells
-    const double b = ctl->baseflow;
-    const long ncell = static_cast<long>(a.imax) * a.jmax;
-    for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < ncell;
-         k += static_cast<long>(gridDim.x) * blockDim.x) {
-      const int x = 1 + static_cast<int>(k / a.jmax);
-      const int y = 1 + static_cast<int>(k % a.jmax);
-      const int idx = x * a.pitch + y;
-      const double v = area[idx] * b * 3 * a.DX * a.DX;
-      if (v > a.MIN_Q && v < a.MIN_Q_MAXVAL) {
-        const int slot = atomicAdd(&ctl->npts, 1);
-        const int rf = rfarea[idx];
-        pts[slot] = idx;
-        prf[slot] = rf;
-        atomicAdd(&ctl->input_counter[rf], 1);
-        atomicMin(&ctl->first_new, idx);
-      }
-    }
-  }
+  if (rebuild) rebuild_points(a, ctl, area, rfarea, pts, prf);
 }
 
 __global__ void k_loop_reset_iter(LoopCtl* ctl) { ctl->iter = 0; }
+
+// ---------------------------------------------------------------- ensemble
+// One block per member: that member's loop_control on its current depth.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ens_control(const EnsMember* __restrict__ mem,
+                                                     int parity) {
+  const EnsMember& M = mem[blockIdx.x];
+  loop_control<T>(M.la, M.ctl, M.top, M.pts, M.prf, M.rfarea,
+                  static_cast<T*>(M.h[M.cur0 ^ parity]), M.red, M.rec);
+}
+
+// Per member: temptot from its edge partials (one block each), scan_area on
+// every 5th of its own steps, and its input-point rebuild.  Scan work items
+// of all members form one list (item_base holds each member's first item).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ens_scan(const EnsMember* __restrict__ mem, int n,
+                                                  const int* __restrict__ item_base, int R,
+                                                  int parity) {
+  for (int m = blockIdx.x; m < n; m += gridDim.x) {
+    const EnsMember& M = mem[m];
+    if (!M.ctl->halt && !M.ctl->err)
+      finish_drain(M.ctl, M.rec, M.dpart, M.nby, M.nbx, R, M.la.imax, M.la.jmax);
+  }
+  const int total = item_base[n];
+  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < total; it += gridDim.x * 4) {
+    int lo = 0, hi = n;  // item_base[lo] <= it < item_base[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (item_base[mid] <= it) lo = mid;
+      else hi = mid;
+    }
+    const EnsMember& M = mem[lo];
+    const LoopCtl* C = M.ctl;
+    if (C->halt || C->err || (C->counter % M.la.scan_interval) != 0) continue;  // per wave
+    scan_item<T>(it - item_base[lo], M.la.imax, M.la.jmax, M.la.pitch, M.W,
+                 static_cast<const T*>(M.h[M.cur0 ^ parity ^ 1]), M.mask);
+  }
+  for (int m = 0; m < n; ++m) {
+    const EnsMember& M = mem[m];
+    if (!M.ctl->halt && !M.ctl->err && M.ctl->pts_dirty)
+      rebuild_points(M.la, M.ctl, M.area, M.rfarea, M.pts, M.prf);
+  }
+}
+
+__global__ void k_ens_reset(const EnsMember* __restrict__ mem, int n) {
+  for (int m = threadIdx.x; m < n; m += blockDim.x) mem[m].ctl->iter = 0;
+}
 
 // One iteration's launches on the context stream.
 int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
@@ -528,6 +595,349 @@ int hc_loop_sync(hc_ctx* c, hc_loop_state* st, hc_loop_record* rec, int max_reco
     HC_HIP(c, hipMemcpy(rec, c->d_rec, sizeof(LoopRecord) * nr, hipMemcpyDeviceToHost));
   if (nrec) *nrec = C.iter;
   if (C.err) return hc_fail(c, HC_ERR_HALTED, "loop: TOPMODEL table exhausted");
+  return HC_OK;
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------------- ensemble API
+struct hc_ens {
+  std::vector<hc_ctx*> m;
+  int mode = 0;           // 0 FP64, 1 FP32 with FP32 elevations, 2 FP32 with FP64 elevations
+  int R = 8;              // fused tile rows
+  int nblocks = 1024;     // persistent grid of the fused step
+  int scan_blocks = 1024;
+  hipStream_t stream = nullptr;
+  std::vector<EnsMember> mem;
+  EnsMember* d_mem = nullptr;
+  int2* d_tiles = nullptr;
+  size_t tiles_cap = 0;
+  int ntiles = 0;
+  int* d_items = nullptr;  // n+1 prefix sums of scan items
+  hipGraphExec_t graph = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int ens_fail(hc_ens* e, int code, const std::string& msg) {
+  e->err = msg;
+  return code;
+}
+
+#define ENS_HIP(e, call)                                                         \
+  do {                                                                           \
+    hipError_t _r = (call);                                                      \
+    if (_r != hipSuccess)                                                        \
+      return ens_fail((e), HC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_r)); \
+  } while (0)
+
+int ens_mode(const hc_ctx* c) {
+  if (c->prec == HC_FP64) return 0;
+  return c->prm.flat_dz < 0 ? 2 : 1;
+}
+
+void ens_drop_graph(hc_ens* e) {
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  e->graph = nullptr;
+}
+
+// Tile list (member-major) and per-member tile geometry for the current R.
+int ens_layout(hc_ens* e) {
+  std::vector<int2> tiles;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    int nbx, nby;
+    hc_fused_geometry(e->m[i], e->R, &nbx, &nby);
+    e->mem[i].nbx = nbx;
+    e->mem[i].nby = nby;
+    for (int t = 0; t < nbx * nby; ++t) tiles.push_back(make_int2(static_cast<int>(i), t));
+  }
+  e->ntiles = static_cast<int>(tiles.size());
+  if (tiles.size() > e->tiles_cap) {
+    if (e->d_tiles) (void)hipFree(e->d_tiles);
+    e->d_tiles = nullptr;
+    ENS_HIP(e, hipMalloc(reinterpret_cast<void**>(&e->d_tiles), tiles.size() * sizeof(int2)));
+    e->tiles_cap = tiles.size();
+  }
+  ENS_HIP(e, hipMemcpy(e->d_tiles, tiles.data(), tiles.size() * sizeof(int2),
+                       hipMemcpyHostToDevice));
+  ens_drop_graph(e);
+  return HC_OK;
+}
+
+// Refresh the member table (generations, record buffers).  A run (run =
+// true) also marks each member as stepped by it, so that hc_loop_sync on the
+// member afterwards returns the run's records and restores its generation.
+int ens_prepare(hc_ens* e, int steps, bool run) {
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    hc_ctx* c = e->m[i];
+    ENS_HIP(e, hipStreamSynchronize(c->stream));
+    if (run && steps > c->rec_cap) {
+      hc_drop_loop_graphs(c);
+      if (c->d_rec) (void)hipFree(c->d_rec);
+      c->d_rec = nullptr;
+      const int cap = std::max(steps, std::max(1024, 2 * c->rec_cap));
+      ENS_HIP(e, hipMalloc(reinterpret_cast<void**>(&c->d_rec), sizeof(LoopRecord) * (cap + 1)));
+      c->rec_cap = cap;
+      ens_drop_graph(e);  // the table changes, not the graph's own arguments; kept simple
+    }
+    if (run) {
+      c->loop_cur0 = c->cur;
+      c->loop_enqueued = steps;
+    }
+    e->mem[i].rec = c->d_rec;
+    e->mem[i].cur0 = c->cur;
+  }
+  ENS_HIP(e, hipMemcpyAsync(e->d_mem, e->mem.data(), e->mem.size() * sizeof(EnsMember),
+                            hipMemcpyHostToDevice, e->stream));
+  return HC_OK;
+}
+
+int ens_step(hc_ens* e, int parity) {
+  const int n = static_cast<int>(e->m.size());
+  if (e->mode == 0)
+    hipLaunchKernelGGL(k_ens_control<double>, dim3(n), dim3(256), 0, e->stream, e->d_mem, parity);
+  else
+    hipLaunchKernelGGL(k_ens_control<float>, dim3(n), dim3(256), 0, e->stream, e->d_mem, parity);
+  ENS_HIP(e, hipGetLastError());
+  if (hc_launch_ens_fused(e->stream, e->mode, e->R, e->d_mem, e->d_tiles, e->ntiles, e->nblocks,
+                          parity))
+    return ens_fail(e, HC_ERR_HIP, "ensemble: fused launch failed");
+  if (e->mode == 0)
+    hipLaunchKernelGGL(k_ens_scan<double>, dim3(e->scan_blocks), dim3(256), 0, e->stream,
+                       e->d_mem, n, e->d_items, e->R, parity);
+  else
+    hipLaunchKernelGGL(k_ens_scan<float>, dim3(e->scan_blocks), dim3(256), 0, e->stream,
+                       e->d_mem, n, e->d_items, e->R, parity);
+  ENS_HIP(e, hipGetLastError());
+  return HC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hc_ens_create(hc_ens** out, hc_ctx* const* members, int n) {
+  if (!out || !members || n < 1) return HC_ERR_ARG;
+  *out = nullptr;
+  hc_ens* e = new (std::nothrow) hc_ens;
+  if (!e) return HC_ERR_ARG;
+  for (int i = 0; i < n; ++i) {
+    hc_ctx* c = members[i];
+    if (!c || !c->loop_ready || c->g.sediment) {
+      delete e;
+      return HC_ERR_STATE;
+    }
+    if (i > 0 && ens_mode(c) != ens_mode(members[0])) {
+      delete e;
+      return HC_ERR_ARG;
+    }
+    for (int j = 0; j < i; ++j)
+      if (members[j] == c) {
+        delete e;
+        return HC_ERR_ARG;
+      }
+    e->m.push_back(c);
+  }
+  e->mode = ens_mode(members[0]);
+  e->mem.resize(n);
+  std::vector<int> items(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    hc_ctx* c = members[i];
+    EnsMember& M = e->mem[i];
+    std::memset(&M, 0, sizeof(M));
+    M.la = c->largs;
+    M.ha = hc_hydro_args(c);
+    M.ctl = c->d_ctl;
+    M.rec = c->d_rec;
+    M.top = c->d_top;
+    M.pts = c->d_lpts;
+    M.prf = c->d_lrf;
+    M.rfarea = c->d_rfarea;
+    M.area = c->d_area;
+    for (int k = 0; k < 2; ++k) {
+      M.h[k] = c->h[k];
+      M.qx[k] = c->qx[k];
+      M.qy[k] = c->qy[k];
+    }
+    M.z = hc_fused_z(c);
+    M.z64 = c->z64;
+    M.nman = c->nman;
+    M.mask = c->mask;
+    M.red = c->d_red;
+    M.dpart = c->d_dpart;
+    M.thr = c->prm.water_depth_erosion_threshold;
+    M.W = c->W;
+    items[i + 1] = items[i] + hcdev::scan_items(c->g.imax, c->W);
+  }
+  int rc = HC_OK;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&e->d_mem), n * sizeof(EnsMember)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&e->d_items), (n + 1) * sizeof(int)) != hipSuccess ||
+      hipMemcpy(e->d_items, items.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice) !=
+          hipSuccess)
+    rc = HC_ERR_HIP;
+  if (rc == HC_OK) rc = ens_layout(e);
+  if (rc != HC_OK) {
+    hc_ens_destroy(e);
+    return rc;
+  }
+  e->scan_blocks = std::max(1, std::min((items[n] + 3) / 4, 1024));
+  *out = e;
+  return HC_OK;
+}
+
+int hc_ens_destroy(hc_ens* e) {
+  if (!e) return HC_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  ens_drop_graph(e);
+  if (e->d_mem) (void)hipFree(e->d_mem);
+  if (e->d_tiles) (void)hipFree(e->d_tiles);
+  if (e->d_items) (void)hipFree(e->d_items);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return HC_OK;
+}
+
+const char* hc_ens_last_error(const hc_ens* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int hc_ens_set_tiles(hc_ens* e, int rows, int blocks) {
+  if (!e || (rows != 4 && rows != 8 && rows != 16) || blocks < 1 || blocks > 65536)
+    return HC_ERR_ARG;
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
+  e->nblocks = blocks;
+  ens_drop_graph(e);
+  if (rows != e->R) {
+    e->R = rows;
+    return ens_layout(e);
+  }
+  return HC_OK;
+}
+
+int hc_ens_run(hc_ens* e, int steps, int use_graph) {
+  if (!e || steps < 0) return HC_ERR_ARG;
+  int rc = ens_prepare(e, steps, true);
+  if (rc) return rc;
+  const int n = static_cast<int>(e->m.size());
+  hipLaunchKernelGGL(k_ens_reset, dim3(1), dim3(256), 0, e->stream, e->d_mem, n);
+  ENS_HIP(e, hipGetLastError());
+  constexpr int kBlock = 10;  // even: every block starts on parity 0
+  int s = 0;
+  if (use_graph && steps >= kBlock) {
+    if (!e->graph) {
+      hipGraph_t g;
+      ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      int crc = HC_OK;
+      for (int k = 0; k < kBlock && crc == HC_OK; ++k) crc = ens_step(e, k & 1);
+      const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+      if (crc) return crc;
+      if (ce != hipSuccess) return ens_fail(e, HC_ERR_HIP, "ensemble: graph capture failed");
+      const hipError_t ie = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return ens_fail(e, HC_ERR_HIP, "ensemble: graph instantiate failed");
+    }
+    for (; s + kBlock <= steps; s += kBlock) ENS_HIP(e, hipGraphLaunch(e->graph, e->stream));
+  }
+  for (; s < steps; ++s) {
+    rc = ens_step(e, s & 1);
+    if (rc) return rc;
+  }
+  return HC_OK;
+}
+
+int hc_ens_sync(hc_ens* e) {
+  if (!e) return HC_ERR_ARG;
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
+  return HC_OK;
+}
+
+int hc_ens_tune(hc_ens* e, int* best_rows, int* best_blocks, double* best_us) {
+  if (!e) return HC_ERR_ARG;
+  int rc = ens_prepare(e, 0, false);
+  if (rc) return rc;
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
+  // the reduction slots carry each member's last maxdepth into its next step
+  std::vector<unsigned long long> saved(e->m.size());
+  for (size_t i = 0; i < e->m.size(); ++i)
+    ENS_HIP(e, hipMemcpy(&saved[i], e->m[i]->d_red, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+  hipEvent_t e0, e1;
+  ENS_HIP(e, hipEventCreate(&e0));
+  ENS_HIP(e, hipEventCreate(&e1));
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  double best = 1e30;
+  int bR = e->R, bB = e->nblocks;
+  for (int R : {4, 8, 16}) {
+    e->R = R;
+    rc = ens_layout(e);
+    if (rc) return rc;
+    ENS_HIP(e, hipMemcpyAsync(e->d_mem, e->mem.data(), e->mem.size() * sizeof(EnsMember),
+                              hipMemcpyHostToDevice, e->stream));
+    for (int per_cu : {2, 4, 8}) {
+      const int nb = std::max(8, std::min(ncu * per_cu, e->ntiles));
+      // parity 0 reads each member's current generation and writes the other:
+      // repeated launches leave the state unchanged
+      if (hc_launch_ens_fused(e->stream, e->mode, R, e->d_mem, e->d_tiles, e->ntiles, nb, 0))
+        return ens_fail(e, HC_ERR_HIP, "ensemble tune: launch failed");
+      ENS_HIP(e, hipEventRecord(e0, e->stream));
+      const int reps = 5;
+      for (int k = 0; k < reps; ++k)
+        (void)hc_launch_ens_fused(e->stream, e->mode, R, e->d_mem, e->d_tiles, e->ntiles, nb, 0);
+      ENS_HIP(e, hipEventRecord(e1, e->stream));
+      ENS_HIP(e, hipEventSynchronize(e1));
+      float ms = 0;
+      ENS_HIP(e, hipEventElapsedTime(&ms, e0, e1));
+      const double us = 1000.0 * ms / reps;
+      if (us < best) {
+        best = us;
+        bR = R;
+        bB = nb;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (size_t i = 0; i < e->m.size(); ++i)
+    ENS_HIP(e, hipMemcpy(e->m[i]->d_red, &saved[i], sizeof(unsigned long long),
+                         hipMemcpyHostToDevice));
+  e->R = bR;
+  e->nblocks = bB;
+  rc = ens_layout(e);
+  if (rc) return rc;
+  if (best_rows) *best_rows = bR;
+  if (best_blocks) *best_blocks = bB;
+  if (best_us) *best_us = best;
+  return HC_OK;
+}
+
+int hc_ens_bench_fused(hc_ens* e, int reps, double* us) {
+  if (!e || reps < 1 || !us) return HC_ERR_ARG;
+  int rc = ens_prepare(e, 0, false);
+  if (rc) return rc;
+  std::vector<unsigned long long> saved(e->m.size());
+  ENS_HIP(e, hipStreamSynchronize(e->stream));
+  for (size_t i = 0; i < e->m.size(); ++i)
+    ENS_HIP(e, hipMemcpy(&saved[i], e->m[i]->d_red, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+  hipEvent_t e0, e1;
+  ENS_HIP(e, hipEventCreate(&e0));
+  ENS_HIP(e, hipEventCreate(&e1));
+  (void)hc_launch_ens_fused(e->stream, e->mode, e->R, e->d_mem, e->d_tiles, e->ntiles, e->nblocks, 0);
+  ENS_HIP(e, hipEventRecord(e0, e->stream));
+  for (int k = 0; k < reps; ++k)
+    (void)hc_launch_ens_fused(e->stream, e->mode, e->R, e->d_mem, e->d_tiles, e->ntiles, e->nblocks, 0);
+  ENS_HIP(e, hipEventRecord(e1, e->stream));
+  ENS_HIP(e, hipEventSynchronize(e1));
+  float ms = 0;
+  ENS_HIP(e, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (size_t i = 0; i < e->m.size(); ++i)
+    ENS_HIP(e, hipMemcpy(e->m[i]->d_red, &saved[i], sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+  *us = 1000.0 * ms / reps;
   return HC_OK;
 }
 

@@ -35,10 +35,13 @@ SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upl
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
            "hc_synchronize", "hc_check_health", "hc_loop_begin", "hc_loop_run",
            "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
-           "hc_set_tiles", "hc_bench_fused"]
+           "hc_set_tiles", "hc_bench_fused", "hc_ens_create", "hc_ens_destroy",
+           "hc_ens_last_error", "hc_ens_run", "hc_ens_sync", "hc_ens_tune", "hc_ens_set_tiles",
+           "hc_ens_bench_fused"]
 MODEL_SYMBOLS = ["hcm_create", "hcm_create_arrays", "hcm_destroy", "hcm_last_error",
                  "hcm_iterate", "hcm_download", "hcm_field_count", "hcm_scalar",
-                 "hcm_series", "hcm_set_graph", "hcm_context"]
+                 "hcm_series", "hcm_set_graph", "hcm_context", "hcm_loop_begin",
+                 "hcm_loop_collect"]
 RF_MAX = 17
 
 
@@ -140,6 +143,15 @@ def lib():
         L.hc_bench_fused.argtypes = [vp, d, i, ctypes.POINTER(d)]
         L.hc_loop_profile.argtypes = [vp, i]
         L.hc_loop_kernel_time.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i)]
+        L.hc_ens_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), i]
+        L.hc_ens_destroy.argtypes = [vp]
+        L.hc_ens_last_error.argtypes = [vp]
+        L.hc_ens_last_error.restype = ctypes.c_char_p
+        L.hc_ens_run.argtypes = [vp, i, i]
+        L.hc_ens_sync.argtypes = [vp]
+        L.hc_ens_tune.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(d)]
+        L.hc_ens_set_tiles.argtypes = [vp, i, i]
+        L.hc_ens_bench_fused.argtypes = [vp, i, ctypes.POINTER(d)]
         _lib = L
     return _lib
 

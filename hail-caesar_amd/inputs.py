@@ -391,3 +391,31 @@ def grid4096_arrays(size: int = 4096, seed: int = 1003, wet_depth: float = 0.2,
                               rain_step_min=15, run_hours=hours)
     arrays = {"dem": dem, "mannings": n, "water_init": wi, "rain": rain, "DX": 2.0}
     return arrays, params
+
+
+def ensemble_members(n: int = 64, lo: int = 128, hi: int = 1024, hours: int = 6,
+                     wet_depth: Optional[float] = None, seed0: int = 4000,
+                     sizes: Optional[List[int]] = None):
+    """The batched-ensemble workload in memory: n independent catchments with
+    side lengths L = round(exp(U(ln lo, ln hi))) cells at 5 m (member m seeded
+    seed0+m), each a fractal DEM (5 m relief) on a 1 % slope to the south edge
+    with a carved channel, a uniform Manning n ~ U(0.02, 0.08), and `hours` of
+    15-minute Gamma(0.8, mean 8 mm/h) rain.  With wet_depth every member starts
+    at that uniform depth (fully wet measurement state); otherwise dry.
+    Returns [(arrays, params)]."""
+    out = []
+    for m in range(n):
+        rng = np.random.Generator(np.random.PCG64(seed0 + m))
+        L = sizes[m] if sizes is not None else int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        dem = planar_dem(L, L, 5.0, 0.01, 5.0, 0.6, seed0 + m, channel_depth=2.0)
+        nman = float(rng.uniform(0.02, 0.08))
+        rain = gamma_rain(hours, 15, 0.8, 8.0, rng)
+        rain = np.vstack([rain, np.zeros((200, 1))]).astype(np.float32)
+        over = {"mannings_n": repr(nman)}
+        arrays = {"dem": dem, "rain": rain, "DX": 5.0}
+        if wet_depth is not None:
+            arrays["water_init"] = np.where(dem > NODATA, wet_depth, 0.0)
+            over.update({"water_init_from_raster_on": "yes",
+                         "water_init_raster_file": "water_init.asc"})
+        out.append((arrays, params_text_dict(over, rain_step_min=15, run_hours=hours)))
+    return out

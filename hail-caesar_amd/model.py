@@ -61,6 +61,8 @@ def _lib():
         L.hcm_set_graph.argtypes = [vp, i]
         L.hcm_context.argtypes = [vp]
         L.hcm_context.restype = vp
+        L.hcm_loop_begin.argtypes = [vp]
+        L.hcm_loop_collect.argtypes = [vp, i, vp, vp, vp, ctypes.POINTER(i), ctypes.POINTER(i)]
         L._hcm_ready = True
     return L
 
@@ -176,3 +178,76 @@ class Model:
 
     def synchronize(self) -> None:
         self.context().synchronize()
+
+
+class Ensemble:
+    """Independent catchments stepped together by the batched kernels
+    (include/hc.h, hc_ens_*).  Every member is a device-loop Model; after each
+    ensemble run every member's clock, records and output series are brought
+    up to date as if it had been stepped alone."""
+
+    def __init__(self, models):
+        self.L = _lib()
+        self.models = list(models)
+        for m in self.models:
+            if self.L.hcm_loop_begin(m.h) != 0:
+                raise hc.HCError(hc.HC_ERR_STATE, self.L.hcm_last_error(m.h).decode())
+        ctxs = (ctypes.c_void_p * len(self.models))(*[self.L.hcm_context(m.h) for m in self.models])
+        e = ctypes.c_void_p()
+        rc = self.L.hc_ens_create(ctypes.byref(e), ctxs, len(self.models))
+        if rc != 0:
+            raise hc.HCError(rc, "hc_ens_create failed")
+        self.e = e
+
+    def close(self):
+        if getattr(self, "e", None) and self.e.value:
+            self.L.hc_ens_destroy(self.e)
+            self.e = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise hc.HCError(rc, self.L.hc_ens_last_error(self.e).decode())
+
+    def run(self, steps: int, use_graph: bool = True):
+        """Enqueue `steps` ensemble steps and wait (no per-member collection)."""
+        self._check(self.L.hc_ens_run(self.e, steps, 1 if use_graph else 0))
+        self._check(self.L.hc_ens_sync(self.e))
+
+    def collect(self, steps: int):
+        """Per member (temptot, cycle, dt_flow) of the last run's steps."""
+        out = []
+        for m in self.models:
+            tt, cy, dt = np.zeros(steps), np.zeros(steps), np.zeros(steps)
+            done, fin = ctypes.c_int(), ctypes.c_int()
+            with _in_dir(m.workdir):
+                rc = self.L.hcm_loop_collect(m.h, steps, tt.ctypes.data, cy.ctypes.data,
+                                             dt.ctypes.data, ctypes.byref(done), ctypes.byref(fin))
+            if rc != 0:
+                raise hc.HCError(rc, self.L.hcm_last_error(m.h).decode())
+            m.finished = bool(fin.value)
+            k = done.value
+            out.append((tt[:k], cy[:k], dt[:k]))
+        return out
+
+    def iterate(self, steps: int, use_graph: bool = True):
+        self.run(steps, use_graph)
+        return self.collect(steps)
+
+    def tune(self):
+        r, b, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(self.L.hc_ens_tune(self.e, ctypes.byref(r), ctypes.byref(b), ctypes.byref(us)))
+        return r.value, b.value, us.value
+
+    def set_tiles(self, rows: int, blocks: int):
+        self._check(self.L.hc_ens_set_tiles(self.e, rows, blocks))
+
+    def bench_fused(self, reps: int = 10) -> float:
+        us = ctypes.c_double()
+        self._check(self.L.hc_ens_bench_fused(self.e, reps, ctypes.byref(us)))
+        return us.value

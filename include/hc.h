@@ -253,6 +253,35 @@ int hc_bench_fused(hc_ctx* ctx, double dt_flow, int reps, double* us);
 int hc_loop_profile(hc_ctx* ctx, int on);
 int hc_loop_kernel_time(hc_ctx* ctx, double* total_ms, int* launches);
 
+/* ---- batched ensemble ----------------------------------------------------
+ * No counterpart in the reference, whose ensembles are independent processes
+ * (parallel_use.md:67-68).  An ensemble steps several independent catchment
+ * contexts together: each step is one control block per member (its own
+ * Courant dt, clocks and rain input, as hc_loop_run), one persistent fused
+ * flow_route+depth_update+water_flux_out kernel over the tile list of all
+ * members, and one scan/drain/rebuild launch.  Each member's results are
+ * those of hc_loop_run on that member alone.  Members must be hydro-only, of
+ * one precision mode, and have had hc_loop_begin; the ensemble borrows their
+ * device state (contexts outlive the ensemble).  A member that reaches the end
+ * of its run stops while the others continue. */
+typedef struct hc_ens hc_ens;
+int hc_ens_create(hc_ens** ens, hc_ctx* const* members, int n);
+int hc_ens_destroy(hc_ens* ens);
+const char* hc_ens_last_error(const hc_ens* ens);
+/* Enqueue `steps` ensemble steps on the ensemble's own stream (with
+ * use_graph, blocks of 10 steps replay one captured HIP graph). */
+int hc_ens_run(hc_ens* ens, int steps, int use_graph);
+/* Wait for the run; then hc_loop_sync on each member returns that member's
+ * state and per-step records, exactly as after hc_loop_run. */
+int hc_ens_sync(hc_ens* ens);
+/* Autotuner: times the fused ensemble kernel for tile rows {4, 8, 16} and
+ * persistent grids of {2, 4, 8} blocks per CU on the current state (left
+ * unchanged) and keeps the fastest.  Call with members synced. */
+int hc_ens_tune(hc_ens* ens, int* best_rows, int* best_blocks, double* best_us);
+int hc_ens_set_tiles(hc_ens* ens, int rows, int blocks);
+/* Average duration (us) of `reps` fused ensemble launches; state unchanged. */
+int hc_ens_bench_fused(hc_ens* ens, int reps, double* us);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,15 @@ struct hc_ctx* hcm_context(hcm_model* m);
  * Qs, Qg[1..9]); copies at most max_rows; returns the row count. */
 int hcm_series(hcm_model* m, double* out, int max_rows);
 
+/* Device-resident stepping driven from outside the model (an ensemble,
+ * hc_ens_*): hcm_loop_begin uploads the loop state (hc_loop_begin) if the
+ * model has not started it; after the outside run of n steps,
+ * hcm_loop_collect reads the state and replays output_data per step, with
+ * hcm_iterate's outputs. */
+int hcm_loop_begin(hcm_model* m);
+int hcm_loop_collect(hcm_model* m, int n, double* temptot, double* cycle, double* dtflow,
+                     int* done, int* finished);
+
 #ifdef __cplusplus
 }
 #endif
