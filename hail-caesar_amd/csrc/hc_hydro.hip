@@ -226,6 +226,7 @@ struct FusedPtrs {
   T* qx1;
   T* qy1;
   unsigned long long* red;
+  hc::InjectArgs inj;  // rain input at the input points (device loop only)
 };
 
 // One tile: R output rows x (four wavefronts x 62 owned columns).  Every
@@ -260,6 +261,17 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   Z zt[R + 2];
   T qxo[R + 2], qyo[R + 2], nn[R + 2];
   uint32_t mbits = 0;
+  // rain input at input points: one amount for the whole grid when every
+  // cell has the same rain cell (added with a select, no branch)
+  const bool inject = p.inj.pmask != nullptr;
+  const double amt_u = (inject && p.inj.rf_uniform >= 0) ? p.inj.amt[p.inj.rf_uniform] : 0.0;
+  // The wavefront's 64 columns touch at most two words of each bit mask
+  // (active cells, input points): both are fetched once per row with
+  // wave-uniform (scalar) loads and each lane picks its bit.
+  const int w0 = (62 * __builtin_amdgcn_readfirstlane(w)) >> 6;
+  const bool two_words = (w0 + 1) < a.mwords;
+  const bool hi_word = (cl >> 6) != w0;
+  const int bit = cl & 63;
   const T nu = static_cast<T>(a.mannings);
   T md = 0;
   // water_flux_out (LCM.cpp:3005-3047) on this tile's edge cells, after
@@ -274,13 +286,32 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     const bool v = (x >= 0 && x < NR);
     const size_t o = static_cast<size_t>(v ? x : 0) * P + cl;
     hh[k] = v ? h0[o] : T(0);
+    if (inject && v) {
+      const size_t ro = static_cast<size_t>(x) * a.mwords + w0;
+      const uint64_t p0 = p.inj.pmask[ro];
+      const uint64_t p1 = two_words ? p.inj.pmask[ro + 1] : 0ull;
+      if ((p0 | p1) != 0ull) {  // uniform: most rows of most tiles hold no point
+        const bool pt = ((hi_word ? p1 : p0) >> bit) & 1ull;
+        if (p.inj.rf_uniform >= 0) {
+          const T hp = static_cast<T>(static_cast<double>(hh[k]) + amt_u);
+          hh[k] = pt ? hp : hh[k];
+        } else if (pt) {
+          hh[k] = static_cast<T>(static_cast<double>(hh[k]) + p.inj.amt[p.inj.rfarea[o]]);
+        }
+      }
+    }
     zt[k] = v ? zz[o] : Z(-9999);
     if (k >= 1) {
       qxo[k] = v ? qx0[o] : T(0);
       qyo[k] = v ? qy0[o] : T(0);
       nn[k] = (a.spatial_n && v) ? nman[o] : nu;
-      const uint64_t word = v ? mask[static_cast<size_t>(x) * a.mwords + (cl >> 6)] : 0ull;
-      mbits |= static_cast<uint32_t>((word >> (cl & 63)) & 1ull) << k;
+      uint64_t m0 = 0ull, m1 = 0ull;
+      if (v) {
+        const size_t ro = static_cast<size_t>(x) * a.mwords + w0;
+        m0 = mask[ro];
+        m1 = two_words ? mask[ro + 1] : 0ull;
+      }
+      mbits |= static_cast<uint32_t>(((hi_word ? m1 : m0) >> bit) & 1ull) << k;
     }
   }
 
@@ -418,6 +449,10 @@ __global__ __launch_bounds__(256) void k_ens_fused(const EnsMember* __restrict__
     p.qx1 = static_cast<T*>(M.qx[g ^ 1]);
     p.qy1 = static_cast<T*>(M.qy[g ^ 1]);
     p.red = M.red;
+    p.inj.pmask = M.pmask;
+    p.inj.rf_uniform = M.rf_uniform;
+    p.inj.rfarea = M.rfarea;
+    p.inj.amt = M.ctl->amt;
     DrainArgs d;
     d.ts = &M.ctl->time_step;
     d.thr = M.thr;
@@ -580,7 +615,7 @@ void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby) {
 }
 
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* haltp, bool swap,
-                         const DrainArgs* drain) {
+                         const DrainArgs* drain, const hc::InjectArgs* inject) {
   if (c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
   const HydroArgs a = hydro_args(c);
@@ -614,6 +649,7 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     p.qx1 = (T*)c->qx[o];
     p.qy1 = (T*)c->qy[o];
     p.red = c->d_red;
+    p.inj = inject ? *inject : hc::InjectArgs{};
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, p, dtp,
                        haltp, dr);
   };

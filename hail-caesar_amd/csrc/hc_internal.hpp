@@ -65,6 +65,18 @@ using LoopRecord = hc_loop_record;
 struct LoopArgs {
   double courant, DX, in_out_allowed, osi, ERODEFACTOR, MIN_Q, MIN_Q_MAXVAL;
   int max_time_step, maxcycle, rfnum, imax, jmax, pitch, ntop, scan_interval;
+  int mwords;  // words per row of the active and input-point bit masks
+};
+
+// Rain input folded into the fused kernel of the device-resident loop: the
+// input points are a bit mask over the padded grid, and a point cell's depth
+// gets amt[rfarea] as the kernel loads it (LCM.cpp:3463-3477 adds it in place
+// before flow_route; the points are distinct cells, so the order is free).
+struct InjectArgs {
+  const uint64_t* pmask = nullptr;  // null: no injection in this launch
+  const int* rfarea = nullptr;      // rain cell per cell (pitched)
+  const double* amt = nullptr;      // per rain cell amount of this step
+  int rf_uniform = -1;              // >= 0: every cell has this rain cell
 };
 
 // water_flux_out folded into the fused kernel of the device-resident loop:
@@ -84,8 +96,8 @@ struct EnsMember {
   LoopCtl* ctl;
   LoopRecord* rec;
   const double* top;
-  int* pts;
-  int* prf;
+  uint64_t* pmask;       // input points
+  int rf_uniform;        // >= 0: the one rain cell of every cell
   const int* rfarea;
   const double* area;
   void* h[2];
@@ -192,8 +204,8 @@ struct hc_ctx {
   double* d_top = nullptr;              // TOPMODEL new_j_mean table [ntop][kRfMax]
   double* d_area = nullptr;             // drainage area (pitched)
   int* d_rfarea = nullptr;              // rain cell index (pitched)
-  int* d_lpts = nullptr;                // loop input points (pitched index)
-  int* d_lrf = nullptr;                 // their rain cell
+  uint64_t* d_pmask = nullptr;          // loop input points: one bit per padded cell
+  int rf_uniform = -1;                  // >= 0: every cell has this rain cell index
   hc::LoopArgs largs{};
   bool loop_ready = false;
   int loop_cur0 = 0;                    // `cur` when the last run was enqueued
@@ -235,7 +247,8 @@ int hc_launch_flow_route(hc_ctx* c, double dt);
 int hc_launch_depth_update(hc_ctx* c, double dt);
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
                          const int* haltp = nullptr, bool swap = true,
-                         const hc::DrainArgs* drain = nullptr);
+                         const hc::DrainArgs* drain = nullptr,
+                         const hc::InjectArgs* inject = nullptr);
 // Geometry of the fused kernel's launch for the context's tile rows.
 void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby);
 hc::HydroArgs hc_hydro_args(const hc_ctx* c);

@@ -46,31 +46,28 @@ __device__ __forceinline__ double cfl(const LoopArgs& a, double maxdepth) {
   return a.courant * (a.DX / sqrt(kGravity * (maxdepth)));  // LCM.cpp:1884-1887
 }
 
-// The scalar part of one loop body for one catchment; one block of 256
-// threads runs it (thread 0 does the scalar chain, the block injects).
-template <typename T>
+// The scalar part of one loop body for one catchment (thread 0 of the block
+// calling it).  The rain input itself is added by the fused kernel as it
+// loads the input-point cells (amounts in ctl->amt).
 __device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restrict__ ctl,
                                              const double* __restrict__ top,
-                                             int* __restrict__ pts, int* __restrict__ prf,
-                                             const int* __restrict__ rfarea, T* __restrict__ h,
+                                             uint64_t* __restrict__ pmask,
                                              unsigned long long* __restrict__ red,
                                              LoopRecord* __restrict__ rec) {
-  __shared__ int s_n, s_stop;
-  __shared__ double s_amt[kRfMax];
   if (threadIdx.x == 0) {
     LoopCtl& C = *ctl;
-    s_stop = (C.halt || C.err) ? 1 : 0;
-    if (!s_stop && C.halt_pending) {  // the previous body ended the run
+    bool stop = C.halt || C.err;
+    if (!stop && C.halt_pending) {  // the previous body ended the run
       C.halt = 1;
-      s_stop = 1;
+      stop = true;
     }
-    if (!s_stop) {
+    if (!stop) {
       // a list rebuilt by the previous iteration: an empty one keeps a single
       // stale point, in_x[1]/in_y[1] of the last non-empty list (LCM.cpp:3764)
       if (C.pts_dirty) {
         if (C.npts == 0) {
-          pts[0] = C.last_first;
-          prf[0] = rfarea[C.last_first];
+          const int px = C.last_first / a.pitch, py = C.last_first % a.pitch;
+          pmask[static_cast<size_t>(px) * a.mwords + (py >> 6)] |= 1ull << (py & 63);
           C.npts = 1;
         } else {
           C.last_first = C.first_new;
@@ -99,9 +96,8 @@ __device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restr
       for (int rf = 0; rf <= a.rfnum; ++rf) {
         double amt = (C.j_mean[rf] * C.nActual[rf]) / (C.input_counter[rf]) * fts;
         if (amt > a.ERODEFACTOR) amt = a.ERODEFACTOR;
-        s_amt[rf] = amt;
+        C.amt[rf] = amt;
       }
-      s_n = C.npts;
       if (C.cycle >= C.time_1) {
         do {
           C.time_1++;
@@ -127,9 +123,7 @@ __device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restr
           }
         } while (C.time_1 < C.cycle);
       }
-      if (C.err) {
-        s_stop = 1;
-      }
+
       const double tt = C.time_1 - C.cycle;  // calchydrograph (LCM.cpp:3717-3724)
       for (int n = 1; n <= a.rfnum; ++n)
         C.j_mean[n] = C.old_j_mean[n] + (((C.new_j_mean[n] - C.old_j_mean[n]) / 2) * (2 - tt));
@@ -169,52 +163,66 @@ __device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restr
       if (!(C.cycle / 60 < a.maxcycle)) C.halt_pending = 1;
     }
   }
-  __syncthreads();
-  if (s_stop) return;
-  // water_depth[i][j] += water_add_amt at every input point (LCM.cpp:3463-3477),
-  // with the list as it was before any rebuild of this iteration
-  for (int k = threadIdx.x; k < s_n; k += blockDim.x) {
-    T* p = &h[pts[k]];
-    *p = static_cast<T>(static_cast<double>(*p) + s_amt[prf[k]]);
-  }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_loop_control(LoopArgs a, LoopCtl* __restrict__ ctl,
-                                                      const double* __restrict__ top,
-                                                      int* __restrict__ pts,
-                                                      int* __restrict__ prf,
-                                                      const int* __restrict__ rfarea,
-                                                      T* __restrict__ h,
-                                                      unsigned long long* __restrict__ red,
-                                                      LoopRecord* __restrict__ rec) {
-  loop_control<T>(a, ctl, top, pts, prf, rfarea, h, red, rec);
+__global__ __launch_bounds__(64) void k_loop_control(LoopArgs a, LoopCtl* __restrict__ ctl,
+                                                     const double* __restrict__ top,
+                                                     uint64_t* __restrict__ pmask,
+                                                     unsigned long long* __restrict__ red,
+                                                     LoopRecord* __restrict__ rec) {
+  loop_control(a, ctl, top, pmask, red, rec);
 }
 
-// get_catchment_input_points (LCM.cpp:3741-3767) after a baseflow trigger,
-// grid-stride over the catchment's cells.  The slot order of the rebuilt list
-// is arbitrary: every cell appears at most once and injection adds to
-// distinct cells.
+// get_catchment_input_points (LCM.cpp:3741-3767) after a baseflow trigger:
+// rewrites the whole input-point mask (one 64-column word per wavefront,
+// grid-stride) and accumulates the point count, the per-rain-cell counters
+// and the first point in list order (row-major) through the block.  Every
+// thread of the block calls it.
 __device__ __forceinline__ void rebuild_points(const LoopArgs& a, LoopCtl* __restrict__ ctl,
                                                const double* __restrict__ area,
                                                const int* __restrict__ rfarea,
-                                               int* __restrict__ pts, int* __restrict__ prf) {
+                                               uint64_t* __restrict__ pmask) {
+  __shared__ int s_cnt[kRfMax];
+  __shared__ int s_n, s_first;
+  if (threadIdx.x < kRfMax) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    s_n = 0;
+    s_first = INT_MAX;
+  }
+  __syncthreads();
   const double b = ctl->baseflow;
-  const long ncell = static_cast<long>(a.imax) * a.jmax;
-  for (long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; k < ncell;
-       k += static_cast<long>(gridDim.x) * blockDim.x) {
-    const int x = 1 + static_cast<int>(k / a.jmax);
-    const int y = 1 + static_cast<int>(k % a.jmax);
-    const int idx = x * a.pitch + y;
-    const double v = area[idx] * b * 3 * a.DX * a.DX;
-    if (v > a.MIN_Q && v < a.MIN_Q_MAXVAL) {
-      const int slot = atomicAdd(&ctl->npts, 1);
-      const int rf = rfarea[idx];
-      pts[slot] = idx;
-      prf[slot] = rf;
-      atomicAdd(&ctl->input_counter[rf], 1);
-      atomicMin(&ctl->first_new, idx);
+  const int lane = threadIdx.x & 63;
+  const long nwords = static_cast<long>(a.imax + 2) * a.mwords;
+  const long nwaves = static_cast<long>(gridDim.x) * (blockDim.x >> 6);
+  int my_n = 0, my_first = INT_MAX;
+  for (long wi = static_cast<long>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       wi < nwords; wi += nwaves) {
+    const int x = static_cast<int>(wi / a.mwords);
+    const int y = static_cast<int>(wi % a.mwords) * 64 + lane;
+    bool pt = false;
+    int idx = 0;
+    if (x >= 1 && x <= a.imax && y >= 1 && y <= a.jmax) {
+      idx = x * a.pitch + y;
+      const double v = area[idx] * b * 3 * a.DX * a.DX;
+      pt = v > a.MIN_Q && v < a.MIN_Q_MAXVAL;
     }
+    const uint64_t bits = __ballot(pt);
+    if (lane == 0) pmask[wi] = bits;
+    if (pt) {
+      atomicAdd(&s_cnt[rfarea[idx]], 1);
+      ++my_n;
+      my_first = min(my_first, idx);
+    }
+  }
+  if (my_n) {
+    atomicAdd(&s_n, my_n);
+    atomicMin(&s_first, my_first);
+  }
+  __syncthreads();
+  if (threadIdx.x < kRfMax && s_cnt[threadIdx.x]) atomicAdd(&ctl->input_counter[threadIdx.x], s_cnt[threadIdx.x]);
+  if (threadIdx.x == 0 && s_n) {
+    atomicAdd(&ctl->npts, s_n);
+    atomicMin(&ctl->first_new, s_first);
   }
 }
 
@@ -254,7 +262,7 @@ __global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restri
                                                    uint64_t* __restrict__ mask,
                                                    const double* __restrict__ area,
                                                    const int* __restrict__ rfarea,
-                                                   int* __restrict__ pts, int* __restrict__ prf,
+                                                   uint64_t* __restrict__ pmask,
                                                    const double* __restrict__ partials,
                                                    int nby, int nbx, int R,
                                                    LoopRecord* __restrict__ rec) {
@@ -270,19 +278,16 @@ __global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restri
     for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += gridDim.x * 4)
       scan_item<T>(it, a.imax, a.jmax, a.pitch, W, h, mask);
   }
-  if (rebuild) rebuild_points(a, ctl, area, rfarea, pts, prf);
+  if (rebuild) rebuild_points(a, ctl, area, rfarea, pmask);
 }
 
 __global__ void k_loop_reset_iter(LoopCtl* ctl) { ctl->iter = 0; }
 
 // ---------------------------------------------------------------- ensemble
 // One block per member: that member's loop_control on its current depth.
-template <typename T>
-__global__ __launch_bounds__(256) void k_ens_control(const EnsMember* __restrict__ mem,
-                                                     int parity) {
+__global__ __launch_bounds__(64) void k_ens_control(const EnsMember* __restrict__ mem) {
   const EnsMember& M = mem[blockIdx.x];
-  loop_control<T>(M.la, M.ctl, M.top, M.pts, M.prf, M.rfarea,
-                  static_cast<T*>(M.h[M.cur0 ^ parity]), M.red, M.rec);
+  loop_control(M.la, M.ctl, M.top, M.pmask, M.red, M.rec);
 }
 
 // Per member: temptot from its edge partials (one block each), scan_area on
@@ -314,7 +319,7 @@ __global__ __launch_bounds__(256) void k_ens_scan(const EnsMember* __restrict__ 
   for (int m = 0; m < n; ++m) {
     const EnsMember& M = mem[m];
     if (!M.ctl->halt && !M.ctl->err && M.ctl->pts_dirty)
-      rebuild_points(M.la, M.ctl, M.area, M.rfarea, M.pts, M.prf);
+      rebuild_points(M.la, M.ctl, M.area, M.rfarea, M.pmask);
   }
 }
 
@@ -325,15 +330,8 @@ __global__ void k_ens_reset(const EnsMember* __restrict__ mem, int n) {
 // One iteration's launches on the context stream.
 int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   const LoopArgs& a = c->largs;
-  const int k = c->cur;
-  if (c->prec == HC_FP64)
-    hipLaunchKernelGGL(k_loop_control<double>, dim3(1), dim3(256), 0, c->stream, a, c->d_ctl,
-                       c->d_top, c->d_lpts, c->d_lrf, c->d_rfarea, (double*)c->h[k], c->d_red,
-                       c->d_rec);
-  else
-    hipLaunchKernelGGL(k_loop_control<float>, dim3(1), dim3(256), 0, c->stream, a, c->d_ctl,
-                       c->d_top, c->d_lpts, c->d_lrf, c->d_rfarea, (float*)c->h[k], c->d_red,
-                       c->d_rec);
+  hipLaunchKernelGGL(k_loop_control, dim3(1), dim3(64), 0, c->stream, a, c->d_ctl, c->d_top,
+                     c->d_pmask, c->d_red, c->d_rec);
   HC_HIP(c, hipGetLastError());
   const bool prof = c->prof_on && k_in_run >= 0 && 2 * k_in_run + 1 < static_cast<int>(c->prof_ev.size());
   if (prof) HC_HIP(c, hipEventRecord(c->prof_ev[2 * k_in_run], c->stream));
@@ -341,7 +339,12 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   dr.ts = &c->d_ctl->time_step;
   dr.thr = c->prm.water_depth_erosion_threshold;
   dr.partials = c->d_dpart;
-  int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt, true, &dr);
+  hc::InjectArgs inj;
+  inj.pmask = c->d_pmask;
+  inj.rfarea = c->d_rfarea;
+  inj.amt = c->d_ctl->amt;
+  inj.rf_uniform = c->rf_uniform;
+  int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt, true, &dr, &inj);
   if (rc) return rc;
   if (prof) {
     HC_HIP(c, hipEventRecord(c->prof_ev[2 * k_in_run + 1], c->stream));
@@ -354,12 +357,12 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   hc_fused_geometry(c, R, &nbx, &nby);
   if (c->prec == HC_FP64)
     hipLaunchKernelGGL(k_loop_scan<double>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
-                       (const double*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_lpts,
-                       c->d_lrf, c->d_dpart, nby, nbx, R, c->d_rec);
+                       (const double*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
+                       c->d_dpart, nby, nbx, R, c->d_rec);
   else
     hipLaunchKernelGGL(k_loop_scan<float>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
-                       (const float*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_lpts,
-                       c->d_lrf, c->d_dpart, nby, nbx, R, c->d_rec);
+                       (const float*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
+                       c->d_dpart, nby, nbx, R, c->d_rec);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
@@ -387,6 +390,8 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
     return hc_fail(c, HC_ERR_ARG, "loop: rfnum out of range (1..16)");
   if (in->npoints < 0 || in->npoints > c->g.imax * c->g.jmax + 1)
     return hc_fail(c, HC_ERR_ARG, "loop: bad input point count");
+  if (in->first_x < 0 || in->first_x >= c->NR || in->first_y < 0 || in->first_y >= c->NC)
+    return hc_fail(c, HC_ERR_ARG, "loop: first input point outside the padded grid");
   HC_HIP(c, hipStreamSynchronize(c->stream));
   LoopArgs& a = c->largs;
   a.courant = in->courant_number;
@@ -404,6 +409,7 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
   a.pitch = c->P;
   a.ntop = in->ntop;
   a.scan_interval = 5;
+  a.mwords = c->W;
   // tables and inputs
   if (!c->d_ctl) HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_ctl), sizeof(LoopCtl)));
   {
@@ -429,24 +435,24 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
   if (rc) return rc;
   rc = upload_pitched(c, &c->d_rfarea, in->rfarea);
   if (rc) return rc;
-  const size_t cap = static_cast<size_t>(c->g.imax) * c->g.jmax + 2;
-  if (!c->d_lpts) {
-    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_lpts), cap * sizeof(int)));
-    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_lrf), cap * sizeof(int)));
+  // every rain cell index must address the amounts table (any cell can
+  // become an input point after a rebuild)
+  c->rf_uniform = in->rfarea[0];
+  for (size_t k = 0; k < static_cast<size_t>(c->NR) * c->NC; ++k) {
+    if (in->rfarea[k] < 0 || in->rfarea[k] >= kRfMax)
+      return hc_fail(c, HC_ERR_ARG, "loop: rain cell index out of range (0..16)");
+    if (in->rfarea[k] != c->rf_uniform) c->rf_uniform = -1;
   }
-  std::vector<int> pts(std::max(1, in->npoints)), prf(std::max(1, in->npoints));
+  const size_t nwords = static_cast<size_t>(c->NR) * c->W;
+  if (!c->d_pmask) HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_pmask), nwords * sizeof(uint64_t)));
+  std::vector<uint64_t> pm(nwords, 0ull);
   for (int k = 0; k < in->npoints; ++k) {
     const int x = in->in_x[k], y = in->in_y[k];
     if (x < 0 || x >= c->NR || y < 0 || y >= c->NC)
       return hc_fail(c, HC_ERR_ARG, "loop: input point outside the padded grid");
-    pts[k] = x * c->P + y;
-    prf[k] = in->rfarea[static_cast<size_t>(x) * c->NC + y];
-    if (prf[k] < 0 || prf[k] >= kRfMax) return hc_fail(c, HC_ERR_ARG, "loop: rain cell index out of range");
+    pm[static_cast<size_t>(x) * c->W + (y >> 6)] |= 1ull << (y & 63);
   }
-  if (in->npoints > 0) {
-    HC_HIP(c, hipMemcpy(c->d_lpts, pts.data(), in->npoints * sizeof(int), hipMemcpyHostToDevice));
-    HC_HIP(c, hipMemcpy(c->d_lrf, prf.data(), in->npoints * sizeof(int), hipMemcpyHostToDevice));
-  }
+  HC_HIP(c, hipMemcpy(c->d_pmask, pm.data(), nwords * sizeof(uint64_t), hipMemcpyHostToDevice));
   LoopCtl C;
   std::memset(&C, 0, sizeof(C));
   C.time_step = in->time_step;
@@ -695,10 +701,7 @@ int ens_prepare(hc_ens* e, int steps, bool run) {
 
 int ens_step(hc_ens* e, int parity) {
   const int n = static_cast<int>(e->m.size());
-  if (e->mode == 0)
-    hipLaunchKernelGGL(k_ens_control<double>, dim3(n), dim3(256), 0, e->stream, e->d_mem, parity);
-  else
-    hipLaunchKernelGGL(k_ens_control<float>, dim3(n), dim3(256), 0, e->stream, e->d_mem, parity);
+  hipLaunchKernelGGL(k_ens_control, dim3(n), dim3(64), 0, e->stream, e->d_mem);
   ENS_HIP(e, hipGetLastError());
   if (hc_launch_ens_fused(e->stream, e->mode, e->R, e->d_mem, e->d_tiles, e->ntiles, e->nblocks,
                           parity))
@@ -751,8 +754,8 @@ int hc_ens_create(hc_ens** out, hc_ctx* const* members, int n) {
     M.ctl = c->d_ctl;
     M.rec = c->d_rec;
     M.top = c->d_top;
-    M.pts = c->d_lpts;
-    M.prf = c->d_lrf;
+    M.pmask = c->d_pmask;
+    M.rf_uniform = c->rf_uniform;
     M.rfarea = c->d_rfarea;
     M.area = c->d_area;
     for (int k = 0; k < 2; ++k) {
