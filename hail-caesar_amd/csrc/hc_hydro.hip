@@ -756,6 +756,53 @@ extern "C" int hc_set_tiles(hc_ctx* c, int rows) {
   return HC_OK;
 }
 
+int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double* qx0,
+                           const double* qy0, double* h1, double* qx1, double* qy1,
+                           const uint64_t* mask) {
+  if (c->prec != HC_FP64 || c->g.sediment)
+    return hc_fail(c, HC_ERR_STATE, "fused64: FP64 hydro-only contexts");
+  const HydroArgs a = hydro_args(c);
+  const int R = c->fused_rows;
+  int nbx, nby;
+  hc_fused_geometry(c, R, &nbx, &nby);
+  const int nblocks = nbx * nby;
+  FusedPtrs<0> p;
+  p.h0 = h0;
+  p.zz = static_cast<const double*>(c->z);
+  p.z64 = nullptr;
+  p.qx0 = qx0;
+  p.qy0 = qy0;
+  p.nman = static_cast<const double*>(c->nman);
+  p.mask = mask;
+  p.h1 = h1;
+  p.qx1 = qx1;
+  p.qy1 = qy1;
+  p.red = c->d_red;
+  p.inj = hc::InjectArgs{};
+  const DrainArgs dr{};
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, p,
+                       nullptr, nullptr, dr);
+  };
+  if (R == 4) go(k_fused<0, 4>);
+  else if (R == 16) go(k_fused<0, 16>);
+  else go(k_fused<0, 8>);
+  HC_HIP(c, hipGetLastError());
+  return HC_OK;
+}
+
+int hc_launch_scan_ptr(hc_ctx* c, const void* h, uint64_t* mask) {
+  const int nb = std::max(1, std::min((scan_items(c->g.imax, c->W) + 3) / 4, 2048));
+  if (c->prec == HC_FP64)
+    hipLaunchKernelGGL(k_scan<double>, dim3(nb), dim3(256), 0, c->stream, c->g.imax, c->g.jmax,
+                       c->P, c->W, static_cast<const double*>(h), mask);
+  else
+    hipLaunchKernelGGL(k_scan<float>, dim3(nb), dim3(256), 0, c->stream, c->g.imax, c->g.jmax,
+                       c->P, c->W, static_cast<const float*>(h), mask);
+  HC_HIP(c, hipGetLastError());
+  return HC_OK;
+}
+
 int hc_launch_scan(hc_ctx* c) {
   const int k = c->cur;
   const int nb = std::max(1, std::min((scan_items(c->g.imax, c->W) + 3) / 4, 2048));

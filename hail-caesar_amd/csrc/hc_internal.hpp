@@ -258,6 +258,12 @@ const void* hc_fused_z(const hc_ctx* c);
 int hc_launch_ens_fused(hipStream_t s, int mode, int R, const hc::EnsMember* mem,
                         const int2* tiles, int ntiles, int nblocks, int parity);
 int hc_launch_scan(hc_ctx* c);
+// FP64 fused flow_route + depth_update between explicit state buffers (fixed
+// dt, no drain or injection), and scan_area of an explicit depth buffer.
+int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double* qx0,
+                           const double* qy0, double* h1, double* qx1, double* qy1,
+                           const uint64_t* mask);
+int hc_launch_scan_ptr(hc_ctx* c, const void* h, uint64_t* mask);
 int hc_launch_flux_out(hc_ctx* c, double ts);
 int hc_launch_points(hc_ctx* c, int n);
 int hc_launch_health(hc_ctx* c);

@@ -37,7 +37,8 @@ SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upl
            "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
            "hc_set_tiles", "hc_bench_fused", "hc_ens_create", "hc_ens_destroy",
            "hc_ens_last_error", "hc_ens_run", "hc_ens_sync", "hc_ens_tune", "hc_ens_set_tiles",
-           "hc_ens_bench_fused"]
+           "hc_ens_bench_fused", "hc_adj_create", "hc_adj_destroy", "hc_adj_last_error",
+           "hc_adj_set_target", "hc_adj_forward", "hc_adj_backward"]
 MODEL_SYMBOLS = ["hcm_create", "hcm_create_arrays", "hcm_destroy", "hcm_last_error",
                  "hcm_iterate", "hcm_download", "hcm_field_count", "hcm_scalar",
                  "hcm_series", "hcm_set_graph", "hcm_context", "hcm_loop_begin",
@@ -152,6 +153,13 @@ def lib():
         L.hc_ens_tune.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(d)]
         L.hc_ens_set_tiles.argtypes = [vp, i, i]
         L.hc_ens_bench_fused.argtypes = [vp, i, ctypes.POINTER(d)]
+        L.hc_adj_create.argtypes = [ctypes.POINTER(vp), vp, i, d, i, vp, i, vp]
+        L.hc_adj_destroy.argtypes = [vp]
+        L.hc_adj_last_error.argtypes = [vp]
+        L.hc_adj_last_error.restype = ctypes.c_char_p
+        L.hc_adj_set_target.argtypes = [vp, vp]
+        L.hc_adj_forward.argtypes = [vp, vp, ctypes.POINTER(d), vp]
+        L.hc_adj_backward.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -338,3 +346,60 @@ class Context:
         us = ctypes.c_double()
         self._check(self.L.hc_bench_fused(self.h, dt, reps, ctypes.byref(us)))
         return us.value
+
+
+class Adjoint:
+    """Outlet-hydrograph misfit and its gradients (include/hc.h, hc_adj_*) on
+    an FP64 hydro-only context with spatially variable n.  The context's state
+    when this object is made is the initial state of every forward run."""
+
+    def __init__(self, ctx: "Context", nsteps: int, dt: float, block: np.ndarray,
+                 rain: np.ndarray, checkpoint_every: int = 100):
+        self.L = lib()
+        self.ctx = ctx
+        self.nsteps = nsteps
+        self.block = np.ascontiguousarray(block, dtype=np.int32)
+        assert self.block.shape == ctx.shape
+        self.nb = int(self.block.max()) + 1
+        rain = np.ascontiguousarray(rain, dtype=np.float64)
+        assert rain.shape == (nsteps,)
+        h = ctypes.c_void_p()
+        rc = self.L.hc_adj_create(ctypes.byref(h), ctx.h, nsteps, dt, checkpoint_every,
+                                  self.block.ctypes.data, self.nb, rain.ctypes.data)
+        if rc != HC_OK:
+            raise HCError(rc, "hc_adj_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.hc_adj_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != HC_OK:
+            raise HCError(rc, self.L.hc_adj_last_error(self.h).decode())
+
+    def set_target(self, target: np.ndarray) -> None:
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        assert t.shape == (self.nsteps,)
+        self._check(self.L.hc_adj_set_target(self.h, t.ctypes.data))
+
+    def forward(self, mult: Sequence[float]):
+        m = np.ascontiguousarray(mult, dtype=np.float64)
+        assert m.shape == (self.nb,)
+        loss = ctypes.c_double()
+        q = np.zeros(self.nsteps)
+        self._check(self.L.hc_adj_forward(self.h, m.ctypes.data, ctypes.byref(loss), q.ctypes.data))
+        return loss.value, q
+
+    def backward(self):
+        gn = np.zeros(self.ctx.shape)
+        gm = np.zeros(self.nb)
+        self._check(self.L.hc_adj_backward(self.h, gn.ctypes.data, gm.ctypes.data))
+        return gn, gm

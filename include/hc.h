@@ -282,6 +282,35 @@ int hc_ens_set_tiles(hc_ens* ens, int rows, int blocks);
 /* Average duration (us) of `reps` fused ensemble launches; state unchanged. */
 int hc_ens_bench_fused(hc_ens* ens, int reps, double* us);
 
+/* ---- adjoint -------------------------------------------------------------
+ * No counterpart in the reference (it has no derivatives).  Gradient of an
+ * outlet-hydrograph misfit L = sum_s (Q_s - Qtarget_s)^2, Q_s the edge outflow
+ * of water_flux_out (LCM.cpp:3005-3047) at step s, through nsteps of the FP64
+ * flow_route + depth_update (LCM.cpp:3063-3299) at a fixed dt (the reference's
+ * set_maximum_timestep bypassed), with the active mask rebuilt every 5th
+ * step and rain added at the start of each step to every cell of a rain block:
+ * h += dt * rain[s] * mult[block].  Gradients are with respect to the
+ * per-cell Manning n (the context's HC_FIELD_MANNINGS) and the block
+ * multipliers.  The context must be FP64, hydro-only, with spatially variable
+ * n; its state at hc_adj_create is the initial state of every forward run.
+ * Two-level checkpointing: the forward keeps every checkpoint_every-th state;
+ * the reverse sweep re-runs each segment once into per-step device slots. */
+typedef struct hc_adj hc_adj;
+/* block: rain block per cell, reference layout ((imax+2) x (jmax+2)), values
+ * 0..nblocks-1 or -1 for no rain (nblocks <= 16); rain: nsteps rates (m/s). */
+int hc_adj_create(hc_adj** adj, hc_ctx* ctx, int nsteps, double dt, int checkpoint_every,
+                  const int* block, int nblocks, const double* rain);
+int hc_adj_destroy(hc_adj* adj);
+const char* hc_adj_last_error(const hc_adj* adj);
+/* target discharge series (nsteps values); zero until set */
+int hc_adj_set_target(hc_adj* adj, const double* target);
+/* forward run with the given multipliers and the context's current n:
+ * loss, and the discharge series when Q is not null */
+int hc_adj_forward(hc_adj* adj, const double* mult, double* loss, double* Q);
+/* gradients of the last forward's loss: grad_n on the reference layout,
+ * grad_mult[nblocks] (either may be null) */
+int hc_adj_backward(hc_adj* adj, double* grad_n, double* grad_mult);
+
 #ifdef __cplusplus
 }
 #endif

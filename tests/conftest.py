@@ -2,6 +2,13 @@ import importlib.util
 import os
 import sys
 
+try:
+    # torch first: it brings its own HIP runtime libraries, and loading them
+    # after libhc.so has initialised the system HIP runtime aborts the process
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
