@@ -7,6 +7,18 @@
 
 namespace hcdev {
 
+// A pointer read from device memory (a table of buffers) is generic to the
+// compiler, which then emits flat loads and stores; those also count against
+// the LDS wait counter, so every cross-lane shuffle waits for them.  Reading
+// the pointer value itself as a global-address-space pointer lets the
+// compiler use global_* accesses through it.
+template <typename T, typename F>
+__device__ __forceinline__ T* table_ptr(const F& field) {
+  using GP = __attribute__((address_space(1))) T*;
+  const GP g = *reinterpret_cast<const GP*>(&field);
+  return (T*)g;
+}
+
 __device__ __forceinline__ double dmax_ref(double a, double b) {
   return (a < b) ? b : a;  // std::max(a, b)
 }

@@ -435,30 +435,31 @@ __global__ __launch_bounds__(256) void k_ens_fused(const EnsMember* __restrict__
   for (long t = t0; t < t1; ++t) {
     const int2 tile = tiles[t];
     const EnsMember& M = mem[tile.x];
-    if (M.ctl->halt) continue;  // uniform per block
+    const hc::LoopCtl* ctl = table_ptr<const hc::LoopCtl>(M.ctl);
+    if (ctl->halt) continue;  // uniform per block
     const int g = M.cur0 ^ step_parity;
     FusedPtrs<MODE> p;
-    p.h0 = static_cast<const T*>(M.h[g]);
-    p.zz = static_cast<const Z*>(M.z);
-    p.z64 = M.z64;
-    p.qx0 = static_cast<const T*>(M.qx[g]);
-    p.qy0 = static_cast<const T*>(M.qy[g]);
-    p.nman = static_cast<const T*>(M.nman);
-    p.mask = M.mask;
-    p.h1 = static_cast<T*>(M.h[g ^ 1]);
-    p.qx1 = static_cast<T*>(M.qx[g ^ 1]);
-    p.qy1 = static_cast<T*>(M.qy[g ^ 1]);
-    p.red = M.red;
-    p.inj.pmask = M.pmask;
+    p.h0 = table_ptr<const T>(M.h[g]);
+    p.zz = table_ptr<const Z>(M.z);
+    p.z64 = table_ptr<const double>(M.z64);
+    p.qx0 = table_ptr<const T>(M.qx[g]);
+    p.qy0 = table_ptr<const T>(M.qy[g]);
+    p.nman = table_ptr<const T>(M.nman);
+    p.mask = table_ptr<const uint64_t>(M.mask);
+    p.h1 = table_ptr<T>(M.h[g ^ 1]);
+    p.qx1 = table_ptr<T>(M.qx[g ^ 1]);
+    p.qy1 = table_ptr<T>(M.qy[g ^ 1]);
+    p.red = table_ptr<unsigned long long>(M.red);
+    p.inj.pmask = table_ptr<const uint64_t>(M.pmask);
     p.inj.rf_uniform = M.rf_uniform;
-    p.inj.rfarea = M.rfarea;
-    p.inj.amt = M.ctl->amt;
+    p.inj.rfarea = table_ptr<const int>(M.rfarea);
+    p.inj.amt = ctl->amt;
     DrainArgs d;
-    d.ts = &M.ctl->time_step;
+    d.ts = &ctl->time_step;
     d.thr = M.thr;
-    d.partials = M.dpart;
+    d.partials = table_ptr<double>(M.dpart);
     __syncthreads();  // the block's shared scratch is reused tile to tile
-    fused_tile<MODE, R>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, M.ctl->dt_flow, p, d);
+    fused_tile<MODE, R>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, ctl->dt_flow, p, d);
   }
 }
 

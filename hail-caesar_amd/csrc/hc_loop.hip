@@ -287,7 +287,9 @@ __global__ void k_loop_reset_iter(LoopCtl* ctl) { ctl->iter = 0; }
 // One block per member: that member's loop_control on its current depth.
 __global__ __launch_bounds__(64) void k_ens_control(const EnsMember* __restrict__ mem) {
   const EnsMember& M = mem[blockIdx.x];
-  loop_control(M.la, M.ctl, M.top, M.pmask, M.red, M.rec);
+  loop_control(M.la, table_ptr<LoopCtl>(M.ctl), table_ptr<const double>(M.top),
+               table_ptr<uint64_t>(M.pmask), table_ptr<unsigned long long>(M.red),
+               table_ptr<LoopRecord>(M.rec));
 }
 
 // Per member: temptot from its edge partials (one block each), scan_area on
@@ -299,8 +301,10 @@ __global__ __launch_bounds__(256) void k_ens_scan(const EnsMember* __restrict__ 
                                                   int parity) {
   for (int m = blockIdx.x; m < n; m += gridDim.x) {
     const EnsMember& M = mem[m];
-    if (!M.ctl->halt && !M.ctl->err)
-      finish_drain(M.ctl, M.rec, M.dpart, M.nby, M.nbx, R, M.la.imax, M.la.jmax);
+    LoopCtl* ctl = table_ptr<LoopCtl>(M.ctl);
+    if (!ctl->halt && !ctl->err)
+      finish_drain(ctl, table_ptr<LoopRecord>(M.rec), table_ptr<const double>(M.dpart), M.nby,
+                   M.nbx, R, M.la.imax, M.la.jmax);
   }
   const int total = item_base[n];
   for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < total; it += gridDim.x * 4) {
@@ -311,20 +315,22 @@ __global__ __launch_bounds__(256) void k_ens_scan(const EnsMember* __restrict__ 
       else hi = mid;
     }
     const EnsMember& M = mem[lo];
-    const LoopCtl* C = M.ctl;
+    const LoopCtl* C = table_ptr<const LoopCtl>(M.ctl);
     if (C->halt || C->err || (C->counter % M.la.scan_interval) != 0) continue;  // per wave
     scan_item<T>(it - item_base[lo], M.la.imax, M.la.jmax, M.la.pitch, M.W,
-                 static_cast<const T*>(M.h[M.cur0 ^ parity ^ 1]), M.mask);
+                 table_ptr<const T>(M.h[M.cur0 ^ parity ^ 1]), table_ptr<uint64_t>(M.mask));
   }
   for (int m = 0; m < n; ++m) {
     const EnsMember& M = mem[m];
-    if (!M.ctl->halt && !M.ctl->err && M.ctl->pts_dirty)
-      rebuild_points(M.la, M.ctl, M.area, M.rfarea, M.pmask);
+    LoopCtl* ctl = table_ptr<LoopCtl>(M.ctl);
+    if (!ctl->halt && !ctl->err && ctl->pts_dirty)
+      rebuild_points(M.la, ctl, table_ptr<const double>(M.area), table_ptr<const int>(M.rfarea),
+                     table_ptr<uint64_t>(M.pmask));
   }
 }
 
 __global__ void k_ens_reset(const EnsMember* __restrict__ mem, int n) {
-  for (int m = threadIdx.x; m < n; m += blockDim.x) mem[m].ctl->iter = 0;
+  for (int m = threadIdx.x; m < n; m += blockDim.x) table_ptr<LoopCtl>(mem[m].ctl)->iter = 0;
 }
 
 // One iteration's launches on the context stream.
