@@ -78,7 +78,7 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
     setup_s = time.time() - t0
     m.iterate(warmup)
     # the project's tile tuner picks rows per wavefront on the warmed state
-    rows, tune_us = m.context().tune_tiles(0.05)
+    rows, strip, tune_us = m.context().tune_tiles(0.05)
     m.iterate(2)
     # timed region: K loop bodies, synchronised on both sides
     m.synchronize()
@@ -101,7 +101,7 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
     return {
         "wall_s": wall, "setup_s": setup_s, "kernel_ms": ktot / max(kn, 1), "kernel_launches": kn,
         "active_fraction": active, "h": h, "counter": m.scalar("counter"), "cycle": m.scalar("cycle"),
-        "model": m, "tile_rows": rows, "tune_us": tune_us,
+        "model": m, "tile_rows": rows, "tile_strip": strip, "tune_us": tune_us,
     }
 
 
@@ -297,7 +297,7 @@ def main():
         res[prec] = r
         print(f"[bench] {prec}: {r['cell_steps_per_s']:.4e} cell-steps/s, {r['ms_per_step']:.4f} ms/step, "
               f"fused kernel {r['kernel_ms']*1e3:.1f} us = {r['kernel_gbs']:.0f} GB/s, "
-              f"active {r['active_fraction']:.3f}, tile rows {r['tile_rows']}", file=sys.stderr, flush=True)
+              f"active {r['active_fraction']:.3f}, tiles {r['tile_rows']}x{r['tile_strip']}", file=sys.stderr, flush=True)
     head = res.get("fp32") or res.get("fp64")
     prec = "fp32" if "fp32" in res else "fp64"
     traffic = None
@@ -306,14 +306,15 @@ def main():
             t = json.load(f)
         # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs,
         # scaled by the same counters' reading of a plain copy of known size
-        traffic = t.get(f"{prec}_R{head['tile_rows']}_bytes_per_launch")
+        traffic = t.get(f"{prec}_R{head['tile_rows']}_S{head['tile_strip']}_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(head["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(head["kernel_gbs"] / HBM_PEAK_GBS, 4),
                 "traffic": traffic,
                 "kernel": "k_flow_depth32 (fused flow_route+depth_update)" if prec == "fp32"
                 else "k_flow_depth64 (fused flow_route+depth_update)",
                 "bytes_per_cell_step": head["bytes_per_cell_step"], "cells_per_launch": cells,
-                "kernel_us": round(head["kernel_ms"] * 1e3, 2), "tile_rows": head["tile_rows"]}
+                "kernel_us": round(head["kernel_ms"] * 1e3, 2), "tile_rows": head["tile_rows"],
+                "tile_strip": head["tile_strip"]}
     out = {
         "metric": base["metric"], "value": round(head["cell_steps_per_s"], 1), "unit": "cell-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -229,11 +229,19 @@ struct FusedPtrs {
   hc::InjectArgs inj;  // rain input at the input points (device loop only)
 };
 
-// One tile: R output rows x (four wavefronts x 62 owned columns).  Every
-// thread of the block calls it (it ends in block reductions).
+// One tile: a strip of nsub x R output rows by four wavefronts x 62 owned
+// columns.  A wavefront covers 64 columns and owns the middle 62 (lane 0 of
+// wavefront 0 also owns column 0); y-neighbours come from lane shuffles.  The
+// strip is walked in sub-tiles of R rows: the first loads rows r0-1 .. r0+R
+// of every field into registers up front, each later one loads only its R new
+// rows and carries the two rows above it and the new fluxes of its first row
+// from the sub-tile before (no halo rows are re-read and no face is computed
+// twice).  Every thread of the block calls this function (it ends in block
+// reductions).
 template <int MODE, int R>
-__device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, int nbx, double dtd,
-                                           const FusedPtrs<MODE>& p, const DrainArgs& drain) {
+__device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, int nbx, int nsub,
+                                           double dtd, const FusedPtrs<MODE>& p,
+                                           const DrainArgs& drain) {
   using T = typename FusedTypes<MODE>::T;
   using Z = typename FusedTypes<MODE>::Z;
   const T* __restrict__ h0 = p.h0;
@@ -250,12 +258,12 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   const int lane = threadIdx.x & 63;
   const int w = bx * 4 + (threadIdx.x >> 6);
   const int NR = a.imax + 2, NC = a.jmax + 2;
+  const int H = R * nsub;  // rows per block
   const bool active_wave = (62 * w + 1 <= NC - 1);  // has an owned column
   const int c = 62 * w + lane;
   const bool owned = ((lane >= 1 && lane <= 62) || (w == 0 && lane == 0)) && c <= NC - 1;
   const size_t P = a.pitch;
   const int cl = min(c, a.pitch - 1);  // load column (lanes past the row reuse the last)
-  const int r0 = by * R;
 
   T hh[R + 2];
   Z zt[R + 2];
@@ -276,12 +284,16 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   T md = 0;
   // water_flux_out (LCM.cpp:3005-3047) on this tile's edge cells, after
   // depth_update and before the store; maxdepth is taken before the drain
-  const bool drains = drain.ts != nullptr && fused_edge_tile(bx, by, R, a.imax, a.jmax);
+  const bool drains = drain.ts != nullptr && fused_edge_tile(bx, by, H, a.imax, a.jmax);
   const double ts = drains ? *drain.ts : 1.0;
   double dacc = 0;
-  if (active_wave) {
-#pragma unroll
-  for (int k = 0; k < R + 2; ++k) {
+  const T dt = static_cast<T>(dtd);
+  const float invdt = 1.0f / static_cast<float>(dtd);
+  const float fthr = static_cast<float>(a.flat_dz);
+  T qxNp = 0, qyNp = 0, qyRp = 0;  // new fluxes of the row above (carried)
+
+  // load row k (x = r0 - 1 + k) of every field
+  auto load_row = [&](int k, int r0) {
     const int x = r0 - 1 + k;
     const bool v = (x >= 0 && x < NR);
     const size_t o = static_cast<size_t>(v ? x : 0) * P + cl;
@@ -313,14 +325,10 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       }
       mbits |= static_cast<uint32_t>(((hi_word ? m1 : m0) >> bit) & 1ull) << k;
     }
-  }
+  };
 
-  const T dt = static_cast<T>(dtd);
-  const float invdt = 1.0f / static_cast<float>(dtd);
-  const float fthr = static_cast<float>(a.flat_dz);
-  T qxNp = 0, qyNp = 0, qyRp = 0;
-#pragma unroll
-  for (int k = 1; k < R + 2; ++k) {
+  // faces of row k, and depth_update + store of row k-1 (k >= 2)
+  auto row = [&](int k, int r0) {
     const int x = r0 - 1 + k;
     const T hl = __shfl_up(hh[k], 1, 64);
     const Z zl = __shfl_up(zt[k], 1, 64);
@@ -385,8 +393,31 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     qxNp = qxN;
     qyNp = qyN;
     qyRp = qyR;
+  };
+
+  if (active_wave) {
+    // first sub-tile: rows r0-1 .. r0+R loaded, faces of rows r0 .. r0+R
+    int r0 = by * H;
+#pragma unroll
+    for (int k = 0; k < R + 2; ++k) load_row(k, r0);
+#pragma unroll
+    for (int k = 1; k < R + 2; ++k) row(k, r0);
+    for (int t = 1; t < nsub; ++t) {
+      r0 += R;
+      if (r0 > NR) break;  // uniform: no row of this sub-tile is in the grid
+      // rows r0-1 and r0 were the last two rows of the sub-tile before, and
+      // the faces of row r0 are already computed (qxNp, qyNp, qyRp)
+      hh[0] = hh[R];
+      hh[1] = hh[R + 1];
+      zt[0] = zt[R];
+      zt[1] = zt[R + 1];
+      mbits = ((mbits >> (R + 1)) & 1u) << 1;
+#pragma unroll
+      for (int k = 2; k < R + 2; ++k) load_row(k, r0);
+#pragma unroll
+      for (int k = 2; k < R + 2; ++k) row(k, r0);
+    }
   }
-  }  // active_wave
   // the halo lanes' depths are not this wavefront's results
   block_max_pos(red, owned && active_wave ? static_cast<double>(md) : 0.0);
   if (drains) {  // uniform per block
@@ -395,7 +426,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     if (lane == 0) s_dw[threadIdx.x >> 6] = dacc;
     __syncthreads();
     if (threadIdx.x == 0)
-      drain.partials[hc::EdgeSlots(nbx, R, a.imax, a.jmax).slot(bx, by)] =
+      drain.partials[hc::EdgeSlots(nbx, H, a.imax, a.jmax).slot(bx, by)] =
           (s_dw[0] + s_dw[1]) + (s_dw[2] + s_dw[3]);
   }
 }
@@ -404,7 +435,8 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
 // b runs on XCD b mod 8) takes a contiguous band of tiles.
 template <int MODE, int R>
 __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int nbx, int nblocks,
-                                               FusedPtrs<MODE> p, const double* __restrict__ dtp,
+                                               int nsub, FusedPtrs<MODE> p,
+                                               const double* __restrict__ dtp,
                                                const int* __restrict__ haltp,
                                                const DrainArgs drain) {
   if (haltp && *haltp) return;  // device-resident loop has finished
@@ -413,7 +445,7 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   const int q8 = nblocks / 8, r8 = nblocks % 8;
   const int xcd = b % 8;
   const int bb = xcd * q8 + min(xcd, r8) + b / 8;
-  fused_tile<MODE, R>(a, bb % nbx, bb / nbx, nbx, dtd, p, drain);
+  fused_tile<MODE, R>(a, bb % nbx, bb / nbx, nbx, nsub, dtd, p, drain);
 }
 
 // Ensemble: a persistent grid walks the ensemble's tile list (member-major).
@@ -459,7 +491,7 @@ __global__ __launch_bounds__(256) void k_ens_fused(const EnsMember* __restrict__
     d.thr = M.thr;
     d.partials = table_ptr<double>(M.dpart);
     __syncthreads();  // the block's shared scratch is reused tile to tile
-    fused_tile<MODE, R>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, ctl->dt_flow, p, d);
+    fused_tile<MODE, R>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, 1, ctl->dt_flow, p, d);
   }
 }
 
@@ -621,11 +653,12 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
   const HydroArgs a = hydro_args(c);
   const int R = c->fused_rows;
+  const int ns = c->fused_strip;
   int nbx, nby;
-  hc_fused_geometry(c, R, &nbx, &nby);
+  hc_fused_geometry(c, R * ns, &nbx, &nby);
   const int nblocks = nbx * nby;
   const DrainArgs dr = drain ? *drain : DrainArgs{};
-  if (dr.ts && static_cast<size_t>(hc::EdgeSlots(nbx, R, c->g.imax, c->g.jmax).count(nby)) >
+  if (dr.ts && static_cast<size_t>(hc::EdgeSlots(nbx, R * ns, c->g.imax, c->g.jmax).count(nby)) >
                    c->dpart_cap)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: drain partials buffer too small");
   const int k = c->cur, o = c->cur ^ 1;
@@ -651,8 +684,8 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     p.qy1 = (T*)c->qy[o];
     p.red = c->d_red;
     p.inj = inject ? *inject : hc::InjectArgs{};
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, p, dtp,
-                       haltp, dr);
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, ns, p,
+                       dtp, haltp, dr);
   };
   double* td = nullptr;
   float* tf = nullptr;
@@ -677,10 +710,10 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
 // Tile tuner: time the fused kernel for each candidate row count on the
 // context's current state (the kernel reads one generation and writes the
 // other, so repeated launches without swapping leave the state unchanged).
-extern "C" int hc_tune_tiles(hc_ctx* c, double dt, int* best_rows, double* best_us) {
+extern "C" int hc_tune_tiles(hc_ctx* c, double dt, int* best_rows, int* best_strip,
+                             double* best_us) {
   if (!c) return HC_ERR_ARG;
   if (c->g.sediment) return hc_fail(c, HC_ERR_STATE, "tune: hydro-only contexts");
-  const int cand[3] = {4, 8, 16};
   // the reduction slot carries the last maxdepth into a device-resident loop
   unsigned long long saved = 0;
   HC_HIP(c, hipStreamSynchronize(c->stream));
@@ -689,41 +722,51 @@ extern "C" int hc_tune_tiles(hc_ctx* c, double dt, int* best_rows, double* best_
   HC_HIP(c, hipEventCreate(&e0));
   HC_HIP(c, hipEventCreate(&e1));
   double best = 1e30;
-  int bestR = c->fused_rows;
-  const int keep = c->fused_rows;
-  for (int R : cand) {
-    c->fused_rows = R;
-    int rc = hc_launch_flow_depth(c, dt, nullptr, nullptr, false);  // warm
-    if (rc) return rc;
-    HC_HIP(c, hipEventRecord(e0, c->stream));
-    const int reps = 10;
-    for (int k = 0; k < reps; ++k) {
-      rc = hc_launch_flow_depth(c, dt, nullptr, nullptr, false);
+  int bestR = c->fused_rows, bestS = c->fused_strip;
+  for (int R : {4, 8, 16}) {
+    for (int S : {1, 2, 4, 8}) {
+      c->fused_rows = R;
+      c->fused_strip = S;
+      int rc = hc_launch_flow_depth(c, dt, nullptr, nullptr, false);  // warm
       if (rc) return rc;
-    }
-    HC_HIP(c, hipEventRecord(e1, c->stream));
-    HC_HIP(c, hipEventSynchronize(e1));
-    float ms = 0;
-    HC_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-    const double us = 1000.0 * ms / reps;
-    if (us < best) {
-      best = us;
-      bestR = R;
+      HC_HIP(c, hipEventRecord(e0, c->stream));
+      const int reps = 10;
+      for (int k = 0; k < reps; ++k) {
+        rc = hc_launch_flow_depth(c, dt, nullptr, nullptr, false);
+        if (rc) return rc;
+      }
+      HC_HIP(c, hipEventRecord(e1, c->stream));
+      HC_HIP(c, hipEventSynchronize(e1));
+      float ms = 0;
+      HC_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+      const double us = 1000.0 * ms / reps;
+      if (us < best) {
+        best = us;
+        bestR = R;
+        bestS = S;
+      }
     }
   }
-  (void)keep;
   HC_HIP(c, hipMemcpy(c->d_red, &saved, sizeof(saved), hipMemcpyHostToDevice));
   hc_drop_loop_graphs(c);  // recapture with the chosen tiles
   c->fused_rows = bestR;
+  c->fused_strip = bestS;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (best_rows) *best_rows = bestR;
+  if (best_strip) *best_strip = bestS;
   if (best_us) *best_us = best;
   return HC_OK;
 }
 
-// Average duration (us) of `reps` fused launches on the current state,
-// without swapping generations (state unchanged).
+extern "C" int hc_set_tiles(hc_ctx* c, int rows, int strip) {
+  if (!c || (rows != 4 && rows != 8 && rows != 16) || strip < 1 || strip > 64) return HC_ERR_ARG;
+  if (rows != c->fused_rows || strip != c->fused_strip) hc_drop_loop_graphs(c);
+  c->fused_rows = rows;
+  c->fused_strip = strip;
+  return HC_OK;
+}
+
 extern "C" int hc_bench_fused(hc_ctx* c, double dt, int reps, double* us) {
   if (!c || reps < 1) return HC_ERR_ARG;
   unsigned long long saved = 0;
@@ -750,12 +793,6 @@ extern "C" int hc_bench_fused(hc_ctx* c, double dt, int reps, double* us) {
   return HC_OK;
 }
 
-extern "C" int hc_set_tiles(hc_ctx* c, int rows) {
-  if (!c || (rows != 4 && rows != 8 && rows != 16)) return HC_ERR_ARG;
-  if (rows != c->fused_rows) hc_drop_loop_graphs(c);
-  c->fused_rows = rows;
-  return HC_OK;
-}
 
 int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double* qx0,
                            const double* qy0, double* h1, double* qx1, double* qy1,
@@ -764,8 +801,9 @@ int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double*
     return hc_fail(c, HC_ERR_STATE, "fused64: FP64 hydro-only contexts");
   const HydroArgs a = hydro_args(c);
   const int R = c->fused_rows;
+  const int ns = c->fused_strip;
   int nbx, nby;
-  hc_fused_geometry(c, R, &nbx, &nby);
+  hc_fused_geometry(c, R * ns, &nbx, &nby);
   const int nblocks = nbx * nby;
   FusedPtrs<0> p;
   p.h0 = h0;
@@ -782,7 +820,7 @@ int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double*
   p.inj = hc::InjectArgs{};
   const DrainArgs dr{};
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, p,
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, ns, p,
                        nullptr, nullptr, dr);
   };
   if (R == 4) go(k_fused<0, 4>);

@@ -165,7 +165,8 @@ struct hc_ctx {
   void* qx[2] = {nullptr, nullptr};
   void* qy[2] = {nullptr, nullptr};
   int cur = 0;
-  int fused_rows = 8;         // output rows per wavefront of the fused kernel (tuned)
+  int fused_rows = 8;         // output rows per sub-tile of the fused kernel (tuned)
+  int fused_strip = 1;        // sub-tiles per block: a block covers rows * strip rows
   void* z = nullptr;          // elevation (state type)
   double* z64 = nullptr;      // FP32 form: FP64 elevation for near-flat faces
   void* nman = nullptr;       // Manning n (state type), spatial only

@@ -358,7 +358,7 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   }
   const int o = c->cur;  // flow_depth flipped the generation
   const int nb = std::max(1, std::min((hcdev::scan_items(c->g.imax, c->W) + 3) / 4, 1024));
-  const int R = c->fused_rows;
+  const int R = c->fused_rows * c->fused_strip;  // rows per fused block
   int nbx, nby;
   hc_fused_geometry(c, R, &nbx, &nby);
   if (c->prec == HC_FP64)

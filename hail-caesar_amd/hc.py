@@ -139,8 +139,8 @@ def lib():
         L.hc_loop_begin.argtypes = [vp, ctypes.POINTER(hc_loop_init)]
         L.hc_loop_run.argtypes = [vp, i, i]
         L.hc_loop_sync.argtypes = [vp, ctypes.POINTER(hc_loop_state), vp, i, ctypes.POINTER(i)]
-        L.hc_tune_tiles.argtypes = [vp, d, ctypes.POINTER(i), ctypes.POINTER(d)]
-        L.hc_set_tiles.argtypes = [vp, i]
+        L.hc_tune_tiles.argtypes = [vp, d, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(d)]
+        L.hc_set_tiles.argtypes = [vp, i, i]
         L.hc_bench_fused.argtypes = [vp, d, i, ctypes.POINTER(d)]
         L.hc_loop_profile.argtypes = [vp, i]
         L.hc_loop_kernel_time.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i)]
@@ -334,13 +334,15 @@ class Context:
         return t.value, n.value
 
     def tune_tiles(self, dt: float):
+        """(rows, strip, us) of the fastest fused-kernel tiling, now in use."""
         r = ctypes.c_int()
+        st = ctypes.c_int()
         us = ctypes.c_double()
-        self._check(self.L.hc_tune_tiles(self.h, dt, ctypes.byref(r), ctypes.byref(us)))
-        return r.value, us.value
+        self._check(self.L.hc_tune_tiles(self.h, dt, ctypes.byref(r), ctypes.byref(st), ctypes.byref(us)))
+        return r.value, st.value, us.value
 
-    def set_tiles(self, rows: int) -> None:
-        self._check(self.L.hc_set_tiles(self.h, rows))
+    def set_tiles(self, rows: int, strip: int = 1) -> None:
+        self._check(self.L.hc_set_tiles(self.h, rows, strip))
 
     def bench_fused(self, dt: float, reps: int = 20) -> float:
         us = ctypes.c_double()

@@ -237,11 +237,13 @@ int hc_loop_sync(hc_ctx* ctx, hc_loop_state* state, hc_loop_record* records,
                  int max_records, int* nrecords);
 
 /* Tile tuner: times the fused flow_route+depth_update kernel for each
- * supported rows-per-wavefront (4, 8, 16) on the context's current state,
- * which it leaves unchanged, and keeps the fastest for this context.  Call
- * before hc_loop_run captures a graph.  hc_set_tiles fixes the choice. */
-int hc_tune_tiles(hc_ctx* ctx, double dt_flow, int* best_rows, double* best_us);
-int hc_set_tiles(hc_ctx* ctx, int rows);
+ * supported sub-tile height (4, 8, 16 rows held in registers) and strip
+ * length (1, 2, 4, 8 sub-tiles walked by one block) on the context's current
+ * state, which it leaves unchanged, and keeps the fastest for this context.
+ * Call before hc_loop_run captures a graph.  hc_set_tiles fixes the choice. */
+int hc_tune_tiles(hc_ctx* ctx, double dt_flow, int* best_rows, int* best_strip,
+                  double* best_us);
+int hc_set_tiles(hc_ctx* ctx, int rows, int strip);
 /* Average duration in microseconds of `reps` fused launches on the current
  * state (which is left unchanged); for the tile tuner and benchmarks. */
 int hc_bench_fused(hc_ctx* ctx, double dt_flow, int reps, double* us);
