@@ -306,7 +306,8 @@ def main():
             t = json.load(f)
         # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs,
         # scaled by the same counters' reading of a plain copy of known size
-        traffic = t.get(f"{prec}_R{head['tile_rows']}_S{head['tile_strip']}_bytes_per_launch")
+        traffic = t.get(f"{prec}_R{head['tile_rows']}_S{head['tile_strip']}_bytes_per_launch",
+                        t.get(f"{prec}_bytes_per_launch"))
     roofline = {"bound": "hbm", "achieved": round(head["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(head["kernel_gbs"] / HBM_PEAK_GBS, 4),
                 "traffic": traffic,
