@@ -231,6 +231,54 @@ def run_adjoint(size: int = 512, steps: int = 2000, dt: float = 0.5, every: int 
     return out
 
 
+def accuracy_run(size: int, hours: int):
+    """The large single grid as specified: dry start, `hours` of rain, run to
+    the end in the FP32 form (FP64 elevations) and the FP64 form; per-field
+    errors at the end and the outlet-hydrograph error over the run."""
+    from hail_caesar_amd import hc, inputs
+    from hail_caesar_amd.model import Model
+    arrays, params = inputs.grid4096_arrays(size=size, wet_depth=None, hours=hours)
+    out = {"workload": f"{size}^2 at 2 m, dry start, {hours} h of 15-min Gamma rain", "fields": {}}
+    res = {}
+    pert = dict(arrays)
+    pert["dem"] = np.asarray(arrays["dem"], dtype=np.float32).astype(np.float64)
+    for prec, name, arr in ((hc.HC_FP64, "fp64", arrays), (hc.HC_FP32, "fp32", arrays),
+                            (hc.HC_FP64, "fp64_floor", pert)):
+        m = Model(None, None, precision=prec, arrays=arr, params=params, device_loop=True,
+                  use_graph=True, flat_dz=-1.0 if prec == hc.HC_FP32 else 0.0)
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            tt, cy, dt = m.iterate(20000)
+            n += len(tt)
+            print(f"[accuracy] {name}: {n} steps, {m.scalar('cycle') / 60:.2f} h", file=sys.stderr,
+                  flush=True)
+            if m.finished:
+                break
+        res[name] = {"model": m, "series": m.series(), "steps": n,
+                     "wall_s": time.perf_counter() - t0}
+    a, b, fl = res["fp32"], res["fp64"], res["fp64_floor"]
+    out["fp64_floor"] = {"what": "the FP64 form from the DEM rounded to float32, against the FP64 form",
+                         "fields": {}}
+    for f in ("water_depth", "qx", "qy"):
+        ref = b["model"].download(f)
+        out["fields"][f] = _field_errors(a["model"].download(f), ref)
+        out["fp64_floor"]["fields"][f] = _field_errors(fl["model"].download(f), ref)
+
+    def hyd(x, y):
+        k = min(len(x), len(y))
+        return {"rows": int(k), "max_rel": float(np.abs(x[:k] - y[:k]).max() / np.abs(y[:k]).max()),
+                "l2_rel": float(np.sqrt(((x[:k] - y[:k]) ** 2).sum() / (y[:k] ** 2).sum())),
+                "peak": float(x[:k].max()), "peak_fp64": float(y[:k].max())}
+    out["hydrograph"] = hyd(a["series"][:, 1], b["series"][:, 1])
+    out["fp64_floor"]["hydrograph"] = hyd(fl["series"][:, 1], b["series"][:, 1])
+    out["steps"] = {"fp32": a["steps"], "fp64": b["steps"]}
+    out["wall_s"] = {"fp32": round(a["wall_s"], 1), "fp64": round(b["wall_s"], 1)}
+    vol = [float(r["model"].download("water_depth")[1:-1, 1:-1].sum()) for r in (a, b)]
+    out["volume_rel"] = abs(vol[0] - vol[1]) / max(vol[1], 1e-300)
+    return out
+
+
 def cpu_baseline(arrays, params, budget_s: float = 20.0):
     """The oracle (FP64 CPU restatement, OpenMP) on the same 4096^2 state:
     timed over whole loop bodies after the active mask is built."""
@@ -273,12 +321,18 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--accuracy-hours", type=int, default=0, metavar="H",
+                    help="instead of the benchmark, run the large grid from dry for H hours in "
+                         "FP32 and FP64 and report their differences")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU row-strip decomposition is a design, not a build).", file=sys.stderr)
         sys.exit(2)
     load_package()
+    if args.accuracy_hours:
+        print(json.dumps({"accuracy": accuracy_run(args.size, args.accuracy_hours)}))
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
     from hail_caesar_amd import inputs
@@ -321,7 +375,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(head["ms_per_step"], 5), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
-        "config": {"workload": f"{args.size}^2 grid at 2 m, fractal DEM (30 m) + carved channel, "
+        "config": {"workload": f"{args.size}^2 grid at 2 m, fractal DEM (30 m) + D8-carved drainage network, "
                                "log-normal Manning n, 15-min Gamma rain, flow only, fully wet "
                                "(0.2 m initial depth); one step = one loop body, device-resident",
                    "cells": cells, "active_fraction": round(head["active_fraction"], 4)},

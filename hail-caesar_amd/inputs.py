@@ -87,6 +87,108 @@ def carve_channel(z: np.ndarray, depth: float, width: int = 3) -> np.ndarray:
     return z
 
 
+def carve_d8_network(z: np.ndarray, block: int = 64, min_acc: int = 6, depth: float = 3.0,
+                     bed_drop: float = 0.002) -> np.ndarray:
+    """Carve a D8-consistent drainage network draining to the south edge.
+
+    A coarse grid of ``block`` x ``block`` cell means is priority-flooded from
+    its south row, giving every coarse cell a D8 receiver; coarse cells with a
+    flow accumulation of at least ``min_acc`` form the network.  Each network
+    link is rasterised as a D8 path between block centres (the outlets run
+    straight to the south edge) and given a bed that falls strictly
+    downstream: at least ``bed_drop`` metres per cell and at least ``depth``
+    below the lowest terrain the link crosses.  Path cells take the bed
+    elevation and their 8-neighbours are lowered to at most bed + depth/2, so
+    every path cell has a strictly lower D8 neighbour on the network, down to
+    the south edge.  NODATA cells are left alone.  Returns (z, channel mask)."""
+    import heapq
+    z = z.copy()
+    nr, nc = z.shape
+    gr, gc = max(1, nr // block), max(1, nc // block)
+    valid = z > NODATA
+    zf = np.where(valid, z, np.nan)
+    zc = np.nanmean(zf[:gr * block, :gc * block].reshape(gr, block, gc, block), axis=(1, 3))
+    zc = np.where(np.isnan(zc), np.nanmax(zc), zc)
+    recv = -np.ones(gr * gc, dtype=np.int64)
+    seen = np.zeros(gr * gc, dtype=bool)
+    heap = []
+    for j in range(gc):  # the south row drains off the grid
+        i = (gr - 1) * gc + j
+        seen[i] = True
+        heapq.heappush(heap, (float(zc[gr - 1, j]), i))
+    order = []
+    while heap:
+        e, i = heapq.heappop(heap)
+        order.append(i)
+        r, c = divmod(i, gc)
+        for dr in (-1, 0, 1):
+            for dc in (-1, 0, 1):
+                rr, cc = r + dr, c + dc
+                if (dr or dc) and 0 <= rr < gr and 0 <= cc < gc:
+                    k = rr * gc + cc
+                    if not seen[k]:
+                        seen[k] = True
+                        recv[k] = i
+                        heapq.heappush(heap, (max(float(zc[rr, cc]), e), k))
+    acc = np.ones(gr * gc)
+    for i in reversed(order):
+        if recv[i] >= 0:
+            acc[recv[i]] += acc[i]
+    centre = lambda i: (divmod(i, gc)[0] * block + block // 2, divmod(i, gc)[1] * block + block // 2)
+
+    def path(a, b):
+        (r, c), (r1, c1) = a, b
+        out = [(r, c)]
+        while (r, c) != (r1, c1):
+            r += int(np.sign(r1 - r))
+            c += int(np.sign(c1 - c))
+            out.append((r, c))
+        return out
+
+    bed = {}
+    links = []
+    for i in order:  # outlet first: a link's receiver has its bed already
+        if acc[i] < min_acc:
+            continue
+        a = centre(i)
+        if recv[i] < 0:
+            pts = path(a, (nr - 1, a[1]))
+            end = None
+        else:
+            pts = path(a, centre(recv[i]))
+            end = bed.get(int(recv[i]))
+        terr = [z[p] for p in pts if valid[p]]
+        low = (min(terr) if terr else 0.0) - depth
+        n = len(pts) - 1
+        b_end = end if end is not None else low
+        b_top = max(low, b_end + bed_drop * max(n, 1))
+        bed[int(i)] = b_top
+        if end is not None:
+            pts = pts[:-1]  # the receiver's own cell keeps its bed
+        links.append([(r, c, b_top + (b_end - b_top) * (k / max(n, 1))) for k, (r, c) in enumerate(pts)
+                      if valid[r, c]])
+    # banks first, then the beds: a cell on several links takes the lowest
+    # bed, so from every path cell some link continues strictly downhill
+    for pts in links:
+        for r, c, zb in pts:
+            for dr in (-1, 0, 1):
+                for dc in (-1, 0, 1):
+                    rr, cc = r + dr, c + dc
+                    if (dr or dc) and 0 <= rr < nr and 0 <= cc < nc and valid[rr, cc]:
+                        z[rr, cc] = min(z[rr, cc], max(zb + depth / 2, z[rr, cc] - depth))
+    for pts in links:
+        for r, c, zb in pts:
+            z[r, c] = zb
+    chan = np.zeros(z.shape, dtype=bool)
+    for pts in links:
+        for r, c, zb in pts:
+            chan[r, c] = True
+    for pts in links:
+        for r, c, zb in pts:
+            z[r, c] = min(z[r, c], zb)
+    return z, chan
+
+
 def planar_dem(nrows: int, ncols: int, dx: float, slope: float, amplitude: float,
                roughness: float, seed: int, channel_depth: float = 2.0,
                south_only: bool = True, base: float = 10.0) -> np.ndarray:
@@ -338,6 +440,15 @@ def lognormal_mannings(nrows: int, ncols: int, median: float, sigma: float,
     return median * np.exp(sigma * rng.standard_normal((nrows, ncols)))
 
 
+def grid4096_dem(size: int = 4096, seed: int = 1003) -> np.ndarray:
+    """The large single grid's DEM: 2 m cells, 0.2 % slope to the south edge,
+    fractal relief of 30 m (roughness 0.6) and a D8-consistent drainage
+    network carved 3 m deep (carve_d8_network on 64-cell blocks)."""
+    dem = planar_dem(size, size, 2.0, 0.002, 30.0, 0.6, seed, channel_depth=0.0)
+    block = max(8, min(64, size // 16))
+    return carve_d8_network(dem, block=block, depth=3.0)[0]
+
+
 def large_4096(path: str, size: int = 4096, hours: int = 12, seed: int = 1003,
                wet_depth: Optional[float] = None) -> Workload:
     """Large single grid: size^2 at 2 m, fractal amplitude 30 m with carved
@@ -345,7 +456,7 @@ def large_4096(path: str, size: int = 4096, hours: int = 12, seed: int = 1003,
     Gamma(0.8, mean 8 mm/h) 15-minute rain.  ``wet_depth`` adds a uniform
     initial depth (fully wet measurement state)."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    dem = planar_dem(size, size, 2.0, 0.002, 30.0, 0.6, seed, channel_depth=3.0)
+    dem = grid4096_dem(size, seed)
     n = lognormal_mannings(size, size, 0.035, 0.3, rng)
     rain = gamma_rain(hours, 15, 0.8, 8.0, rng)
     wi = None
@@ -369,27 +480,28 @@ def params_text_dict(overrides: Dict[str, str] = None, rain_step_min: float = 60
     return p
 
 
-def grid4096_arrays(size: int = 4096, seed: int = 1003, wet_depth: float = 0.2,
+def grid4096_arrays(size: int = 4096, seed: int = 1003, wet_depth: Optional[float] = 0.2,
                     hours: int = 12):
     """The large single-grid workload in memory: size^2 cells at 2 m, fractal
-    relief (30 m) on a 0.2 % slope to the south edge with a carved channel,
+    relief (30 m) on a 0.2 % slope to the south edge with a carved D8 network,
     log-normal Manning n (median 0.035, sigma 0.3), 12 h of 15-minute
-    Gamma(0.8, mean 8 mm/h) rain, flow only, and a uniform initial depth
-    (fully wet measurement state).  Input points use the reference's default
+    Gamma(0.8, mean 8 mm/h) rain, flow only.  With ``wet_depth`` every cell
+    starts at that uniform depth (fully wet measurement state); with None the
+    grid starts dry.  Input points use the reference's default
     min_q_for_depth_calc (0.01)."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    dem = planar_dem(size, size, 2.0, 0.002, 30.0, 0.6, seed, channel_depth=3.0)
+    dem = grid4096_dem(size, seed)
     n = lognormal_mannings(size, size, 0.035, 0.3, rng)
     rain = gamma_rain(hours, 15, 0.8, 8.0, rng)
     rain = np.vstack([rain, np.zeros((200, 1))]).astype(np.float32)
-    wi = np.where(dem > NODATA, wet_depth, 0.0)
-    params = params_text_dict({"spatially_variable_mannings_on": "yes",
-                               "spatial_mannings_dem_file": "mannings.asc",
-                               "water_init_from_raster_on": "yes",
-                               "water_init_raster_file": "water_init.asc",
-                               "min_q_for_depth_calc": "0.01"},
-                              rain_step_min=15, run_hours=hours)
-    arrays = {"dem": dem, "mannings": n, "water_init": wi, "rain": rain, "DX": 2.0}
+    over = {"spatially_variable_mannings_on": "yes", "spatial_mannings_dem_file": "mannings.asc",
+            "min_q_for_depth_calc": "0.01"}
+    arrays = {"dem": dem, "mannings": n, "rain": rain, "DX": 2.0}
+    if wet_depth is not None:
+        arrays["water_init"] = np.where(dem > NODATA, wet_depth, 0.0)
+        over.update({"water_init_from_raster_on": "yes",
+                     "water_init_raster_file": "water_init.asc"})
+    params = params_text_dict(over, rain_step_min=15, run_hours=hours)
     return arrays, params
 
 
