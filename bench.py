@@ -231,6 +231,39 @@ def run_adjoint(size: int = 512, steps: int = 2000, dt: float = 0.5, every: int 
     return out
 
 
+def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50):
+    """Flow + erosion (FP64, host-stepped as the reference's loop): size^2 at
+    10 m, fractal relief 20 m on a 1 % slope, 9 grain fractions (d = 0.0005
+    .. 0.128 m) in the reference's default surface proportions, Wilcock-Crowe,
+    1 m active layer, a 40 mm/h storm, from a uniform 0.2 m start so that
+    flow and entrainment are active everywhere.  Times whole loop bodies
+    (flow_route, depth_update, scan, erode at the reference's erosion
+    interval, water_flux_out, output)."""
+    from hail_caesar_amd import hc, inputs
+    from hail_caesar_amd.model import Model
+    dem = inputs.planar_dem(size, size, 10.0, 0.01, 20.0, 0.6, 1002)
+    arrays = {"dem": dem, "water_init": np.where(dem > inputs.NODATA, 0.2, 0.0),
+              "rain": np.vstack([inputs.triangular_storm(6, 40.0, 15)[:, None],
+                                 np.zeros((200, 1))]).astype(np.float32), "DX": 10.0}
+    params = inputs.params_text_dict({
+        "hydro_model_only": "no", "active_layer_thickness": "1", "transport_law": "wilcock",
+        "water_init_from_raster_on": "yes", "water_init_raster_file": "water_init.asc",
+        "hc_grain_sizes": ",".join(repr(d) for d in inputs.EROSION_GRAIN_D)},
+        rain_step_min=15, run_hours=24)
+    m = Model(None, None, precision=hc.HC_FP64, arrays=arrays, params=params)
+    m.iterate(warmup)
+    m.synchronize()
+    t = time.perf_counter()
+    m.iterate(steps)
+    m.synchronize()
+    wall = time.perf_counter() - t
+    cells = size * size
+    return {"workload": f"{size}^2 at 10 m, 9 fractions, Wilcock-Crowe, wet start", "dtype": "f64",
+            "steps": steps, "value": round(cells * steps / wall, 1), "unit": "cell-steps/s",
+            "ms_per_step": round(1000 * wall / steps, 4),
+            "sim_minutes": round(m.scalar("cycle"), 3)}
+
+
 def accuracy_run(size: int, hours: int):
     """The large single grid as specified: dry start, `hours` of rain, run to
     the end in the FP32 form (FP64 elevations) and the FP64 form; per-field
@@ -319,6 +352,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel from the host")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-catchment ensemble")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-erosion", action="store_true", help="skip the flow + erosion timing")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--accuracy-hours", type=int, default=0, metavar="H",
@@ -395,6 +429,10 @@ def main():
         print(f"[bench] ensemble: {out['ensemble']['value']:.4e} cell-steps/s, "
               f"{out['ensemble']['ms_per_step']:.4f} ms/step, fused {out['ensemble']['fused_kernel_us']} us",
               file=sys.stderr, flush=True)
+    if not args.no_erosion and args.only is None:
+        out["erosion"] = run_erosion()
+        print(f"[bench] erosion: {out['erosion']['value']:.4e} cell-steps/s, "
+              f"{out['erosion']['ms_per_step']} ms/step", file=sys.stderr, flush=True)
     if not args.no_adjoint and args.only is None:
         out["adjoint"] = run_adjoint()
         print(f"[bench] adjoint: forward {out['adjoint']['forward_ms_per_step']} ms/step, reverse "

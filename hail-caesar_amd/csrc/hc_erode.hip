@@ -368,7 +368,7 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
                                                 const double* __restrict__ sdir, Grain gr,
                                                 double* __restrict__ etot,
                                                 double* __restrict__ etot3) {
-  const int y = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = 2 + blockIdx.x * blockDim.x + threadIdx.x;  // one column per wavefront
   if (y >= a.jmax) return;
   const size_t P = a.pitch, nc = gr.nc;
   auto S = [&](int d, int n, size_t cell) { return sdir[(d * 10 + n) * nc + cell]; };
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
                                                   const uint64_t* __restrict__ mask,
                                                   const double* __restrict__ Tau, Grain gr,
                                                   const double* __restrict__ etot3) {
-  const int x = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // one row per wavefront
   if (x > a.imax) return;
   const size_t P = a.pitch;
   for (int y = 2; y < a.jmax; ++y) {
@@ -554,14 +554,16 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
   HC_HIP(c, hipMemsetAsync(c->etot, 0, c->ncell * sizeof(double), c->stream));
   HC_HIP(c, hipMemsetAsync(c->etot3, 0, c->ncell * sizeof(double), c->stream));
   const int ncol = std::max(0, c->g.jmax - 2);
+  // The deposition sweep runs each column top to bottom and the y-slide each
+  // row left to right (the reference's in-place order): one wavefront per
+  // column / row, so lanes never wait on another column's branches.
   if (ncol > 0) {
-    hipLaunchKernelGGL(k_deposit, dim3((ncol + 63) / 64), dim3(64), 0, c->stream, a, ts, mult, h,
-                       elev, c->bedrock, c->mask, c->V, c->Tau, c->ss, c->sdir, gr, c->etot,
-                       c->etot3);
+    hipLaunchKernelGGL(k_deposit, dim3(ncol), dim3(1), 0, c->stream, a, ts, mult, h, elev,
+                       c->bedrock, c->mask, c->V, c->Tau, c->ss, c->sdir, gr, c->etot, c->etot3);
     HC_HIP(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_lateral_y, dim3((c->g.imax + 63) / 64), dim3(64), 0, c->stream, a, ts, mult,
-                     h, elev, c->bedrock, c->mask, c->Tau, gr, c->etot3);
+  hipLaunchKernelGGL(k_lateral_y, dim3(c->g.imax), dim3(1), 0, c->stream, a, ts, mult, h, elev,
+                     c->bedrock, c->mask, c->Tau, gr, c->etot3);
   HC_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(k_edges, dim3(1), dim3(64), 0, c->stream, a, h, c->V, c->sdir, c->ncell,
                      c->gtot);
