@@ -268,18 +268,41 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   T hh[R + 2];
   Z zt[R + 2];
   T qxo[R + 2], qyo[R + 2], nn[R + 2];
-  uint32_t mbits = 0;
+  uint32_t mbits = 0, pbits = 0;
   // rain input at input points: one amount for the whole grid when every
   // cell has the same rain cell (added with a select, no branch)
   const bool inject = p.inj.pmask != nullptr;
   const double amt_u = (inject && p.inj.rf_uniform >= 0) ? p.inj.amt[p.inj.rf_uniform] : 0.0;
-  // The wavefront's 64 columns touch at most two words of each bit mask
-  // (active cells, input points): both are fetched once per row with
-  // wave-uniform (scalar) loads and each lane picks its bit.
-  const int w0 = (62 * __builtin_amdgcn_readfirstlane(w)) >> 6;
-  const bool two_words = (w0 + 1) < a.mwords;
-  const bool hi_word = (cl >> 6) != w0;
-  const int bit = cl & 63;
+  // Bit masks (active cells, input points).  The wavefront's 64 columns
+  // span at most two 64-bit words of a row.  One load per sub-tile: lane k
+  // (< R+2) fetches both words of the active mask in row r0-1+k, lane 32+k
+  // those of the input-point mask, and funnels out the 64 bits under the
+  // wavefront (bit l = column 62w+l; columns past the last word read 0).
+  // Each row's bits are then broadcast with v_readlane.  Per-lane loads of
+  // the words cost a vector load per row and mask; holding the words of
+  // every row in scalar registers spills them.
+  static_assert(R + 2 <= 32, "mask rows are fetched by lanes 0..31 and 32..63");
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  const int w0 = (62 * wu) >> 6;
+  const int off = 62 * wu - 64 * w0;
+  auto load_win = [&](int r0) -> uint64_t {
+    const int k = lane & 31;
+    const bool pm = lane >= 32;
+    const int x = r0 - 1 + k;
+    uint64_t m0 = 0, m1 = 0;
+    if (k < R + 2 && x >= 0 && x < NR && (!pm || inject)) {
+      const uint64_t* base = pm ? p.inj.pmask : mask;
+      const size_t ro = static_cast<size_t>(x) * a.mwords + w0;
+      m0 = base[ro];
+      if (w0 + 1 < a.mwords) m1 = base[ro + 1];
+    }
+    return off ? ((m0 >> off) | (m1 << (64 - off))) : m0;
+  };
+  auto bit_of = [&](uint64_t win, int src) -> uint32_t {
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(win), src);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(win >> 32), src);
+    return ((lane < 32 ? lo : hi) >> (lane & 31)) & 1u;
+  };
   const T nu = static_cast<T>(a.mannings);
   T md = 0;
   // water_flux_out (LCM.cpp:3005-3047) on this tile's edge cells, after
@@ -289,6 +312,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   double dacc = 0;
   const T dt = static_cast<T>(dtd);
   const float invdt = 1.0f / static_cast<float>(dtd);
+  const float dtDX = static_cast<float>(dtd / a.DX);  // FP32 form's depth-update factor
   const float fthr = static_cast<float>(a.flat_dz);
   T qxNp = 0, qyNp = 0, qyRp = 0;  // new fluxes of the row above (carried)
 
@@ -298,32 +322,23 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     const bool v = (x >= 0 && x < NR);
     const size_t o = static_cast<size_t>(v ? x : 0) * P + cl;
     hh[k] = v ? h0[o] : T(0);
-    if (inject && v) {
-      const size_t ro = static_cast<size_t>(x) * a.mwords + w0;
-      const uint64_t p0 = p.inj.pmask[ro];
-      const uint64_t p1 = two_words ? p.inj.pmask[ro + 1] : 0ull;
-      if ((p0 | p1) != 0ull) {  // uniform: most rows of most tiles hold no point
-        const bool pt = ((hi_word ? p1 : p0) >> bit) & 1ull;
-        if (p.inj.rf_uniform >= 0) {
-          const T hp = static_cast<T>(static_cast<double>(hh[k]) + amt_u);
-          hh[k] = pt ? hp : hh[k];
-        } else if (pt) {
-          hh[k] = static_cast<T>(static_cast<double>(hh[k]) + p.inj.amt[p.inj.rfarea[o]]);
-        }
-      }
-    }
     zt[k] = v ? zz[o] : Z(-9999);
     if (k >= 1) {
       qxo[k] = v ? qx0[o] : T(0);
       qyo[k] = v ? qy0[o] : T(0);
       nn[k] = (a.spatial_n && v) ? nman[o] : nu;
-      uint64_t m0 = 0ull, m1 = 0ull;
-      if (v) {
-        const size_t ro = static_cast<size_t>(x) * a.mwords + w0;
-        m0 = mask[ro];
-        m1 = two_words ? mask[ro + 1] : 0ull;
-      }
-      mbits |= static_cast<uint32_t>(((hi_word ? m1 : m0) >> bit) & 1ull) << k;
+    }
+  };
+
+  // rain input of row k at this lane's cell (LCM.cpp:3463-3477)
+  auto inject_row = [&](int k, int r0) {
+    const bool pt = (pbits >> k) & 1u;
+    if (p.inj.rf_uniform >= 0) {
+      const T hp = static_cast<T>(static_cast<double>(hh[k]) + amt_u);
+      hh[k] = pt ? hp : hh[k];
+    } else if (pt) {
+      const size_t o = static_cast<size_t>(r0 - 1 + k) * P + cl;
+      hh[k] = static_cast<T>(static_cast<double>(hh[k]) + p.inj.amt[p.inj.rfarea[o]]);
     }
   };
 
@@ -368,7 +383,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
         if constexpr (MODE == 0) {
           hn = hh[k - 1] + dtd * (qxN - qxNp + qyRp - qyNp) / a.DX;
         } else {
-          hn = hh[k - 1] + static_cast<float>(dtd / a.DX) * ((qxN - qxNp) + (qyRp - qyNp));
+          hn = hh[k - 1] + dtDX * ((qxN - qxNp) + (qyRp - qyNp));
         }
         if (hn > T(0)) {
           if (zt[k - 1] == Z(-9999)) hn = T(0);
@@ -398,8 +413,19 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   if (active_wave) {
     // first sub-tile: rows r0-1 .. r0+R loaded, faces of rows r0 .. r0+R
     int r0 = by * H;
+    uint64_t win = load_win(r0);
 #pragma unroll
     for (int k = 0; k < R + 2; ++k) load_row(k, r0);
+#pragma unroll
+    for (int k = 1; k < R + 2; ++k) mbits |= bit_of(win, k) << k;
+    if (inject) {
+#pragma unroll
+      for (int k = 0; k < R + 2; ++k) pbits |= bit_of(win, 32 + k) << k;
+    }
+    if (inject) {
+#pragma unroll
+      for (int k = 0; k < R + 2; ++k) inject_row(k, r0);
+    }
 #pragma unroll
     for (int k = 1; k < R + 2; ++k) row(k, r0);
     for (int t = 1; t < nsub; ++t) {
@@ -412,8 +438,20 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       zt[0] = zt[R];
       zt[1] = zt[R + 1];
       mbits = ((mbits >> (R + 1)) & 1u) << 1;
+      pbits = 0;  // the carried rows already hold their rain input
+      win = load_win(r0);
 #pragma unroll
       for (int k = 2; k < R + 2; ++k) load_row(k, r0);
+#pragma unroll
+      for (int k = 2; k < R + 2; ++k) mbits |= bit_of(win, k) << k;
+      if (inject) {
+#pragma unroll
+        for (int k = 2; k < R + 2; ++k) pbits |= bit_of(win, 32 + k) << k;
+      }
+      if (inject) {
+#pragma unroll
+        for (int k = 2; k < R + 2; ++k) inject_row(k, r0);
+      }
 #pragma unroll
       for (int k = 2; k < R + 2; ++k) row(k, r0);
     }
