@@ -14,9 +14,58 @@ namespace hcdev {
 // compiler use global_* accesses through it.
 template <typename T, typename F>
 __device__ __forceinline__ T* table_ptr(const F& field) {
+  static_assert(sizeof(F) == sizeof(uintptr_t), "a table field holds one pointer");
   using GP = __attribute__((address_space(1))) T*;
-  const GP g = *reinterpret_cast<const GP*>(&field);
-  return (T*)g;
+  const uintptr_t bits = *reinterpret_cast<const uintptr_t*>(&field);
+  return (T*)(GP)bits;
+}
+
+// Element i of a field addressed by a 32-bit byte offset from its base, so
+// the access is one global instruction with a scalar base and a vector
+// offset (the launcher checks that every offset fits in 32 bits).
+template <typename T>
+__device__ __forceinline__ T ld_elem(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     i * static_cast<uint32_t>(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st_elem(T* base, uint32_t i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<uint32_t>(sizeof(T))) = v;
+}
+
+// A value materialised in a register at this point of the program (the
+// compiler may not recompute it at, or sink it into, each use after here).
+__device__ __forceinline__ float pin_value(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Lane shifts by one across the whole wavefront (DPP wave_shr:1 / wave_shl:1):
+// lane_up1 gives lane l the value of lane l-1 (lane 0 keeps its own),
+// lane_down1 the value of lane l+1 (lane 63 keeps its own).
+__device__ __forceinline__ int dpp_up1(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_down1(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float lane_up1(float v) {
+  return __int_as_float(dpp_up1(__float_as_int(v)));
+}
+__device__ __forceinline__ float lane_down1(float v) {
+  return __int_as_float(dpp_down1(__float_as_int(v)));
+}
+__device__ __forceinline__ double lane_up1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_up1(static_cast<int>(b));
+  const int hi = dpp_up1(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<uint32_t>(lo));
+}
+__device__ __forceinline__ double lane_down1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_down1(static_cast<int>(b));
+  const int hi = dpp_down1(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<uint32_t>(lo));
 }
 
 __device__ __forceinline__ double dmax_ref(double a, double b) {

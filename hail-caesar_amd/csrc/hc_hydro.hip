@@ -76,31 +76,35 @@ __device__ __forceinline__ void face64(double& q, double& qs, double hc, double 
   }
 }
 
-// One face of the FP32 form.  dz = z_nb - z_cell.  Same branch structure as
-// face64; limiter tests are written as products instead of quotients.
-__device__ __forceinline__ float face32(float q, float hc, float hn, float dz, float n, float dt,
-                                        float invdt, const HydroArgs& a, int edge) {
-  const float hflow = (dz >= 0.0f) ? fmaxf(hc - dz, hn) : fmaxf(hc, hn + dz);
-  if (!(hflow > static_cast<float>(a.hthr))) return 0.0f;
-  const float DX = static_cast<float>(a.DX);
-  const float invDX = 1.0f / DX;
-  float slope = (dz + (hn - hc)) * invDX;
-  if (edge > 0) slope = static_cast<float>(a.edgeslope);
-  if (edge < 0) slope = -static_cast<float>(a.edgeslope);
-  const float gh = static_cast<float>(kGravity) * hflow;
-  const float ghdt = gh * dt;
-  const float h3 = hflow * hflow * hflow;
-  const float num = q - ghdt * slope;
-  const float den = 1.0f + ghdt * (n * n) * fabsf(q) * __builtin_amdgcn_rcpf(h3);
-  float qv = num * __builtin_amdgcn_rcpf(den);
-  const float qF = hflow * (__builtin_amdgcn_sqrtf(gh) * static_cast<float>(a.froude));
-  if (qv > qF) qv = qF;
-  if (qv < -qF) qv = -qF;
-  const float k4 = 0.25f * DX * invdt;   // q*dt/DX > h/4  <=>  q > h*DX/(4 dt)
-  const float k5 = 0.2f * DX * invdt;
-  if (qv > 0.0f && qv > hc * k4) qv = hc * k5;
-  if (qv < 0.0f && -qv > hn * k4) qv = -(hn * k5);
-  return qv;
+// One face of the FP32 form, branch-free: every lane evaluates it and the
+// caller selects.  dz = z_nb - z_cell.  The friction denominator is cleared
+// of 1/h^3,
+//   (q - g h dt S) / (1 + g h dt n^2 |q| / h^3)
+//     = (q - g h dt S) h^3 / (h^3 + g h dt n^2 |q|),
+// so one reciprocal serves; the Froude cap is a clamp to [-qF, qF]
+// (qF = h sqrt(g h) Fr > 0 wherever the result is kept) and the depth limiter
+// tests are products instead of quotients.  0 where hflow <= the flow
+// threshold (LSDCatchmentModel.cpp:3094-3191).  The slope is taken as
+// slope * smul + sadd: (1, 0) inside, (0, +-edgeslope) on the edge faces.
+struct Face32K {
+  float g, dt, invDX, froude, hthr, k4, k5;
+};
+
+__device__ __forceinline__ float face32(float q, float hc, float hn, float dz, float n2,
+                                        float smul, float sadd, const Face32K& K) {
+  // dz >= 0 ? max(hc - dz, hn) : max(hc, hn + dz)
+  const float hflow = fmaxf(hc - fmaxf(dz, 0.0f), hn + fminf(dz, 0.0f));
+  const float slope = __builtin_fmaf((dz + (hn - hc)) * K.invDX, smul, sadd);
+  const float gh = K.g * hflow;
+  const float ghdt = gh * K.dt;
+  const float h3 = (hflow * hflow) * hflow;
+  const float num = __builtin_fmaf(-ghdt, slope, q);
+  float qv = (num * h3) * __builtin_amdgcn_rcpf(__builtin_fmaf(ghdt * n2, fabsf(q), h3));
+  const float qF = hflow * (__builtin_amdgcn_sqrtf(gh) * K.froude);
+  qv = __builtin_amdgcn_fmed3f(qv, -qF, qF);
+  qv = (qv > 0.0f && qv > hc * K.k4) ? hc * K.k5 : qv;
+  qv = (qv < 0.0f && -qv > hn * K.k4) ? -(hn * K.k5) : qv;
+  return hflow > K.hthr ? qv : 0.0f;
 }
 
 __device__ __forceinline__ int edge_code(int i, int imaxlike) {
@@ -231,13 +235,19 @@ struct FusedPtrs {
 
 // One tile: a strip of nsub x R output rows by four wavefronts x 62 owned
 // columns.  A wavefront covers 64 columns and owns the middle 62 (lane 0 of
-// wavefront 0 also owns column 0); y-neighbours come from lane shuffles.  The
-// strip is walked in sub-tiles of R rows: the first loads rows r0-1 .. r0+R
-// of every field into registers up front, each later one loads only its R new
-// rows and carries the two rows above it and the new fluxes of its first row
-// from the sub-tile before (no halo rows are re-read and no face is computed
-// twice).  Every thread of the block calls this function (it ends in block
-// reductions).
+// wavefront 0 also owns column 0); y-neighbours come from DPP lane shifts.
+// (Wavefronts aligned to 64 columns need the halo columns from extra narrow
+// loads and a lane-batched east face per sub-tile; measured, that costs more
+// than the aligned rows save.)  The strip is walked in sub-tiles of R rows:
+// the first loads rows r0-1 .. r0+R of every field into registers up front,
+// each later one loads only its R new rows and carries the two rows above it
+// and the new fluxes of its first row from the sub-tile before (no halo rows
+// are re-read and no face is computed twice).  Rows past the grid's first
+// and last rows load the nearest grid row instead: their mask bits are 0, so
+// no face or depth of theirs is used or stored, and the loads need no
+// branch.  Loads and stores address a row by a 32-bit byte offset from the
+// field's base.  Every thread of the block calls this function (it ends in
+// block reductions).
 template <int MODE, int R>
 __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, int nbx, int nsub,
                                            double dtd, const FusedPtrs<MODE>& p,
@@ -256,33 +266,38 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   T* __restrict__ qy1 = p.qy1;
   unsigned long long* red = p.red;
   const int lane = threadIdx.x & 63;
-  const int w = bx * 4 + (threadIdx.x >> 6);
+  const int wu = __builtin_amdgcn_readfirstlane(bx * 4 + (threadIdx.x >> 6));
   const int NR = a.imax + 2, NC = a.jmax + 2;
   const int H = R * nsub;  // rows per block
-  const bool active_wave = (62 * w + 1 <= NC - 1);  // has an owned column
-  const int c = 62 * w + lane;
-  const bool owned = ((lane >= 1 && lane <= 62) || (w == 0 && lane == 0)) && c <= NC - 1;
-  const size_t P = a.pitch;
-  const int cl = min(c, a.pitch - 1);  // load column (lanes past the row reuse the last)
+  const bool active_wave = (62 * wu + 1 <= NC - 1);  // has an owned column
+  const int c = 62 * wu + lane;
+  const bool owned = ((lane >= 1 && lane <= 62) || (wu == 0 && lane == 0)) && c <= NC - 1;
+  const uint32_t P = static_cast<uint32_t>(a.pitch);
+  const uint32_t cl = static_cast<uint32_t>(min(c, a.pitch - 1));  // lanes past the row reuse the last column
 
   T hh[R + 2];
   Z zt[R + 2];
   T qxo[R + 2], qyo[R + 2], nn[R + 2];
-  uint32_t mbits = 0, pbits = 0;
+  uint32_t ob[R + 2];  // element offset of the lane's cell in row k
   // rain input at input points: one amount for the whole grid when every
   // cell has the same rain cell (added with a select, no branch)
   const bool inject = p.inj.pmask != nullptr;
   const double amt_u = (inject && p.inj.rf_uniform >= 0) ? p.inj.amt[p.inj.rf_uniform] : 0.0;
+  auto rain = [&](T v, bool pt, uint32_t o) -> T {  // LCM.cpp:3463-3477
+    if (p.inj.rf_uniform >= 0) {
+      const T hp = static_cast<T>(static_cast<double>(v) + amt_u);
+      return pt ? hp : v;
+    }
+    return pt ? static_cast<T>(static_cast<double>(v) + p.inj.amt[p.inj.rfarea[o]]) : v;
+  };
   // Bit masks (active cells, input points).  The wavefront's 64 columns
   // span at most two 64-bit words of a row.  One load per sub-tile: lane k
   // (< R+2) fetches both words of the active mask in row r0-1+k, lane 32+k
   // those of the input-point mask, and funnels out the 64 bits under the
   // wavefront (bit l = column 62w+l; columns past the last word read 0).
-  // Each row's bits are then broadcast with v_readlane.  Per-lane loads of
-  // the words cost a vector load per row and mask; holding the words of
-  // every row in scalar registers spills them.
+  // A row's 64 bits, read back with v_readlane, are then the lane mask of
+  // its predicate as they stand (inverse ballot): no per-lane bit work.
   static_assert(R + 2 <= 32, "mask rows are fetched by lanes 0..31 and 32..63");
-  const int wu = __builtin_amdgcn_readfirstlane(w);
   const int w0 = (62 * wu) >> 6;
   const int off = 62 * wu - 64 * w0;
   auto load_win = [&](int r0) -> uint64_t {
@@ -298,10 +313,10 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     }
     return off ? ((m0 >> off) | (m1 << (64 - off))) : m0;
   };
-  auto bit_of = [&](uint64_t win, int src) -> uint32_t {
+  auto row_pred = [&](uint64_t win, int src) -> bool {
     const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(win), src);
     const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(win >> 32), src);
-    return ((lane < 32 ? lo : hi) >> (lane & 31)) & 1u;
+    return __builtin_amdgcn_inverse_ballot_w64((static_cast<uint64_t>(hi) << 32) | lo);
   };
   const T nu = static_cast<T>(a.mannings);
   T md = 0;
@@ -310,85 +325,122 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   const bool drains = drain.ts != nullptr && fused_edge_tile(bx, by, H, a.imax, a.jmax);
   const double ts = drains ? *drain.ts : 1.0;
   double dacc = 0;
-  const T dt = static_cast<T>(dtd);
-  const float invdt = 1.0f / static_cast<float>(dtd);
-  const float dtDX = static_cast<float>(dtd / a.DX);  // FP32 form's depth-update factor
+  float dtDX = static_cast<float>(dtd / a.DX);  // FP32 form's depth-update factor
   const float fthr = static_cast<float>(a.flat_dz);
-  T qxNp = 0, qyNp = 0, qyRp = 0;  // new fluxes of the row above (carried)
+  Face32K K;
+  if constexpr (MODE != 0) {
+    const float DX = static_cast<float>(a.DX);
+    const float invdt = 1.0f / static_cast<float>(dtd);
+    K.g = static_cast<float>(kGravity);
+    K.dt = static_cast<float>(dtd);
+    K.invDX = 1.0f / DX;
+    K.froude = static_cast<float>(a.froude);
+    K.hthr = static_cast<float>(a.hthr);
+    K.k4 = 0.25f * DX * invdt;  // q*dt/DX > h/4  <=>  q > h*DX/(4 dt)
+    K.k5 = 0.2f * DX * invdt;
+    // computed once here: left to itself the compiler repeats the divisions
+    // at every use
+    K.dt = pin_value(K.dt);
+    K.invDX = pin_value(K.invDX);
+    K.k4 = pin_value(K.k4);
+    K.k5 = pin_value(K.k5);
+    dtDX = pin_value(dtDX);
+  }
+  const float es = static_cast<float>(a.edgeslope);
+  const int ycode = edge_code(c, a.jmax);  // y-faces: per column
+  // y-face slope = slope * ymul + yadd: the edge override without a lane mask
+  const float ymul = ycode != 0 ? 0.0f : 1.0f;
+  const float yadd = ycode < 0 ? -es : (ycode > 0 ? es : 0.0f);
 
   // load row k (x = r0 - 1 + k) of every field
   auto load_row = [&](int k, int r0) {
-    const int x = r0 - 1 + k;
-    const bool v = (x >= 0 && x < NR);
-    const size_t o = static_cast<size_t>(v ? x : 0) * P + cl;
-    hh[k] = v ? h0[o] : T(0);
-    zt[k] = v ? zz[o] : Z(-9999);
+    const int xc = min(max(r0 - 1 + k, 0), NR - 1);
+    ob[k] = static_cast<uint32_t>(xc) * P + cl;
+    hh[k] = ld_elem(h0, ob[k]);
+    zt[k] = ld_elem(zz, ob[k]);
     if (k >= 1) {
-      qxo[k] = v ? qx0[o] : T(0);
-      qyo[k] = v ? qy0[o] : T(0);
-      nn[k] = (a.spatial_n && v) ? nman[o] : nu;
+      qxo[k] = ld_elem(qx0, ob[k]);
+      qyo[k] = ld_elem(qy0, ob[k]);
+    }
+  };
+  auto load_n = [&](int k0) {
+    if (a.spatial_n) {
+#pragma unroll
+      for (int k = k0; k < R + 2; ++k) {
+        uint32_t o = ob[k];
+        asm volatile("" : "+v"(o));  // offset made in this block: scalar-base load
+        nn[k] = ld_elem(nman, o);
+      }
+    } else {
+#pragma unroll
+      for (int k = k0; k < R + 2; ++k) nn[k] = nu;
     }
   };
 
-  // rain input of row k at this lane's cell (LCM.cpp:3463-3477)
-  auto inject_row = [&](int k, int r0) {
-    const bool pt = (pbits >> k) & 1u;
-    if (p.inj.rf_uniform >= 0) {
-      const T hp = static_cast<T>(static_cast<double>(hh[k]) + amt_u);
-      hh[k] = pt ? hp : hh[k];
-    } else if (pt) {
-      const size_t o = static_cast<size_t>(r0 - 1 + k) * P + cl;
-      hh[k] = static_cast<T>(static_cast<double>(hh[k]) + p.inj.amt[p.inj.rfarea[o]]);
-    }
-  };
-
+  T qxNp = 0, qyNp = 0, qyRp = 0;  // new fluxes of the row above (carried)
+  bool mprev = false;                // active-mask predicate of the row above
   // faces of row k, and depth_update + store of row k-1 (k >= 2)
-  auto row = [&](int k, int r0) {
-    const int x = r0 - 1 + k;
-    const T hl = __shfl_up(hh[k], 1, 64);
-    const Z zl = __shfl_up(zt[k], 1, 64);
+  auto row = [&](uint64_t win, int k, int r0) {
+    const int xr = r0 - 1 + k;
+    const T hl = lane_up1(hh[k]);
+    const Z zl = lane_up1(zt[k]);
+    const bool m = row_pred(win, k);
     T qxN = qxo[k], qyN = qyo[k];
-    const bool m = (mbits >> k) & 1u;
-    if (m && zt[k] > Z(-9999)) {
-      if constexpr (MODE == 0) {
+    if constexpr (MODE == 0) {
+      if (m && zt[k] > Z(-9999)) {
         double dummy = 0;
         face64<false>(qxN, dummy, hh[k], hh[k - 1], zt[k], zt[k - 1], nn[k], dtd, a,
-                      edge_code(x, a.imax), true, 0, 0, nullptr, nullptr);
-        face64<false>(qyN, dummy, hh[k], hl, zt[k], zl, nn[k], dtd, a, edge_code(c, a.jmax), false,
-                      0, 0, nullptr, nullptr);
-      } else {
-        auto dz = [&](Z zn, Z zc, size_t cn, size_t cc) -> float {
-          if constexpr (MODE == 2) {
-            return static_cast<float>(zn - zc);
-          } else {
-            float d = zn - zc;
-            if (fthr > 0.0f && fabsf(d) < fthr) d = static_cast<float>(z64[cn] - z64[cc]);
-            return d;
-          }
-        };
-        const size_t oc = static_cast<size_t>(x) * P + cl;
-        if ((hh[k] > 0.0f || hh[k - 1] > 0.0f) && zt[k - 1] > Z(-9999))
-          qxN = face32(qxo[k], hh[k], hh[k - 1], dz(zt[k - 1], zt[k], oc - P, oc), nn[k], dt, invdt,
-                       a, edge_code(x, a.imax));
-        if ((hh[k] > 0.0f || hl > 0.0f) && zl > Z(-9999))
-          qyN = face32(qyo[k], hh[k], hl, dz(zl, zt[k], oc - 1, oc), nn[k], dt, invdt, a,
-                       edge_code(c, a.jmax));
+                      edge_code(xr, a.imax), true, 0, 0, nullptr, nullptr);
+        face64<false>(qyN, dummy, hh[k], hl, zt[k], zl, nn[k], dtd, a, ycode, false, 0, 0,
+                      nullptr, nullptr);
       }
+    } else {
+      // both faces on every lane, then a select: in a wet grid nearly every
+      // lane takes them, and branches would only add exec-mask work
+      float dzx, dzy;
+      if constexpr (MODE == 2) {
+        dzx = static_cast<float>(zt[k - 1] - zt[k]);
+        dzy = static_cast<float>(zl - zt[k]);
+      } else {
+        dzx = zt[k - 1] - zt[k];
+        dzy = zl - zt[k];
+        if (fthr > 0.0f && m) {  // near-flat faces take dz from the FP64 DEM
+          if (fabsf(dzx) < fthr) dzx = static_cast<float>(z64[ob[k - 1]] - z64[ob[k]]);
+          if (fabsf(dzy) < fthr) dzy = static_cast<float>(z64[ob[k] - 1] - z64[ob[k]]);
+        }
+      }
+      const float n2 = nn[k] * nn[k];
+      const int xcode = edge_code(xr, a.imax);  // uniform
+      float fx = face32(qxo[k], hh[k], hh[k - 1], dzx, n2, xcode != 0 ? 0.0f : 1.0f,
+                        xcode < 0 ? -es : (xcode > 0 ? es : 0.0f), K);
+      float fy = face32(qyo[k], hh[k], hl, dzy, n2, ymul, yadd, K);
+      // computed here on every lane: left to itself the compiler moves them
+      // under the selects below as exec-masked branches
+      asm volatile("" : "+v"(fx), "+v"(fy));
+      const bool mc = m && zt[k] > Z(-9999);
+      const bool wc = hh[k] > 0.0f;
+      qxN = (mc && (wc || hh[k - 1] > 0.0f) && zt[k - 1] > Z(-9999)) ? fx : qxo[k];
+      qyN = (mc && (wc || hl > 0.0f) && zl > Z(-9999)) ? fy : qyo[k];
     }
-    const T qyR = __shfl_down(qyN, 1, 64);
-    if (k >= 2) {  // row x-1 has both of its x-faces and y-faces now
-      const int xo = x - 1;
+    const T qyR = lane_down1(qyN);
+    if (k >= 2) {  // row k-1 has both of its x-faces and y-faces now
+      const int xo = xr - 1;
+      const bool mu = mprev;
       T hn = hh[k - 1];
-      if ((mbits >> (k - 1)) & 1u) {
-        if constexpr (MODE == 0) {
+      if constexpr (MODE == 0) {
+        if (mu) {
           hn = hh[k - 1] + dtd * (qxN - qxNp + qyRp - qyNp) / a.DX;
-        } else {
-          hn = hh[k - 1] + dtDX * ((qxN - qxNp) + (qyRp - qyNp));
+          if (hn > T(0)) {
+            if (zt[k - 1] == Z(-9999)) hn = T(0);
+            if (hn > md) md = hn;
+          }
         }
-        if (hn > T(0)) {
-          if (zt[k - 1] == Z(-9999)) hn = T(0);
-          if (hn > md) md = hn;
-        }
+      } else {
+        const float t = __builtin_fmaf(dtDX, (qxN - qxNp) + (qyRp - qyNp), hh[k - 1]);
+        const bool pos = t > 0.0f;
+        const bool zbad = zt[k - 1] == Z(-9999);
+        hn = mu ? ((pos && zbad) ? 0.0f : t) : hh[k - 1];
+        md = fmaxf(md, (mu && pos && !zbad) ? t : 0.0f);
       }
       if (drains && owned && xo >= 1 && xo <= a.imax && c >= 1 && c <= a.jmax &&
           (xo == 1 || xo == a.imax || c == 1 || c == a.jmax)) {
@@ -399,15 +451,17 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
         }
       }
       if (owned && xo < NR) {
-        const size_t o = static_cast<size_t>(xo) * P + c;
-        h1[o] = hn;
-        qx1[o] = qxNp;
-        qy1[o] = qyNp;
+        uint32_t o = ob[k - 1];
+        asm volatile("" : "+v"(o));  // offset made in this block: scalar-base stores
+        st_elem(h1, o, hn);
+        st_elem(qx1, o, qxNp);
+        st_elem(qy1, o, qyNp);
       }
     }
     qxNp = qxN;
     qyNp = qyN;
     qyRp = qyR;
+    mprev = m;
   };
 
   if (active_wave) {
@@ -416,44 +470,34 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     uint64_t win = load_win(r0);
 #pragma unroll
     for (int k = 0; k < R + 2; ++k) load_row(k, r0);
-#pragma unroll
-    for (int k = 1; k < R + 2; ++k) mbits |= bit_of(win, k) << k;
+    load_n(1);
     if (inject) {
 #pragma unroll
-      for (int k = 0; k < R + 2; ++k) pbits |= bit_of(win, 32 + k) << k;
-    }
-    if (inject) {
-#pragma unroll
-      for (int k = 0; k < R + 2; ++k) inject_row(k, r0);
+      for (int k = 0; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), ob[k]);
     }
 #pragma unroll
-    for (int k = 1; k < R + 2; ++k) row(k, r0);
+    for (int k = 1; k < R + 2; ++k) row(win, k, r0);
     for (int t = 1; t < nsub; ++t) {
       r0 += R;
       if (r0 > NR) break;  // uniform: no row of this sub-tile is in the grid
-      // rows r0-1 and r0 were the last two rows of the sub-tile before, and
-      // the faces of row r0 are already computed (qxNp, qyNp, qyRp)
+      // rows r0-1 and r0 were the last two rows of the sub-tile before (their
+      // rain input included), and the faces of row r0 are already computed
+      // (qxNp, qyNp, qyRp)
       hh[0] = hh[R];
       hh[1] = hh[R + 1];
       zt[0] = zt[R];
       zt[1] = zt[R + 1];
-      mbits = ((mbits >> (R + 1)) & 1u) << 1;
-      pbits = 0;  // the carried rows already hold their rain input
+      ob[1] = ob[R + 1];
       win = load_win(r0);
 #pragma unroll
       for (int k = 2; k < R + 2; ++k) load_row(k, r0);
-#pragma unroll
-      for (int k = 2; k < R + 2; ++k) mbits |= bit_of(win, k) << k;
+      load_n(2);
       if (inject) {
 #pragma unroll
-        for (int k = 2; k < R + 2; ++k) pbits |= bit_of(win, 32 + k) << k;
-      }
-      if (inject) {
-#pragma unroll
-        for (int k = 2; k < R + 2; ++k) inject_row(k, r0);
+        for (int k = 2; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), ob[k]);
       }
 #pragma unroll
-      for (int k = 2; k < R + 2; ++k) row(k, r0);
+      for (int k = 2; k < R + 2; ++k) row(win, k, r0);
     }
   }
   // the halo lanes' depths are not this wavefront's results
@@ -689,6 +733,8 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
                          const DrainArgs* drain, const hc::InjectArgs* inject) {
   if (c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
+  if (!hc::fused_offsets_fit(c->NR, c->P))
+    return hc_fail(c, HC_ERR_ARG, "flow_depth: grid too large for the fused kernel's 32-bit offsets");
   const HydroArgs a = hydro_args(c);
   const int R = c->fused_rows;
   const int ns = c->fused_strip;
@@ -837,6 +883,8 @@ int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double*
                            const uint64_t* mask) {
   if (c->prec != HC_FP64 || c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "fused64: FP64 hydro-only contexts");
+  if (!hc::fused_offsets_fit(c->NR, c->P))
+    return hc_fail(c, HC_ERR_ARG, "fused64: grid too large for the fused kernel's 32-bit offsets");
   const HydroArgs a = hydro_args(c);
   const int R = c->fused_rows;
   const int ns = c->fused_strip;

@@ -115,6 +115,12 @@ struct EnsMember {
   int W;                // mask words per row
 };
 
+// The fused kernels address a field by 32-bit byte offsets (8-byte elements
+// at most).
+inline bool fused_offsets_fit(int NR, int P) {
+  return static_cast<uint64_t>(NR) * static_cast<uint64_t>(P) * 8ull <= 0xffffffffull;
+}
+
 // Fused-kernel tile (bx, by) holds an edge cell: row 1 or imax, column 1 or
 // jmax.  Tile rows by*R .. by*R+R-1; wavefront w owns columns 62w+1 .. 62w+62
 // (and column 0 for w = 0), four wavefronts per tile column bx.

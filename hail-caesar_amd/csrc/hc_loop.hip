@@ -737,7 +737,7 @@ int hc_ens_create(hc_ens** out, hc_ctx* const* members, int n) {
       delete e;
       return HC_ERR_STATE;
     }
-    if (i > 0 && ens_mode(c) != ens_mode(members[0])) {
+    if ((i > 0 && ens_mode(c) != ens_mode(members[0])) || !hc::fused_offsets_fit(c->NR, c->P)) {
       delete e;
       return HC_ERR_ARG;
     }
