@@ -46,10 +46,10 @@ def load_package():
 
 def bytes_per_cell_step(precision: str, spatial_n: bool = True) -> int:
     """Algorithmic bytes of one fused flux+depth step per cell: reads h, qx,
-    qy, z (and n), writes h, qx, qy.  The FP32 form keeps elevations in FP64
-    (exact elevation differences), so z is 8 bytes in both forms."""
+    qy, z (and n), writes h, qx, qy, each field 8 bytes in the FP64 form and 4
+    in the FP32 form (whose elevations are 32-bit fixed point)."""
     esz = 4 if precision == "fp32" else 8
-    return esz * (3 + 3 + (1 if spatial_n else 0)) + 8
+    return esz * (3 + 3 + 1 + (1 if spatial_n else 0))
 
 
 def copy_probe(mib: int):
@@ -74,7 +74,7 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
     prec = hc.HC_FP32 if precision == "fp32" else hc.HC_FP64
     t0 = time.time()
     m = Model(None, None, precision=prec, arrays=arrays, params=params, device_loop=True,
-              use_graph=use_graph, flat_dz=-1.0 if precision == "fp32" else 0.0)
+              use_graph=use_graph, elev_fixed=precision == "fp32")
     setup_s = time.time() - t0
     m.iterate(warmup)
     # the project's tile tuner picks rows per wavefront on the warmed state
@@ -148,14 +148,14 @@ def fp32_error(m32, m64, arrays, params, nsteps):
 def run_ensemble(steps: int, warmup: int, n: int = 64):
     """The batched ensemble: n independent catchments (side lengths
     log-uniform in [128, 1024] cells at 5 m, own DEM, n and rain), FP32 with
-    FP64 elevations, each with its own adaptive dt, stepped together from a
+    fixed-point elevations, each with its own adaptive dt, stepped together from a
     captured graph; started fully wet (0.1 m) so that every cell is active."""
     from hail_caesar_amd import hc, inputs
     from hail_caesar_amd.model import Ensemble, Model
     specs = inputs.ensemble_members(n, hours=6, wet_depth=0.1)
     cells = int(sum(a["dem"].shape[0] * a["dem"].shape[1] for a, _ in specs))
     ms = [Model(None, None, precision=hc.HC_FP32, arrays=a, params=p, device_loop=True,
-                flat_dz=-1.0) for a, p in specs]
+                elev_fixed=True) for a, p in specs]
     ens = Ensemble(ms)
     ens.iterate(warmup)
     rows, blocks, _ = ens.tune()
@@ -266,7 +266,7 @@ def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50):
 
 def accuracy_run(size: int, hours: int):
     """The large single grid as specified: dry start, `hours` of rain, run to
-    the end in the FP32 form (FP64 elevations) and the FP64 form; per-field
+    the end in the FP32 form (fixed-point elevations) and the FP64 form; per-field
     errors at the end and the outlet-hydrograph error over the run."""
     from hail_caesar_amd import hc, inputs
     from hail_caesar_amd.model import Model
@@ -278,7 +278,7 @@ def accuracy_run(size: int, hours: int):
     for prec, name, arr in ((hc.HC_FP64, "fp64", arrays), (hc.HC_FP32, "fp32", arrays),
                             (hc.HC_FP64, "fp64_floor", pert)):
         m = Model(None, None, precision=prec, arrays=arr, params=params, device_loop=True,
-                  use_graph=True, flat_dz=-1.0 if prec == hc.HC_FP32 else 0.0)
+                  use_graph=True, elev_fixed=prec == hc.HC_FP32)
         t0 = time.perf_counter()
         n = 0
         while True:

@@ -12,7 +12,7 @@
 int main(int argc, char* argv[]) {
   if (argc < 3) {
     std::cout << "usage: " << argv[0]
-              << " <path> <parameter file> [--fp32] [--separate] [--device-loop] [--graph]\n";
+              << " <path> <parameter file> [--fp32] [--fixed-elev] [--separate] [--device-loop] [--graph]\n";
     return 0;
   }
   hcm_options opt{};
@@ -23,11 +23,13 @@ int main(int argc, char* argv[]) {
   opt.flat_dz = 0.0;
   opt.device_loop = 0;
   opt.use_graph = 0;
+  opt.elev_fixed = 0;
   for (int a = 3; a < argc; ++a) {
     if (!std::strcmp(argv[a], "--fp32")) opt.precision = HC_FP32;
     else if (!std::strcmp(argv[a], "--device-loop")) opt.device_loop = 1;
     else if (!std::strcmp(argv[a], "--graph")) { opt.device_loop = 1; opt.use_graph = 1; }
     else if (!std::strcmp(argv[a], "--separate")) opt.fused = 0;
+    else if (!std::strcmp(argv[a], "--fixed-elev")) opt.elev_fixed = 1;
   }
   const auto t0 = std::chrono::steady_clock::now();
   hcm_model* m = nullptr;

@@ -193,6 +193,7 @@ class CatchmentModel {
     p.chann_lateral_erosion = chann_lateral_erosion_;
     p.veg_lat_restriction = veg_lat_restriction_;
     p.flat_dz = opt_.flat_dz;
+    p.elev_fixed = opt_.elev_fixed;
     check(hc_create(&ctx_, &g, &p), "hc_create");
   }
 
@@ -994,6 +995,7 @@ static hcm_options default_options(const hcm_options* opt) {
   o.flat_dz = 0.0;
   o.device_loop = 0;
   o.use_graph = 0;
+  o.elev_fixed = 0;
   if (opt) o = *opt;
   return o;
 }

@@ -1,6 +1,7 @@
 // C-ABI entry points of libhc.so (include/hc.h): context lifetime, host <->
 // device layout conversion, and the per-method calls of the hot path.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -62,6 +63,39 @@ int put_plane(hc_ctx* c, void* dev, bool dev_float, const double* host, int hstr
       HC_HIP(c, hipMemcpy(dev, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
     }
   }
+  return HC_OK;
+}
+
+// Fixed-point elevations: q = round((z - base) * 2^shift) with base the
+// middle of the valid range and the largest shift (<= 40) that keeps |q| <=
+// 2^30 - 1, so differences of two valid cells fit in an int32; no-data
+// (z <= -9999, which the reference's `> -9999` tests exclude) is hc::kZqNodata.
+int put_fixed_elev(hc_ctx* c, const double* host) {
+  double lo = 0.0, hi = 0.0;
+  bool any = false;
+  for (size_t i = 0; i < static_cast<size_t>(c->NR) * c->NC; ++i) {
+    const double z = host[i];
+    if (!(z > -9999.0)) continue;
+    if (!std::isfinite(z)) return hc_fail(c, HC_ERR_ARG, "upload: non-finite elevation");
+    lo = any ? std::min(lo, z) : z;
+    hi = any ? std::max(hi, z) : z;
+    any = true;
+  }
+  const double base = 0.5 * (lo + hi);
+  const double half = std::max(hi - base, base - lo);
+  int shift = 40;
+  while (shift > -60 && half * std::ldexp(1.0, shift) > 1073741823.0) --shift;
+  std::vector<int32_t> tmp(c->ncell, hc::kZqNodata);
+  for (int x = 0; x < c->NR; ++x)
+    for (int y = 0; y < c->NC; ++y) {
+      const double z = host[static_cast<size_t>(x) * c->NC + y];
+      if (z > -9999.0)
+        tmp[static_cast<size_t>(x) * c->P + y] =
+            static_cast<int32_t>(std::llrint((z - base) * std::ldexp(1.0, shift)));
+    }
+  HC_HIP(c, hipMemcpy(c->zq, tmp.data(), c->ncell * sizeof(int32_t), hipMemcpyHostToDevice));
+  c->zq_base = base;
+  c->zq_shift = shift;
   return HC_OK;
 }
 
@@ -177,6 +211,12 @@ int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
   } else {
     if ((rc = dalloc(c, reinterpret_cast<float**>(&c->z), n))) return fail(rc);
     if ((rc = dalloc(c, &c->z64, n))) return fail(rc);
+    if (params->elev_fixed) {
+      if ((rc = dalloc(c, &c->zq, n))) return fail(rc);
+      if (hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(c->zq), hc::kZqNodata, n, c->stream) !=
+          hipSuccess)
+        return fail(HC_ERR_HIP);
+    }
     if (params->spatially_var_mannings &&
         (rc = dalloc(c, reinterpret_cast<float**>(&c->nman), n)))
       return fail(rc);
@@ -209,7 +249,7 @@ int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
 int hc_destroy(hc_ctx* c) {
   if (!c) return HC_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->h[0], c->h[1], c->qx[0], c->qx[1], c->qy[0], c->qy[1], c->z, c->z64,
+  void* ptrs[] = {c->h[0], c->h[1], c->qx[0], c->qx[1], c->qy[0], c->qy[1], c->z, c->z64, c->zq,
                   c->nman, c->mask, c->V, c->qxs, c->qys, c->veldir, c->grain, c->strata,
                   c->hasgrain, c->needgrain, c->bedrock, c->Tau, c->Vel, c->ss, c->sdir,
                   c->etot, c->etot3, c->gtot, c->d_red, c->d_partials, c->d_count, c->d_out,
@@ -249,6 +289,7 @@ int hc_upload(hc_ctx* c, int field, const void* host, size_t count) {
     case HC_FIELD_ELEV: {
       int rc = put_plane(c, c->z, f32, d, 1, 0);
       if (rc == HC_OK && f32) rc = put_plane(c, c->z64, false, d, 1, 0);
+      if (rc == HC_OK && c->zq) rc = put_fixed_elev(c, d);
       return rc;
     }
     case HC_FIELD_MANNINGS:

@@ -55,6 +55,7 @@ __device__ __forceinline__ float lane_up1(float v) {
 __device__ __forceinline__ float lane_down1(float v) {
   return __int_as_float(dpp_down1(__float_as_int(v)));
 }
+__device__ __forceinline__ int32_t lane_up1(int32_t v) { return dpp_up1(v); }
 __device__ __forceinline__ double lane_up1(double v) {
   const long long b = __double_as_longlong(v);
   const int lo = dpp_up1(static_cast<int>(b));

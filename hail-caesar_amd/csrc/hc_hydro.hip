@@ -15,6 +15,7 @@
 //    slope = (dz + (h_nb - h_cell)) / DX
 // and faces with |dz| < flat_dz take dz from the FP64 DEM.
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "hc_device.hpp"
@@ -207,12 +208,33 @@ __global__ __launch_bounds__(256) void k_depth64(HydroArgs a, double dt, double*
 //
 // MODE 0: FP64 reference form.  MODE 1: FP32 state with FP32 elevations (and
 // the FP64 fallback for near-flat faces when flat_dz > 0).  MODE 2: FP32
-// state with FP64 elevations (exact face differences).
+// state with FP64 elevations (exact face differences).  MODE 3: FP32 state
+// with 32-bit fixed-point elevations (differences exact to one quantum).
 template <int MODE>
 struct FusedTypes {
   using T = typename std::conditional<MODE == 0, double, float>::type;
-  using Z = typename std::conditional<MODE == 1, float, double>::type;
+  using Z = typename std::conditional<
+      MODE == 3, int32_t, typename std::conditional<MODE == 1, float, double>::type>::type;
 };
+
+// Elevation predicates of the reference (`elev > -9999`, `elev == -9999`),
+// for every elevation type of the fused kernels.
+template <typename Z>
+__device__ __forceinline__ bool z_ok(Z z) {
+  return z > Z(-9999);
+}
+template <>
+__device__ __forceinline__ bool z_ok<int32_t>(int32_t z) {
+  return z != hc::kZqNodata;
+}
+template <typename Z>
+__device__ __forceinline__ bool z_nodata(Z z) {
+  return z == Z(-9999);
+}
+template <>
+__device__ __forceinline__ bool z_nodata<int32_t>(int32_t z) {
+  return z == hc::kZqNodata;
+}
 
 // Device pointers of one fused launch (one generation in, the other out).
 template <int MODE>
@@ -401,6 +423,14 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       if constexpr (MODE == 2) {
         dzx = static_cast<float>(zt[k - 1] - zt[k]);
         dzy = static_cast<float>(zl - zt[k]);
+      } else if constexpr (MODE == 3) {
+        // exact integer differences (wrapping only where a no-data cell is
+        // involved, whose result is discarded), scaled by a power of two
+        auto qd = [](int32_t zn, int32_t zc) {
+          return static_cast<int32_t>(static_cast<uint32_t>(zn) - static_cast<uint32_t>(zc));
+        };
+        dzx = static_cast<float>(qd(zt[k - 1], zt[k])) * a.zq_inv;
+        dzy = static_cast<float>(qd(zl, zt[k])) * a.zq_inv;
       } else {
         dzx = zt[k - 1] - zt[k];
         dzy = zl - zt[k];
@@ -417,10 +447,10 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       // computed here on every lane: left to itself the compiler moves them
       // under the selects below as exec-masked branches
       asm volatile("" : "+v"(fx), "+v"(fy));
-      const bool mc = m && zt[k] > Z(-9999);
+      const bool mc = m && z_ok(zt[k]);
       const bool wc = hh[k] > 0.0f;
-      qxN = (mc && (wc || hh[k - 1] > 0.0f) && zt[k - 1] > Z(-9999)) ? fx : qxo[k];
-      qyN = (mc && (wc || hl > 0.0f) && zl > Z(-9999)) ? fy : qyo[k];
+      qxN = (mc && (wc || hh[k - 1] > 0.0f) && z_ok(zt[k - 1])) ? fx : qxo[k];
+      qyN = (mc && (wc || hl > 0.0f) && z_ok(zl)) ? fy : qyo[k];
     }
     const T qyR = lane_down1(qyN);
     if (k >= 2) {  // row k-1 has both of its x-faces and y-faces now
@@ -438,7 +468,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       } else {
         const float t = __builtin_fmaf(dtDX, (qxN - qxNp) + (qyRp - qyNp), hh[k - 1]);
         const bool pos = t > 0.0f;
-        const bool zbad = zt[k - 1] == Z(-9999);
+        const bool zbad = z_nodata(zt[k - 1]);
         hn = mu ? ((pos && zbad) ? 0.0f : t) : hh[k - 1];
         md = fmaxf(md, (mu && pos && !zbad) ? t : 0.0f);
       }
@@ -645,6 +675,7 @@ HydroArgs hydro_args(const hc_ctx* c) {
   a.mannings = c->prm.mannings;
   a.spatial_n = c->prm.spatially_var_mannings;
   a.flat_dz = c->prm.flat_dz;
+  a.zq_inv = std::ldexp(1.0f, -c->zq_shift);
   return a;
 }
 
@@ -699,6 +730,7 @@ int hc_launch_depth_update(hc_ctx* c, double dt) {
 hc::HydroArgs hc_hydro_args(const hc_ctx* c) { return hydro_args(c); }
 
 const void* hc_fused_z(const hc_ctx* c) {
+  if (c->prec == HC_FP32 && c->zq) return c->zq;
   return (c->prec == HC_FP32 && c->prm.flat_dz < 0) ? static_cast<const void*>(c->z64) : c->z;
 }
 
@@ -715,6 +747,10 @@ int hc_launch_ens_fused(hipStream_t st, int mode, int R, const EnsMember* mem, c
     if (R == 4) go(k_ens_fused<2, 4>);
     else if (R == 16) go(k_ens_fused<2, 16>);
     else go(k_ens_fused<2, 8>);
+  } else if (mode == 3) {
+    if (R == 4) go(k_ens_fused<3, 4>);
+    else if (R == 16) go(k_ens_fused<3, 16>);
+    else go(k_ens_fused<3, 8>);
   } else {
     if (R == 4) go(k_ens_fused<1, 4>);
     else if (R == 16) go(k_ens_fused<1, 16>);
@@ -751,10 +787,10 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
   auto go = [&](auto kern, auto* tag_t, auto* tag_z) {
     using T = std::remove_pointer_t<decltype(tag_t)>;
     using Z = std::remove_pointer_t<decltype(tag_z)>;
-    const Z* zp = std::is_same<Z, double>::value && c->prec == HC_FP32
-                      ? reinterpret_cast<const Z*>(c->z64)
-                      : reinterpret_cast<const Z*>(c->z);
-    constexpr int M = std::is_same<T, double>::value ? 0 : (std::is_same<Z, double>::value ? 2 : 1);
+    const Z* zp = reinterpret_cast<const Z*>(hc_fused_z(c));
+    constexpr int M = std::is_same<T, double>::value
+                          ? 0
+                          : (std::is_same<Z, double>::value ? 2 : (std::is_same<Z, float>::value ? 1 : 3));
     FusedPtrs<M> p;
     p.h0 = (const T*)c->h[k];
     p.zz = zp;
@@ -773,10 +809,15 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
   };
   double* td = nullptr;
   float* tf = nullptr;
+  int32_t* tq = nullptr;
   if (c->prec == HC_FP64) {
     if (R == 4) go(k_fused<0, 4>, td, td);
     else if (R == 16) go(k_fused<0, 16>, td, td);
     else go(k_fused<0, 8>, td, td);
+  } else if (c->zq) {
+    if (R == 4) go(k_fused<3, 4>, tf, tq);
+    else if (R == 16) go(k_fused<3, 16>, tf, tq);
+    else go(k_fused<3, 8>, tf, tq);
   } else if (c->prm.flat_dz < 0) {
     if (R == 4) go(k_fused<2, 4>, tf, td);
     else if (R == 16) go(k_fused<2, 16>, tf, td);

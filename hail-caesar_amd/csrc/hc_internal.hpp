@@ -35,6 +35,7 @@ struct HydroArgs {
   double mannings;
   int spatial_n;
   double flat_dz; // FP32 form only
+  float zq_inv;   // FP32 form, fixed-point elevations: metres per quantum (a power of two)
 };
 
 // Erosion arguments.
@@ -115,6 +116,9 @@ struct EnsMember {
   int W;                // mask words per row
 };
 
+// No-data value of the fixed-point elevations.
+constexpr int32_t kZqNodata = INT32_MIN;
+
 // The fused kernels address a field by 32-bit byte offsets (8-byte elements
 // at most).
 inline bool fused_offsets_fit(int NR, int P) {
@@ -175,6 +179,9 @@ struct hc_ctx {
   int fused_strip = 1;        // sub-tiles per block: a block covers rows * strip rows
   void* z = nullptr;          // elevation (state type)
   double* z64 = nullptr;      // FP32 form: FP64 elevation for near-flat faces
+  int32_t* zq = nullptr;      // FP32 form, elev_fixed: fixed-point elevations
+  double zq_base = 0.0;       // elevation of quantum 0
+  int zq_shift = 0;           // quanta per metre = 2^zq_shift
   void* nman = nullptr;       // Manning n (state type), spatial only
   uint64_t* mask = nullptr;
 
@@ -261,7 +268,8 @@ void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby);
 hc::HydroArgs hc_hydro_args(const hc_ctx* c);
 // Elevation plane the fused kernel reads for this context's mode.
 const void* hc_fused_z(const hc_ctx* c);
-// Ensemble fused step over a tile list (mode: 0 FP64, 1 FP32 z32, 2 FP32 z64).
+// Ensemble fused step over a tile list (mode: 0 FP64, 1 FP32 z32, 2 FP32 z64,
+// 3 FP32 fixed-point z).
 int hc_launch_ens_fused(hipStream_t s, int mode, int R, const hc::EnsMember* mem,
                         const int2* tiles, int ntiles, int nblocks, int parity);
 int hc_launch_scan(hc_ctx* c);

@@ -646,6 +646,7 @@ int ens_fail(hc_ens* e, int code, const std::string& msg) {
 
 int ens_mode(const hc_ctx* c) {
   if (c->prec == HC_FP64) return 0;
+  if (c->zq) return 3;
   return c->prm.flat_dz < 0 ? 2 : 1;
 }
 

@@ -63,6 +63,7 @@ class hc_params(ctypes.Structure):
         ("bedrock_erosion_threshold", ctypes.c_double), ("bedrock_erosion_rate", ctypes.c_double),
         ("lateral_constant", ctypes.c_double), ("chann_lateral_erosion", ctypes.c_double),
         ("veg_lat_restriction", ctypes.c_double), ("flat_dz", ctypes.c_double),
+        ("elev_fixed", ctypes.c_int),
     ]
 
 
@@ -193,6 +194,7 @@ def default_params(**kw) -> hc_params:
     p.chann_lateral_erosion = 20.0
     p.veg_lat_restriction = 0.1
     p.flat_dz = 0.0
+    p.elev_fixed = 0
     for k, v in kw.items():
         if k in ("d", "dprop", "fallVelocity"):
             arr = getattr(p, k)

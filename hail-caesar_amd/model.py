@@ -30,7 +30,8 @@ def _in_dir(path):
 class hcm_options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("fused", ctypes.c_int), ("quiet", ctypes.c_int),
                 ("write_files", ctypes.c_int), ("flat_dz", ctypes.c_double),
-                ("device_loop", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("device_loop", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("elev_fixed", ctypes.c_int)]
 
 
 class hcm_arrays(ctypes.Structure):
@@ -73,14 +74,14 @@ class Model:
     def __init__(self, path: Optional[str], params_file: Optional[str], precision: int = hc.HC_FP64,
                  fused: bool = True, write_files: bool = False, flat_dz: float = 0.0,
                  device_loop: bool = False, use_graph: bool = False, arrays: Optional[dict] = None,
-                 params: Optional[dict] = None):
+                 params: Optional[dict] = None, elev_fixed: bool = False):
         """Files (path + params_file, as the reference's CLI) or, with
         ``arrays`` and ``params``, in-memory inputs: arrays holds 'dem' and
         optionally 'mannings', 'water_init', 'hydroindex', 'rain' (rows x
         cells, mm/h) plus 'DX'."""
         self.L = _lib()
         opt = hcm_options(precision, 1 if fused else 0, 1, 1 if write_files else 0, flat_dz,
-                          1 if device_loop else 0, 1 if use_graph else 0)
+                          1 if device_loop else 0, 1 if use_graph else 0, 1 if elev_fixed else 0)
         h = ctypes.c_void_p()
         self.workdir = None
         if arrays is None:

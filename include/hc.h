@@ -88,6 +88,12 @@ typedef struct hc_params {
    * the difference from the FP64 DEM; < 0: every face takes its elevation
    * difference from the FP64 DEM (4 more bytes per cell and step). */
   double flat_dz;
+  /* FP32 form only.  1: elevations held as 32-bit fixed point, a power-of-two
+   * scale about the DEM's mid-range chosen on upload so that the relief fits
+   * in 2^31 quanta; every elevation difference is then exact to one quantum
+   * (relief / 2^31, about 2e-8 m for 40 m of relief) at 4 bytes per cell.
+   * flat_dz is ignored.  Cells at -9999 or below stay no-data. */
+  int elev_fixed;
 } hc_params;
 
 /* Field identifiers for hc_upload / hc_download. */

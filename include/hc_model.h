@@ -29,6 +29,7 @@ typedef struct hcm_options {
   int device_loop;/* 1: hydro-only runs step with hc_loop_* (scalars on the device;
                      rasters are not written in this mode) */
   int use_graph;  /* with device_loop: replay captured HIP graphs */
+  int elev_fixed; /* FP32 form: see hc_params.elev_fixed */
 } hcm_options;
 
 /* Constructor + initialise_model_domain_extents + initialise_arrays +

@@ -12,7 +12,7 @@ from pyoracle import OracleModel, orc_params
 STATE_FIELDS = ("water_depth", "elev", "qx", "qy")
 
 
-def hc_params_from(p: orc_params, flat_dz: float = 0.0):
+def hc_params_from(p: orc_params, flat_dz: float = 0.0, elev_fixed: bool = False):
     from hail_caesar_amd import hc
     return hc.default_params(
         courant_number=p.courant_number, froude_limit=p.froude_limit,
@@ -24,7 +24,7 @@ def hc_params_from(p: orc_params, flat_dz: float = 0.0):
         fallVelocity=list(p.fallVelocity), bedrock_erosion_threshold=p.bedrock_erosion_threshold,
         bedrock_erosion_rate=p.bedrock_erosion_rate, lateral_constant=p.lateral_constant,
         chann_lateral_erosion=p.chann_lateral_erosion, veg_lat_restriction=p.veg_lat_restriction,
-        flat_dz=flat_dz)
+        flat_dz=flat_dz, elev_fixed=1 if elev_fixed else 0)
 
 
 def wet_flow_state(tmp, size=64, steps=400, seed=1001, spatial_n=False):
@@ -41,10 +41,12 @@ def wet_flow_state(tmp, size=64, steps=400, seed=1001, spatial_n=False):
     return w, o
 
 
-def to_context(o: OracleModel, precision=0, flat_dz: float = 0.0, sediment: bool = False):
+def to_context(o: OracleModel, precision=0, flat_dz: float = 0.0, sediment: bool = False,
+               elev_fixed: bool = False):
     from hail_caesar_amd import hc
     p = o.p
-    ctx = hc.Context(p.imax, p.jmax, p.DX, hc_params_from(p, flat_dz), precision, sediment)
+    ctx = hc.Context(p.imax, p.jmax, p.DX, hc_params_from(p, flat_dz, elev_fixed), precision,
+                     sediment)
     for f in STATE_FIELDS:
         ctx.upload(f, o.field(f))
     if p.spatially_var_mannings:

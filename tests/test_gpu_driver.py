@@ -162,14 +162,16 @@ def test_fp32_storage_form_error(tmp_path):
     ref.iterate(10 ** 7)
     h64 = ref.download("water_depth")
     res = {}
-    for name, fdz in (("z32", 0.0), ("flat0.01", 0.01), ("exact_z", -1.0)):
-        f32 = Model(w.path, w.params_file, precision=hc.HC_FP32, flat_dz=fdz)
+    for name, fdz, fixed in (("z32", 0.0, False), ("flat0.01", 0.01, False), ("exact_z", -1.0, False),
+                             ("fixed_z", 0.0, True)):
+        f32 = Model(w.path, w.params_file, precision=hc.HC_FP32, flat_dz=fdz, elev_fixed=fixed)
         f32.iterate(10 ** 7)
         res[name] = _errs(f32.download("water_depth"), h64)
     print(f"fp32 depth error vs fp64 (floor {floor}): {res}")
-    assert res["exact_z"]["l2"] <= max(1e-4, 4 * floor["l2"])
-    assert res["exact_z"]["max"] <= max(1e-4, 4 * floor["max"])
-    assert res["exact_z"]["vol"] < 1e-6
+    for name in ("exact_z", "fixed_z"):
+        assert res[name]["l2"] <= max(1e-4, 4 * floor["l2"]), name
+        assert res[name]["max"] <= max(1e-4, 4 * floor["max"]), name
+        assert res[name]["vol"] < 1e-6, name
 
 
 def test_fp32_fully_wet_error(tmp_path):
@@ -186,11 +188,12 @@ def test_fp32_fully_wet_error(tmp_path):
     ref.iterate(200)
     h64 = ref.download("water_depth")
     res = {}
-    for name, fdz in (("z32", 0.0), ("exact_z", -1.0)):
-        f32 = Model(w.path, w.params_file, precision=hc.HC_FP32, flat_dz=fdz)
+    for name, fdz, fixed in (("z32", 0.0, False), ("exact_z", -1.0, False), ("fixed_z", 0.0, True)):
+        f32 = Model(w.path, w.params_file, precision=hc.HC_FP32, flat_dz=fdz, elev_fixed=fixed)
         f32.iterate(200)
         res[name] = _errs(f32.download("water_depth"), h64)
         res[name]["qx_max"] = float(np.abs(f32.download("qx") - ref.download("qx")).max()
                                     / np.abs(ref.download("qx")).max())
     print(f"fp32 fully wet depth error vs fp64: {res}")
     assert res["exact_z"]["max"] < 1e-4
+    assert res["fixed_z"]["max"] < 1e-4

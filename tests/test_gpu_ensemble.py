@@ -25,8 +25,15 @@ def _members(wet):
     return specs
 
 
+# precision forms: FP64; FP32 with FP64 elevations; FP32 with fixed-point elevations
+FIXED = "fp32_fixed"
+
+
 def _model(spec, prec):
     arrays, params = spec
+    if prec == FIXED:
+        return Model(None, None, precision=hc.HC_FP32, arrays=arrays, params=params,
+                     device_loop=True, use_graph=False, elev_fixed=True)
     return Model(None, None, precision=prec, arrays=arrays, params=params, device_loop=True,
                  use_graph=False, flat_dz=-1.0 if prec == hc.HC_FP32 else 0.0)
 
@@ -57,7 +64,8 @@ def _run_ensemble(specs, prec, chunk, tune=False):
     return ms, ens, [[np.concatenate(a[k]) for k in range(3)] for a in acc]
 
 
-@pytest.mark.parametrize("prec,wet", [(hc.HC_FP64, False), (hc.HC_FP64, True), (hc.HC_FP32, True)])
+@pytest.mark.parametrize("prec,wet", [(hc.HC_FP64, False), (hc.HC_FP64, True), (hc.HC_FP32, True),
+                                      (FIXED, True)])
 def test_ensemble_equals_members_alone(prec, wet):
     specs = _members(wet)
     alone = [_run_alone(s, prec) for s in specs]
@@ -86,9 +94,10 @@ def test_ensemble_tuned_tiles_same_results():
             assert np.array_equal(mb.download(f), ma.download(f)), f
 
 
-def test_ensemble_rejects_mixed_modes():
+@pytest.mark.parametrize("pa,pb", [(hc.HC_FP64, hc.HC_FP32), (hc.HC_FP32, FIXED)])
+def test_ensemble_rejects_mixed_modes(pa, pb):
     specs = _members(True)[:2]
-    a = _model(specs[0], hc.HC_FP64)
-    b = _model(specs[1], hc.HC_FP32)
+    a = _model(specs[0], pa)
+    b = _model(specs[1], pb)
     with pytest.raises(hc.HCError):
         Ensemble([a, b])

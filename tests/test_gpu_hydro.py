@@ -102,6 +102,41 @@ def test_multistep_fused_vs_oracle(tmp_path):
         assert rel_err(ctx.download(f), o.field(f), 1e-12) < 1e-10, f
 
 
+def test_fixed_point_elevations_high_datum(tmp_path):
+    """Fixed-point elevations keep the FP32 form's face differences exact where
+    single-precision elevations cannot: the oracle state raised by 2000 m
+    (float32 spacing there 1.2e-4 m).  30 steps on the oracle's own dt
+    sequence: the fixed-point and FP64-elevation forms stay within 1e-5 of the
+    oracle (FP64) in relative depth, and within 5e-6 of each other (both carry
+    the FP32 arithmetic's own ~1e-6); the single-precision-elevation form is at
+    least ten times further off."""
+    w, o = wet_flow_state(tmp_path, size=64, steps=600)
+    e = o.field("elev")
+    e[e > -9999] += 2000.0
+    forms = {"fixed": dict(elev_fixed=True), "z64": dict(flat_dz=-1.0), "z32": dict(flat_dz=0.0)}
+    ctxs = {k: to_context(o, 1, **kw) for k, kw in forms.items()}
+    for k in range(30):
+        dt = o.call("get_flow_timestep")
+        o.call("flow_route")
+        o.call("depth_update")
+        for c in ctxs.values():
+            c.flow_depth(dt)
+        if k % 5 == 4:
+            o.call("scan_area")
+            for c in ctxs.values():
+                c.scan_area()
+    h = o.field("water_depth")
+    wet = h > 1e-3
+    err = {k: float(np.max(np.abs(c.download("water_depth") - h)[wet] / h[wet]))
+           for k, c in ctxs.items()}
+    hf, hz = ctxs["fixed"].download("water_depth"), ctxs["z64"].download("water_depth")
+    between = float(np.max(np.abs(hf - hz)[wet] / h[wet]))
+    print(f"relative depth error vs the oracle: {err}, fixed vs z64 {between}")
+    assert err["fixed"] < 1e-5 and err["z64"] < 1e-5
+    assert between < 5e-6
+    assert err["z32"] > 10 * err["fixed"]
+
+
 def test_health_check(tmp_path):
     w, o = wet_flow_state(tmp_path, size=32, steps=0)
     ctx = to_context(o)
