@@ -68,7 +68,8 @@ def copy_probe(mib: int):
     del a, b
 
 
-def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: bool = True):
+def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: bool = True,
+            tiles=None):
     from hail_caesar_amd import hc
     from hail_caesar_amd.model import Model
     prec = hc.HC_FP32 if precision == "fp32" else hc.HC_FP64
@@ -78,7 +79,28 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
     setup_s = time.time() - t0
     m.iterate(warmup)
     # the project's tile tuner picks rows per wavefront on the warmed state
-    rows, strip, tune_us = m.context().tune_tiles(0.05)
+    # (isolated launches on a fixed state); the pick and the usual
+    # candidates are then timed inside the loop itself, where each step reads
+    # what the step before wrote, and the fastest there is kept (untimed steps)
+    ctx = m.context()
+    if tiles:  # fixed tiling (profiler runs that must all see the same kernel)
+        rows, strip, tune_us = tiles[0], tiles[1], None
+        cands = [tuple(tiles)]
+    else:
+        rows, strip, tune_us = ctx.tune_tiles(0.05)
+        cands = [(rows, strip), (4, 1), (8, 1), (4, 2), (8, 2), (16, 1)]
+    loop_us = {}
+    m.set_graph(False)
+    for rs in dict.fromkeys(cands):
+        ctx.set_tiles(*rs)
+        ctx.loop_profile(True)
+        m.iterate(20)
+        t, n = ctx.loop_kernel_time()
+        ctx.loop_profile(False)
+        loop_us[rs] = 1e3 * t / max(n, 1)
+    m.set_graph(use_graph)
+    rows, strip = min(loop_us, key=loop_us.get)
+    ctx.set_tiles(rows, strip)
     m.iterate(2)
     # timed region: K loop bodies, synchronised on both sides
     m.synchronize()
@@ -88,7 +110,6 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
     wall = time.perf_counter() - t1
     # kernel timing pass: the same loop without graphs, HIP events around
     # every fused flow_route+depth_update launch on the context stream
-    ctx = m.context()
     ctx.loop_profile(True)
     m.set_graph(False)
     m.iterate(steps)
@@ -102,6 +123,7 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
         "wall_s": wall, "setup_s": setup_s, "kernel_ms": ktot / max(kn, 1), "kernel_launches": kn,
         "active_fraction": active, "h": h, "counter": m.scalar("counter"), "cycle": m.scalar("cycle"),
         "model": m, "tile_rows": rows, "tile_strip": strip, "tune_us": tune_us,
+        "loop_tile_us": {f"{r}x{s}": round(u, 1) for (r, s), u in loop_us.items()},
     }
 
 
@@ -350,6 +372,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel from the host")
+    ap.add_argument("--tiles", default=None, metavar="RxS",
+                    help="fixed fused-kernel tiling (rows x strip) instead of tuning")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-catchment ensemble")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--no-erosion", action="store_true", help="skip the flow + erosion timing")
@@ -376,7 +400,8 @@ def main():
     cells = args.size * args.size
     res = {}
     for prec in (["fp32", "fp64"] if args.only is None else [args.only]):
-        r = run_gpu(prec, arrays, params, args.steps, args.warmup, use_graph=not args.no_graph)
+        r = run_gpu(prec, arrays, params, args.steps, args.warmup, use_graph=not args.no_graph,
+                    tiles=tuple(int(v) for v in args.tiles.split("x")) if args.tiles else None)
         bpc = bytes_per_cell_step(prec)
         r["cell_steps_per_s"] = cells * args.steps / r["wall_s"]
         r["ms_per_step"] = 1000.0 * r["wall_s"] / args.steps
@@ -399,10 +424,11 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(head["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(head["kernel_gbs"] / HBM_PEAK_GBS, 4),
                 "traffic": traffic,
-                "kernel": "k_flow_depth32 (fused flow_route+depth_update)" if prec == "fp32"
-                else "k_flow_depth64 (fused flow_route+depth_update)",
+                "kernel": "k_fused<3,R> (fused flow_route+depth_update, FP32 state, fixed-point z)"
+                if prec == "fp32" else "k_fused<0,R> (fused flow_route+depth_update, FP64)",
                 "bytes_per_cell_step": head["bytes_per_cell_step"], "cells_per_launch": cells,
-                "kernel_us": round(head["kernel_ms"] * 1e3, 2), "tile_rows": head["tile_rows"],
+                "kernel_us": round(head["kernel_ms"] * 1e3, 2),
+                "tile_loop_us": head["loop_tile_us"], "tile_rows": head["tile_rows"],
                 "tile_strip": head["tile_strip"]}
     out = {
         "metric": base["metric"], "value": round(head["cell_steps_per_s"], 1), "unit": "cell-steps/s",

@@ -272,8 +272,8 @@ struct FusedPtrs {
 // block reductions).
 template <int MODE, int R>
 __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, int nbx, int nsub,
-                                           double dtd, const FusedPtrs<MODE>& p,
-                                           const DrainArgs& drain) {
+                                           double dt_arg, const double* dtp, const int* haltp,
+                                           const FusedPtrs<MODE>& p, const DrainArgs& drain) {
   using T = typename FusedTypes<MODE>::T;
   using Z = typename FusedTypes<MODE>::Z;
   const T* __restrict__ h0 = p.h0;
@@ -304,7 +304,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   // rain input at input points: one amount for the whole grid when every
   // cell has the same rain cell (added with a select, no branch)
   const bool inject = p.inj.pmask != nullptr;
-  const double amt_u = (inject && p.inj.rf_uniform >= 0) ? p.inj.amt[p.inj.rf_uniform] : 0.0;
+  double amt_u = 0.0;  // read with the loop state below
   auto rain = [&](T v, bool pt, uint32_t o) -> T {  // LCM.cpp:3463-3477
     if (p.inj.rf_uniform >= 0) {
       const T hp = static_cast<T>(static_cast<double>(v) + amt_u);
@@ -345,29 +345,38 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   // water_flux_out (LCM.cpp:3005-3047) on this tile's edge cells, after
   // depth_update and before the store; maxdepth is taken before the drain
   const bool drains = drain.ts != nullptr && fused_edge_tile(bx, by, H, a.imax, a.jmax);
-  const double ts = drains ? *drain.ts : 1.0;
+  double ts = 1.0;
   double dacc = 0;
-  float dtDX = static_cast<float>(dtd / a.DX);  // FP32 form's depth-update factor
+  // In the device-resident loop dt, the halt flag, the time step and the
+  // rain amount are read from the loop state.  They are read after the first
+  // sub-tile's loads are issued: scalar loads complete out of order, so a
+  // wait for any of them is a wait for all, and the vector loads would
+  // otherwise queue behind that chain.
+  double dtd = dt_arg;
+  float dtDX = 0.0f;  // FP32 form's depth-update factor
   const float fthr = static_cast<float>(a.flat_dz);
   Face32K K;
-  if constexpr (MODE != 0) {
-    const float DX = static_cast<float>(a.DX);
-    const float invdt = 1.0f / static_cast<float>(dtd);
-    K.g = static_cast<float>(kGravity);
-    K.dt = static_cast<float>(dtd);
-    K.invDX = 1.0f / DX;
-    K.froude = static_cast<float>(a.froude);
-    K.hthr = static_cast<float>(a.hthr);
-    K.k4 = 0.25f * DX * invdt;  // q*dt/DX > h/4  <=>  q > h*DX/(4 dt)
-    K.k5 = 0.2f * DX * invdt;
-    // computed once here: left to itself the compiler repeats the divisions
-    // at every use
-    K.dt = pin_value(K.dt);
-    K.invDX = pin_value(K.invDX);
-    K.k4 = pin_value(K.k4);
-    K.k5 = pin_value(K.k5);
-    dtDX = pin_value(dtDX);
-  }
+  auto dt_setup = [&]() {
+    if constexpr (MODE != 0) {
+      const float DX = static_cast<float>(a.DX);
+      const float invdt = 1.0f / static_cast<float>(dtd);
+      dtDX = static_cast<float>(dtd / a.DX);
+      K.g = static_cast<float>(kGravity);
+      K.dt = static_cast<float>(dtd);
+      K.invDX = 1.0f / DX;
+      K.froude = static_cast<float>(a.froude);
+      K.hthr = static_cast<float>(a.hthr);
+      K.k4 = 0.25f * DX * invdt;  // q*dt/DX > h/4  <=>  q > h*DX/(4 dt)
+      K.k5 = 0.2f * DX * invdt;
+      // computed once here: left to itself the compiler repeats the
+      // divisions at every use
+      K.dt = pin_value(K.dt);
+      K.invDX = pin_value(K.invDX);
+      K.k4 = pin_value(K.k4);
+      K.k5 = pin_value(K.k5);
+      dtDX = pin_value(dtDX);
+    }
+  };
   const float es = static_cast<float>(a.edgeslope);
   const int ycode = edge_code(c, a.jmax);  // y-faces: per column
   // y-face slope = slope * ymul + yadd: the edge override without a lane mask
@@ -494,13 +503,22 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     mprev = m;
   };
 
+  // first sub-tile: rows r0-1 .. r0+R loaded, faces of rows r0 .. r0+R
+  int r0 = by * H;
+  uint64_t win = 0;
   if (active_wave) {
-    // first sub-tile: rows r0-1 .. r0+R loaded, faces of rows r0 .. r0+R
-    int r0 = by * H;
-    uint64_t win = load_win(r0);
+    win = load_win(r0);
 #pragma unroll
     for (int k = 0; k < R + 2; ++k) load_row(k, r0);
     load_n(1);
+  }
+  asm volatile("" ::: "memory");  // the loads above stay ahead of the loop-state reads
+  if (haltp && *haltp) return;  // the loop has finished: uniform over the grid
+  if (dtp) dtd = *dtp;
+  if (drains) ts = *drain.ts;
+  if (inject && p.inj.rf_uniform >= 0) amt_u = p.inj.amt[p.inj.rf_uniform];
+  dt_setup();
+  if (active_wave) {
     if (inject) {
 #pragma unroll
       for (int k = 0; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), ob[k]);
@@ -544,20 +562,21 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
 }
 
 // One grid: one block per tile, block ids remapped so that each XCD (block
-// b runs on XCD b mod 8) takes a contiguous band of tiles.
+// b runs on XCD b mod 8) takes a contiguous band of tiles in row-major order.
+// (Orders that walk groups of tile columns, so that vertically adjacent
+// tiles run close together, measured slower: the row-major stream keeps the
+// DRAM accesses sequential.)
 template <int MODE, int R>
 __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int nbx, int nblocks,
                                                int nsub, FusedPtrs<MODE> p,
                                                const double* __restrict__ dtp,
                                                const int* __restrict__ haltp,
                                                const DrainArgs drain) {
-  if (haltp && *haltp) return;  // device-resident loop has finished
-  const double dtd = dtp ? *dtp : dt_arg;
   const int b = blockIdx.x;
   const int q8 = nblocks / 8, r8 = nblocks % 8;
   const int xcd = b % 8;
   const int bb = xcd * q8 + min(xcd, r8) + b / 8;
-  fused_tile<MODE, R>(a, bb % nbx, bb / nbx, nbx, nsub, dtd, p, drain);
+  fused_tile<MODE, R>(a, bb % nbx, bb / nbx, nbx, nsub, dt_arg, dtp, haltp, p, drain);
 }
 
 // Ensemble: a persistent grid walks the ensemble's tile list (member-major).
@@ -580,6 +599,7 @@ __global__ __launch_bounds__(256) void k_ens_fused(const EnsMember* __restrict__
     const int2 tile = tiles[t];
     const EnsMember& M = mem[tile.x];
     const hc::LoopCtl* ctl = table_ptr<const hc::LoopCtl>(M.ctl);
+    // a finished member's tiles are skipped before any of their loads
     if (ctl->halt) continue;  // uniform per block
     const int g = M.cur0 ^ step_parity;
     FusedPtrs<MODE> p;
@@ -603,7 +623,8 @@ __global__ __launch_bounds__(256) void k_ens_fused(const EnsMember* __restrict__
     d.thr = M.thr;
     d.partials = table_ptr<double>(M.dpart);
     __syncthreads();  // the block's shared scratch is reused tile to tile
-    fused_tile<MODE, R>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, 1, ctl->dt_flow, p, d);
+    fused_tile<MODE, R>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, 1, 0.0, &ctl->dt_flow,
+                        nullptr, p, d);
   }
 }
 
