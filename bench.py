@@ -76,6 +76,7 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
     t0 = time.time()
     m = Model(None, None, precision=prec, arrays=arrays, params=params, device_loop=True,
               use_graph=use_graph, elev_fixed=precision == "fp32")
+    counter0 = m.scalar("counter")
     setup_s = time.time() - t0
     m.iterate(warmup)
     # the project's tile tuner picks rows per wavefront on the warmed state
@@ -122,6 +123,7 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
     return {
         "wall_s": wall, "setup_s": setup_s, "kernel_ms": ktot / max(kn, 1), "kernel_launches": kn,
         "active_fraction": active, "h": h, "counter": m.scalar("counter"), "cycle": m.scalar("cycle"),
+        "bodies": int(round(m.scalar("counter") - counter0)),
         "model": m, "tile_rows": rows, "tile_strip": strip, "tune_us": tune_us,
         "loop_tile_us": {f"{r}x{s}": round(u, 1) for (r, s), u in loop_us.items()},
     }
@@ -140,7 +142,7 @@ def _field_errors(a, b):
             "p99_9_abs": float(np.percentile(d, 99.9))}
 
 
-def fp32_error(m32, m64, arrays, params, nsteps):
+def fp32_error(r32, r64, arrays, params):
     """Per-field error of the FP32 form against the FP64 form after the same
     loop bodies, beside the FP64 form's own sensitivity floor: the FP64 form
     started from the DEM rounded to float32 (a perturbation below one FP32
@@ -149,6 +151,12 @@ def fp32_error(m32, m64, arrays, params, nsteps):
     cells near the threshold; the floor shows how far that alone carries."""
     from hail_caesar_amd import hc
     from hail_caesar_amd.model import Model
+    # both forms to the same number of loop bodies (their tile trials differ)
+    m32, m64 = r32["model"], r64["model"]
+    nsteps = max(r32["bodies"], r64["bodies"])
+    for r in (r32, r64):
+        if r["bodies"] < nsteps:
+            r["model"].iterate(nsteps - r["bodies"])
     pert = dict(arrays)
     pert["dem"] = np.asarray(arrays["dem"], dtype=np.float32).astype(np.float64)
     mf = Model(None, None, precision=hc.HC_FP64, arrays=pert, params=params, device_loop=True,
@@ -448,8 +456,7 @@ def main():
                        "kernel_gbs": round(f64["kernel_gbs"], 1),
                        "frac": round(f64["kernel_gbs"] / HBM_PEAK_GBS, 4)}
         out["fp32_over_fp64"] = round(f32["cell_steps_per_s"] / f64["cell_steps_per_s"], 3)
-        out["fp32_error"] = fp32_error(f32["model"], f64["model"], arrays, params,
-                                       args.warmup + 2 + 2 * args.steps)
+        out["fp32_error"] = fp32_error(f32, f64, arrays, params)
     if not args.no_ensemble and args.only is None:
         out["ensemble"] = run_ensemble(args.steps, args.warmup)
         print(f"[bench] ensemble: {out['ensemble']['value']:.4e} cell-steps/s, "
