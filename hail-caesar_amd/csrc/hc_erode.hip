@@ -13,12 +13,14 @@
 //                             (addGS is deterministic, so order is moot);
 //                             tempbmax is a max-reduction; repeated on the
 //                             host while tempbmax > ERODEFACTOR.
-//   deposition + x slide      one thread per column y walks x in the
+//   deposition + x slide      one wavefront per column y walks x in the
 //   (4570-4682)               reference's order: every write stays in the
 //                             column, and each cell sees its upper
 //                             neighbour's slide, exactly as the serial loop.
-//   y slide (4684-4747)       one thread per row x walks y in order (writes
-//                             stay in the row).
+//                             The cells around the sweep position are held
+//                             in registers (see the sweep section below).
+//   y slide (4684-4747)       one wavefront per row x walks y in order
+//                             (writes stay in the row).
 //   edge outputs (4750-4827)  nine lanes, one per fraction, accumulate in
 //                             float in the reference order: a zeroed private
 //                             sum per loop, added to the total afterwards, as
@@ -356,7 +358,245 @@ __device__ __forceinline__ void lateral_slide(const ErodeArgs& a, const Grain& g
   }
 }
 
+// ------------------------------------------ register-window sweeps
+// The deposition and y-slide sweeps are sequential along a column (row) in
+// the reference, so one wavefront walks one column (row).  Each cell's
+// grain row, elevation, bedrock, depth and allocation flag are held in
+// registers from one cell before the sweep reaches it until one cell after
+// (the only cells a step reads or changes are the current one and its two
+// neighbours along the sweep), so the per-cell work runs on registers; the
+// next cell's inputs are issued one step ahead.  The strata (10 layers x 9
+// fractions) stay in memory and are moved only on the rare layer shifts and
+// allocations, spread over the lanes: element i = z*9+n lives in lane i % 64,
+// slot i / 64, and every load and store of an element is made by that lane,
+// so program order within the lane orders all accesses to it.  All other
+// values are uniform: every lane computes them alike.
+
+// A cell within one step of the sweep position.  (No member leaves padding:
+// copies of padding bytes keep the compiler from holding the cell in
+// registers.)
+struct CellW {
+  double G[kG + 1];
+  double elev, bedrock, h;
+  uint64_t has;
+  size_t c;
+};
+
+__device__ __forceinline__ double bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | lo);
+}
+
+// A cell's state as fetched (lane n holds grain plane n); turned into a
+// CellW only when the cell enters the window, so the fetch does not wait.
+struct CellRaw {
+  double gv, elev, bedrock, h;
+  size_t c;
+  int64_t has;
+};
+__device__ __forceinline__ void cell_fetch(CellRaw& r, const Grain& gr, size_t c, const double* h,
+                                           const double* elev, const double* bedrock, int lane) {
+  r.c = c;
+  r.gv = (lane <= kG) ? gr.g[static_cast<size_t>(lane) * gr.nc + c] : 0.0;
+  r.elev = elev[c];
+  r.bedrock = bedrock[c];
+  r.h = h[c];
+  r.has = gr.has[c];
+}
+__device__ __forceinline__ void cell_take(CellW& w, const CellRaw& r) {
+  w.c = r.c;
+#pragma unroll
+  for (int n = 0; n <= kG; ++n) w.G[n] = bcast(r.gv, n);
+  w.elev = r.elev;
+  w.bedrock = r.bedrock;
+  w.h = r.h;
+  w.has = r.has != 0 ? 1u : 0u;
+}
+
+// cell state back to memory (plane n by lane n, the rest by lane 0; one
+// store per plane: selecting w.G[lane] would be compiled as a dynamic index,
+// which puts the cell in scratch memory)
+__device__ __forceinline__ void cell_store(const CellW& w, const Grain& gr, double* elev,
+                                           int lane) {
+#pragma unroll
+  for (int n = 0; n <= kG; ++n)
+    if (lane == n) gr.g[static_cast<size_t>(n) * gr.nc + w.c] = w.G[n];
+  if (lane == 0) {
+    elev[w.c] = w.elev;
+    gr.has[w.c] = w.has ? 1 : 0;
+  }
+}
+
+// strata of a cell into the lanes (slot 0: element lane, slot 1: lane + 64)
+__device__ __forceinline__ void strata_load(const Grain& gr, size_t c, int lane, double& s0,
+                                            double& s1) {
+  s0 = gr.s[static_cast<size_t>(lane) * gr.nc + c];
+  s1 = (lane + 64 < 90) ? gr.s[static_cast<size_t>(lane + 64) * gr.nc + c] : 0.0;
+}
+__device__ __forceinline__ void strata_store(const Grain& gr, size_t c, int lane, double s0,
+                                             double s1) {
+  gr.s[static_cast<size_t>(lane) * gr.nc + c] = s0;
+  if (lane + 64 < 90) gr.s[static_cast<size_t>(lane + 64) * gr.nc + c] = s1;
+}
+// element j of the strata held in (s0, s1) of lane j % 64
+__device__ __forceinline__ double strata_elem(double s0, double s1, int j) {
+  return j < 64 ? __shfl(s0, j, 64) : __shfl(s1, j - 64, 64);
+}
+// new element i = old element i + d (i + d in [0, 90)), for both slots
+__device__ __forceinline__ void strata_move(double& s0, double& s1, int lane, int d, int lo,
+                                            int hi) {
+  const int i0 = lane, i1 = lane + 64;
+  const int j0 = min(max(i0 + d, 0), 89), j1 = min(max(i1 + d, 0), 89);
+  const double a0 = __shfl(s0, j0 & 63, 64), b0 = __shfl(s1, j0 & 63, 64);
+  const double a1 = __shfl(s0, j1 & 63, 64), b1 = __shfl(s1, j1 & 63, 64);
+  const double n0 = j0 < 64 ? a0 : b0;
+  const double n1 = j1 < 64 ? a1 : b1;
+  if (i0 >= lo && i0 < hi) s0 = n0;
+  if (i1 >= lo && i1 < hi) s1 = n1;
+}
+
+// sort_active (LCM.cpp:3874-3947) on a register cell
+__device__ __forceinline__ void sort_active_w(const Grain& gr, CellW& w, const ErodeArgs& a, int lane) {
+  double total = 0.0;
+#pragma unroll
+  for (int n = 0; n <= kG; ++n) total += w.G[n];
+  if (total > (a.active * 1.5)) {
+    double s0, s1;
+    strata_load(gr, w.c, lane, s0, s1);
+    strata_move(s0, s1, lane, -9, 9, 90);  // layers 9..1 take the layer above (z-1)
+    const double coeff = a.active / total;
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n) {
+      const double gv = w.G[n];
+      if (gv > 0.0) {
+        const double amount = coeff * (gv);
+        if (lane == n - 1) s0 = amount;  // layer 0, fraction n-1
+        w.G[n] = gv - amount;
+      }
+    }
+    strata_store(gr, w.c, lane, s0, s1);
+  }
+  if (total < (a.active / 4)) {
+    double s0, s1;
+    strata_load(gr, w.c, lane, s0, s1);
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n) w.G[n] += bcast(s0, n - 1);
+    strata_move(s0, s1, lane, 9, 0, 81);  // layers 0..8 take the layer below (z+1)
+    const double amount = a.active;
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n)
+      if (lane + 64 == 81 + n - 1) s1 = amount * a.dprop[n];  // layer 9, fraction n-1
+    strata_store(gr, w.c, lane, s0, s1);
+  }
+}
+
+// addGS (LCM.cpp:3949-3983) on a register cell
+__device__ __forceinline__ void add_gs_w(const Grain& gr, CellW& w, const ErodeArgs& a, int lane) {
+  w.has = 1;
+  w.G[0] = 0;
+#pragma unroll
+  for (int n = 1; n <= kG - 1; ++n) w.G[n] = a.active * a.dprop[n];
+  w.G[kG] = 0;
+  auto val = [&](int i) {
+    const int z = i / 9, n2 = i % 9;
+    const bool zero = w.elev - (a.active * (z + 1)) < (w.bedrock - a.active);
+    return zero ? 0.0 : (a.active) * a.dprop[n2 + 1];
+  };
+  strata_store(gr, w.c, lane, val(lane), (lane + 64 < 90) ? val(lane + 64) : 0.0);
+  sort_active_w(gr, w, a, lane);
+}
+
+__device__ __forceinline__ void sort_active_alloc_w(const Grain& gr, CellW& w, const ErodeArgs& a,
+                                                    int lane) {
+  if (!w.has) add_gs_w(gr, w, a, lane);
+  sort_active_w(gr, w, a, lane);
+}
+
+// slide_GS (LCM.cpp:4077-4171) between register cells: `src` to `dst`
+__device__ __forceinline__ void slide_gs_w(const Grain& gr, CellW& src, CellW& dst, double amount,
+                           const ErodeArgs& a, int lane) {
+  double total = 0;
+  if (src.has && dst.has) {
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n)
+      if (src.G[n] > 0) total += src.G[n];
+    if (amount > total) {
+#pragma unroll
+      for (int n = 1; n <= kG - 1; ++n) dst.G[n] += (amount - total) * a.dprop[n];
+      amount = total;
+    }
+    if (total > 0) {
+#pragma unroll
+      for (int n = 1; n <= kG - 1; ++n) {
+        const double tr = amount * (src.G[n] / total);
+        dst.G[n] += tr;
+        src.G[n] -= tr;
+        if (src.G[n] < 0) src.G[n] = 0;
+      }
+    }
+    sort_active_alloc_w(gr, src, a, lane);
+    sort_active_alloc_w(gr, dst, a, lane);
+    return;
+  }
+  if (!src.has && dst.has) {
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n) dst.G[n] += (amount) * a.dprop[n];
+    sort_active_alloc_w(gr, dst, a, lane);
+    return;
+  }
+  if (src.has && !dst.has) {
+    add_gs_w(gr, dst, a, lane);
+    if (amount > a.active) {
+#pragma unroll
+      for (int n = 1; n <= kG - 1; ++n) dst.G[n] += amount * a.dprop[n];
+      amount = a.active;
+    }
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n)
+      if (src.G[n] > 0) total += src.G[n];
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n) {
+      if (total > 0) {
+        dst.G[n] += amount * (src.G[n] / total);
+        if (src.G[n] > 0.0001) src.G[n] -= amount * (src.G[n] / total);
+        if (src.G[n] < 0) src.G[n] = 0;
+      }
+    }
+    sort_active_alloc_w(gr, src, a, lane);
+    sort_active_alloc_w(gr, dst, a, lane);
+    return;
+  }
+}
+
+// one lateral slide from neighbour `n` into cell `w` (LCM.cpp:4619-4664,
+// 4689-4742; veg and edge are zero in this build): as lateral_slide above
+__device__ __forceinline__ void lateral_slide_w(const ErodeArgs& a, const Grain& gr, double mult,
+                                                double time_step, double tau_c, double e3,
+                                                CellW& nb, CellW& w, bool at_edge,
+                                                double& elev_update, int lane) {
+  double amt = 0;
+  if (nb.h < a.wdet)
+    amt = mult * a.lateral_constant * tau_c * 0.0 * time_step / a.DX;
+  else
+    amt = a.chann_lateral * e3 * (nb.elev - w.elev) / a.DX * 0.1;
+  if (amt > 0) {
+    amt *= 1 - (0.0 * (1 - a.veg_lat));
+    if ((nb.elev - amt) < nb.bedrock || at_edge) amt = 0;
+    if (amt > a.ERODEFACTOR * 0.1) amt = a.ERODEFACTOR * 0.1;
+    elev_update += amt;
+    nb.elev -= amt;
+    slide_gs_w(gr, nb, w, amt, a, lane);
+  }
+}
+
 // ------------------------------------------- deposition + x-lateral slide
+// One wavefront per column y, walking x = 2 .. imax-1 (LCM.cpp:4570-4682).
+// Lane l fetches one of the 64 bedload terms of the next cell (fraction
+// 2 + l/8; terms 0-3 its own su, sr, sd, sl, terms 4-7 the neighbours' su of
+// y+1, sd of y-1, sl of x+1, sr of x-1), summed in the reference's order by
+// lanes 8k and 8k+4.
 __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, double mult,
                                                 const double* __restrict__ h,
                                                 double* __restrict__ elev,
@@ -368,55 +608,110 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
                                                 const double* __restrict__ sdir, Grain gr,
                                                 double* __restrict__ etot,
                                                 double* __restrict__ etot3) {
-  const int y = 2 + blockIdx.x * blockDim.x + threadIdx.x;  // one column per wavefront
-  if (y >= a.jmax) return;
+  const int y = 2 + blockIdx.x;
+  if (y >= a.jmax || a.imax < 3) return;
+  const int lane = threadIdx.x;
   const size_t P = a.pitch, nc = gr.nc;
-  auto S = [&](int d, int n, size_t cell) { return sdir[(d * 10 + n) * nc + cell]; };
-  for (int x = 2; x < a.imax; ++x) {
-    if (!mask_bit(mask, a.mwords, x, y)) continue;
+  const int tn = 2 + (lane >> 3), tt = lane & 7;  // this lane's bedload term
+  const int td = (tt < 4) ? tt : (tt == 4 ? 0 : (tt == 5 ? 2 : (tt == 6 ? 3 : 1)));
+  const long toff = (tt < 4) ? 0 : (tt == 4 ? 1 : (tt == 5 ? -1 : (tt == 6 ? static_cast<long>(P)
+                                                                            : -static_cast<long>(P))));
+  // inputs of one cell that nothing in the sweep changes
+  struct In {
+    double V, ss, tau, term;
+    uint64_t act;
+  };
+  auto fetch = [&](int x, In& in) {
     const size_t c = static_cast<size_t>(x) * P + y;
-    if (!(h[c] > a.wdet)) continue;
-    if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
-    double et = 0, et3 = 0;
-    for (int n = 1; n <= kG - 1; ++n) {
-      if (n == 1) {
-        double v = V[c] + ss[c];
-        gr.G(c, n) -= ss[c];
-        et -= ss[c];
-        double coeff = (a.fall[n] * time_step) / h[c];
+    in.act = mask_bit(mask, a.mwords, x, y) && (h[c] > a.wdet);
+    in.V = V[c];
+    in.ss = ss[c];
+    in.tau = Tau[c];
+    in.term = sdir[(static_cast<size_t>(td) * 10 + tn) * nc + c + toff];
+  };
+  CellW wm, wc, wp;
+  CellRaw rn;
+  In in, inn;
+  cell_fetch(rn, gr, static_cast<size_t>(1) * P + y, h, elev, bedrock, lane);
+  cell_take(wm, rn);
+  cell_fetch(rn, gr, static_cast<size_t>(2) * P + y, h, elev, bedrock, lane);
+  cell_take(wc, rn);
+  cell_fetch(rn, gr, static_cast<size_t>(3) * P + y, h, elev, bedrock, lane);
+  cell_take(wp, rn);
+  fetch(2, in);
+  inn = in;
+  for (int x = 2; x < a.imax; ++x) {
+    // issued one step ahead: the next cell's inputs and the cell entering the window
+    if (x + 1 < a.imax) fetch(x + 1, inn);
+    if (x + 2 <= a.imax) cell_fetch(rn, gr, static_cast<size_t>(x + 2) * P + y, h, elev, bedrock, lane);
+    if (in.act) {
+      const size_t c = wc.c;
+      if (!wc.has) add_gs_w(gr, wc, a, lane);
+      // bedload sums of fractions 2..9 in the reference's order
+      const double t1 = in.term + __shfl_down(in.term, 1, 8);
+      const double t2 = t1 + __shfl_down(in.term, 2, 8);
+      const double sum4 = t2 + __shfl_down(in.term, 3, 8);
+      double et = 0, et3 = 0;
+      {  // fraction 1: suspended load
+        const int n = 1;
+        const double v = in.V + in.ss;
+        wc.G[n] -= in.ss;
+        et -= in.ss;
+        double coeff = (a.fall[n] * time_step) / wc.h;
         if (coeff > 1) coeff = 1;
         double drop = coeff * v;
         if (drop > 0.001) drop = 0.001;
-        gr.G(c, n) += drop;
+        wc.G[n] += drop;
         et += drop;
-        V[c] = v - drop;
-      } else {
-        const double val1 = (S(0, n, c) + S(1, n, c) + S(2, n, c) + S(3, n, c));
-        const double val2 = (S(0, n, c + 1) + S(2, n, c - 1) + S(3, n, c + P) + S(1, n, c - P));
-        gr.G(c, n) += val2 - val1;
+        if (lane == 0) V[c] = v - drop;
+      }
+#pragma unroll
+      for (int n = 2; n <= kG - 1; ++n) {
+        const double val1 = bcast(sum4, 8 * (n - 2));
+        const double val2 = bcast(sum4, 8 * (n - 2) + 4);
+        wc.G[n] += val2 - val1;
         et += val2 - val1;
         et3 += val1;
       }
+      if (lane == 0) {
+        etot[c] = et;
+        etot3[c] = et3;
+      }
+      wc.elev += et;
+      if (et < 0) sort_active_alloc_w(gr, wc, a, lane);
+      if (et3 > 0) {
+        double elev_update = 0;
+        // (the neighbours go through the slide by value: two calls on
+        // references to different cells are merged by the compiler into one
+        // call on a selected address, which puts the cells in scratch memory)
+        if (wm.elev > wc.elev && x > 2) {
+          CellW nb = wm;
+          lateral_slide_w(a, gr, mult, time_step, in.tau, et3, nb, wc, x - 1 == 1, elev_update,
+                          lane);
+          wm = nb;
+        }
+        if (wp.elev > wc.elev && x < a.imax - 1) {
+          CellW nb = wp;
+          lateral_slide_w(a, gr, mult, time_step, in.tau, et3, nb, wc, x + 1 == a.imax,
+                          elev_update, lane);
+          wp = nb;
+        }
+        wc.elev += elev_update;
+      }
     }
-    etot[c] = et;
-    etot3[c] = et3;
-    elev[c] += et;
-    if (et < 0) sort_active_alloc(gr, c, elev, bedrock, a);
-    if (et3 > 0) {
-      double elev_update = 0;
-      const size_t cu = c - P, cd = c + P;
-      if (elev[cu] > elev[c] && x > 2)
-        lateral_slide(a, gr, mult, time_step, h, elev, bedrock, Tau[c], et3, cu, c, x - 1 == 1,
-                      elev_update);
-      if (elev[cd] > elev[c] && x < a.imax - 1)
-        lateral_slide(a, gr, mult, time_step, h, elev, bedrock, Tau[c], et3, cd, c,
-                      x + 1 == a.imax, elev_update);
-      elev[c] += elev_update;
-    }
+    cell_store(wm, gr, elev, lane);  // row x-1 is final
+    wm = wc;
+    wc = wp;
+    if (x + 2 <= a.imax) cell_take(wp, rn);
+    in = inn;
   }
+  // rows imax-1 and imax (the last two of the window)
+  cell_store(wm, gr, elev, lane);
+  if (a.imax >= 3) cell_store(wc, gr, elev, lane);
 }
 
 // --------------------------------------------------------- y-lateral slide
+// One wavefront per row x, walking y = 2 .. jmax-1 (LCM.cpp:4684-4747).
 __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step, double mult,
                                                   const double* __restrict__ h,
                                                   double* __restrict__ elev,
@@ -424,23 +719,58 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
                                                   const uint64_t* __restrict__ mask,
                                                   const double* __restrict__ Tau, Grain gr,
                                                   const double* __restrict__ etot3) {
-  const int x = 1 + blockIdx.x * blockDim.x + threadIdx.x;  // one row per wavefront
-  if (x > a.imax) return;
-  const size_t P = a.pitch;
+  const int x = 1 + blockIdx.x;
+  if (x > a.imax || a.jmax < 3) return;
+  const int lane = threadIdx.x;
+  const size_t row = static_cast<size_t>(x) * a.pitch;
+  struct In {
+    double e3, tau;
+    uint64_t act;
+  };
+  auto fetch = [&](int y, In& in) {
+    const size_t c = row + y;
+    in.e3 = etot3[c];
+    in.act = (in.e3 > 0) && mask_bit(mask, a.mwords, x, y);
+    in.tau = Tau[c];
+  };
+  CellW wl, wc, wr;
+  CellRaw rn;
+  In in, inn;
+  cell_fetch(rn, gr, row + 1, h, elev, bedrock, lane);
+  cell_take(wl, rn);
+  cell_fetch(rn, gr, row + 2, h, elev, bedrock, lane);
+  cell_take(wc, rn);
+  cell_fetch(rn, gr, row + 3, h, elev, bedrock, lane);
+  cell_take(wr, rn);
+  fetch(2, in);
+  inn = in;
   for (int y = 2; y < a.jmax; ++y) {
-    const size_t c = static_cast<size_t>(x) * P + y;
-    const double et3 = etot3[c];
-    if (!(et3 > 0) || !mask_bit(mask, a.mwords, x, y)) continue;
-    double elev_update = 0;
-    const size_t cl = c - 1, cr = c + 1;
-    if (elev[cl] > elev[c])
-      lateral_slide(a, gr, mult, time_step, h, elev, bedrock, Tau[c], et3, cl, c, y - 1 == 1,
-                    elev_update);
-    if (elev[cr] > elev[c])
-      lateral_slide(a, gr, mult, time_step, h, elev, bedrock, Tau[c], et3, cr, c, y + 1 == a.jmax,
-                    elev_update);
-    elev[c] += elev_update;
+    if (y + 1 < a.jmax) fetch(y + 1, inn);
+    if (y + 2 <= a.jmax) cell_fetch(rn, gr, row + y + 2, h, elev, bedrock, lane);
+    if (in.act) {
+      double elev_update = 0;
+      if (wl.elev > wc.elev) {  // (by value: as in the deposition sweep)
+        CellW nb = wl;
+        lateral_slide_w(a, gr, mult, time_step, in.tau, in.e3, nb, wc, y - 1 == 1, elev_update,
+                        lane);
+        wl = nb;
+      }
+      if (wr.elev > wc.elev) {
+        CellW nb = wr;
+        lateral_slide_w(a, gr, mult, time_step, in.tau, in.e3, nb, wc, y + 1 == a.jmax,
+                        elev_update, lane);
+        wr = nb;
+      }
+      wc.elev += elev_update;
+    }
+    cell_store(wl, gr, elev, lane);  // column y-1 is final
+    wl = wc;
+    wc = wr;
+    if (y + 2 <= a.jmax) cell_take(wr, rn);
+    in = inn;
   }
+  cell_store(wl, gr, elev, lane);
+  if (a.jmax >= 3) cell_store(wc, gr, elev, lane);
 }
 
 // ------------------------------------------------------------ edge outputs
@@ -452,39 +782,49 @@ __global__ __launch_bounds__(64) void k_edges(ErodeArgs a, const double* __restr
   const bool act = (n >= 1 && n <= kG - 1);
   const int I = a.imax, J = a.jmax;
   const size_t P = a.pitch;
-  auto S = [&](int d, size_t cell) { return sdir[(d * 10 + n) * nc + cell]; };
+  const int nn = act ? n : 1;  // lanes without a fraction read a valid plane
+  auto S = [&](int d, size_t cell) { return sdir[(d * 10 + nn) * nc + cell]; };
+  // The edge cells are distinct and each is read before it is cleared, so
+  // the sums read V as it stands and the clearing follows in a second pass;
+  // with no stores or barriers in the summing loops their loads run ahead
+  // of the (sequential, float) sums.
   float part1 = 0.0f, part2 = 0.0f;
+#pragma unroll 8
   for (int y = 2; y < J; ++y) {
     const size_t cA = static_cast<size_t>(I) * P + y;
     const double vA = V[cA];
+    const double sA = S(1, cA - P);
     const bool condA = (h[cA] > a.wdet || vA > 0);
-    if (act && condA) part1 = static_cast<float>(static_cast<double>(part1) + (n == 1 ? vA : S(1, cA - P)));
-    __syncthreads();
-    if (n == 1 && condA) V[cA] = 0;
-    __syncthreads();
     const size_t cB = static_cast<size_t>(1) * P + y;
     const double vB = V[cB];
+    const double sB = S(3, cB + P);
     const bool condB = (h[cB] > a.wdet || vB > 0);
-    if (act && condB) part1 = static_cast<float>(static_cast<double>(part1) + (n == 1 ? vB : S(3, cB + P)));
-    __syncthreads();
-    if (n == 1 && condB) V[cB] = 0;
-    __syncthreads();
+    if (act && condA) part1 = static_cast<float>(static_cast<double>(part1) + (n == 1 ? vA : sA));
+    if (act && condB) part1 = static_cast<float>(static_cast<double>(part1) + (n == 1 ? vB : sB));
   }
+#pragma unroll 8
   for (int x = 2; x < I; ++x) {
     const size_t cA = static_cast<size_t>(x) * P + J;
     const double vA = V[cA];
+    const double sA = S(2, cA - 1);
     const bool condA = (h[cA] > a.wdet || vA > 0);
-    if (act && condA) part2 = static_cast<float>(static_cast<double>(part2) + (n == 1 ? vA : S(2, cA - 1)));
-    __syncthreads();
-    if (n == 1 && condA) V[cA] = 0;
-    __syncthreads();
     const size_t cB = static_cast<size_t>(x) * P + 1;
     const double vB = V[cB];
+    const double sB = S(0, cB + 1);
     const bool condB = (h[cB] > a.wdet || vB > 0);
-    if (act && condB) part2 = static_cast<float>(static_cast<double>(part2) + (n == 1 ? vB : S(0, cB + 1)));
-    __syncthreads();
-    if (n == 1 && condB) V[cB] = 0;
-    __syncthreads();
+    if (act && condA) part2 = static_cast<float>(static_cast<double>(part2) + (n == 1 ? vA : sA));
+    if (act && condB) part2 = static_cast<float>(static_cast<double>(part2) + (n == 1 ? vB : sB));
+  }
+  __syncthreads();  // every lane's reads of V precede the clearing
+  for (int y = 2 + n; y < J; y += 64) {
+    const size_t cA = static_cast<size_t>(I) * P + y, cB = static_cast<size_t>(1) * P + y;
+    if (h[cA] > a.wdet || V[cA] > 0) V[cA] = 0;
+    if (h[cB] > a.wdet || V[cB] > 0) V[cB] = 0;
+  }
+  for (int x = 2 + n; x < I; x += 64) {
+    const size_t cA = static_cast<size_t>(x) * P + J, cB = static_cast<size_t>(x) * P + 1;
+    if (h[cA] > a.wdet || V[cA] > 0) V[cA] = 0;
+    if (h[cB] > a.wdet || V[cB] > 0) V[cB] = 0;
   }
   if (n < 20) gtot[n] = act ? (part1 + part2) : 0.0f;
 }
@@ -556,13 +896,13 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
   const int ncol = std::max(0, c->g.jmax - 2);
   // The deposition sweep runs each column top to bottom and the y-slide each
   // row left to right (the reference's in-place order): one wavefront per
-  // column / row, so lanes never wait on another column's branches.
+  // column / row, its lanes sharing the per-cell loads and strata moves.
   if (ncol > 0) {
-    hipLaunchKernelGGL(k_deposit, dim3(ncol), dim3(1), 0, c->stream, a, ts, mult, h, elev,
+    hipLaunchKernelGGL(k_deposit, dim3(ncol), dim3(64), 0, c->stream, a, ts, mult, h, elev,
                        c->bedrock, c->mask, c->V, c->Tau, c->ss, c->sdir, gr, c->etot, c->etot3);
     HC_HIP(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_lateral_y, dim3(c->g.imax), dim3(1), 0, c->stream, a, ts, mult, h, elev,
+  hipLaunchKernelGGL(k_lateral_y, dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
                      c->bedrock, c->mask, c->Tau, gr, c->etot3);
   HC_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(k_edges, dim3(1), dim3(64), 0, c->stream, a, h, c->V, c->sdir, c->ncell,
