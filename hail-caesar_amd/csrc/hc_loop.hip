@@ -285,19 +285,34 @@ __global__ void k_loop_reset_iter(LoopCtl* ctl) { ctl->iter = 0; }
 
 // ---------------------------------------------------------------- ensemble
 // One block per member: that member's loop_control on its current depth.
-__global__ __launch_bounds__(64) void k_ens_control(const EnsMember* __restrict__ mem) {
-  const EnsMember& M = mem[blockIdx.x];
-  loop_control(M.la, table_ptr<LoopCtl>(M.ctl), table_ptr<const double>(M.top),
-               table_ptr<uint64_t>(M.pmask), table_ptr<unsigned long long>(M.red),
-               table_ptr<LoopRecord>(M.rec));
+// It also leaves the member's two per-step flags for the scan kernel:
+// flags[m] = this step scans the active area (counter % 5 == 0, as
+// k_loop_scan tests it), flags[n + m] = this step rebuilds the input points.
+__global__ __launch_bounds__(64) void k_ens_control(const EnsMember* __restrict__ mem, int n,
+                                                    int* __restrict__ flags) {
+  const int m = blockIdx.x;
+  const EnsMember& M = mem[m];
+  LoopCtl* ctl = table_ptr<LoopCtl>(M.ctl);
+  loop_control(M.la, ctl, table_ptr<const double>(M.top), table_ptr<uint64_t>(M.pmask),
+               table_ptr<unsigned long long>(M.red), table_ptr<LoopRecord>(M.rec));
+  if (threadIdx.x == 0) {
+    const bool live = !ctl->halt && !ctl->err;
+    flags[m] = (live && (ctl->counter % M.la.scan_interval) == 0) ? 1 : 0;
+    flags[n + m] = (live && ctl->pts_dirty) ? 1 : 0;
+  }
 }
 
 // Per member: temptot from its edge partials (one block each), scan_area on
 // every 5th of its own steps, and its input-point rebuild.  Scan work items
-// of all members form one list (item_base holds each member's first item).
+// of all members form one list (item_base holds each member's first item,
+// item_member each item's member); the control kernel's flags say which
+// members scan or rebuild in this step, so an item of a member that does not
+// scan costs two loads.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ens_scan(const EnsMember* __restrict__ mem, int n,
-                                                  const int* __restrict__ item_base, int R,
+                                                  const int* __restrict__ item_base,
+                                                  const int* __restrict__ item_member,
+                                                  const int* __restrict__ flags, int R,
                                                   int parity) {
   for (int m = blockIdx.x; m < n; m += gridDim.x) {
     const EnsMember& M = mem[m];
@@ -308,24 +323,27 @@ __global__ __launch_bounds__(256) void k_ens_scan(const EnsMember* __restrict__ 
   }
   const int total = item_base[n];
   for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < total; it += gridDim.x * 4) {
-    int lo = 0, hi = n;  // item_base[lo] <= it < item_base[hi]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (item_base[mid] <= it) lo = mid;
-      else hi = mid;
-    }
-    const EnsMember& M = mem[lo];
-    const LoopCtl* C = table_ptr<const LoopCtl>(M.ctl);
-    if (C->halt || C->err || (C->counter % M.la.scan_interval) != 0) continue;  // per wave
-    scan_item<T>(it - item_base[lo], M.la.imax, M.la.jmax, M.la.pitch, M.W,
+    const int m = item_member[it];
+    if (!flags[m]) continue;  // per wave
+    const EnsMember& M = mem[m];
+    scan_item<T>(it - item_base[m], M.la.imax, M.la.jmax, M.la.pitch, M.W,
                  table_ptr<const T>(M.h[M.cur0 ^ parity ^ 1]), table_ptr<uint64_t>(M.mask));
   }
-  for (int m = 0; m < n; ++m) {
-    const EnsMember& M = mem[m];
-    LoopCtl* ctl = table_ptr<LoopCtl>(M.ctl);
-    if (!ctl->halt && !ctl->err && ctl->pts_dirty)
-      rebuild_points(M.la, ctl, table_ptr<const double>(M.area), table_ptr<const int>(M.rfarea),
-                     table_ptr<uint64_t>(M.pmask));
+  // input-point rebuilds: the flags of a chunk of members are read at once
+  // and the chunk skipped when none is set (the usual case)
+  __shared__ int s_dirty[256];
+  for (int base = 0; base < n; base += 256) {
+    const int m = base + static_cast<int>(threadIdx.x);
+    const int d = (m < n) ? flags[n + m] : 0;
+    s_dirty[threadIdx.x] = d;
+    if (!__syncthreads_or(d)) continue;
+    for (int k = 0; k < 256 && base + k < n; ++k) {
+      if (!s_dirty[k]) continue;  // uniform
+      const EnsMember& M = mem[base + k];
+      rebuild_points(M.la, table_ptr<LoopCtl>(M.ctl), table_ptr<const double>(M.area),
+                     table_ptr<const int>(M.rfarea), table_ptr<uint64_t>(M.pmask));
+    }
+    __syncthreads();  // s_dirty is rewritten by the next chunk
   }
 }
 
@@ -615,7 +633,8 @@ int hc_loop_sync(hc_ctx* c, hc_loop_state* st, hc_loop_record* rec, int max_reco
 // -------------------------------------------------------------- ensemble API
 struct hc_ens {
   std::vector<hc_ctx*> m;
-  int mode = 0;           // 0 FP64, 1 FP32 with FP32 elevations, 2 FP32 with FP64 elevations
+  int mode = 0;           // 0 FP64, 1 FP32 with FP32 elevations, 2 FP32 with FP64 elevations,
+                          // 3 FP32 with fixed-point elevations
   int R = 8;              // fused tile rows
   int nblocks = 1024;     // persistent grid of the fused step
   int scan_blocks = 1024;
@@ -626,6 +645,8 @@ struct hc_ens {
   size_t tiles_cap = 0;
   int ntiles = 0;
   int* d_items = nullptr;  // n+1 prefix sums of scan items
+  int* d_item_member = nullptr;  // member of each scan item
+  int* d_flags = nullptr;        // 2n per-step flags from the control kernel
   hipGraphExec_t graph = nullptr;
   std::string err;
 };
@@ -708,17 +729,17 @@ int ens_prepare(hc_ens* e, int steps, bool run) {
 
 int ens_step(hc_ens* e, int parity) {
   const int n = static_cast<int>(e->m.size());
-  hipLaunchKernelGGL(k_ens_control, dim3(n), dim3(64), 0, e->stream, e->d_mem);
+  hipLaunchKernelGGL(k_ens_control, dim3(n), dim3(64), 0, e->stream, e->d_mem, n, e->d_flags);
   ENS_HIP(e, hipGetLastError());
   if (hc_launch_ens_fused(e->stream, e->mode, e->R, e->d_mem, e->d_tiles, e->ntiles, e->nblocks,
                           parity))
     return ens_fail(e, HC_ERR_HIP, "ensemble: fused launch failed");
   if (e->mode == 0)
     hipLaunchKernelGGL(k_ens_scan<double>, dim3(e->scan_blocks), dim3(256), 0, e->stream,
-                       e->d_mem, n, e->d_items, e->R, parity);
+                       e->d_mem, n, e->d_items, e->d_item_member, e->d_flags, e->R, parity);
   else
     hipLaunchKernelGGL(k_ens_scan<float>, dim3(e->scan_blocks), dim3(256), 0, e->stream,
-                       e->d_mem, n, e->d_items, e->R, parity);
+                       e->d_mem, n, e->d_items, e->d_item_member, e->d_flags, e->R, parity);
   ENS_HIP(e, hipGetLastError());
   return HC_OK;
 }
@@ -780,12 +801,21 @@ int hc_ens_create(hc_ens** out, hc_ctx* const* members, int n) {
     M.W = c->W;
     items[i + 1] = items[i] + hcdev::scan_items(c->g.imax, c->W);
   }
+  std::vector<int> item_member(std::max(items[n], 1), 0);
+  for (int i = 0; i < n; ++i)
+    for (int t = items[i]; t < items[i + 1]; ++t) item_member[t] = i;
   int rc = HC_OK;
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&e->d_mem), n * sizeof(EnsMember)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&e->d_items), (n + 1) * sizeof(int)) != hipSuccess ||
       hipMemcpy(e->d_items, items.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice) !=
-          hipSuccess)
+          hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&e->d_item_member), item_member.size() * sizeof(int)) !=
+          hipSuccess ||
+      hipMemcpy(e->d_item_member, item_member.data(), item_member.size() * sizeof(int),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&e->d_flags), 2 * n * sizeof(int)) != hipSuccess ||
+      hipMemset(e->d_flags, 0, 2 * n * sizeof(int)) != hipSuccess)
     rc = HC_ERR_HIP;
   if (rc == HC_OK) rc = ens_layout(e);
   if (rc != HC_OK) {
@@ -804,6 +834,8 @@ int hc_ens_destroy(hc_ens* e) {
   if (e->d_mem) (void)hipFree(e->d_mem);
   if (e->d_tiles) (void)hipFree(e->d_tiles);
   if (e->d_items) (void)hipFree(e->d_items);
+  if (e->d_item_member) (void)hipFree(e->d_item_member);
+  if (e->d_flags) (void)hipFree(e->d_flags);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return HC_OK;
@@ -885,8 +917,8 @@ int hc_ens_tune(hc_ens* e, int* best_rows, int* best_blocks, double* best_us) {
     if (rc) return rc;
     ENS_HIP(e, hipMemcpyAsync(e->d_mem, e->mem.data(), e->mem.size() * sizeof(EnsMember),
                               hipMemcpyHostToDevice, e->stream));
-    for (int per_cu : {2, 4, 8}) {
-      const int nb = std::max(8, std::min(ncu * per_cu, e->ntiles));
+    for (int per_cu : {2, 4, 8, 0}) {  // 0: one block per tile (no persistent loop)
+      const int nb = per_cu ? std::max(8, std::min(ncu * per_cu, e->ntiles)) : e->ntiles;
       // parity 0 reads each member's current generation and writes the other:
       // repeated launches leave the state unchanged
       if (hc_launch_ens_fused(e->stream, e->mode, R, e->d_mem, e->d_tiles, e->ntiles, nb, 0))

@@ -283,8 +283,9 @@ int hc_ens_run(hc_ens* ens, int steps, int use_graph);
  * state and per-step records, exactly as after hc_loop_run. */
 int hc_ens_sync(hc_ens* ens);
 /* Autotuner: times the fused ensemble kernel for tile rows {4, 8, 16} and
- * persistent grids of {2, 4, 8} blocks per CU on the current state (left
- * unchanged) and keeps the fastest.  Call with members synced. */
+ * grids of {2, 4, 8} blocks per CU (each block walking a run of tiles) or one
+ * block per tile, on the current state (left unchanged), and keeps the
+ * fastest.  Call with members synced. */
 int hc_ens_tune(hc_ens* ens, int* best_rows, int* best_blocks, double* best_us);
 int hc_ens_set_tiles(hc_ens* ens, int rows, int blocks);
 /* Average duration (us) of `reps` fused ensemble launches; state unchanged. */
