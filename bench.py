@@ -387,6 +387,8 @@ def main():
     ap.add_argument("--no-erosion", action="store_true", help="skip the flow + erosion timing")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--section", choices=["ensemble", "erosion", "adjoint"], default=None,
+                    help="run only this part (its JSON object alone; for profiler runs)")
     ap.add_argument("--accuracy-hours", type=int, default=0, metavar="H",
                     help="instead of the benchmark, run the large grid from dry for H hours in "
                          "FP32 and FP64 and report their differences")
@@ -398,6 +400,15 @@ def main():
     load_package()
     if args.accuracy_hours:
         print(json.dumps({"accuracy": accuracy_run(args.size, args.accuracy_hours)}))
+        return
+    if args.section == "ensemble":
+        print(json.dumps({"ensemble": run_ensemble(args.steps, args.warmup)}))
+        return
+    if args.section == "erosion":
+        print(json.dumps({"erosion": run_erosion(steps=args.steps, warmup=args.warmup)}))
+        return
+    if args.section == "adjoint":
+        print(json.dumps({"adjoint": run_adjoint()}))
         return
     if args.copy_probe:
         copy_probe(args.copy_probe)
