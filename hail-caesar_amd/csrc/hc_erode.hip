@@ -22,7 +22,8 @@
 //   y slide (4684-4747)       one wavefront per row x walks y in order
 //                             (writes stay in the row).
 //   edge outputs (4750-4827)  nine lanes, one per fraction, accumulate in
-//                             float in the reference order: a zeroed private
+//                             float in the reference order (from LDS chunks
+//                             staged by the whole block): a zeroed private
 //                             sum per loop, added to the total afterwards, as
 //                             GCC's OpenMP array reduction does.
 // Transcendentals (pow, exp, log, sqrt) are the device library's, so values
@@ -774,59 +775,70 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
 }
 
 // ------------------------------------------------------------ edge outputs
-__global__ __launch_bounds__(64) void k_edges(ErodeArgs a, const double* __restrict__ h,
-                                              double* __restrict__ V,
-                                              const double* __restrict__ sdir, size_t nc,
-                                              float* __restrict__ gtot) {
-  const int n = threadIdx.x;  // fraction
-  const bool act = (n >= 1 && n <= kG - 1);
+__global__ __launch_bounds__(256) void k_edges(ErodeArgs a, const double* __restrict__ h,
+                                               double* __restrict__ V,
+                                               const double* __restrict__ sdir, size_t nc,
+                                               float* __restrict__ gtot) {
+  // The reference's two loops visit the edge cells in a fixed order (entry
+  // e: y or x = 2 + e/2, the far edge for even e, the near edge for odd e),
+  // and each fraction's float total must add them in that order.  Chunks of
+  // 256 entries are staged in LDS by all threads (one entry each: its
+  // condition, V, and the eight bedload terms), then lane n adds fraction n
+  // from LDS in entry order.  The edge cells are distinct and each is read
+  // before it is cleared, so the clearing follows in a last pass.
+  constexpr int C = 256;
+  __shared__ double s_val[kG - 1][C];  // fraction n at [n-1]: V for n = 1, bedload otherwise
+  __shared__ int s_cond[C];
+  const int t = threadIdx.x;
   const int I = a.imax, J = a.jmax;
   const size_t P = a.pitch;
-  const int nn = act ? n : 1;  // lanes without a fraction read a valid plane
-  auto S = [&](int d, size_t cell) { return sdir[(d * 10 + nn) * nc + cell]; };
-  // The edge cells are distinct and each is read before it is cleared, so
-  // the sums read V as it stands and the clearing follows in a second pass;
-  // with no stores or barriers in the summing loops their loads run ahead
-  // of the (sequential, float) sums.
-  float part1 = 0.0f, part2 = 0.0f;
-#pragma unroll 8
-  for (int y = 2; y < J; ++y) {
-    const size_t cA = static_cast<size_t>(I) * P + y;
-    const double vA = V[cA];
-    const double sA = S(1, cA - P);
-    const bool condA = (h[cA] > a.wdet || vA > 0);
-    const size_t cB = static_cast<size_t>(1) * P + y;
-    const double vB = V[cB];
-    const double sB = S(3, cB + P);
-    const bool condB = (h[cB] > a.wdet || vB > 0);
-    if (act && condA) part1 = static_cast<float>(static_cast<double>(part1) + (n == 1 ? vA : sA));
-    if (act && condB) part1 = static_cast<float>(static_cast<double>(part1) + (n == 1 ? vB : sB));
+  const bool sums = (t >= 1 && t <= kG - 1);  // lane t adds fraction t
+  float part[2] = {0.0f, 0.0f};
+  for (int phase = 0; phase < 2; ++phase) {
+    const int ne = 2 * max((phase == 0 ? J : I) - 2, 0);
+    float acc = 0.0f;
+    for (int base = 0; base < ne; base += C) {
+      const int e = base + t;
+      if (e < ne) {
+        size_t c, src;
+        int d;
+        const int i = 2 + e / 2;
+        if (phase == 0) {  // rows imax (sr of the row above) and 1 (sl of the row below)
+          c = (e % 2 == 0) ? static_cast<size_t>(I) * P + i : P + i;
+          src = (e % 2 == 0) ? c - P : c + P;
+          d = (e % 2 == 0) ? 1 : 3;
+        } else {  // columns jmax (sd of the column left) and 1 (su of the column right)
+          c = (e % 2 == 0) ? static_cast<size_t>(i) * P + J : static_cast<size_t>(i) * P + 1;
+          src = (e % 2 == 0) ? c - 1 : c + 1;
+          d = (e % 2 == 0) ? 2 : 0;
+        }
+        const double v = V[c];
+        s_cond[t] = (h[c] > a.wdet || v > 0) ? 1 : 0;
+        s_val[0][t] = v;
+#pragma unroll
+        for (int n = 2; n <= kG - 1; ++n) s_val[n - 1][t] = sdir[(d * 10 + n) * nc + src];
+      }
+      __syncthreads();
+      if (sums) {
+        const int cnt = min(C, ne - base);
+        for (int k = 0; k < cnt; ++k)
+          if (s_cond[k]) acc = static_cast<float>(static_cast<double>(acc) + s_val[t - 1][k]);
+      }
+      __syncthreads();  // the chunk's LDS is rewritten by the next one
+    }
+    part[phase] = acc;
   }
-#pragma unroll 8
-  for (int x = 2; x < I; ++x) {
-    const size_t cA = static_cast<size_t>(x) * P + J;
-    const double vA = V[cA];
-    const double sA = S(2, cA - 1);
-    const bool condA = (h[cA] > a.wdet || vA > 0);
-    const size_t cB = static_cast<size_t>(x) * P + 1;
-    const double vB = V[cB];
-    const double sB = S(0, cB + 1);
-    const bool condB = (h[cB] > a.wdet || vB > 0);
-    if (act && condA) part2 = static_cast<float>(static_cast<double>(part2) + (n == 1 ? vA : sA));
-    if (act && condB) part2 = static_cast<float>(static_cast<double>(part2) + (n == 1 ? vB : sB));
-  }
-  __syncthreads();  // every lane's reads of V precede the clearing
-  for (int y = 2 + n; y < J; y += 64) {
-    const size_t cA = static_cast<size_t>(I) * P + y, cB = static_cast<size_t>(1) * P + y;
+  for (int y = 2 + t; y < J; y += C) {
+    const size_t cA = static_cast<size_t>(I) * P + y, cB = P + y;
     if (h[cA] > a.wdet || V[cA] > 0) V[cA] = 0;
     if (h[cB] > a.wdet || V[cB] > 0) V[cB] = 0;
   }
-  for (int x = 2 + n; x < I; x += 64) {
+  for (int x = 2 + t; x < I; x += C) {
     const size_t cA = static_cast<size_t>(x) * P + J, cB = static_cast<size_t>(x) * P + 1;
     if (h[cA] > a.wdet || V[cA] > 0) V[cA] = 0;
     if (h[cB] > a.wdet || V[cB] > 0) V[cB] = 0;
   }
-  if (n < 20) gtot[n] = act ? (part1 + part2) : 0.0f;
+  if (t < 20) gtot[t] = sums ? (part[0] + part[1]) : 0.0f;
 }
 
 ErodeArgs erode_args(const hc_ctx* c) {
@@ -905,7 +917,7 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
   hipLaunchKernelGGL(k_lateral_y, dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
                      c->bedrock, c->mask, c->Tau, gr, c->etot3);
   HC_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(k_edges, dim3(1), dim3(64), 0, c->stream, a, h, c->V, c->sdir, c->ncell,
+  hipLaunchKernelGGL(k_edges, dim3(1), dim3(256), 0, c->stream, a, h, c->V, c->sdir, c->ncell,
                      c->gtot);
   HC_HIP(c, hipGetLastError());
   float g[20];
