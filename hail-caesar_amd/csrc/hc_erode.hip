@@ -373,11 +373,14 @@ __device__ __forceinline__ void lateral_slide(const ErodeArgs& a, const Grain& g
 // so program order within the lane orders all accesses to it.  All other
 // values are uniform: every lane computes them alike.
 
-// A cell within one step of the sweep position.  (No member leaves padding:
-// copies of padding bytes keep the compiler from holding the cell in
-// registers.)
+// A cell within one step of the sweep position.  Its grain row is held one
+// fraction per lane (lane n holds fraction n, n <= 10; other lanes hold 0),
+// so per-fraction work is one vector operation; sums over fractions are
+// taken in the reference's order through v_readlane.  (No member leaves
+// padding: copies of padding bytes keep the compiler from holding the cell
+// in registers.)
 struct CellW {
-  double G[kG + 1];
+  double g;
   double elev, bedrock, h;
   uint64_t has;
   size_t c;
@@ -390,8 +393,17 @@ __device__ __forceinline__ double bcast(double v, int l) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | lo);
 }
 
-// A cell's state as fetched (lane n holds grain plane n); turned into a
-// CellW only when the cell enters the window, so the fetch does not wait.
+// sum of lanes lo .. hi of v, added in lane order from 0.0 (as the
+// reference's loops over n add)
+__device__ __forceinline__ double lane_sum(double v, int lo, int hi) {
+  double t = 0.0;
+#pragma unroll
+  for (int n = lo; n <= hi; ++n) t += bcast(v, n);
+  return t;
+}
+
+// A cell's state as fetched; becomes a CellW when the cell enters the
+// window, so the fetch does not wait for its loads.
 struct CellRaw {
   double gv, elev, bedrock, h;
   size_t c;
@@ -408,22 +420,17 @@ __device__ __forceinline__ void cell_fetch(CellRaw& r, const Grain& gr, size_t c
 }
 __device__ __forceinline__ void cell_take(CellW& w, const CellRaw& r) {
   w.c = r.c;
-#pragma unroll
-  for (int n = 0; n <= kG; ++n) w.G[n] = bcast(r.gv, n);
+  w.g = r.gv;
   w.elev = r.elev;
   w.bedrock = r.bedrock;
   w.h = r.h;
   w.has = r.has != 0 ? 1u : 0u;
 }
 
-// cell state back to memory (plane n by lane n, the rest by lane 0; one
-// store per plane: selecting w.G[lane] would be compiled as a dynamic index,
-// which puts the cell in scratch memory)
+// cell state back to memory (fraction n by lane n, the rest by lane 0)
 __device__ __forceinline__ void cell_store(const CellW& w, const Grain& gr, double* elev,
                                            int lane) {
-#pragma unroll
-  for (int n = 0; n <= kG; ++n)
-    if (lane == n) gr.g[static_cast<size_t>(n) * gr.nc + w.c] = w.G[n];
+  if (lane <= kG) gr.g[static_cast<size_t>(lane) * gr.nc + w.c] = w.g;
   if (lane == 0) {
     elev[w.c] = w.elev;
     gr.has[w.c] = w.has ? 1 : 0;
@@ -441,10 +448,6 @@ __device__ __forceinline__ void strata_store(const Grain& gr, size_t c, int lane
   gr.s[static_cast<size_t>(lane) * gr.nc + c] = s0;
   if (lane + 64 < 90) gr.s[static_cast<size_t>(lane + 64) * gr.nc + c] = s1;
 }
-// element j of the strata held in (s0, s1) of lane j % 64
-__device__ __forceinline__ double strata_elem(double s0, double s1, int j) {
-  return j < 64 ? __shfl(s0, j, 64) : __shfl(s1, j - 64, 64);
-}
 // new element i = old element i + d (i + d in [0, 90)), for both slots
 __device__ __forceinline__ void strata_move(double& s0, double& s1, int lane, int d, int lo,
                                             int hi) {
@@ -458,125 +461,135 @@ __device__ __forceinline__ void strata_move(double& s0, double& s1, int lane, in
   if (i1 >= lo && i1 < hi) s1 = n1;
 }
 
-// sort_active (LCM.cpp:3874-3947) on a register cell
-__device__ __forceinline__ void sort_active_w(const Grain& gr, CellW& w, const ErodeArgs& a, int lane) {
-  double total = 0.0;
+// Per-lane constants of the grain helpers: this lane's fraction, whether it
+// is one of the moving fractions 1..9, its default proportion, and the
+// proportion of fraction lane-16 (the lane that holds that fraction's
+// bottom strata layer).
+struct LaneG {
+  double dprop;   // dprop[lane] for lanes 0..10, else 0
+  double dprop9;  // dprop[lane - 16] for lanes 17..25, else 0
+  int lane;
+  int frac;       // 1 <= lane <= 9
+};
+__device__ __forceinline__ LaneG make_lane(const ErodeArgs& a, int lane) {
+  LaneG L;
+  L.lane = lane;
+  L.frac = (lane >= 1 && lane <= kG - 1) ? 1 : 0;
+  double d = 0.0, d9 = 0.0;
 #pragma unroll
-  for (int n = 0; n <= kG; ++n) total += w.G[n];
+  for (int n = 0; n <= kG; ++n) {
+    d = (lane == n) ? a.dprop[n] : d;
+    if (n >= 1 && n <= kG - 1) d9 = (lane == 16 + n) ? a.dprop[n] : d9;
+  }
+  L.dprop = d;
+  L.dprop9 = d9;
+  return L;
+}
+
+// sort_active (LCM.cpp:3874-3947) on a register cell
+__device__ __forceinline__ void sort_active_w(const Grain& gr, CellW& w, const ErodeArgs& a,
+                                              const LaneG& L) {
+  const double total = lane_sum(w.g, 0, kG);
   if (total > (a.active * 1.5)) {
     double s0, s1;
-    strata_load(gr, w.c, lane, s0, s1);
-    strata_move(s0, s1, lane, -9, 9, 90);  // layers 9..1 take the layer above (z-1)
+    strata_load(gr, w.c, L.lane, s0, s1);
+    strata_move(s0, s1, L.lane, -9, 9, 90);  // layers 9..1 take the layer above (z-1)
     const double coeff = a.active / total;
-#pragma unroll
-    for (int n = 1; n <= kG - 1; ++n) {
-      const double gv = w.G[n];
-      if (gv > 0.0) {
-        const double amount = coeff * (gv);
-        if (lane == n - 1) s0 = amount;  // layer 0, fraction n-1
-        w.G[n] = gv - amount;
-      }
-    }
-    strata_store(gr, w.c, lane, s0, s1);
+    const bool pos = L.frac && w.g > 0.0;
+    const double amount = coeff * (w.g);
+    // layer 0, fraction n-1 lives in lane n-1: it takes lane n's amount
+    const double am_next = __shfl_down(amount, 1, 64);
+    const bool pos_next = __shfl_down(pos ? 1 : 0, 1, 64) != 0;
+    if (L.lane <= kG - 2 && pos_next) s0 = am_next;
+    if (pos) w.g = w.g - amount;
+    strata_store(gr, w.c, L.lane, s0, s1);
   }
   if (total < (a.active / 4)) {
     double s0, s1;
-    strata_load(gr, w.c, lane, s0, s1);
-#pragma unroll
-    for (int n = 1; n <= kG - 1; ++n) w.G[n] += bcast(s0, n - 1);
-    strata_move(s0, s1, lane, 9, 0, 81);  // layers 0..8 take the layer below (z+1)
-    const double amount = a.active;
-#pragma unroll
-    for (int n = 1; n <= kG - 1; ++n)
-      if (lane + 64 == 81 + n - 1) s1 = amount * a.dprop[n];  // layer 9, fraction n-1
-    strata_store(gr, w.c, lane, s0, s1);
+    strata_load(gr, w.c, L.lane, s0, s1);
+    const double s_prev = __shfl_up(s0, 1, 64);  // element (0, n-1) for lane n
+    if (L.frac) w.g += s_prev;
+    strata_move(s0, s1, L.lane, 9, 0, 81);  // layers 0..8 take the layer below (z+1)
+    // layer 9, fraction n-1: element 80 + n, lane 16 + n slot 1
+    const int n9 = L.lane - 16;
+    if (n9 >= 1 && n9 <= kG - 1) s1 = a.active * L.dprop9;
+    strata_store(gr, w.c, L.lane, s0, s1);
   }
 }
 
 // addGS (LCM.cpp:3949-3983) on a register cell
-__device__ __forceinline__ void add_gs_w(const Grain& gr, CellW& w, const ErodeArgs& a, int lane) {
+__device__ __forceinline__ void add_gs_w(const Grain& gr, CellW& w, const ErodeArgs& a,
+                                         const LaneG& L) {
   w.has = 1;
-  w.G[0] = 0;
-#pragma unroll
-  for (int n = 1; n <= kG - 1; ++n) w.G[n] = a.active * a.dprop[n];
-  w.G[kG] = 0;
+  w.g = L.frac ? a.active * L.dprop : 0.0;  // fractions 0 and 10 start empty
   auto val = [&](int i) {
-    const int z = i / 9, n2 = i % 9;
+    const int z = i / 9;
     const bool zero = w.elev - (a.active * (z + 1)) < (w.bedrock - a.active);
-    return zero ? 0.0 : (a.active) * a.dprop[n2 + 1];
+    return zero ? 0.0 : (a.active) * a.dprop[i % 9 + 1];
   };
-  strata_store(gr, w.c, lane, val(lane), (lane + 64 < 90) ? val(lane + 64) : 0.0);
-  sort_active_w(gr, w, a, lane);
+  strata_store(gr, w.c, L.lane, val(L.lane), (L.lane + 64 < 90) ? val(L.lane + 64) : 0.0);
+  sort_active_w(gr, w, a, L);
 }
 
 __device__ __forceinline__ void sort_active_alloc_w(const Grain& gr, CellW& w, const ErodeArgs& a,
-                                                    int lane) {
-  if (!w.has) add_gs_w(gr, w, a, lane);
-  sort_active_w(gr, w, a, lane);
+                                                    const LaneG& L) {
+  if (!w.has) add_gs_w(gr, w, a, L);
+  sort_active_w(gr, w, a, L);
 }
 
-// slide_GS (LCM.cpp:4077-4171) between register cells: `src` to `dst`
+// slide_GS (LCM.cpp:4077-4171) between register cells: `src` to `dst`, one
+// fraction per lane (lanes 1..9)
 __device__ __forceinline__ void slide_gs_w(const Grain& gr, CellW& src, CellW& dst, double amount,
-                           const ErodeArgs& a, int lane) {
-  double total = 0;
+                                           const ErodeArgs& a, const LaneG& L) {
+  const bool f = L.frac != 0;
+  // sum over n of G(src, n) where positive, in n order (adding 0.0 elsewhere
+  // leaves a non-negative running sum unchanged)
+  auto pos_total = [&]() { return lane_sum((f && src.g > 0) ? src.g : 0.0, 1, kG - 1); };
   if (src.has && dst.has) {
-#pragma unroll
-    for (int n = 1; n <= kG - 1; ++n)
-      if (src.G[n] > 0) total += src.G[n];
+    const double total = pos_total();
     if (amount > total) {
-#pragma unroll
-      for (int n = 1; n <= kG - 1; ++n) dst.G[n] += (amount - total) * a.dprop[n];
+      if (f) dst.g += (amount - total) * L.dprop;
       amount = total;
     }
-    if (total > 0) {
-#pragma unroll
-      for (int n = 1; n <= kG - 1; ++n) {
-        const double tr = amount * (src.G[n] / total);
-        dst.G[n] += tr;
-        src.G[n] -= tr;
-        if (src.G[n] < 0) src.G[n] = 0;
-      }
+    if (total > 0 && f) {
+      const double tr = amount * (src.g / total);
+      dst.g += tr;
+      src.g -= tr;
+      if (src.g < 0) src.g = 0;
     }
-    sort_active_alloc_w(gr, src, a, lane);
-    sort_active_alloc_w(gr, dst, a, lane);
+    sort_active_alloc_w(gr, src, a, L);
+    sort_active_alloc_w(gr, dst, a, L);
     return;
   }
   if (!src.has && dst.has) {
-#pragma unroll
-    for (int n = 1; n <= kG - 1; ++n) dst.G[n] += (amount) * a.dprop[n];
-    sort_active_alloc_w(gr, dst, a, lane);
+    if (f) dst.g += (amount) * L.dprop;
+    sort_active_alloc_w(gr, dst, a, L);
     return;
   }
   if (src.has && !dst.has) {
-    add_gs_w(gr, dst, a, lane);
+    add_gs_w(gr, dst, a, L);
     if (amount > a.active) {
-#pragma unroll
-      for (int n = 1; n <= kG - 1; ++n) dst.G[n] += amount * a.dprop[n];
+      if (f) dst.g += amount * L.dprop;
       amount = a.active;
     }
-#pragma unroll
-    for (int n = 1; n <= kG - 1; ++n)
-      if (src.G[n] > 0) total += src.G[n];
-#pragma unroll
-    for (int n = 1; n <= kG - 1; ++n) {
-      if (total > 0) {
-        dst.G[n] += amount * (src.G[n] / total);
-        if (src.G[n] > 0.0001) src.G[n] -= amount * (src.G[n] / total);
-        if (src.G[n] < 0) src.G[n] = 0;
-      }
+    const double total = pos_total();
+    if (total > 0 && f) {
+      dst.g += amount * (src.g / total);
+      if (src.g > 0.0001) src.g -= amount * (src.g / total);
+      if (src.g < 0) src.g = 0;
     }
-    sort_active_alloc_w(gr, src, a, lane);
-    sort_active_alloc_w(gr, dst, a, lane);
+    sort_active_alloc_w(gr, src, a, L);
+    sort_active_alloc_w(gr, dst, a, L);
     return;
   }
 }
 
-// one lateral slide from neighbour `n` into cell `w` (LCM.cpp:4619-4664,
+// one lateral slide from neighbour `nb` into cell `w` (LCM.cpp:4619-4664,
 // 4689-4742; veg and edge are zero in this build): as lateral_slide above
 __device__ __forceinline__ void lateral_slide_w(const ErodeArgs& a, const Grain& gr, double mult,
                                                 double time_step, double tau_c, double e3,
                                                 CellW& nb, CellW& w, bool at_edge,
-                                                double& elev_update, int lane) {
+                                                double& elev_update, const LaneG& L) {
   double amt = 0;
   if (nb.h < a.wdet)
     amt = mult * a.lateral_constant * tau_c * 0.0 * time_step / a.DX;
@@ -588,7 +601,7 @@ __device__ __forceinline__ void lateral_slide_w(const ErodeArgs& a, const Grain&
     if (amt > a.ERODEFACTOR * 0.1) amt = a.ERODEFACTOR * 0.1;
     elev_update += amt;
     nb.elev -= amt;
-    slide_gs_w(gr, nb, w, amt, a, lane);
+    slide_gs_w(gr, nb, w, amt, a, L);
   }
 }
 
@@ -612,6 +625,7 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
   const int y = 2 + blockIdx.x;
   if (y >= a.jmax || a.imax < 3) return;
   const int lane = threadIdx.x;
+  const LaneG L = make_lane(a, lane);
   const size_t P = a.pitch, nc = gr.nc;
   const int tn = 2 + (lane >> 3), tt = lane & 7;  // this lane's bedload term
   const int td = (tt < 4) ? tt : (tt == 4 ? 0 : (tt == 5 ? 2 : (tt == 6 ? 3 : 1)));
@@ -647,39 +661,43 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
     if (x + 2 <= a.imax) cell_fetch(rn, gr, static_cast<size_t>(x + 2) * P + y, h, elev, bedrock, lane);
     if (in.act) {
       const size_t c = wc.c;
-      if (!wc.has) add_gs_w(gr, wc, a, lane);
-      // bedload sums of fractions 2..9 in the reference's order
+      if (!wc.has) add_gs_w(gr, wc, a, L);
+      // bedload sums of fractions 2..9 in the reference's order (lanes 8k
+      // and 8k+4), then moved to lane n = 2 + k
       const double t1 = in.term + __shfl_down(in.term, 1, 8);
       const double t2 = t1 + __shfl_down(in.term, 2, 8);
       const double sum4 = t2 + __shfl_down(in.term, 3, 8);
+      const bool bl = lane >= 2 && lane <= kG - 1;
+      const int src1 = bl ? 8 * (lane - 2) : 0;
+      const double val1 = __shfl(sum4, src1, 64);
+      const double val2 = __shfl(sum4, src1 + 4, 64);
+      const double dv = val2 - val1;
       double et = 0, et3 = 0;
       {  // fraction 1: suspended load
         const int n = 1;
         const double v = in.V + in.ss;
-        wc.G[n] -= in.ss;
+        if (lane == n) wc.g -= in.ss;
         et -= in.ss;
         double coeff = (a.fall[n] * time_step) / wc.h;
         if (coeff > 1) coeff = 1;
         double drop = coeff * v;
         if (drop > 0.001) drop = 0.001;
-        wc.G[n] += drop;
+        if (lane == n) wc.g += drop;
         et += drop;
         if (lane == 0) V[c] = v - drop;
       }
+      if (bl) wc.g += dv;  // fractions 2..9 at once
 #pragma unroll
-      for (int n = 2; n <= kG - 1; ++n) {
-        const double val1 = bcast(sum4, 8 * (n - 2));
-        const double val2 = bcast(sum4, 8 * (n - 2) + 4);
-        wc.G[n] += val2 - val1;
-        et += val2 - val1;
-        et3 += val1;
+      for (int n = 2; n <= kG - 1; ++n) {  // the running sums in fraction order
+        et += bcast(dv, n);
+        et3 += bcast(val1, n);
       }
       if (lane == 0) {
         etot[c] = et;
         etot3[c] = et3;
       }
       wc.elev += et;
-      if (et < 0) sort_active_alloc_w(gr, wc, a, lane);
+      if (et < 0) sort_active_alloc_w(gr, wc, a, L);
       if (et3 > 0) {
         double elev_update = 0;
         // (the neighbours go through the slide by value: two calls on
@@ -688,13 +706,13 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
         if (wm.elev > wc.elev && x > 2) {
           CellW nb = wm;
           lateral_slide_w(a, gr, mult, time_step, in.tau, et3, nb, wc, x - 1 == 1, elev_update,
-                          lane);
+                          L);
           wm = nb;
         }
         if (wp.elev > wc.elev && x < a.imax - 1) {
           CellW nb = wp;
           lateral_slide_w(a, gr, mult, time_step, in.tau, et3, nb, wc, x + 1 == a.imax,
-                          elev_update, lane);
+                          elev_update, L);
           wp = nb;
         }
         wc.elev += elev_update;
@@ -723,6 +741,7 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
   const int x = 1 + blockIdx.x;
   if (x > a.imax || a.jmax < 3) return;
   const int lane = threadIdx.x;
+  const LaneG L = make_lane(a, lane);
   const size_t row = static_cast<size_t>(x) * a.pitch;
   struct In {
     double e3, tau;
@@ -753,13 +772,13 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
       if (wl.elev > wc.elev) {  // (by value: as in the deposition sweep)
         CellW nb = wl;
         lateral_slide_w(a, gr, mult, time_step, in.tau, in.e3, nb, wc, y - 1 == 1, elev_update,
-                        lane);
+                        L);
         wl = nb;
       }
       if (wr.elev > wc.elev) {
         CellW nb = wr;
         lateral_slide_w(a, gr, mult, time_step, in.tau, in.e3, nb, wc, y + 1 == a.jmax,
-                        elev_update, lane);
+                        elev_update, L);
         wr = nb;
       }
       wc.elev += elev_update;
