@@ -137,6 +137,49 @@ def test_fixed_point_elevations_high_datum(tmp_path):
     assert err["z32"] > 10 * err["fixed"]
 
 
+@pytest.mark.parametrize("relief", ["flat", "alpine"])
+def test_fixed_point_elevations_extreme_relief(relief):
+    """Fixed-point quantisation at the ends of its range: a DEM with no relief
+    (the scale is capped, so both forms see dz = 0 and agree exactly), and a
+    gently sloping plateau at 8000 m with one pit at -500 m, so the quantum is
+    coarse (8500 m / 2^31 = 4e-6 m, against face differences of ~5 cm).  The
+    uploaded DEM downloads unchanged; 20 steps track the FP64-elevation form
+    to 1e-4 relative depth (flat: exactly)."""
+    from hail_caesar_amd import hc
+    n = 96
+    rng = np.random.Generator(np.random.PCG64(7))
+    if relief == "flat":
+        dem = np.full((n, n), 12.5)
+    else:
+        yy, xx = np.mgrid[0:n, 0:n]
+        dem = 8000.0 - 0.05 * yy + rng.normal(0, 0.01, (n, n))
+        dem[n // 2, 5] = -500.0
+    e = np.full((n + 2, n + 2), -9999.0)
+    e[1:-1, 1:-1] = dem
+    e[1:-1, 1] = -9999.0  # a no-data column inside the grid
+    hgt = np.zeros_like(e)
+    hgt[1:-1, 1:-1] = 0.3 + 0.1 * rng.random((n, n))
+    mask = np.zeros(e.shape, np.int32)
+    mask[1:-1, 1:-1] = 1
+    ctxs = {}
+    for name, kw in (("fixed", dict(elev_fixed=1)), ("z64", dict(flat_dz=-1.0))):
+        p = hc.default_params(hflow_threshold=0.001, edgeslope=0.005, mannings=0.04, **kw)
+        c = hc.Context(n, n, 5.0, p, hc.HC_FP32)
+        c.upload("elev", e)
+        c.upload("water_depth", hgt)
+        c.upload("mask", mask)
+        ctxs[name] = c
+    assert np.array_equal(ctxs["fixed"].download("elev"), e)
+    for _ in range(20):
+        for c in ctxs.values():
+            c.flow_depth(0.02)
+    hf, hz = ctxs["fixed"].download("water_depth"), ctxs["z64"].download("water_depth")
+    wet = hz > 1e-3
+    err = float(np.max(np.abs(hf - hz)[wet] / hz[wet]))
+    print(f"{relief}: fixed vs FP64 elevations, max relative depth difference {err:.3g}")
+    assert err == 0.0 if relief == "flat" else err < 1e-4
+
+
 def test_health_check(tmp_path):
     w, o = wet_flow_state(tmp_path, size=32, steps=0)
     ctx = to_context(o)
