@@ -69,6 +69,49 @@ __device__ __forceinline__ double lane_down1(double v) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<uint32_t>(lo));
 }
 
+// The same shifts with the vacated lane (lane 0 for lane_up1, lane 63 for
+// lane_down1) taking `edge` instead: with bound_ctrl off, a lane whose source
+// lies outside the wavefront keeps the instruction's old value.
+__device__ __forceinline__ int dpp_up1(int v, int edge) {
+  return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_down1(int v, int edge) {
+  return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float lane_up1(float v, float edge) {
+  return __int_as_float(dpp_up1(__float_as_int(v), __float_as_int(edge)));
+}
+__device__ __forceinline__ float lane_down1(float v, float edge) {
+  return __int_as_float(dpp_down1(__float_as_int(v), __float_as_int(edge)));
+}
+__device__ __forceinline__ int32_t lane_up1(int32_t v, int32_t edge) { return dpp_up1(v, edge); }
+__device__ __forceinline__ double lane_up1(double v, double edge) {
+  const long long b = __double_as_longlong(v), e = __double_as_longlong(edge);
+  const int lo = dpp_up1(static_cast<int>(b), static_cast<int>(e));
+  const int hi = dpp_up1(static_cast<int>(b >> 32), static_cast<int>(e >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<uint32_t>(lo));
+}
+__device__ __forceinline__ double lane_down1(double v, double edge) {
+  const long long b = __double_as_longlong(v), e = __double_as_longlong(edge);
+  const int lo = dpp_down1(static_cast<int>(b), static_cast<int>(e));
+  const int hi = dpp_down1(static_cast<int>(b >> 32), static_cast<int>(e >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<uint32_t>(lo));
+}
+
+// Lane k's value, uniform over the wavefront (v_readlane; k uniform).
+__device__ __forceinline__ float lane_value(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ int32_t lane_value(int32_t v, int k) {
+  return __builtin_amdgcn_readlane(v, k);
+}
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), k);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), k);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<uint32_t>(lo));
+}
+
 __device__ __forceinline__ double dmax_ref(double a, double b) {
   return (a < b) ? b : a;  // std::max(a, b)
 }

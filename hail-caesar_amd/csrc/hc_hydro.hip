@@ -194,14 +194,12 @@ __global__ __launch_bounds__(256) void k_depth64(HydroArgs a, double dt, double*
 // buffers and writing generation 1 for every cell (unmasked cells are copied,
 // which keeps the reference's stale-flux semantics).
 //
-// Tiling: a wavefront covers 64 consecutive columns starting at 62*w and owns
-// the 62 in the middle; its first and last lanes are halo columns that only
-// feed the y-face neighbours through lane shuffles (the left neighbour's
-// depth and elevation, the right neighbour's new y-face flux), so no lane
-// issues a special load.  Column 0 (padding) is owned by lane 0 of wavefront
-// 0.  Each wavefront takes R output rows and first loads its whole tile
-// (R+2 rows of h and z, R+1 rows of qx, qy, n and the mask) so all of a
-// tile's loads are in flight together; then it computes the rows in order.
+// Tiling: a wavefront owns 64 consecutive columns starting at 64*w; the
+// y-faces on its two sides take their neighbour values from the adjacent
+// wavefronts through LDS, or from one extra column load on a block's sides.
+// Each wavefront takes R output rows and first loads its whole tile (R+2
+// rows of h and z, R+1 rows of qx, qy, n and the mask) so all of a tile's
+// loads are in flight together; then it computes the rows in order.
 // Blocks are four wavefronts side by side; consecutive block ids go to the
 // eight XCDs in turn, so the block id is remapped to give each XCD a
 // contiguous band of tiles (whose halo rows then hit the same L2).
@@ -255,22 +253,39 @@ struct FusedPtrs {
   hc::InjectArgs inj;  // rain input at the input points (device loop only)
 };
 
-// One tile: a strip of nsub x R output rows by four wavefronts x 62 owned
-// columns.  A wavefront covers 64 columns and owns the middle 62 (lane 0 of
-// wavefront 0 also owns column 0); y-neighbours come from DPP lane shifts.
-// (Wavefronts aligned to 64 columns need the halo columns from extra narrow
-// loads and a lane-batched east face per sub-tile; measured, that costs more
-// than the aligned rows save.)  The strip is walked in sub-tiles of R rows:
-// the first loads rows r0-1 .. r0+R of every field into registers up front,
-// each later one loads only its R new rows and carries the two rows above it
-// and the new fluxes of its first row from the sub-tile before (no halo rows
-// are re-read and no face is computed twice).  Rows past the grid's first
-// and last rows load the nearest grid row instead: their mask bits are 0, so
-// no face or depth of theirs is used or stored, and the loads need no
-// branch.  Loads and stores address a row by a 32-bit byte offset from the
-// field's base.  Every thread of the block calls this function (it ends in
-// block reductions).
-template <int MODE, int R>
+// One tile: a strip of nsub x R output rows by four wavefronts side by side.
+//
+// FP32 forms (aligned): wavefront w owns the 64 columns 64w .. 64w+63 (its
+// rows are whole aligned 256-byte segments).  The y-faces need one column on
+// either side: column 64w-1's depth and elevation for lane 0's west face,
+// and the new flux of the face east of lane 63, which belongs to column
+// 64w+64.  Inside the block the neighbouring wavefronts supply them through
+// LDS (one barrier per sub-tile); on the block's own west and east sides
+// wavefronts 0 and 3 fetch that column with one row per lane.  The east
+// faces of all the sub-tile's rows are computed together, lane k taking row
+// k, by the same y-face code as the row's own faces, and enter the row's
+// lane shift as the value shifted into lane 63 (DPP with the old value kept).
+//
+// Overlapping (stride 62): wavefront w covers the 64 columns from 62w and
+// owns the 62 in the middle (lane 0 of wavefront 0 also owns column 0); its
+// first and last lanes are halo columns, so the y-neighbours come from lane
+// shifts alone, with no exchange or barrier.  The FP64 form is bound by its
+// arithmetic, which the aligned form's recomputed east faces would add to
+// (1/(2R) more y-faces), and the ensemble's tiles are small and mostly on a
+// member's edge, where the aligned form's per-tile exchange does not pay;
+// both use this form.  The FP32 single grid is bound by its loads, which the
+// aligned rows make cheaper.
+//
+// The strip is walked in sub-tiles of R rows: the first loads rows r0-1 ..
+// r0+R of every field into registers up front, each later one loads only
+// its R new rows and carries the two rows above it and the new fluxes of its
+// first row from the sub-tile before (no halo rows are re-read and no face
+// is computed twice).  Rows past the grid's first and last rows load the
+// nearest grid row instead: their mask bits are 0, so no face or depth of
+// theirs is used or stored, and the loads need no branch.  Loads and stores
+// address a row by a 32-bit byte offset from the field's base.  Every thread
+// of the block calls this function (it has block barriers).
+template <int MODE, int R, bool AL>
 __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, int nbx, int nsub,
                                            double dt_arg, const double* dtp, const int* haltp,
                                            const FusedPtrs<MODE>& p, const DrainArgs& drain) {
@@ -287,20 +302,27 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   T* __restrict__ qx1 = p.qx1;
   T* __restrict__ qy1 = p.qy1;
   unsigned long long* red = p.red;
+  static_assert(R + 2 <= 32, "rows of a sub-tile are indexed by lanes 0..31");
   const int lane = threadIdx.x & 63;
-  const int wu = __builtin_amdgcn_readfirstlane(bx * 4 + (threadIdx.x >> 6));
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wavefront in the block
+  const int kk = lane & 31;         // row of the lane-indexed vectors
+  const int wu = __builtin_amdgcn_readfirstlane(bx * 4 + wv);
   const int NR = a.imax + 2, NC = a.jmax + 2;
   const int H = R * nsub;  // rows per block
-  const bool active_wave = (62 * wu + 1 <= NC - 1);  // has an owned column
-  const int c = 62 * wu + lane;
-  const bool owned = ((lane >= 1 && lane <= 62) || (wu == 0 && lane == 0)) && c <= NC - 1;
+  constexpr int SW = AL ? hc::kStrideAligned : hc::kStrideOverlap;  // columns per wavefront
+  const bool active_wave = AL ? (64 * wu <= NC - 1) : (62 * wu + 1 <= NC - 1);
+  const int c = SW * wu + lane;
+  const bool owned = (AL || (lane >= 1 && lane <= 62) || (wu == 0 && lane == 0)) && c <= NC - 1;
   const uint32_t P = static_cast<uint32_t>(a.pitch);
-  const uint32_t cl = static_cast<uint32_t>(min(c, a.pitch - 1));  // lanes past the row reuse the last column
+  const uint32_t cl = static_cast<uint32_t>(min(c, a.pitch - 1));
+  // the block's outer halo columns (column 0 stands in west of column 0,
+  // whose faces are never used)
+  const uint32_t cw = static_cast<uint32_t>(max(64 * wu - 1, 0));
+  const uint32_t ce = static_cast<uint32_t>(min(64 * wu + 64, a.pitch - 1));
 
   T hh[R + 2];
   Z zt[R + 2];
   T qxo[R + 2], qyo[R + 2], nn[R + 2];
-  uint32_t ob[R + 2];  // element offset of the lane's cell in row k
   // rain input at input points: one amount for the whole grid when every
   // cell has the same rain cell (added with a select, no branch)
   const bool inject = p.inj.pmask != nullptr;
@@ -312,28 +334,34 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     }
     return pt ? static_cast<T>(static_cast<double>(v) + p.inj.amt[p.inj.rfarea[o]]) : v;
   };
-  // Bit masks (active cells, input points).  The wavefront's 64 columns
-  // span at most two 64-bit words of a row.  One load per sub-tile: lane k
-  // (< R+2) fetches both words of the active mask in row r0-1+k, lane 32+k
-  // those of the input-point mask, and funnels out the 64 bits under the
-  // wavefront (bit l = column 62w+l; columns past the last word read 0).
-  // A row's 64 bits, read back with v_readlane, are then the lane mask of
-  // its predicate as they stand (inverse ballot): no per-lane bit work.
-  static_assert(R + 2 <= 32, "mask rows are fetched by lanes 0..31 and 32..63");
-  const int w0 = (62 * wu) >> 6;
-  const int off = 62 * wu - 64 * w0;
+  // Bit masks (active cells, input points).  The wavefront's 64 columns are
+  // one 64-bit word of a row (aligned) or span two (stride 62).  One load per
+  // sub-tile: lane k (< R+2) fetches the active-mask words of row r0-1+k,
+  // lane 32+k those of the input-point mask, and funnels out the 64 bits
+  // under the wavefront (bit l = column SW*w+l; columns past the last word
+  // read 0).  A row's 64 bits, read back with v_readlane, are then the lane
+  // mask of its predicate as they stand (inverse ballot): no per-lane bit
+  // work.  Aligned, lane k also keeps bit 0 of the next word: the east
+  // column's active bit.
+  const int w0 = AL ? wu : (62 * wu) >> 6;
+  const int wsh = AL ? 0 : 62 * wu - 64 * w0;  // bit of column SW*w in word w0
+  uint32_t east_m = 0;
   auto load_win = [&](int r0) -> uint64_t {
-    const int k = lane & 31;
     const bool pm = lane >= 32;
-    const int x = r0 - 1 + k;
+    const int x = r0 - 1 + kk;
     uint64_t m0 = 0, m1 = 0;
-    if (k < R + 2 && x >= 0 && x < NR && (!pm || inject)) {
+    if (kk < R + 2 && x >= 0 && x < NR && (!pm || inject)) {
       const uint64_t* base = pm ? p.inj.pmask : mask;
       const size_t ro = static_cast<size_t>(x) * a.mwords + w0;
       m0 = base[ro];
       if (w0 + 1 < a.mwords) m1 = base[ro + 1];
     }
-    return off ? ((m0 >> off) | (m1 << (64 - off))) : m0;
+    if constexpr (AL) {
+      east_m = static_cast<uint32_t>(m1) & 1u;
+      return m0;
+    } else {
+      return wsh ? ((m0 >> wsh) | (m1 << (64 - wsh))) : m0;
+    }
   };
   auto row_pred = [&](uint64_t win, int src) -> bool {
     const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(win), src);
@@ -344,7 +372,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   T md = 0;
   // water_flux_out (LCM.cpp:3005-3047) on this tile's edge cells, after
   // depth_update and before the store; maxdepth is taken before the drain
-  const bool drains = drain.ts != nullptr && fused_edge_tile(bx, by, H, a.imax, a.jmax);
+  const bool drains = drain.ts != nullptr && fused_edge_tile(bx, by, H, a.imax, a.jmax, SW);
   double ts = 1.0;
   double dacc = 0;
   // In the device-resident loop dt, the halt flag, the time step and the
@@ -382,30 +410,154 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   // y-face slope = slope * ymul + yadd: the edge override without a lane mask
   const float ymul = ycode != 0 ? 0.0f : 1.0f;
   const float yadd = ycode < 0 ? -es : (ycode > 0 ? es : 0.0f);
+  const int ycode_e = edge_code(64 * wu + 64, a.jmax);  // the east column's
+  const float ymul_e = ycode_e != 0 ? 0.0f : 1.0f;
+  const float yadd_e = ycode_e < 0 ? -es : (ycode_e > 0 ? es : 0.0f);
 
-  // load row k (x = r0 - 1 + k) of every field
+  // New flux of the y-face between a cell (qo, h, z, n, active bit m, offset
+  // o, slope terms) and its west neighbour (hl, zl): the face of row() below,
+  // also used for the lane-batched east faces, so both give the same bits.
+  auto yface = [&](T qo, T hcell, T hl, Z zc, Z zl, T ncell, bool m, int code, float smul,
+                   float sadd, uint32_t o) -> T {
+    T qN = qo;
+    if constexpr (MODE == 0) {
+      if (m && zc > Z(-9999)) {
+        double dummy = 0;
+        face64<false>(qN, dummy, hcell, hl, zc, zl, ncell, dtd, a, code, false, 0, 0, nullptr,
+                      nullptr);
+      }
+    } else {
+      float dz;
+      if constexpr (MODE == 2) {
+        dz = static_cast<float>(zl - zc);
+      } else if constexpr (MODE == 3) {
+        dz = static_cast<float>(static_cast<int32_t>(static_cast<uint32_t>(zl) -
+                                                     static_cast<uint32_t>(zc))) *
+             a.zq_inv;
+      } else {
+        dz = zl - zc;
+        if (fthr > 0.0f && m && fabsf(dz) < fthr) dz = static_cast<float>(z64[o - 1] - z64[o]);
+      }
+      float fy = face32(qo, hcell, hl, dz, ncell * ncell, smul, sadd, K);
+      asm volatile("" : "+v"(fy));  // on every lane, not under the select
+      qN = (m & z_ok(zc) & ((hcell > 0.0f) | (hl > 0.0f)) & z_ok(zl)) ? fy : qo;
+    }
+    return qN;
+  };
+
+  // Element offset of the lane's cell in row k (x = r0 - 1 + k; rows past
+  // the grid: the nearest grid row).  The row part is uniform and made
+  // afresh in a scalar register at each use, so no per-row offsets stay live
+  // in vector registers.
+  auto off = [&](int k, int r0) -> uint32_t {
+    uint32_t rb = __builtin_amdgcn_readfirstlane(min(max(r0 - 1 + k, 0), NR - 1) * P);
+    asm volatile("" : "+s"(rb));
+    return rb + cl;
+  };
+  // load row k of every field
   auto load_row = [&](int k, int r0) {
-    const int xc = min(max(r0 - 1 + k, 0), NR - 1);
-    ob[k] = static_cast<uint32_t>(xc) * P + cl;
-    hh[k] = ld_elem(h0, ob[k]);
-    zt[k] = ld_elem(zz, ob[k]);
+    const uint32_t o = off(k, r0);
+    hh[k] = ld_elem(h0, o);
+    zt[k] = ld_elem(zz, o);
     if (k >= 1) {
-      qxo[k] = ld_elem(qx0, ob[k]);
-      qyo[k] = ld_elem(qy0, ob[k]);
+      qxo[k] = ld_elem(qx0, o);
+      qyo[k] = ld_elem(qy0, o);
     }
   };
-  auto load_n = [&](int k0) {
+  auto load_n = [&](int k0, int r0) {
     if (a.spatial_n) {
 #pragma unroll
-      for (int k = k0; k < R + 2; ++k) {
-        uint32_t o = ob[k];
-        asm volatile("" : "+v"(o));  // offset made in this block: scalar-base load
-        nn[k] = ld_elem(nman, o);
-      }
+      for (int k = k0; k < R + 2; ++k) nn[k] = ld_elem(nman, off(k, r0));
     } else {
 #pragma unroll
       for (int k = k0; k < R + 2; ++k) nn[k] = nu;
     }
+  };
+
+  // Halo columns as lane-indexed vectors (lane k = row r0-1+k): west column
+  // (depth, elevation), east column (depth, elevation, old y-flux, n, offset)
+  // and this wavefront's own column 63 (depth, elevation).
+  T hw = 0, he = 0, qe = 0, ne = 0, h63 = 0;
+  Z zw = 0, ze = 0, z63 = 0;
+  uint32_t ork = 0;  // lane k: offset of row r0-1+k (rows past the grid: the nearest)
+  // the block's outer columns, one row per lane (wavefronts 0 and 3).  The
+  // halo variables are assigned on every path by value selects: a store to
+  // one of two variables in the arms of a branch becomes a store through a
+  // selected address, which puts both variables in scratch memory.
+  auto load_outer = [&](int r0) {
+    ork = static_cast<uint32_t>(min(max(r0 - 1 + min(kk, R + 1), 0), NR - 1)) * P;
+    if (wv == 0 || wv == 3) {
+      const bool west = wv == 0;
+      const uint32_t o = ork + (west ? cw : ce);
+      const T hv = ld_elem(h0, o);
+      const Z zv = ld_elem(zz, o);
+      hw = west ? hv : hw;
+      zw = west ? zv : zw;
+      he = west ? he : hv;
+      ze = west ? ze : zv;
+    }
+    if (wv == 3) {
+      qe = ld_elem(qy0, ork + ce);
+      ne = nu;
+      if (a.spatial_n) ne = ld_elem(nman, ork + ce);
+    }
+  };
+  auto rain_outer = [&](int r0) {
+    if (wv != 0 && wv != 3) return;
+    const bool west = wv == 0;
+    const uint32_t col = west ? cw : ce;
+    const int x = r0 - 1 + kk;
+    bool pt = false;
+    if (kk < R + 2 && x >= 0 && x < NR)
+      pt = (p.inj.pmask[static_cast<size_t>(x) * a.mwords + (col >> 6)] >> (col & 63)) & 1u;
+    const T hv = rain(west ? hw : he, pt, ork + col);
+    hw = west ? hv : hw;
+    he = west ? he : hv;
+  };
+  // Edge values through LDS, double-buffered by sub-tile parity so one
+  // barrier per sub-tile separates each write from the reads of it and the
+  // reads from the next writes to the same buffer.
+  constexpr int LB = AL ? 2 : 1, LW = AL ? 4 : 1, LR = AL ? 32 : 1;  // none for stride 62
+  __shared__ T s_eh[LB][LW][LR], s_wh[LB][LW][LR], s_wq[LB][LW][LR], s_wn[LB][LW][LR];
+  __shared__ Z s_ez[LB][LW][LR], s_wz[LB][LW][LR];
+  auto exchange = [&](auto k0c, int par) {
+    constexpr int k0 = decltype(k0c)::value;  // first row written: a constant
+    if (active_wave && (lane == 0 || lane == 63)) {
+      T* dh = lane == 0 ? s_wh[par][wv] : s_eh[par][wv];
+      Z* dz = lane == 0 ? s_wz[par][wv] : s_ez[par][wv];
+#pragma unroll
+      for (int k = k0; k < R + 2; ++k) {
+        dh[k] = hh[k];
+        dz[k] = zt[k];
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int k = k0; k < R + 2; ++k) {
+          s_wq[par][wv][k] = qyo[k];
+          s_wn[par][wv][k] = nn[k];
+        }
+      }
+    }
+    __syncthreads();
+    if (active_wave) {
+      h63 = s_eh[par][wv][kk];
+      z63 = s_ez[par][wv][kk];
+      if (wv > 0) {
+        hw = s_eh[par][wv - 1][kk];
+        zw = s_ez[par][wv - 1][kk];
+      }
+      if (wv < 3) {
+        he = s_wh[par][wv + 1][kk];
+        ze = s_wz[par][wv + 1][kk];
+        qe = s_wq[par][wv + 1][kk];
+        ne = s_wn[par][wv + 1][kk];
+      }
+    }
+  };
+  // new fluxes of the faces east of lane 63, lane k = row k
+  T qye = 0;
+  auto east_faces = [&]() {
+    qye = yface(qe, he, h63, ze, z63, ne, east_m != 0, ycode_e, ymul_e, yadd_e, ork + ce);
   };
 
   T qxNp = 0, qyNp = 0, qyRp = 0;  // new fluxes of the row above (carried)
@@ -413,12 +565,20 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   // faces of row k, and depth_update + store of row k-1 (k >= 2)
   auto row = [&](uint64_t win, int k, int r0) {
     const int xr = r0 - 1 + k;
-    const T hl = lane_up1(hh[k]);
-    const Z zl = lane_up1(zt[k]);
+    T hl;
+    Z zl;
+    if constexpr (AL) {
+      hl = lane_up1(hh[k], lane_value(hw, k));
+      zl = lane_up1(zt[k], lane_value(zw, k));
+    } else {
+      hl = lane_up1(hh[k]);
+      zl = lane_up1(zt[k]);
+    }
     const bool m = row_pred(win, k);
-    T qxN = qxo[k], qyN = qyo[k];
+    T qxN = qxo[k];
+    T qyN = qyo[k];
     if constexpr (MODE == 0) {
-      if (m && zt[k] > Z(-9999)) {
+      if (m && zt[k] > Z(-9999)) {  // both faces in one branch
         double dummy = 0;
         face64<false>(qxN, dummy, hh[k], hh[k - 1], zt[k], zt[k - 1], nn[k], dtd, a,
                       edge_code(xr, a.imax), true, 0, 0, nullptr, nullptr);
@@ -428,40 +588,32 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     } else {
       // both faces on every lane, then a select: in a wet grid nearly every
       // lane takes them, and branches would only add exec-mask work
-      float dzx, dzy;
+      float dzx;
       if constexpr (MODE == 2) {
         dzx = static_cast<float>(zt[k - 1] - zt[k]);
-        dzy = static_cast<float>(zl - zt[k]);
       } else if constexpr (MODE == 3) {
         // exact integer differences (wrapping only where a no-data cell is
         // involved, whose result is discarded), scaled by a power of two
-        auto qd = [](int32_t zn, int32_t zc) {
-          return static_cast<int32_t>(static_cast<uint32_t>(zn) - static_cast<uint32_t>(zc));
-        };
-        dzx = static_cast<float>(qd(zt[k - 1], zt[k])) * a.zq_inv;
-        dzy = static_cast<float>(qd(zl, zt[k])) * a.zq_inv;
+        dzx = static_cast<float>(static_cast<int32_t>(static_cast<uint32_t>(zt[k - 1]) -
+                                                      static_cast<uint32_t>(zt[k]))) *
+              a.zq_inv;
       } else {
         dzx = zt[k - 1] - zt[k];
-        dzy = zl - zt[k];
-        if (fthr > 0.0f && m) {  // near-flat faces take dz from the FP64 DEM
-          if (fabsf(dzx) < fthr) dzx = static_cast<float>(z64[ob[k - 1]] - z64[ob[k]]);
-          if (fabsf(dzy) < fthr) dzy = static_cast<float>(z64[ob[k] - 1] - z64[ob[k]]);
-        }
+        if (fthr > 0.0f && m && fabsf(dzx) < fthr)  // near-flat: dz from the FP64 DEM
+          dzx = static_cast<float>(z64[off(k - 1, r0)] - z64[off(k, r0)]);
       }
-      const float n2 = nn[k] * nn[k];
       const int xcode = edge_code(xr, a.imax);  // uniform
-      float fx = face32(qxo[k], hh[k], hh[k - 1], dzx, n2, xcode != 0 ? 0.0f : 1.0f,
+      float fx = face32(qxo[k], hh[k], hh[k - 1], dzx, nn[k] * nn[k], xcode != 0 ? 0.0f : 1.0f,
                         xcode < 0 ? -es : (xcode > 0 ? es : 0.0f), K);
-      float fy = face32(qyo[k], hh[k], hl, dzy, n2, ymul, yadd, K);
-      // computed here on every lane: left to itself the compiler moves them
-      // under the selects below as exec-masked branches
-      asm volatile("" : "+v"(fx), "+v"(fy));
-      const bool mc = m && z_ok(zt[k]);
-      const bool wc = hh[k] > 0.0f;
-      qxN = (mc && (wc || hh[k - 1] > 0.0f) && z_ok(zt[k - 1])) ? fx : qxo[k];
-      qyN = (mc && (wc || hl > 0.0f) && z_ok(zl)) ? fy : qyo[k];
+      asm volatile("" : "+v"(fx));  // on every lane, not under the select
+      qxN = (m & z_ok(zt[k]) & ((hh[k] > 0.0f) | (hh[k - 1] > 0.0f)) & z_ok(zt[k - 1])) ? fx
+                                                                                    : qxo[k];
     }
-    const T qyR = lane_down1(qyN);
+    if constexpr (MODE != 0)
+      qyN = yface(qyo[k], hh[k], hl, zt[k], zl, nn[k], m, ycode, ymul, yadd, off(k, r0));
+    T qyR;
+    if constexpr (AL) qyR = lane_down1(qyN, lane_value(qye, k));
+    else qyR = lane_down1(qyN);
     if (k >= 2) {  // row k-1 has both of its x-faces and y-faces now
       const int xo = xr - 1;
       const bool mu = mprev;
@@ -490,8 +642,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
         }
       }
       if (owned && xo < NR) {
-        uint32_t o = ob[k - 1];
-        asm volatile("" : "+v"(o));  // offset made in this block: scalar-base stores
+        const uint32_t o = off(k - 1, r0);
         st_elem(h1, o, hn);
         st_elem(qx1, o, qxNp);
         st_elem(qy1, o, qyNp);
@@ -510,7 +661,8 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     win = load_win(r0);
 #pragma unroll
     for (int k = 0; k < R + 2; ++k) load_row(k, r0);
-    load_n(1);
+    load_n(1, r0);
+    if constexpr (AL) load_outer(r0);
   }
   asm volatile("" ::: "memory");  // the loads above stay ahead of the loop-state reads
   if (haltp && *haltp) return;  // the loop has finished: uniform over the grid
@@ -518,16 +670,21 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   if (drains) ts = *drain.ts;
   if (inject && p.inj.rf_uniform >= 0) amt_u = p.inj.amt[p.inj.rf_uniform];
   dt_setup();
-  if (active_wave) {
-    if (inject) {
+  if (active_wave && inject) {
 #pragma unroll
-      for (int k = 0; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), ob[k]);
-    }
+    for (int k = 0; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), off(k, r0));
+    if constexpr (AL) rain_outer(r0);
+  }
+  if constexpr (AL) exchange(std::integral_constant<int, 1>{}, 0);
+  if (active_wave) {
+    if constexpr (AL) east_faces();
 #pragma unroll
     for (int k = 1; k < R + 2; ++k) row(win, k, r0);
-    for (int t = 1; t < nsub; ++t) {
-      r0 += R;
-      if (r0 > NR) break;  // uniform: no row of this sub-tile is in the grid
+  }
+  for (int t = 1; t < nsub; ++t) {
+    r0 += R;
+    if (r0 > NR) break;  // uniform: no row of this sub-tile is in the grid
+    if (active_wave) {
       // rows r0-1 and r0 were the last two rows of the sub-tile before (their
       // rain input included), and the faces of row r0 are already computed
       // (qxNp, qyNp, qyRp)
@@ -535,20 +692,24 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       hh[1] = hh[R + 1];
       zt[0] = zt[R];
       zt[1] = zt[R + 1];
-      ob[1] = ob[R + 1];
       win = load_win(r0);
 #pragma unroll
       for (int k = 2; k < R + 2; ++k) load_row(k, r0);
-      load_n(2);
+      load_n(2, r0);
+      if constexpr (AL) load_outer(r0);
       if (inject) {
 #pragma unroll
-        for (int k = 2; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), ob[k]);
+        for (int k = 2; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), off(k, r0));
+        if constexpr (AL) rain_outer(r0);
       }
+    }
+    if constexpr (AL) exchange(std::integral_constant<int, 2>{}, t & 1);
+    if (active_wave) {
+      if constexpr (AL) east_faces();
 #pragma unroll
       for (int k = 2; k < R + 2; ++k) row(win, k, r0);
     }
   }
-  // the halo lanes' depths are not this wavefront's results
   block_max_pos(red, owned && active_wave ? static_cast<double>(md) : 0.0);
   if (drains) {  // uniform per block
     __shared__ double s_dw[4];
@@ -556,7 +717,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     if (lane == 0) s_dw[threadIdx.x >> 6] = dacc;
     __syncthreads();
     if (threadIdx.x == 0)
-      drain.partials[hc::EdgeSlots(nbx, H, a.imax, a.jmax).slot(bx, by)] =
+      drain.partials[hc::EdgeSlots(nbx, H, a.imax, a.jmax, SW).slot(bx, by)] =
           (s_dw[0] + s_dw[1]) + (s_dw[2] + s_dw[3]);
   }
 }
@@ -576,7 +737,7 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   const int q8 = nblocks / 8, r8 = nblocks % 8;
   const int xcd = b % 8;
   const int bb = xcd * q8 + min(xcd, r8) + b / 8;
-  fused_tile<MODE, R>(a, bb % nbx, bb / nbx, nbx, nsub, dt_arg, dtp, haltp, p, drain);
+  fused_tile<MODE, R, MODE != 0>(a, bb % nbx, bb / nbx, nbx, nsub, dt_arg, dtp, haltp, p, drain);
 }
 
 // Ensemble: a persistent grid walks the ensemble's tile list (member-major).
@@ -623,8 +784,8 @@ __global__ __launch_bounds__(256) void k_ens_fused(const EnsMember* __restrict__
     d.thr = M.thr;
     d.partials = table_ptr<double>(M.dpart);
     __syncthreads();  // the block's shared scratch is reused tile to tile
-    fused_tile<MODE, R>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, 1, 0.0, &ctl->dt_flow,
-                        nullptr, p, d);
+    fused_tile<MODE, R, false>(M.ha, tile.y % M.nbx, tile.y / M.nbx, M.nbx, 1, 0.0,
+                               &ctl->dt_flow, nullptr, p, d);
   }
 }
 
@@ -780,8 +941,12 @@ int hc_launch_ens_fused(hipStream_t st, int mode, int R, const EnsMember* mem, c
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby) {
-  const int nwaves = (c->NC - 1 + 61) / 62;
+int hc_fused_stride(const hc_ctx* c) {
+  return c->prec == HC_FP64 ? hc::kStrideOverlap : hc::kStrideAligned;
+}
+
+void hc_fused_geometry(const hc_ctx* c, int R, int sw, int* nbx, int* nby) {
+  const int nwaves = sw == hc::kStrideAligned ? (c->NC + 63) / 64 : (c->NC - 1 + 61) / 62;
   *nbx = (nwaves + 3) / 4;
   *nby = (c->NR + R - 1) / R;
 }
@@ -796,10 +961,11 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
   const int R = c->fused_rows;
   const int ns = c->fused_strip;
   int nbx, nby;
-  hc_fused_geometry(c, R * ns, &nbx, &nby);
+  hc_fused_geometry(c, R * ns, hc_fused_stride(c), &nbx, &nby);
   const int nblocks = nbx * nby;
   const DrainArgs dr = drain ? *drain : DrainArgs{};
-  if (dr.ts && static_cast<size_t>(hc::EdgeSlots(nbx, R * ns, c->g.imax, c->g.jmax).count(nby)) >
+  if (dr.ts && static_cast<size_t>(hc::EdgeSlots(nbx, R * ns, c->g.imax, c->g.jmax,
+                                                  hc_fused_stride(c)).count(nby)) >
                    c->dpart_cap)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: drain partials buffer too small");
   const int k = c->cur, o = c->cur ^ 1;
@@ -951,7 +1117,7 @@ int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double*
   const int R = c->fused_rows;
   const int ns = c->fused_strip;
   int nbx, nby;
-  hc_fused_geometry(c, R * ns, &nbx, &nby);
+  hc_fused_geometry(c, R * ns, hc_fused_stride(c), &nbx, &nby);
   const int nblocks = nbx * nby;
   FusedPtrs<0> p;
   p.h0 = h0;

@@ -126,21 +126,28 @@ inline bool fused_offsets_fit(int NR, int P) {
 }
 
 // Fused-kernel tile (bx, by) holds an edge cell: row 1 or imax, column 1 or
-// jmax.  Tile rows by*R .. by*R+R-1; wavefront w owns columns 62w+1 .. 62w+62
-// (and column 0 for w = 0), four wavefronts per tile column bx.
-__host__ __device__ inline bool fused_edge_tile(int bx, int by, int R, int imax, int jmax) {
-  return by == 0 || by == imax / R || bx == 0 || bx == ((jmax - 1) / 62) / 4;
+// jmax.  Tile rows by*R .. by*R+R-1; four wavefronts per tile column bx,
+// wavefront w owning columns 64w .. 64w+63 (sw = 64, FP32 forms) or
+// 62w+1 .. 62w+62 and column 0 for w = 0 (sw = 62, FP64 form).
+// Columns per wavefront of the fused tiles: aligned (FP32 single grid) or
+// overlapping (FP64 form and the ensemble).
+constexpr int kStrideAligned = 64, kStrideOverlap = 62;
+__host__ __device__ inline int fused_last_tile_col(int jmax, int sw) {
+  return sw == 64 ? jmax / 256 : ((jmax - 1) / 62) / 4;
+}
+__host__ __device__ inline bool fused_edge_tile(int bx, int by, int R, int imax, int jmax, int sw) {
+  return by == 0 || by == imax / R || bx == 0 || bx == fused_last_tile_col(jmax, sw);
 }
 
 // Compact slots for the edge tiles' partial sums: the tile rows 0 and imax/R
 // first (nbx slots each), then two slots (one if they coincide) per tile row
-// for tile columns 0 and ((jmax-1)/62)/4.  The latter slots of rows 0 and
-// imax/R are holes that no tile writes and the sum skips.
+// for tile columns 0 and the last one holding column jmax.  The latter slots
+// of rows 0 and imax/R are holes that no tile writes and the sum skips.
 struct EdgeSlots {
   int nbx, r1, c1, nr, nc;
-  __host__ __device__ EdgeSlots(int nbx_, int R, int imax, int jmax)
-      : nbx(nbx_), r1(imax / R), c1(((jmax - 1) / 62) / 4), nr(imax / R == 0 ? 1 : 2),
-        nc(((jmax - 1) / 62) / 4 == 0 ? 1 : 2) {}
+  __host__ __device__ EdgeSlots(int nbx_, int R, int imax, int jmax, int sw)
+      : nbx(nbx_), r1(imax / R), c1(fused_last_tile_col(jmax, sw)), nr(imax / R == 0 ? 1 : 2),
+        nc(fused_last_tile_col(jmax, sw) == 0 ? 1 : 2) {}
   __host__ __device__ int count(int nby) const { return nr * nbx + nby * nc; }
   // slot of tile (bx, by), -1 when it holds no edge cell
   __host__ __device__ int slot(int bx, int by) const {
@@ -264,7 +271,10 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
                          const hc::DrainArgs* drain = nullptr,
                          const hc::InjectArgs* inject = nullptr);
 // Geometry of the fused kernel's launch for the context's tile rows.
-void hc_fused_geometry(const hc_ctx* c, int R, int* nbx, int* nby);
+// Columns per wavefront of this context's single-grid fused kernel.
+int hc_fused_stride(const hc_ctx* c);
+// Tile grid of the fused kernel at R rows per block and sw columns per wavefront.
+void hc_fused_geometry(const hc_ctx* c, int R, int sw, int* nbx, int* nby);
 hc::HydroArgs hc_hydro_args(const hc_ctx* c);
 // Elevation plane the fused kernel reads for this context's mode.
 const void* hc_fused_z(const hc_ctx* c);

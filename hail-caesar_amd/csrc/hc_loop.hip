@@ -230,10 +230,10 @@ __device__ __forceinline__ void rebuild_points(const LoopArgs& a, LoopCtl* __res
 // water_flux_out total for the iteration.
 __device__ void finish_drain(LoopCtl* __restrict__ ctl, LoopRecord* __restrict__ rec,
                              const double* __restrict__ partials, int nby, int nbx, int R,
-                             int imax, int jmax) {
+                             int imax, int jmax, int sw) {
   __shared__ double s_sum[256];
   const int t = threadIdx.x;
-  const hc::EdgeSlots es(nbx, R, imax, jmax);
+  const hc::EdgeSlots es(nbx, R, imax, jmax, sw);
   const int n = es.count(nby);
   double acc = 0;
   for (int k = t; k < n; k += 256) {
@@ -264,14 +264,15 @@ __global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restri
                                                    const int* __restrict__ rfarea,
                                                    uint64_t* __restrict__ pmask,
                                                    const double* __restrict__ partials,
-                                                   int nby, int nbx, int R,
+                                                   int nby, int nbx, int R, int sw,
                                                    LoopRecord* __restrict__ rec) {
   // the control kernel alone sets halt, so the fused kernel of this
   // iteration ran exactly when it is still clear here
   const bool live = !ctl->halt && !ctl->err;
   const bool scan = live && (ctl->counter % a.scan_interval) == 0;
   const bool rebuild = live && ctl->pts_dirty;
-  if (live && blockIdx.x == 0) finish_drain(ctl, rec, partials, nby, nbx, R, a.imax, a.jmax);
+  if (live && blockIdx.x == 0)
+    finish_drain(ctl, rec, partials, nby, nbx, R, a.imax, a.jmax, sw);
   if (!scan && !rebuild) return;  // uniform over the grid
   if (scan) {
     const int items = scan_items(a.imax, W);
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(256) void k_ens_scan(const EnsMember* __restrict__ 
     LoopCtl* ctl = table_ptr<LoopCtl>(M.ctl);
     if (!ctl->halt && !ctl->err)
       finish_drain(ctl, table_ptr<LoopRecord>(M.rec), table_ptr<const double>(M.dpart), M.nby,
-                   M.nbx, R, M.la.imax, M.la.jmax);
+                   M.nbx, R, M.la.imax, M.la.jmax, hc::kStrideOverlap);
   }
   const int total = item_base[n];
   for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < total; it += gridDim.x * 4) {
@@ -378,15 +379,16 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   const int nb = std::max(1, std::min((hcdev::scan_items(c->g.imax, c->W) + 3) / 4, 1024));
   const int R = c->fused_rows * c->fused_strip;  // rows per fused block
   int nbx, nby;
-  hc_fused_geometry(c, R, &nbx, &nby);
+  const int sw = hc_fused_stride(c);
+  hc_fused_geometry(c, R, sw, &nbx, &nby);
   if (c->prec == HC_FP64)
     hipLaunchKernelGGL(k_loop_scan<double>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
                        (const double*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
-                       c->d_dpart, nby, nbx, R, c->d_rec);
+                       c->d_dpart, nby, nbx, R, sw, c->d_rec);
   else
     hipLaunchKernelGGL(k_loop_scan<float>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
                        (const float*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
-                       c->d_dpart, nby, nbx, R, c->d_rec);
+                       c->d_dpart, nby, nbx, R, sw, c->d_rec);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
@@ -437,13 +439,16 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
   // tables and inputs
   if (!c->d_ctl) HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_ctl), sizeof(LoopCtl)));
   {
-    // edge-drain partials: the largest slot count over the tile heights
+    // edge-drain partials: the largest slot count over the tile heights and
+    // both wavefront strides (single grid and ensemble)
     size_t need = 1;
-    for (int R : {4, 8, 16}) {
-      int nbx, nby;
-      hc_fused_geometry(c, R, &nbx, &nby);
-      need = std::max(need, static_cast<size_t>(
-                                hc::EdgeSlots(nbx, R, c->g.imax, c->g.jmax).count(nby)));
+    for (int sw : {hc::kStrideAligned, hc::kStrideOverlap}) {
+      for (int R : {4, 8, 16}) {
+        int nbx, nby;
+        hc_fused_geometry(c, R, sw, &nbx, &nby);
+        need = std::max(need, static_cast<size_t>(
+                                  hc::EdgeSlots(nbx, R, c->g.imax, c->g.jmax, sw).count(nby)));
+      }
     }
     if (need > c->dpart_cap) {
       if (c->d_dpart) (void)hipFree(c->d_dpart);
@@ -681,7 +686,7 @@ int ens_layout(hc_ens* e) {
   std::vector<int2> tiles;
   for (size_t i = 0; i < e->m.size(); ++i) {
     int nbx, nby;
-    hc_fused_geometry(e->m[i], e->R, &nbx, &nby);
+    hc_fused_geometry(e->m[i], e->R, hc::kStrideOverlap, &nbx, &nby);
     e->mem[i].nbx = nbx;
     e->mem[i].nby = nby;
     for (int t = 0; t < nbx * nby; ++t) tiles.push_back(make_int2(static_cast<int>(i), t));
