@@ -189,11 +189,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Rows per scan work item: one wavefront takes one 64-column mask word over
-// a strip of this many rows.
-constexpr int kScanRows = 16;
+// a strip of rows.  With the row above and the row below, a strip is 64 rows
+// of 4-byte depths or 32 of 8-byte ones: the side columns are then fetched
+// one row per lane, and the strip's row loads all stay in flight together.
+__host__ __device__ constexpr int scan_rows(int esz) { return esz == 4 ? 62 : 30; }
 
-__host__ __device__ inline int scan_items(int imax, int W) {
-  return W * ((imax + kScanRows - 1) / kScanRows);
+__host__ __device__ inline int scan_items(int imax, int W, int esz) {
+  const int r = scan_rows(esz);
+  return W * ((imax + r - 1) / r);
 }
 
 // scan_area (LSDCatchmentModel.cpp:3821-3849) for one work item: a cell is
@@ -206,47 +209,68 @@ template <typename T>
 __device__ __forceinline__ void scan_item(int item, int imax, int jmax, int P, int W,
                                           const T* __restrict__ h,
                                           uint64_t* __restrict__ mask) {
+  constexpr int SR = scan_rows(static_cast<int>(sizeof(T)));
+  static_assert(SR + 2 <= 64, "a strip's rows are indexed by lanes");
   const int lane = threadIdx.x & 63;
+  item = __builtin_amdgcn_readfirstlane(item);  // one item per wavefront
   const int word = item % W;
   const int strip = item / W;
-  const int x0 = 1 + strip * kScanRows;
+  const int x0 = 1 + strip * SR;
   const int c = word * 64 + lane;  // < P = 64 W
-  const int cs = (lane == 0) ? c - 1 : c + 1;
-  const bool side_lane = (lane == 0 || lane == 63) && cs >= 0 && cs < P;
-  // every lane issues both loads (no divergent branch around a load); the
-  // second is the column outside the word for the edge lanes, a re-read of
-  // its own column for the rest
-  const int c2 = side_lane ? cs : c;
-  bool v[kScanRows + 2], s[kScanRows + 2];
+  T hv[SR + 2];
 #pragma unroll
-  for (int k = 0; k < kScanRows + 2; ++k) {
+  for (int k = 0; k < SR + 2; ++k) {
     const int x = min(x0 - 1 + k, imax + 1);  // rows past the grid repeat the last
-    const size_t b = static_cast<size_t>(x) * P;
-    const T hv = h[b + c];
-    const T hs = h[b + c2];
-    v[k] = hv > T(0);
-    s[k] = side_lane && (hs > T(0));
+    // a uniform row base and the lane's column: one vector register serves
+    // every row's address
+    hv[k] = ld_elem(h + static_cast<size_t>(x) * P, static_cast<uint32_t>(c));
   }
+  // The columns either side of the word, lane k taking row x0-1+k, as bit
+  // masks over the strip's rows (bit k: that column is wet in row x0-1+k).
+  // Word 0 has no column to its west, the last word none to its east.
+  const uint32_t xk = static_cast<uint32_t>(min(x0 - 1 + min(lane, SR + 1), imax + 1));
+  const int cw = word * 64 - 1, ce = word * 64 + 64;
+  const T* hs = h + static_cast<size_t>(x0 - 1) * P;  // row x0-1 (the lanes' offsets from it)
+  const uint32_t ok = (xk - static_cast<uint32_t>(x0 - 1)) * static_cast<uint32_t>(P);
+  T hw = T(0), he = T(0);
+  if (cw >= 0) hw = ld_elem(hs + cw, ok);  // uniform branches
+  if (ce < P) he = ld_elem(hs + ce, ok);
+  constexpr uint64_t rows = SR + 2 == 64 ? ~0ull : ((1ull << (SR + 2)) - 1);
+  const uint64_t wside = __ballot(hw > T(0)) & rows;
+  const uint64_t eside = __ballot(he > T(0)) & rows;
   const int lo = word * 64;
   uint64_t keep = ~0ull;
   if (lo == 0) keep &= ~1ull;
   const int hi = jmax + 1 - lo;  // first column past jmax, relative to the word
   if (hi <= 0) keep = 0;
   else if (hi < 64) keep &= (1ull << hi) - 1;
+  // The 3x3 dilation on the vector unit: per lane the largest depth of rows
+  // k-1..k+1 (wet when > 0; the largest of three is > 0 exactly when one of
+  // them is, and a NaN counts as dry either way), then the same over the
+  // lane and its two neighbours through lane shifts, the side columns
+  // entering as the value shifted into lanes 0 and 63.  Row k's mask word is
+  // kept in lane k (both halves) and the strip's words go out in one store.
+  uint32_t out_lo = 0, out_hi = 0;
+  int nrows = 0;
 #pragma unroll
-  for (int k = 1; k <= kScanRows; ++k) {
+  for (int k = 1; k <= SR; ++k) {
     const int x = x0 - 1 + k;
     if (x > imax) break;  // uniform per wavefront
-    const uint64_t bits = __ballot(v[k - 1] || v[k] || v[k + 1]);
-    const uint64_t side = __ballot(s[k - 1] || s[k] || s[k + 1]);
-    uint64_t dil = bits | (bits << 1) | (bits >> 1);
-    if (side & 1ull) dil |= 1ull;                // column lo-1 is wet
-    if (side & (1ull << 63)) dil |= 1ull << 63;  // column lo+64 is wet
-    if (lane == 0) mask[static_cast<size_t>(x) * W + word] = dil & keep;
+    const T vm = fmax(fmax(hv[k - 1], hv[k]), hv[k + 1]);
+    const T ws = ((wside >> (k - 1)) & 7ull) ? T(1) : T(0);  // column lo-1 wet
+    const T es = ((eside >> (k - 1)) & 7ull) ? T(1) : T(0);  // column lo+64 wet
+    const T hm = fmax(fmax(vm, lane_up1(vm, ws)), lane_down1(vm, es));
+    const uint64_t dil = __ballot(hm > T(0)) & keep;
+    out_lo = lane == k ? static_cast<uint32_t>(dil) : out_lo;
+    out_hi = lane == k ? static_cast<uint32_t>(dil >> 32) : out_hi;
+    nrows = k;
   }
+  if (lane >= 1 && lane <= nrows)
+    mask[static_cast<size_t>(x0 - 1 + lane) * W + word] =
+        (static_cast<uint64_t>(out_hi) << 32) | out_lo;
   if (lane == 0) {
     if (strip == 0) mask[word] = 0ull;
-    if (x0 + kScanRows > imax) mask[static_cast<size_t>(imax + 1) * W + word] = 0ull;
+    if (x0 + SR > imax) mask[static_cast<size_t>(imax + 1) * W + word] = 0ull;
   }
 }
 

@@ -794,8 +794,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_scan(int imax, int jmax, int P, int W,
                                               const T* __restrict__ h,
                                               uint64_t* __restrict__ mask) {
-  const int items = scan_items(imax, W);
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += gridDim.x * 4)
+  const int items = scan_items(imax, W, static_cast<int>(sizeof(T)));
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
+  for (int it = blockIdx.x * 4 + wv; it < items; it += gridDim.x * 4)
     scan_item<T>(it, imax, jmax, P, W, h, mask);
 }
 
@@ -1145,7 +1146,8 @@ int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double*
 }
 
 int hc_launch_scan_ptr(hc_ctx* c, const void* h, uint64_t* mask) {
-  const int nb = std::max(1, std::min((scan_items(c->g.imax, c->W) + 3) / 4, 2048));
+  const int items = scan_items(c->g.imax, c->W, static_cast<int>(c->esz));
+  const int nb = std::max(1, std::min((items + 3) / 4, 2048));
   if (c->prec == HC_FP64)
     hipLaunchKernelGGL(k_scan<double>, dim3(nb), dim3(256), 0, c->stream, c->g.imax, c->g.jmax,
                        c->P, c->W, static_cast<const double*>(h), mask);
@@ -1158,7 +1160,8 @@ int hc_launch_scan_ptr(hc_ctx* c, const void* h, uint64_t* mask) {
 
 int hc_launch_scan(hc_ctx* c) {
   const int k = c->cur;
-  const int nb = std::max(1, std::min((scan_items(c->g.imax, c->W) + 3) / 4, 2048));
+  const int items = scan_items(c->g.imax, c->W, static_cast<int>(c->esz));
+  const int nb = std::max(1, std::min((items + 3) / 4, 2048));
   if (c->prec == HC_FP64)
     hipLaunchKernelGGL(k_scan<double>, dim3(nb), dim3(256), 0, c->stream, c->g.imax, c->g.jmax,
                        c->P, c->W, (const double*)c->h[k], c->mask);

@@ -275,8 +275,9 @@ __global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restri
     finish_drain(ctl, rec, partials, nby, nbx, R, a.imax, a.jmax, sw);
   if (!scan && !rebuild) return;  // uniform over the grid
   if (scan) {
-    const int items = scan_items(a.imax, W);
-    for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += gridDim.x * 4)
+    const int items = scan_items(a.imax, W, static_cast<int>(sizeof(T)));
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
+    for (int it = blockIdx.x * 4 + wv; it < items; it += gridDim.x * 4)
       scan_item<T>(it, a.imax, a.jmax, a.pitch, W, h, mask);
   }
   if (rebuild) rebuild_points(a, ctl, area, rfarea, pmask);
@@ -323,7 +324,8 @@ __global__ __launch_bounds__(256) void k_ens_scan(const EnsMember* __restrict__ 
                    M.nbx, R, M.la.imax, M.la.jmax, hc::kStrideOverlap);
   }
   const int total = item_base[n];
-  for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < total; it += gridDim.x * 4) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
+  for (int it = blockIdx.x * 4 + wv; it < total; it += gridDim.x * 4) {
     const int m = item_member[it];
     if (!flags[m]) continue;  // per wave
     const EnsMember& M = mem[m];
@@ -376,7 +378,8 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
     c->prof_n = k_in_run + 1;
   }
   const int o = c->cur;  // flow_depth flipped the generation
-  const int nb = std::max(1, std::min((hcdev::scan_items(c->g.imax, c->W) + 3) / 4, 1024));
+  const int items = hcdev::scan_items(c->g.imax, c->W, static_cast<int>(c->esz));
+  const int nb = std::max(1, std::min((items + 3) / 4, 1024));
   const int R = c->fused_rows * c->fused_strip;  // rows per fused block
   int nbx, nby;
   const int sw = hc_fused_stride(c);
@@ -804,7 +807,7 @@ int hc_ens_create(hc_ens** out, hc_ctx* const* members, int n) {
     M.dpart = c->d_dpart;
     M.thr = c->prm.water_depth_erosion_threshold;
     M.W = c->W;
-    items[i + 1] = items[i] + hcdev::scan_items(c->g.imax, c->W);
+    items[i + 1] = items[i] + hcdev::scan_items(c->g.imax, c->W, static_cast<int>(c->esz));
   }
   std::vector<int> item_member(std::max(items[n], 1), 0);
   for (int i = 0; i < n; ++i)
