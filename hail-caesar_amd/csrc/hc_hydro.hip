@@ -83,12 +83,13 @@ __device__ __forceinline__ void face64(double& q, double& qs, double hc, double 
 //   (q - g h dt S) / (1 + g h dt n^2 |q| / h^3)
 //     = (q - g h dt S) h^3 / (h^3 + g h dt n^2 |q|),
 // so one reciprocal serves; the Froude cap is a clamp to [-qF, qF]
-// (qF = h sqrt(g h) Fr > 0 wherever the result is kept) and the depth limiter
+// (qF = h sqrt(g h) Fr = sqrt(g Fr^2 h^3) > 0 wherever the result is kept,
+// from the h^3 already formed) and the depth limiter
 // tests are products instead of quotients.  0 where hflow <= the flow
 // threshold (LSDCatchmentModel.cpp:3094-3191).  The slope is taken as
 // slope * smul + sadd: (1, 0) inside, (0, +-edgeslope) on the edge faces.
 struct Face32K {
-  float g, dt, invDX, froude, hthr, k4, k5;
+  float gdt, gfr2, invDX, hthr, k4, k5;  // g dt, g Fr^2, 1/DX, flow threshold, limiters
 };
 
 __device__ __forceinline__ float face32(float q, float hc, float hn, float dz, float n2,
@@ -96,12 +97,11 @@ __device__ __forceinline__ float face32(float q, float hc, float hn, float dz, f
   // dz >= 0 ? max(hc - dz, hn) : max(hc, hn + dz)
   const float hflow = fmaxf(hc - fmaxf(dz, 0.0f), hn + fminf(dz, 0.0f));
   const float slope = __builtin_fmaf((dz + (hn - hc)) * K.invDX, smul, sadd);
-  const float gh = K.g * hflow;
-  const float ghdt = gh * K.dt;
+  const float ghdt = K.gdt * hflow;
   const float h3 = (hflow * hflow) * hflow;
   const float num = __builtin_fmaf(-ghdt, slope, q);
   float qv = (num * h3) * __builtin_amdgcn_rcpf(__builtin_fmaf(ghdt * n2, fabsf(q), h3));
-  const float qF = hflow * (__builtin_amdgcn_sqrtf(gh) * K.froude);
+  const float qF = __builtin_amdgcn_sqrtf(K.gfr2 * h3);
   qv = __builtin_amdgcn_fmed3f(qv, -qF, qF);
   qv = (qv > 0.0f && qv > hc * K.k4) ? hc * K.k5 : qv;
   qv = (qv < 0.0f && -qv > hn * K.k4) ? -(hn * K.k5) : qv;
@@ -389,16 +389,17 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       const float DX = static_cast<float>(a.DX);
       const float invdt = 1.0f / static_cast<float>(dtd);
       dtDX = static_cast<float>(dtd / a.DX);
-      K.g = static_cast<float>(kGravity);
-      K.dt = static_cast<float>(dtd);
+      const float g = static_cast<float>(kGravity);
+      const float fr = static_cast<float>(a.froude);
+      K.gdt = g * static_cast<float>(dtd);
+      K.gfr2 = (g * fr) * fr;
       K.invDX = 1.0f / DX;
-      K.froude = static_cast<float>(a.froude);
       K.hthr = static_cast<float>(a.hthr);
       K.k4 = 0.25f * DX * invdt;  // q*dt/DX > h/4  <=>  q > h*DX/(4 dt)
       K.k5 = 0.2f * DX * invdt;
       // computed once here: left to itself the compiler repeats the
       // divisions at every use
-      K.dt = pin_value(K.dt);
+      K.gdt = pin_value(K.gdt);
       K.invDX = pin_value(K.invDX);
       K.k4 = pin_value(K.k4);
       K.k5 = pin_value(K.k5);
