@@ -94,6 +94,51 @@ def test_ensemble_tuned_tiles_same_results():
             assert np.array_equal(mb.download(f), ma.download(f)), f
 
 
+@pytest.mark.parametrize("form", ["fp32_z32_flat", "fp32_z64", FIXED])
+def test_wide_members_aligned_equals_overlapping(form):
+    """Members wider than one four-wavefront tile column (300 and 520 cells):
+    stepped alone they use the aligned FP32 tiles, whose edge columns come
+    from the neighbouring wavefronts through LDS and, on a block's sides, from
+    one-row-per-lane column loads, with the east faces recomputed lane-batched;
+    in the ensemble they use the overlapping tiles, whose halo lanes compute
+    every face themselves.  Both must give the same bits, rain input at the
+    halo columns included."""
+    specs = inputs.ensemble_members(2, hours=1, sizes=[300, 520], seed0=4200, wet_depth=0.05)
+
+    def model(spec):
+        arrays, params = spec
+        if form == FIXED:
+            return Model(None, None, precision=hc.HC_FP32, arrays=arrays, params=params,
+                         device_loop=True, use_graph=False, elev_fixed=True)
+        return Model(None, None, precision=hc.HC_FP32, arrays=arrays, params=params,
+                     device_loop=True, use_graph=False,
+                     flat_dz=1e-3 if form == "fp32_z32_flat" else -1.0)
+
+    alone = []
+    for spec in specs:
+        m = model(spec)
+        parts = []
+        while True:
+            parts.append(m.iterate(1000))
+            if m.finished:
+                break
+        alone.append((m, [np.concatenate([p[k] for p in parts]) for k in range(3)]))
+    ms = [model(s) for s in specs]
+    ens = Ensemble(ms)
+    acc = [[[], [], []] for _ in ms]
+    for _ in range(10000):
+        for a, r in zip(acc, ens.iterate(500, use_graph=True)):
+            for k in range(3):
+                a[k].append(r[k])
+        if all(m.finished for m in ms):
+            break
+    for (ma, (tt, cy, dt)), mb, a in zip(alone, ms, acc):
+        cy2, dt2 = np.concatenate(a[1]), np.concatenate(a[2])
+        assert np.array_equal(cy2, cy) and np.array_equal(dt2, dt)
+        for f in ("water_depth", "qx", "qy"):
+            assert np.array_equal(mb.download(f), ma.download(f)), f
+
+
 @pytest.mark.parametrize("pa,pb", [(hc.HC_FP64, hc.HC_FP32), (hc.HC_FP32, FIXED)])
 def test_ensemble_rejects_mixed_modes(pa, pb):
     specs = _members(True)[:2]
