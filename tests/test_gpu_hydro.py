@@ -56,6 +56,28 @@ def test_scan_area_exact(tmp_path):
     assert np.array_equal(ctx.download("mask"), o.field_int("mask"))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_scan_area_exact_ragged(tmp_path, prec):
+    """scan_area over several strips of rows (62 rows per work item for 4-byte
+    depths, 30 for 8-byte ones) and several 64-column mask words, on a ragged
+    wet/dry pattern: 60% of the cells dried at random, and single wet cells
+    in dry surroundings on the word seams (columns 63/64, 127/128), on the
+    strips' first and last rows and on the grid's edges."""
+    from hail_caesar_amd import hc
+    w, o = wet_flow_state(tmp_path, size=203, steps=50)
+    h = o.field("water_depth")  # a view of the oracle's state
+    rng = np.random.Generator(np.random.PCG64(77))
+    h[rng.random(h.shape) < 0.6] = 0.0
+    for x0, y0 in [(40, 63), (40, 64), (100, 127), (100, 128), (62, 20), (63, 150),
+                   (30, 150), (31, 170), (1, 5), (203, 7), (9, 1), (11, 203)]:
+        h[max(x0 - 2, 0):x0 + 3, max(y0 - 2, 0):y0 + 3] = 0.0
+        h[x0, y0] = 0.25
+    ctx = to_context(o, precision=hc.HC_FP32 if prec == "fp32" else hc.HC_FP64)
+    o.call("scan_area")
+    ctx.scan_area()
+    assert np.array_equal(ctx.download("mask"), o.field_int("mask"))
+
+
 def test_water_flux_out(tmp_path):
     w, o = wet_flow_state(tmp_path, size=64, steps=500)
     # wet every edge so that all four edges and the corners drain
