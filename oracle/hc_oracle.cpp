@@ -919,6 +919,49 @@ struct orc_model {
   }
 
   // LCM.cpp:2277-2294
+  // global_landsliding (LCM.cpp:5529-5654; called by call_global_landsliding,
+  // LCM.cpp:1986-1996): row-major sweeps over the interior (x = 2..imax-1
+  // outer, y = 2..jmax-1 inner).  Each cell, with wet_factor = tan(failure
+  // angle) DX (10 DX where the cell is at or below bedrock + active), moves
+  // material to each of its 8 neighbours in the order below whose drop (over
+  // 1.41 on the diagonals) exceeds wet_factor: diff = drop - wet_factor, cut
+  // so the cell stays at bedrock + active, capped at ERODEFACTOR; total sums
+  // the diffs.  Sweeps repeat while total > 0, at most 200.  The dunes
+  // option (sand layer) is not part of this restatement.  Returns the number
+  // of sweeps.
+  int global_landsliding(double failureangle) {
+    const double factor = std::tan((failureangle * (3.141592654 / 180))) * p.DX;
+    static const int ox[8] = {1, 0, -1, -1, -1, 0, 1, 1};
+    static const int oy[8] = {1, 1, 1, 0, -1, -1, -1, 0};
+    int inc = 0;
+    double total;
+    do {
+      total = 0;
+      inc++;
+      for (int x = 2; x < p.imax; ++x) {
+        for (int y = 2; y < p.jmax; ++y) {
+          const size_t c = at(x, y);
+          double wet_factor = factor;
+          if (elev[c] <= (bedrock[c] + p.active)) wet_factor = 10 * p.DX;
+          for (int k = 0; k < 8; ++k) {
+            const size_t nb = at(x + ox[k], y + oy[k]);
+            const bool diag = (ox[k] != 0 && oy[k] != 0);
+            const double drop = diag ? ((elev[c] - elev[nb]) / 1.41) : (elev[c] - elev[nb]);
+            if (drop > wet_factor && elev[nb] > -9999) {
+              double diff = drop - wet_factor;
+              if ((elev[c] - diff) < (bedrock[c] + p.active)) diff = (elev[c] - (bedrock[c] + p.active));
+              if (diff > p.ERODEFACTOR) diff = p.ERODEFACTOR;
+              elev[c] -= diff;
+              elev[nb] += diff;
+              total += diff;
+            }
+          }
+        }
+      }
+    } while (total > 0 && inc < 200);
+    return inc;
+  }
+
   void call_erosion() {
     if (counter >= erode_call) {
       erode_mult = trunc_int(p.ERODEFACTOR / erode(erode_mult));
@@ -1104,6 +1147,9 @@ double orc_get_flow_timestep(orc_model* m) { return m->get_flow_timestep(); }
 void orc_set_maximum_timestep(orc_model* m) { m->set_maximum_timestep(); }
 double orc_erode(orc_model* m, double mult) { return m->erode(mult); }
 void orc_call_erosion(orc_model* m) { m->call_erosion(); }
+int orc_global_landsliding(orc_model* m, double failureangle) {
+  return m->global_landsliding(failureangle);
+}
 void orc_drainage_area(orc_model* m) { m->drainage_area(); }
 int orc_input_points(orc_model* m) { return m->get_catchment_input_points(); }
 

@@ -87,6 +87,9 @@ double orc_get_flow_timestep(orc_model* m);
 void orc_set_maximum_timestep(orc_model* m);
 double orc_erode(orc_model* m, double mult_factor);
 void orc_call_erosion(orc_model* m);
+/* global_landsliding with the given slope failure angle (degrees); returns
+ * the number of sweeps. */
+int orc_global_landsliding(orc_model* m, double failureangle);
 void orc_drainage_area(orc_model* m);
 int orc_input_points(orc_model* m);
 double orc_grain_call(orc_model* m, int which, int x, int y, double amount, int x2, int y2);

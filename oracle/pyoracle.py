@@ -209,6 +209,8 @@ class OracleModel:
             for fn in ("orc_depth_update", "orc_water_flux_out", "orc_get_flow_timestep"):
                 getattr(L, fn).argtypes = [ctypes.c_void_p]
                 getattr(L, fn).restype = ctypes.c_double
+            L.orc_global_landsliding.argtypes = [ctypes.c_void_p, ctypes.c_double]
+            L.orc_global_landsliding.restype = ctypes.c_int
             L.orc_erode.argtypes = [ctypes.c_void_p, ctypes.c_double]
             L.orc_erode.restype = ctypes.c_double
             L.orc_input_points.argtypes = [ctypes.c_void_p]
@@ -387,7 +389,8 @@ class RefModel:
              "set_maximum_timestep": 4, "increment_counters": 5, "catchment_waterinputs": 6,
              "call_erosion": 7, "set_loop_cycle": 8, "set_inputoutput_diff": 9,
              "write_output_timeseries": 10, "drainage_area": 11, "input_points": 12,
-             "erode": 13, "get_flow_timestep": 14, "topmodel_runoff": 15}
+             "erode": 13, "get_flow_timestep": 14, "topmodel_runoff": 15,
+             "global_landsliding": 16}
 
     def __init__(self, path: str, params_file: str, threads: int = 1):
         self.L = self.lib()

@@ -338,3 +338,37 @@ def test_oracle_grain_helpers_vs_reference(tmp_path):
     s = r.snapshot()
     assert np.array_equal(o.field("grain"), s["grain"])
     assert np.array_equal(o.field("strata").reshape(s["strata"].shape), s["strata"])
+
+
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("angle", [45.0, 30.0])
+def test_global_landsliding_vs_compiled_reference(tmp_path, angle):
+    """global_landsliding (LSDCatchmentModel.cpp:5529-5654) on a steep 1 m DEM
+    with 20 m cliffs and cells held at bedrock + active (their wet factor is
+    10 DX and their moves can be negative, so a sweep's total can cancel):
+    elevations bitwise after the reference's repeated sweeps."""
+    n = 32
+    rng = np.random.Generator(np.random.PCG64(21))
+    dem = inputs.planar_dem(n, n, 1.0, 0.05, 6.0, 0.6, 1003, south_only=False)
+    props = inputs.dirichlet_props(n, n, rng)
+    w = inputs.make_workload(str(tmp_path / "ls"), dem, 1.0, np.array([10.0, 0.0]), 15, 1,
+                             overrides={"hydro_model_only": "no",
+                                        "active_layer_thickness": "0.3",
+                                        "slope_failure_thresh": repr(angle)},
+                             graindata=props, name="ls")
+    r = RefModel(w.path, w.params_file, threads=1)
+    o = OracleModel(params_from_dict(w.params, n, n, 1.0))
+    o.load_snapshot(r.snapshot())
+    for m in (r, o):
+        e, b = m.field("elev"), m.field("bedrock")
+        e[8:12, 8:12] += 20.0                 # a block 20 m proud of its neighbours
+        e[20, 5:25] -= 15.0                   # a trench
+        b[9, 9] = e[9, 9] - 0.1               # at bedrock + active: wet factor 10 DX
+        b[21, 10] = e[21, 10] - 0.2
+        b[19, 12] = e[19, 12] - 0.35
+    e0 = o.field("elev").copy()
+    r.call("global_landsliding")
+    sweeps = o.call("global_landsliding", angle)
+    assert sweeps > 1
+    assert not np.array_equal(o.field("elev"), e0)
+    assert np.array_equal(r.field("elev"), o.field("elev"))
