@@ -254,7 +254,7 @@ int hc_destroy(hc_ctx* c) {
                   c->hasgrain, c->needgrain, c->bedrock, c->Tau, c->Vel, c->ss, c->sdir,
                   c->etot, c->etot3, c->gtot, c->d_red, c->d_partials, c->d_count, c->d_out,
                   c->d_pts, c->d_amt, c->d_ctl, c->d_rec, c->d_top, c->d_area, c->d_rfarea,
-                  c->d_pmask, c->d_dpart};
+                  c->d_pmask, c->d_dpart, c->d_slide_moves};
   for (auto& gexec : c->loop_graph)
     if (gexec) (void)hipGraphExecDestroy(gexec);
   for (auto e : c->prof_ev) (void)hipEventDestroy(e);
@@ -484,6 +484,13 @@ int hc_erode(hc_ctx* c, double mult, double* time_step, double* tempbmax, float 
   if (!c || !time_step) return HC_ERR_ARG;
   if (!c->g.sediment) return hc_fail(c, HC_ERR_STATE, "erode: context has no sediment state");
   return hc_erode_impl(c, mult, time_step, tempbmax, gtot);
+}
+
+int hc_global_landsliding(hc_ctx* c, double failureangle, int* sweeps) {
+  if (!c) return HC_ERR_ARG;
+  if (!c->g.sediment)
+    return hc_fail(c, HC_ERR_STATE, "global_landsliding: context has no sediment state");
+  return hc_global_landsliding_impl(c, failureangle, sweeps);
 }
 
 int hc_synchronize(hc_ctx* c) {

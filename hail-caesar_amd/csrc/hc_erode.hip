@@ -29,6 +29,7 @@
 // Transcendentals (pow, exp, log, sqrt) are the device library's, so values
 // agree with the libm reference to a few ulp rather than bitwise.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "hc_device.hpp"
@@ -893,6 +894,89 @@ ErodeArgs erode_args(const hc_ctx* c) {
   return a;
 }
 
+// global_landsliding (LCM.cpp:5529-5654).  The reference sweeps the
+// interior row-major (x = 2..imax-1 outer, y = 2..jmax-1 inner); a cell
+// reads its 8 neighbours and moves material into them in place, so two cells
+// interact only within Chebyshev distance 2.  Cells on one wavefront
+// t = 3x + y are at least 3 columns apart, and every cell within distance 2
+// that precedes (x, y) in row-major order lies on a smaller t: visiting the
+// wavefronts in order, the cells of each in parallel, reproduces the
+// reference's sweep exactly.  One block walks them with a barrier between
+// wavefronts.  Each applied move is kept in its cell's neighbour slot (0
+// where nothing moved), so that a sweep whose moves are not all positive (a
+// cell held at bedrock + active moves negative amounts) sums its total in
+// the reference's order; otherwise the total is positive exactly when some
+// move is.  Sweeps repeat while the total is positive, at most 200.
+__global__ __launch_bounds__(1024) void k_global_landslide(int imax, int jmax, int P, double DX,
+                                                           double factor, double active,
+                                                           double EF, double* __restrict__ elev,
+                                                           const double* __restrict__ bedrock,
+                                                           double* __restrict__ moves,
+                                                           size_t plane, int* __restrict__ sweeps) {
+  // neighbour order of the reference's checks
+  constexpr int ox[8] = {1, 0, -1, -1, -1, 0, 1, 1};
+  constexpr int oy[8] = {1, 1, 1, 0, -1, -1, -1, 0};
+  __shared__ int s_pos, s_neg, s_again;
+  const int tid = threadIdx.x;
+  const int t0 = 3 * 2 + 2, t1 = 3 * (imax - 1) + (jmax - 1);
+  int inc = 0;
+  for (;;) {
+    ++inc;
+    if (tid == 0) {
+      s_pos = 0;
+      s_neg = 0;
+    }
+    __syncthreads();
+    int pos = 0, neg = 0;
+    for (int t = t0; t <= t1; ++t) {
+      const int xlo = max(2, (t - (jmax - 1) + 2) / 3);  // y = t - 3x <= jmax - 1
+      const int xhi = min(imax - 1, (t - 2) / 3);        // y >= 2
+      for (int x = xlo + tid; x <= xhi; x += blockDim.x) {
+        const int y = t - 3 * x;
+        const size_t c = static_cast<size_t>(x) * P + y;
+        double e = elev[c];
+        const double bpa = bedrock[c] + active;
+        const double wet = (e <= bpa) ? 10 * DX : factor;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const size_t nb = static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k]);
+          const double en = elev[nb];
+          const double drop = (ox[k] != 0 && oy[k] != 0) ? (e - en) / 1.41 : (e - en);
+          double d = 0.0;
+          if (drop > wet && en > -9999) {
+            d = drop - wet;
+            if ((e - d) < bpa) d = e - bpa;
+            if (d > EF) d = EF;
+            e -= d;
+            elev[nb] = en + d;
+            if (d > 0) pos = 1;
+            else neg = 1;
+          }
+          moves[k * plane + c] = d;
+        }
+        elev[c] = e;
+      }
+      __syncthreads();  // this wavefront's moves are seen by the next
+    }
+    if (pos) atomicOr(&s_pos, 1);
+    if (neg) atomicOr(&s_neg, 1);
+    __syncthreads();
+    if (tid == 0) {
+      double total = s_pos ? 1.0 : 0.0;
+      if (s_neg) {  // the reference's sum, in its order
+        total = 0;
+        for (int x = 2; x < imax; ++x)
+          for (int y = 2; y < jmax; ++y)
+            for (int k = 0; k < 8; ++k) total += moves[k * plane + static_cast<size_t>(x) * P + y];
+      }
+      s_again = (total > 0 && inc < 200) ? 1 : 0;
+      if (!s_again) *sweeps = inc;
+    }
+    __syncthreads();
+    if (!s_again) break;
+  }
+}
+
 }  // namespace
 
 int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot_out) {
@@ -946,5 +1030,31 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
   if (tempbmax) *tempbmax = tb;
   if (gtot_out)
     for (int n = 0; n < 10; ++n) gtot_out[n] = g[n];
+  return HC_OK;
+}
+
+int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps) {
+  if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "global_landsliding: FP64 contexts only");
+  if (c->g.imax < 3 || c->g.jmax < 3) {  // no interior cell
+    if (sweeps) *sweeps = 1;
+    return HC_OK;
+  }
+  // the reference's factor, from the host's libm as in the reference
+  const double factor = std::tan((failureangle * (3.141592654 / 180))) * c->g.DX;
+  const size_t plane = c->ncell;
+  if (!c->d_slide_moves) {
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_slide_moves), 8 * plane * sizeof(double)));
+  }
+  hipLaunchKernelGGL(k_global_landslide, dim3(1), dim3(1024), 0, c->stream, c->g.imax,
+                     c->g.jmax, c->P, c->g.DX, factor, c->prm.active, c->prm.ERODEFACTOR,
+                     static_cast<double*>(c->z), c->bedrock, c->d_slide_moves, plane,
+                     reinterpret_cast<int*>(c->d_red + 3));
+  HC_HIP(c, hipGetLastError());
+  int n = 0;
+  HC_HIP(c, hipMemcpyAsync(c->h_pinned, c->d_red + 3, sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  std::memcpy(&n, c->h_pinned, sizeof(int));
+  if (sweeps) *sweeps = n;
   return HC_OK;
 }

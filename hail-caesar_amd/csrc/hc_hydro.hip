@@ -108,6 +108,12 @@ __device__ __forceinline__ float face32(float q, float hc, float hn, float dz, f
   return hflow > K.hthr ? qv : 0.0f;
 }
 
+// a && b && c && d evaluated without branches (the selects it feeds stay
+// selects instead of exec-masked branches)
+__device__ __forceinline__ bool and4(bool a, bool b, bool c, bool d) {
+  return (static_cast<int>(a) & static_cast<int>(b) & static_cast<int>(c) & static_cast<int>(d)) != 0;
+}
+
 __device__ __forceinline__ int edge_code(int i, int imaxlike) {
   return (i <= 2) ? -1 : ((i == imaxlike) ? 1 : 0);
 }
@@ -441,7 +447,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       }
       float fy = face32(qo, hcell, hl, dz, ncell * ncell, smul, sadd, K);
       asm volatile("" : "+v"(fy));  // on every lane, not under the select
-      qN = (m & z_ok(zc) & ((hcell > 0.0f) | (hl > 0.0f)) & z_ok(zl)) ? fy : qo;
+      qN = and4(m, z_ok(zc), (hcell > 0.0f) || (hl > 0.0f), z_ok(zl)) ? fy : qo;
     }
     return qN;
   };
@@ -607,8 +613,8 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       float fx = face32(qxo[k], hh[k], hh[k - 1], dzx, nn[k] * nn[k], xcode != 0 ? 0.0f : 1.0f,
                         xcode < 0 ? -es : (xcode > 0 ? es : 0.0f), K);
       asm volatile("" : "+v"(fx));  // on every lane, not under the select
-      qxN = (m & z_ok(zt[k]) & ((hh[k] > 0.0f) | (hh[k - 1] > 0.0f)) & z_ok(zt[k - 1])) ? fx
-                                                                                    : qxo[k];
+      qxN = and4(m, z_ok(zt[k]), (hh[k] > 0.0f) || (hh[k - 1] > 0.0f), z_ok(zt[k - 1])) ? fx
+                                                                                       : qxo[k];
     }
     if constexpr (MODE != 0)
       qyN = yface(qyo[k], hh[k], hl, zt[k], zl, nn[k], m, ycode, ymul, yadd, off(k, r0));

@@ -177,6 +177,17 @@ int hc_add_point_inputs(hc_ctx* ctx, const int* x, const int* y,
 int hc_erode(hc_ctx* ctx, double mult_factor, double* time_step,
              double* tempbmax, float gtot[10]);
 
+/* LSDCatchmentModel::global_landsliding (LCM.cpp:5529-5654), which the
+ * reference's loop calls through call_global_landsliding (LCM.cpp:1986-1996)
+ * once per simulated day in erosion runs: repeated sweeps moving material
+ * down every slope steeper than `failureangle` degrees (the parameter-file
+ * key slope_failure_thresh), each move capped at ERODEFACTOR and kept above
+ * bedrock + active, until a sweep's total is not positive or 200 sweeps.
+ * Elevations are updated in place, bit-identical to the reference's sweep
+ * order.  FP64 sediment contexts; *sweeps (may be NULL) receives the number
+ * of sweeps.  Synchronous. */
+int hc_global_landsliding(hc_ctx* ctx, double failureangle, int* sweeps);
+
 /* Block until every call enqueued on the context has finished. */
 int hc_synchronize(hc_ctx* ctx);
 
