@@ -919,6 +919,42 @@ struct orc_model {
   }
 
   // LCM.cpp:2277-2294
+  // creep (LCM.cpp:5293-5427; called by slope_creep, LCM.cpp:1976-1984 with
+  // time = 0.028): every interior cell above its bedrock moves
+  // ((z - z_nb) / D) * creep_rate * time / D to each lower neighbour
+  // (D = DX on the sides, the constant root = 7.07 on the diagonals,
+  // LCM.hpp:437), cut so it stays at its bedrock and at 0, in the order
+  // below; moves collect in tempcreep from the unchanged elevations and are
+  // added at the end; a cell with grain slides the moved grain with each
+  // move (slide_GS, in the sweep's order).
+  void creep(double time, double creep_rate) {
+    const double root = 7.07;
+    static const int ox[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+    static const int oy[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+    std::vector<double> tempcreep(elev.size(), 0.0);
+    for (int x = 2; x < p.imax; ++x) {
+      for (int y = 2; y < p.jmax; ++y) {
+        const size_t c = at(x, y);
+        if (!(elev[c] > bedrock[c])) continue;
+        for (int k = 0; k < 8; ++k) {
+          const int x2 = x + ox[k], y2 = y + oy[k];
+          const size_t nb = at(x2, y2);
+          if (elev[nb] < elev[c] && elev[nb] > -9999) {
+            const double D = (ox[k] != 0 && oy[k] != 0) ? root : p.DX;
+            double temp = ((elev[c] - elev[nb]) / D) * creep_rate * time / D;
+            if ((elev[c] - temp) < bedrock[c]) temp = elev[c] - bedrock[c];
+            if (temp < 0) temp = 0;
+            tempcreep[c] -= temp;
+            tempcreep[nb] += temp;
+            if (hasgrain[c]) slide_GS(x, y, temp, x2, y2);
+          }
+        }
+      }
+    }
+    for (int x = 1; x <= p.imax; ++x)
+      for (int y = 1; y <= p.jmax; ++y) elev[at(x, y)] += tempcreep[at(x, y)];
+  }
+
   // global_landsliding (LCM.cpp:5529-5654; called by call_global_landsliding,
   // LCM.cpp:1986-1996): row-major sweeps over the interior (x = 2..imax-1
   // outer, y = 2..jmax-1 inner).  Each cell, with wet_factor = tan(failure
@@ -1147,6 +1183,7 @@ double orc_get_flow_timestep(orc_model* m) { return m->get_flow_timestep(); }
 void orc_set_maximum_timestep(orc_model* m) { m->set_maximum_timestep(); }
 double orc_erode(orc_model* m, double mult) { return m->erode(mult); }
 void orc_call_erosion(orc_model* m) { m->call_erosion(); }
+void orc_creep(orc_model* m, double time, double creep_rate) { m->creep(time, creep_rate); }
 int orc_global_landsliding(orc_model* m, double failureangle) {
   return m->global_landsliding(failureangle);
 }

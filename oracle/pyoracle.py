@@ -211,6 +211,7 @@ class OracleModel:
                 getattr(L, fn).restype = ctypes.c_double
             L.orc_global_landsliding.argtypes = [ctypes.c_void_p, ctypes.c_double]
             L.orc_global_landsliding.restype = ctypes.c_int
+            L.orc_creep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
             L.orc_erode.argtypes = [ctypes.c_void_p, ctypes.c_double]
             L.orc_erode.restype = ctypes.c_double
             L.orc_input_points.argtypes = [ctypes.c_void_p]
@@ -390,7 +391,7 @@ class RefModel:
              "call_erosion": 7, "set_loop_cycle": 8, "set_inputoutput_diff": 9,
              "write_output_timeseries": 10, "drainage_area": 11, "input_points": 12,
              "erode": 13, "get_flow_timestep": 14, "topmodel_runoff": 15,
-             "global_landsliding": 16}
+             "global_landsliding": 16, "creep": 17}
 
     def __init__(self, path: str, params_file: str, threads: int = 1):
         self.L = self.lib()

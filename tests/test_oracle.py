@@ -372,3 +372,39 @@ def test_global_landsliding_vs_compiled_reference(tmp_path, angle):
     assert sweeps > 1
     assert not np.array_equal(o.field("elev"), e0)
     assert np.array_equal(r.field("elev"), o.field("elev"))
+
+
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("rate", [0.0025, 2.5])
+def test_creep_vs_compiled_reference(tmp_path, rate):
+    """creep (LSDCatchmentModel.cpp:5293-5427) with slope_creep's time 0.028 on
+    a grain-bearing steep DEM, with cells within a move of their bedrock (the
+    move is cut there): elevations, grain rows and strata bitwise, the grain
+    slides applied in the reference's sweep order."""
+    n = 24
+    rng = np.random.Generator(np.random.PCG64(22))
+    dem = inputs.planar_dem(n, n, 1.0, 0.05, 6.0, 0.6, 1004, south_only=False)
+    props = inputs.dirichlet_props(n, n, rng)
+    w = inputs.make_workload(str(tmp_path / "cr"), dem, 1.0, np.array([10.0, 0.0]), 15, 1,
+                             overrides={"hydro_model_only": "no",
+                                        "active_layer_thickness": "0.3",
+                                        "creep_rate": repr(rate)},
+                             graindata=props, name="cr")
+    r = RefModel(w.path, w.params_file, threads=1)
+    o = OracleModel(params_from_dict(w.params, n, n, 1.0))
+    o.load_snapshot(r.snapshot())
+    for m in (r, o):
+        e, b = m.field("elev"), m.field("bedrock")
+        e[8:12, 8:12] += 5.0
+        e[15, 4:20] -= 3.0
+        b[9, 9] = e[9, 9] - 1e-5
+        b[12, 14] = e[12, 14] - 2e-4
+        b[5, 5] = e[5, 5] + 1.0               # below its bedrock: no creep from it
+    e0 = o.field("elev").copy()
+    r.call("creep", 0.028)
+    o.call("creep", 0.028, rate)
+    s = r.snapshot()
+    assert not np.array_equal(o.field("elev"), e0)
+    assert np.array_equal(r.field("elev"), o.field("elev"))
+    assert np.array_equal(o.field("grain"), s["grain"])
+    assert np.array_equal(o.field("strata").reshape(s["strata"].shape), s["strata"])
