@@ -269,6 +269,7 @@ class CatchmentModel {
 
   void set_time_counters() {  // LCM.cpp:1862-1871
     save_time_ = cycle_;
+    creep_time_ = cycle_;
     creep_time2_ = cycle_;
     time_1_ = cycle_;
   }
@@ -351,8 +352,11 @@ class CatchmentModel {
   void call_channel_landsliding(int interval) {
     if (!hydro_only_ && (counter_ % interval) == 0) warn_once_landslide();
   }
-  void slope_creep() {  // LCM.cpp:1976-1984: cycle > creep_time holds at the first body
-    if (!hydro_only_) warn_once_landslide();
+  void slope_creep(int interval_days, double coeff) {  // LCM.cpp:1976-1984
+    if (!hydro_only_ && cycle_ > creep_time_) {
+      creep_time_ += interval_days;
+      check(hc_creep(ctx_, coeff, creep_rate_), "creep");
+    }
   }
   void call_global_landsliding(int interval_hours) {  // LCM.cpp:1986-1996
     if (!hydro_only_ && cycle_ > creep_time2_) {
@@ -392,7 +396,7 @@ class CatchmentModel {
     if (!is_hydro_only()) call_erosion();
     water_flux_out();
     call_channel_landsliding(10);
-    slope_creep();
+    slope_creep(14400, 0.028);
     call_global_landsliding(1440);
     write_output_timeseries();
     print_cycle();
@@ -548,8 +552,8 @@ class CatchmentModel {
   }
   void warn_once_landslide() {
     if (!warned_landslide_ && !opt_.quiet)
-      std::cerr << "note: in_channel_landsliding and creep are not part of this build; erosion "
-                   "runs omit them\n";
+      std::cerr << "note: in_channel_landsliding is not part of this build; erosion runs omit "
+                   "it\n";
     warned_landslide_ = true;
   }
 
@@ -597,6 +601,7 @@ class CatchmentModel {
       else if (k == "max_tau_velocity") max_vel_ = std::atof(v);
       else if (k == "active_layer_thickness") active_ = std::atof(v);
       else if (k == "slope_failure_thresh") failureangle_ = std::atof(v);
+      else if (k == "creep_rate") creep_rate_ = std::atof(v);
       else if (k == "chann_lateral_erosion") chann_lateral_erosion_ = std::atof(v);
       else if (k == "erosion_limit") ERODEFACTOR_ = std::atof(v);
       else if (k == "lateral_erosion_const") lateral_constant_ = std::atof(v);
@@ -975,8 +980,8 @@ class CatchmentModel {
   std::vector<double> pts_a_;
   int totalinputpoints_ = 0, counter_ = 0, hours_ = 0;
   double time_step_ = 1, cycle_ = 0, maxdepth_ = 10, temptot_ = 0, waterinput_ = 0,
-         waterOut_ = 0, iod_ = 0, time_1_ = 1, save_time_ = 0, creep_time2_ = 1,
-         failureangle_ = 45, previous_ = 0, new_cycle_ = 0,
+         waterOut_ = 0, iod_ = 0, time_1_ = 1, save_time_ = 0, creep_time_ = 1, creep_time2_ = 1,
+         failureangle_ = 45, creep_rate_ = 0.0025, previous_ = 0, new_cycle_ = 0,
          old_cycle_ = 0, erode_mult_ = 1, erode_call_ = 0, globalsediq_ = 0, sediQ_ = 0,
          last_dt_ = 0;
   double tx_ = 60, tlastcalc_ = 0, Qs_step_ = 0, Qs_hour_ = 0, Qs_over_ = 0, Qs_last_ = 0,

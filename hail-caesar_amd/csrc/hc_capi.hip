@@ -493,6 +493,12 @@ int hc_global_landsliding(hc_ctx* c, double failureangle, int* sweeps) {
   return hc_global_landsliding_impl(c, failureangle, sweeps);
 }
 
+int hc_creep(hc_ctx* c, double time, double creep_rate) {
+  if (!c) return HC_ERR_ARG;
+  if (!c->g.sediment) return hc_fail(c, HC_ERR_STATE, "creep: context has no sediment state");
+  return hc_creep_impl(c, time, creep_rate);
+}
+
 int hc_synchronize(hc_ctx* c) {
   if (!c) return HC_ERR_ARG;
   HC_HIP(c, hipStreamSynchronize(c->stream));

@@ -977,6 +977,102 @@ __global__ __launch_bounds__(1024) void k_global_landslide(int imax, int jmax, i
   }
 }
 
+// creep (LCM.cpp:5293-5427).  Every interior cell above its bedrock moves
+// ((z - z_nb) / D) * rate * time / D to each lower neighbour (D = DX on the
+// sides, the reference's constant root = 7.07 on the diagonals, LCM.hpp:437),
+// cut at its bedrock and at 0; the moves collect in tempcreep from the
+// unchanged elevations and are added at the end, and a cell with grain
+// slides the moved grain with each move, in the sweep's (row-major) order.
+// Neighbour order of the reference's moves:
+__device__ constexpr int kCreepOx[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+__device__ constexpr int kCreepOy[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+
+// The grain slides: the reference's row-major sweep as wavefronts t = 3x + y
+// (a slide touches the cell and one neighbour, so sources interact within
+// Chebyshev distance 2), one block, a barrier per wavefront.  Runs before
+// the elevations change (the slides' grain allocation reads them).
+__global__ __launch_bounds__(1024) void k_creep_grain(ErodeArgs a, int imax, int jmax, int P,
+                                                      double DX, double rate, double time,
+                                                      const double* __restrict__ elev,
+                                                      const double* __restrict__ bedrock,
+                                                      Grain gr) {
+  const int t0 = 3 * 2 + 2, t1 = 3 * (imax - 1) + (jmax - 1);
+  for (int t = t0; t <= t1; ++t) {
+    const int xlo = max(2, (t - (jmax - 1) + 2) / 3);
+    const int xhi = min(imax - 1, (t - 2) / 3);
+    for (int x = xlo + static_cast<int>(threadIdx.x); x <= xhi; x += blockDim.x) {
+      const size_t c = static_cast<size_t>(x) * P + (t - 3 * x);
+      if (!(elev[c] > bedrock[c])) continue;
+      for (int k = 0; k < 8; ++k) {
+        const long off = static_cast<long>(kCreepOx[k]) * P + kCreepOy[k];
+        // rate * time / D as the reference groups it: ((dz / D) * rate) * time / D
+        const double es = elev[c];
+        const double en = elev[static_cast<size_t>(static_cast<long>(c) + off)];
+        if (en < es && en > -9999) {
+          const double D = (kCreepOx[k] != 0 && kCreepOy[k] != 0) ? 7.07 : DX;
+          double temp = ((es - en) / D) * rate * time / D;
+          if ((es - temp) < bedrock[c]) temp = es - bedrock[c];
+          if (temp < 0) temp = 0;
+          if (gr.has[c])
+            slide_gs(gr, c, static_cast<size_t>(static_cast<long>(c) + off), temp, elev, bedrock, a);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// tempcreep of every cell x = 1..imax, y = 1..jmax, summed in the order the
+// reference's sweep adds to it: its 3x3 block's interior cells in row-major
+// order, each contributing its move into this cell, or, for the cell itself,
+// its 8 moves out.
+__global__ __launch_bounds__(256) void k_creep_sum(int imax, int jmax, int P, double DX,
+                                                   double rate, double time,
+                                                   const double* __restrict__ elev,
+                                                   const double* __restrict__ bedrock,
+                                                   double* __restrict__ tempcreep) {
+  const int y = 1 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int x = 1 + static_cast<int>(blockIdx.y);
+  if (y > jmax || x > imax) return;
+  const size_t c = static_cast<size_t>(x) * P + y;
+  double tc = 0;
+  for (int dx = -1; dx <= 1; ++dx) {
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int sx = x + dx, sy = y + dy;
+      if (sx < 2 || sx > imax - 1 || sy < 2 || sy > jmax - 1) continue;
+      const size_t s = static_cast<size_t>(sx) * P + sy;
+      if (!(elev[s] > bedrock[s])) continue;
+      for (int k = 0; k < 8; ++k) {
+        const bool self = (dx == 0 && dy == 0);
+        // the source's move k lands on this cell when its offset is -(dx, dy)
+        if (!self && !(kCreepOx[k] == -dx && kCreepOy[k] == -dy)) continue;
+        const long off = static_cast<long>(kCreepOx[k]) * P + kCreepOy[k];
+        const double es = elev[s];
+        const double en = elev[static_cast<size_t>(static_cast<long>(s) + off)];
+        if (en < es && en > -9999) {
+          const double D = (kCreepOx[k] != 0 && kCreepOy[k] != 0) ? 7.07 : DX;
+          double temp = ((es - en) / D) * rate * time / D;
+          if ((es - temp) < bedrock[s]) temp = es - bedrock[s];
+          if (temp < 0) temp = 0;
+          if (self) tc -= temp;
+          else tc += temp;
+        }
+      }
+    }
+  }
+  tempcreep[c] = tc;
+}
+
+__global__ __launch_bounds__(256) void k_creep_add(int imax, int jmax, int P,
+                                                   double* __restrict__ elev,
+                                                   const double* __restrict__ tempcreep) {
+  const int y = 1 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int x = 1 + static_cast<int>(blockIdx.y);
+  if (y > jmax || x > imax) return;
+  const size_t c = static_cast<size_t>(x) * P + y;
+  elev[c] += tempcreep[c];
+}
+
 }  // namespace
 
 int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot_out) {
@@ -1056,5 +1152,27 @@ int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps) {
   HC_HIP(c, hipStreamSynchronize(c->stream));
   std::memcpy(&n, c->h_pinned, sizeof(int));
   if (sweeps) *sweeps = n;
+  return HC_OK;
+}
+
+int hc_creep_impl(hc_ctx* c, double time, double creep_rate) {
+  if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "creep: FP64 contexts only");
+  const size_t plane = c->ncell;
+  if (!c->d_slide_moves) {
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_slide_moves), 8 * plane * sizeof(double)));
+  }
+  const ErodeArgs a = erode_args(c);
+  const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
+  double* elev = static_cast<double*>(c->z);
+  const int imax = c->g.imax, jmax = c->g.jmax;
+  if (imax >= 3 && jmax >= 3)
+    hipLaunchKernelGGL(k_creep_grain, dim3(1), dim3(1024), 0, c->stream, a, imax, jmax, c->P,
+                       c->g.DX, creep_rate, time, elev, c->bedrock, gr);
+  const dim3 blk(256), grd(static_cast<unsigned>((jmax + 255) / 256), static_cast<unsigned>(imax));
+  hipLaunchKernelGGL(k_creep_sum, grd, blk, 0, c->stream, imax, jmax, c->P, c->g.DX, creep_rate,
+                     time, elev, c->bedrock, c->d_slide_moves);
+  hipLaunchKernelGGL(k_creep_add, grd, blk, 0, c->stream, imax, jmax, c->P, elev,
+                     c->d_slide_moves);
+  HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

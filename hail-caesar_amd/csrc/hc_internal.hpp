@@ -213,7 +213,7 @@ struct hc_ctx {
   // reductions and small device scalars
   unsigned long long* d_red = nullptr;  // [0] maxdepth bits, [1] tempbmax bits, [2] health count,
                                         // [3] global_landsliding sweep count
-  double* d_slide_moves = nullptr;      // global_landsliding: 8 neighbour planes of moves
+  double* d_slide_moves = nullptr;      // hillslope scratch: 8 planes (landslide moves, tempcreep)
   double* d_partials = nullptr;         // per-block partial sums
   unsigned int* d_count = nullptr;      // last-block counters
   double* d_dpart = nullptr;            // fused-kernel edge drain: one slot per tile
@@ -298,3 +298,4 @@ int hc_launch_fill(hc_ctx* c, void* p, size_t n, double v, bool is_float);
 // erosion (hc_erode.hip)
 int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot);
 int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps);
+int hc_creep_impl(hc_ctx* c, double time, double creep_rate);

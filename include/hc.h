@@ -188,6 +188,15 @@ int hc_erode(hc_ctx* ctx, double mult_factor, double* time_step,
  * of sweeps.  Synchronous. */
 int hc_global_landsliding(hc_ctx* ctx, double failureangle, int* sweeps);
 
+/* LSDCatchmentModel::creep(time) (LCM.cpp:5293-5427), which the reference's
+ * loop calls through slope_creep (LCM.cpp:1976-1984) with time = 0.028 every
+ * 14400 simulated minutes in erosion runs: each cell above its bedrock moves
+ * material to its lower neighbours in proportion to the slope and
+ * `creep_rate` (the parameter-file key creep_rate), sliding the moved grain
+ * fractions in the reference's sweep order.  Elevations, grain rows and
+ * strata bit-identical to the reference.  FP64 sediment contexts. */
+int hc_creep(hc_ctx* ctx, double time, double creep_rate);
+
 /* Block until every call enqueued on the context has finished. */
 int hc_synchronize(hc_ctx* ctx);
 

@@ -90,3 +90,38 @@ def test_global_landsliding_vs_oracle(angle, nrows, ncols):
     sweeps_o = o.call("global_landsliding", angle)
     assert sweeps == sweeps_o and sweeps > 1
     assert np.array_equal(ctx.download("elev"), o.field("elev"))
+
+
+@pytest.mark.parametrize("rate,nrows,ncols", [(2.5, 24, 24), (2.5, 40, 70), (0.0025, 140, 50)])
+def test_creep_vs_oracle(tmp_path, rate, nrows, ncols):
+    """creep (LSDCatchmentModel.cpp:5293-5427) on the device, the grain
+    slides as wavefronts t = 3x + y and tempcreep gathered per cell in the
+    sweep's order, against the oracle (pinned bitwise to the compiled
+    reference): elevations, grain rows, strata and grain flags bitwise, on a
+    grain-bearing steep DEM with cells at and below their bedrock."""
+    rng = np.random.Generator(np.random.PCG64(23))
+    dem = inputs.planar_dem(nrows, ncols, 1.0, 0.05, 6.0, 0.6, 1005 + nrows, south_only=False)
+    props = inputs.dirichlet_props(nrows, ncols, rng)
+    w = inputs.make_workload(str(tmp_path / "cr"), dem, 1.0, np.array([10.0, 0.0]), 15, 1,
+                             overrides={"hydro_model_only": "no", "active_layer_thickness": "0.3",
+                                        "creep_rate": repr(rate)},
+                             graindata=props, name="cr")
+    o = OracleModel.from_workload(w)
+    p = o.p
+    e, b = o.field("elev"), o.field("bedrock")
+    e[8:12, 8:12] += 5.0
+    e[15, 4:20] -= 3.0
+    b[9, 9] = e[9, 9] - 1e-5
+    b[12, 14] = e[12, 14] - 2e-4
+    b[5, 5] = e[5, 5] + 1.0
+    ctx = hc.Context(p.imax, p.jmax, p.DX, hc_params_from(p), hc.HC_FP64, sediment=True)
+    for f in ("elev", "bedrock", "grain"):
+        ctx.upload(f, o.field(f))
+    ctx.upload("strata", o.field("strata"))
+    ctx.upload("hasgrain", o.field_int("hasgrain"))
+    ctx.creep(0.028, rate)
+    o.call("creep", 0.028, rate)
+    assert np.array_equal(ctx.download("elev"), o.field("elev"))
+    assert np.array_equal(ctx.download("grain"), o.field("grain"))
+    assert np.array_equal(ctx.download("strata").reshape(o.field("strata").shape), o.field("strata"))
+    assert np.array_equal(ctx.download("hasgrain"), o.field_int("hasgrain"))
