@@ -919,6 +919,44 @@ struct orc_model {
   }
 
   // LCM.cpp:2277-2294
+  // in_channel_landsliding (LCM.cpp:5429-5528; called every 10 iterations by
+  // call_channel_landsliding, LCM.cpp:1968-1974): for each column y =
+  // 2..jmax-1, the rows x of its active list (down_scan, the mask here) in
+  // ascending order, x = imax taken as imax-1 and x = 1 as 2; each moves
+  // material in place to every neighbour, in the order below, whose drop
+  // (over 1.41 on the diagonals) exceeds tan(failure angle) DX (10000 where
+  // the cell is at or below bedrock + active), cut so it stays at bedrock +
+  // active, sliding the moved grain with each move.
+  void in_channel_landsliding(double failureangle) {
+    const double factor = std::tan((failureangle * (3.141592654 / 180))) * p.DX;
+    static const int ox[8] = {1, 0, -1, -1, -1, 0, 1, 1};
+    static const int oy[8] = {1, 1, 1, 0, -1, -1, -1, 0};
+    for (int y = 2; y < p.jmax; ++y) {
+      for (int xl = 1; xl <= p.imax; ++xl) {
+        if (!mask[at(xl, y)]) continue;
+        int x = xl;
+        if (x == p.imax) x = p.imax - 1;
+        if (x == 1) x = 2;
+        const size_t c = at(x, y);
+        double wet_factor = factor;
+        if (elev[c] <= (bedrock[c] + p.active)) wet_factor = 10000;
+        for (int k = 0; k < 8; ++k) {
+          const int x2 = x + ox[k], y2 = y + oy[k];
+          const size_t nb = at(x2, y2);
+          const bool diag = (ox[k] != 0 && oy[k] != 0);
+          const double drop = diag ? ((elev[c] - elev[nb]) / 1.41) : (elev[c] - elev[nb]);
+          if (drop > wet_factor && elev[nb] > -9999) {
+            double diff = drop - wet_factor;
+            if ((elev[c] - diff) < (bedrock[c] + p.active)) diff = (elev[c] - (bedrock[c] + p.active));
+            elev[c] -= diff;
+            elev[nb] += diff;
+            slide_GS(x, y, diff, x2, y2);
+          }
+        }
+      }
+    }
+  }
+
   // creep (LCM.cpp:5293-5427; called by slope_creep, LCM.cpp:1976-1984 with
   // time = 0.028): every interior cell above its bedrock moves
   // ((z - z_nb) / D) * creep_rate * time / D to each lower neighbour
@@ -1184,6 +1222,9 @@ void orc_set_maximum_timestep(orc_model* m) { m->set_maximum_timestep(); }
 double orc_erode(orc_model* m, double mult) { return m->erode(mult); }
 void orc_call_erosion(orc_model* m) { m->call_erosion(); }
 void orc_creep(orc_model* m, double time, double creep_rate) { m->creep(time, creep_rate); }
+void orc_in_channel_landsliding(orc_model* m, double failureangle) {
+  m->in_channel_landsliding(failureangle);
+}
 int orc_global_landsliding(orc_model* m, double failureangle) {
   return m->global_landsliding(failureangle);
 }

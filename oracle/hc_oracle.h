@@ -92,6 +92,8 @@ void orc_call_erosion(orc_model* m);
 int orc_global_landsliding(orc_model* m, double failureangle);
 /* creep(time) with the given creep rate (the parameter key creep_rate). */
 void orc_creep(orc_model* m, double time, double creep_rate);
+/* in_channel_landsliding over the current mask (the active list). */
+void orc_in_channel_landsliding(orc_model* m, double failureangle);
 void orc_drainage_area(orc_model* m);
 int orc_input_points(orc_model* m);
 double orc_grain_call(orc_model* m, int which, int x, int y, double amount, int x2, int y2);

@@ -212,6 +212,7 @@ class OracleModel:
             L.orc_global_landsliding.argtypes = [ctypes.c_void_p, ctypes.c_double]
             L.orc_global_landsliding.restype = ctypes.c_int
             L.orc_creep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+            L.orc_in_channel_landsliding.argtypes = [ctypes.c_void_p, ctypes.c_double]
             L.orc_erode.argtypes = [ctypes.c_void_p, ctypes.c_double]
             L.orc_erode.restype = ctypes.c_double
             L.orc_input_points.argtypes = [ctypes.c_void_p]
@@ -391,7 +392,8 @@ class RefModel:
              "call_erosion": 7, "set_loop_cycle": 8, "set_inputoutput_diff": 9,
              "write_output_timeseries": 10, "drainage_area": 11, "input_points": 12,
              "erode": 13, "get_flow_timestep": 14, "topmodel_runoff": 15,
-             "global_landsliding": 16, "creep": 17}
+             "global_landsliding": 16, "creep": 17,
+             "in_channel_landsliding": 18}
 
     def __init__(self, path: str, params_file: str, threads: int = 1):
         self.L = self.lib()

@@ -170,7 +170,7 @@ int ref_iterate_plain(void* h, int n) {
 //   8 set_loop_cycle    9 set_inputoutput_diff 10 write_output_timeseries
 //  11 zero_and_calc_drainage_area  12 get_catchment_input_points
 //  13 erode  14 get_flow_timestep  15 topmodel_runoff  16 global_landsliding
-//  17 creep(arg)
+//  17 creep(arg)  18 in_channel_landsliding
 double ref_call(void* h, int which, double arg) {
   RefRun* r = static_cast<RefRun*>(h);
   LSDCatchmentModel& s = *r->m;
@@ -193,6 +193,7 @@ double ref_call(void* h, int which, double arg) {
     case 15: s.topmodel_runoff(arg); return s.new_j_mean[1];
     case 16: s.global_landsliding(); return 0;
     case 17: s.creep(arg); return 0;
+    case 18: s.in_channel_landsliding(); return 0;
     default: return -1;
   }
 }

@@ -408,3 +408,48 @@ def test_creep_vs_compiled_reference(tmp_path, rate):
     assert np.array_equal(r.field("elev"), o.field("elev"))
     assert np.array_equal(o.field("grain"), s["grain"])
     assert np.array_equal(o.field("strata").reshape(s["strata"].shape), s["strata"])
+
+
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("angle", [45.0, 20.0])
+def test_in_channel_landsliding_vs_compiled_reference(tmp_path, angle):
+    """in_channel_landsliding (LSDCatchmentModel.cpp:5429-5528) over the
+    active list of a partly wet, grain-bearing steep DEM, with wet cells on
+    the first and last rows (their list entries are taken as rows 2 and
+    imax-1, which then slide twice), cells at bedrock + active and cells
+    whose moves are cut at bedrock + active:
+    elevations, grain rows and strata bitwise."""
+    n = 24
+    rng = np.random.Generator(np.random.PCG64(24))
+    dem = inputs.planar_dem(n, n, 1.0, 0.05, 6.0, 0.6, 1006, south_only=False)
+    props = inputs.dirichlet_props(n, n, rng)
+    water = np.where(rng.random((n, n)) < 0.3, 0.1, 0.0)
+    water[0, 3:9] = 0.2
+    water[n - 1, 10:16] = 0.2
+    w = inputs.make_workload(str(tmp_path / "ic"), dem, 1.0, np.array([10.0, 0.0]), 15, 1,
+                             overrides={"hydro_model_only": "no",
+                                        "active_layer_thickness": "0.3",
+                                        "slope_failure_thresh": repr(angle)},
+                             water_init=water, graindata=props, name="ic")
+    r = RefModel(w.path, w.params_file, threads=1)
+    o = OracleModel(params_from_dict(w.params, n, n, 1.0))
+    o.load_snapshot(r.snapshot())
+    for m in (r, o):
+        e, b = m.field("elev"), m.field("bedrock")
+        e[8:12, 8:12] += 8.0
+        e[2, 4:8] += 6.0                      # steep cells next to the clamped rows
+        e[n - 1, 11:15] += 6.0
+        b[9, 9] = e[9, 9] - 0.1               # at bedrock + active
+        b[8, 8] = e[8, 8] - 2.3               # above it: its moves are cut there
+        b[11, 11] = e[11, 11] - 1.3
+    r.call("scan_area")
+    o.call("scan_area")
+    assert np.array_equal(r.mask(), o.field_int("mask"))
+    e0 = o.field("elev").copy()
+    r.call("in_channel_landsliding")
+    o.call("in_channel_landsliding", angle)
+    s = r.snapshot()
+    assert not np.array_equal(o.field("elev"), e0)
+    assert np.array_equal(r.field("elev"), o.field("elev"))
+    assert np.array_equal(o.field("grain"), s["grain"])
+    assert np.array_equal(o.field("strata").reshape(s["strata"].shape), s["strata"])
