@@ -349,8 +349,9 @@ class CatchmentModel {
     check(hc_water_flux_out(ctx_, time_step_, &temptot_), "water_flux_out");
     waterOut_ = temptot_;
   }
-  void call_channel_landsliding(int interval) {
-    if (!hydro_only_ && (counter_ % interval) == 0) warn_once_landslide();
+  void call_channel_landsliding(int interval) {  // LCM.cpp:1968-1974
+    if (!hydro_only_ && (counter_ % interval) == 0)
+      check(hc_in_channel_landsliding(ctx_, failureangle_), "in_channel_landsliding");
   }
   void slope_creep(int interval_days, double coeff) {  // LCM.cpp:1976-1984
     if (!hydro_only_ && cycle_ > creep_time_) {
@@ -550,12 +551,7 @@ class CatchmentModel {
   void upload(int field, const void* host) {
     check(hc_upload(ctx_, field, host, hc_field_count(ctx_, field)), "upload");
   }
-  void warn_once_landslide() {
-    if (!warned_landslide_ && !opt_.quiet)
-      std::cerr << "note: in_channel_landsliding is not part of this build; erosion runs omit "
-                   "it\n";
-    warned_landslide_ = true;
-  }
+
 
   // ---------------------------------------------------------- parameters
   // LSDCatchmentModel::initialise_variables (LCM.cpp:842-1610) for the keys
@@ -969,7 +965,7 @@ class CatchmentModel {
        spatially_var_rainfall_ = false, spatially_var_mannings_ = false,
        graindata_from_file_ = false, bedrock_layer_on_ = false, wilcock_ = false,
        einstein_ = false, write_elev_file_ = false, write_waterd_file_ = false,
-       debug_print_cycle_ = false, warned_landslide_ = false;
+       debug_print_cycle_ = false;
   // state
   std::vector<double> elev_, area_, area_depth_;
   std::vector<int> rfarea_;

@@ -499,6 +499,13 @@ int hc_creep(hc_ctx* c, double time, double creep_rate) {
   return hc_creep_impl(c, time, creep_rate);
 }
 
+int hc_in_channel_landsliding(hc_ctx* c, double failureangle) {
+  if (!c) return HC_ERR_ARG;
+  if (!c->g.sediment)
+    return hc_fail(c, HC_ERR_STATE, "in_channel_landsliding: context has no sediment state");
+  return hc_in_channel_landsliding_impl(c, failureangle);
+}
+
 int hc_synchronize(hc_ctx* c) {
   if (!c) return HC_ERR_ARG;
   HC_HIP(c, hipStreamSynchronize(c->stream));

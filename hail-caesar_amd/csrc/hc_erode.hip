@@ -1073,6 +1073,59 @@ __global__ __launch_bounds__(256) void k_creep_add(int imax, int jmax, int P,
   elev[c] += tempcreep[c];
 }
 
+// in_channel_landsliding (LCM.cpp:5429-5528).  The reference walks the
+// columns y = 2..jmax-1 in order and, in each, the rows of its active list
+// (down_scan: the mask) ascending, row 1 taken as row 2 and row imax as
+// imax-1; a cell moves material in place to each neighbour, in the order
+// below, whose drop (over 1.41 on the diagonals) exceeds tan(failure angle)
+// DX (10000 where the cell is at or below bedrock + active), cut so the cell
+// stays at bedrock + active, sliding the moved grain with each move.  As for
+// global_landsliding, cells interact only within Chebyshev distance 2, so
+// the column-major order runs as wavefronts t = 3y + x, one block, a barrier
+// per wavefront.  A cell runs once for each list entry taken as its row (its
+// own, and row 1's for row 2, row imax's for row imax-1), consecutively as
+// in the reference.
+__global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int imax, int jmax,
+                                                               int P, int W, double factor,
+                                                               const uint64_t* __restrict__ mask,
+                                                               double* __restrict__ elev,
+                                                               const double* __restrict__ bedrock,
+                                                               Grain gr) {
+  constexpr int ox[8] = {1, 0, -1, -1, -1, 0, 1, 1};
+  constexpr int oy[8] = {1, 1, 1, 0, -1, -1, -1, 0};
+  const int xmax = max(imax - 1, 2);  // the rows list entries are taken as
+  const int t0 = 3 * 2 + 2, t1 = 3 * (jmax - 1) + xmax;
+  for (int t = t0; t <= t1; ++t) {
+    const int ylo = max(2, (t - xmax + 2) / 3);  // x = t - 3y <= xmax
+    const int yhi = min(jmax - 1, (t - 2) / 3);  // x >= 2
+    for (int y = ylo + static_cast<int>(threadIdx.x); y <= yhi; y += blockDim.x) {
+      const int x = t - 3 * y;
+      // entries taken as row x: x itself, row 1 (taken as 2), row imax (taken as imax-1)
+      int runs = mask_bit(mask, W, x, y) ? 1 : 0;
+      if (x == 2 && mask_bit(mask, W, 1, y)) ++runs;
+      if (imax >= 3 && x == imax - 1 && mask_bit(mask, W, imax, y)) ++runs;
+      const size_t c = static_cast<size_t>(x) * P + y;
+      for (int run = 0; run < runs; ++run) {
+        const double bpa = bedrock[c] + a.active;
+        const double wet = (elev[c] <= bpa) ? 10000.0 : factor;
+        for (int k = 0; k < 8; ++k) {
+          const size_t nb = static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k]);
+          const double e = elev[c], en = elev[nb];
+          const double drop = (ox[k] != 0 && oy[k] != 0) ? (e - en) / 1.41 : (e - en);
+          if (drop > wet && en > -9999) {
+            double diff = drop - wet;
+            if ((e - diff) < bpa) diff = (e - bpa);
+            elev[c] = e - diff;
+            elev[nb] = en + diff;
+            slide_gs(gr, c, nb, diff, elev, bedrock, a);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot_out) {
@@ -1173,6 +1226,20 @@ int hc_creep_impl(hc_ctx* c, double time, double creep_rate) {
                      time, elev, c->bedrock, c->d_slide_moves);
   hipLaunchKernelGGL(k_creep_add, grd, blk, 0, c->stream, imax, jmax, c->P, elev,
                      c->d_slide_moves);
+  HC_HIP(c, hipGetLastError());
+  return HC_OK;
+}
+
+int hc_in_channel_landsliding_impl(hc_ctx* c, double failureangle) {
+  if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "in_channel_landsliding: FP64 contexts only");
+  if (c->g.imax < 2 || c->g.jmax < 3) return HC_OK;  // no column 2..jmax-1 or row 2
+  // the reference's factor, from the host's libm as in the reference
+  const double factor = std::tan((failureangle * (3.141592654 / 180))) * c->g.DX;
+  const ErodeArgs a = erode_args(c);
+  const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
+  hipLaunchKernelGGL(k_in_channel_landslide, dim3(1), dim3(1024), 0, c->stream, a, c->g.imax,
+                     c->g.jmax, c->P, c->W, factor, c->mask, static_cast<double*>(c->z),
+                     c->bedrock, gr);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

@@ -299,3 +299,4 @@ int hc_launch_fill(hc_ctx* c, void* p, size_t n, double v, bool is_float);
 int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot);
 int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps);
 int hc_creep_impl(hc_ctx* c, double time, double creep_rate);
+int hc_in_channel_landsliding_impl(hc_ctx* c, double failureangle);

@@ -33,7 +33,7 @@ INT_FIELDS = {"mask", "hasgrain"}
 SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upload",
            "hc_download", "hc_flow_route", "hc_depth_update", "hc_flow_depth",
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
-           "hc_global_landsliding", "hc_creep",
+           "hc_global_landsliding", "hc_creep", "hc_in_channel_landsliding",
            "hc_synchronize", "hc_check_health", "hc_loop_begin", "hc_loop_run",
            "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
            "hc_set_tiles", "hc_bench_fused", "hc_ens_create", "hc_ens_destroy",
@@ -138,6 +138,7 @@ def lib():
         L.hc_erode.argtypes = [vp, d, ctypes.POINTER(d), ctypes.POINTER(d), vp]
         L.hc_global_landsliding.argtypes = [vp, d, ctypes.POINTER(i)]
         L.hc_creep.argtypes = [vp, d, d]
+        L.hc_in_channel_landsliding.argtypes = [vp, d]
         L.hc_synchronize.argtypes = [vp]
         L.hc_check_health.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         L.hc_loop_begin.argtypes = [vp, ctypes.POINTER(hc_loop_init)]
@@ -314,6 +315,10 @@ class Context:
     def creep(self, time: float, creep_rate: float) -> None:
         """LSDCatchmentModel::creep(time) (LCM.cpp:5293-5427)."""
         self._check(self.L.hc_creep(self.h, time, creep_rate))
+
+    def in_channel_landsliding(self, failureangle: float) -> None:
+        """LSDCatchmentModel::in_channel_landsliding (LCM.cpp:5429-5528)."""
+        self._check(self.L.hc_in_channel_landsliding(self.h, failureangle))
 
     def synchronize(self) -> None:
         self._check(self.L.hc_synchronize(self.h))

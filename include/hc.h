@@ -197,6 +197,16 @@ int hc_global_landsliding(hc_ctx* ctx, double failureangle, int* sweeps);
  * strata bit-identical to the reference.  FP64 sediment contexts. */
 int hc_creep(hc_ctx* ctx, double time, double creep_rate);
 
+/* LSDCatchmentModel::in_channel_landsliding (LCM.cpp:5429-5528), which the
+ * reference's loop calls every 10 iterations in erosion runs
+ * (call_channel_landsliding, LCM.cpp:1968-1974): over the active mask as the
+ * scan left it (the reference's down_scan), column by column, each listed
+ * cell moves material down every slope steeper than `failureangle` degrees,
+ * kept above bedrock + active, sliding the moved grain fractions.
+ * Elevations, grain rows and strata bit-identical to the reference.  FP64
+ * sediment contexts. */
+int hc_in_channel_landsliding(hc_ctx* ctx, double failureangle);
+
 /* Block until every call enqueued on the context has finished. */
 int hc_synchronize(hc_ctx* ctx);
 

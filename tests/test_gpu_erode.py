@@ -125,3 +125,46 @@ def test_creep_vs_oracle(tmp_path, rate, nrows, ncols):
     assert np.array_equal(ctx.download("grain"), o.field("grain"))
     assert np.array_equal(ctx.download("strata").reshape(o.field("strata").shape), o.field("strata"))
     assert np.array_equal(ctx.download("hasgrain"), o.field_int("hasgrain"))
+
+
+@pytest.mark.parametrize("angle,nrows,ncols", [(45.0, 24, 24), (20.0, 40, 70), (45.0, 140, 50)])
+def test_in_channel_landsliding_vs_oracle(tmp_path, angle, nrows, ncols):
+    """in_channel_landsliding (LSDCatchmentModel.cpp:5429-5528) on the device,
+    column-major wavefronts t = 3y + x over the active mask, against the oracle
+    (pinned bitwise to the compiled reference), on a partly wet grain-bearing
+    steep DEM with wet first and last rows and cells at and just above
+    bedrock + active: the device's own scan gives the same mask, then
+    elevations, grain rows, strata and grain flags bitwise."""
+    rng = np.random.Generator(np.random.PCG64(25))
+    dem = inputs.planar_dem(nrows, ncols, 1.0, 0.05, 6.0, 0.6, 1007 + nrows, south_only=False)
+    props = inputs.dirichlet_props(nrows, ncols, rng)
+    water = np.where(rng.random((nrows, ncols)) < 0.3, 0.1, 0.0)
+    water[0, 3:9] = 0.2
+    water[nrows - 1, 10:16] = 0.2
+    w = inputs.make_workload(str(tmp_path / "ic"), dem, 1.0, np.array([10.0, 0.0]), 15, 1,
+                             overrides={"hydro_model_only": "no", "active_layer_thickness": "0.3",
+                                        "slope_failure_thresh": repr(angle)},
+                             water_init=water, graindata=props, name="ic")
+    o = OracleModel.from_workload(w)
+    p = o.p
+    e, b = o.field("elev"), o.field("bedrock")
+    e[8:12, 8:12] += 8.0
+    e[2, 4:8] += 6.0
+    e[nrows - 1, 11:15] += 6.0
+    b[9, 9] = e[9, 9] - 0.1
+    b[8, 8] = e[8, 8] - 2.3
+    b[11, 11] = e[11, 11] - 1.3
+    ctx = hc.Context(p.imax, p.jmax, p.DX, hc_params_from(p), hc.HC_FP64, sediment=True)
+    for f in ("water_depth", "elev", "bedrock", "grain"):
+        ctx.upload(f, o.field(f))
+    ctx.upload("strata", o.field("strata"))
+    ctx.upload("hasgrain", o.field_int("hasgrain"))
+    o.call("scan_area")
+    ctx.scan_area()
+    assert np.array_equal(ctx.download("mask"), o.field_int("mask"))
+    ctx.in_channel_landsliding(angle)
+    o.call("in_channel_landsliding", angle)
+    assert np.array_equal(ctx.download("elev"), o.field("elev"))
+    assert np.array_equal(ctx.download("grain"), o.field("grain"))
+    assert np.array_equal(ctx.download("strata").reshape(o.field("strata").shape), o.field("strata"))
+    assert np.array_equal(ctx.download("hasgrain"), o.field_int("hasgrain"))
