@@ -894,6 +894,39 @@ ErodeArgs erode_args(const hc_ctx* c) {
   return a;
 }
 
+// Whether any cell of rows 2..xmax, columns 2..jmax-1 would move material into a neighbour under the
+// present elevations: the drop to the neighbour (over 1.41 on the
+// diagonals) above the cell's threshold (wet_low where the cell is at or
+// below bedrock + active, factor elsewhere), the neighbour above -9999 --
+// the test both landsliding sweeps apply.  The sweeps change elevations only
+// by such moves, so where no cell passes it a sweep in any order moves
+// nothing: *any stays 0 and the sweep kernels return at once.  Neighbour
+// order does not matter here.
+__global__ __launch_bounds__(256) void k_slide_candidates(int xmax, int jmax, int P, double factor,
+                                                          double wet_low, double active,
+                                                          const double* __restrict__ elev,
+                                                          const double* __restrict__ bedrock,
+                                                          unsigned long long* __restrict__ any) {
+  const int y = 2 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int x = 2 + static_cast<int>(blockIdx.y);
+  if (y > jmax - 1 || x > xmax) return;
+  const size_t c = static_cast<size_t>(x) * P + y;
+  const double e = elev[c];
+  const double wet = (e <= bedrock[c] + active) ? wet_low : factor;
+  bool found = false;
+#pragma unroll
+  for (int dx = -1; dx <= 1; ++dx) {
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      if (dx == 0 && dy == 0) continue;
+      const double en = elev[static_cast<size_t>(static_cast<long>(c) + dx * P + dy)];
+      const double drop = (dx != 0 && dy != 0) ? (e - en) / 1.41 : (e - en);
+      if (drop > wet && en > -9999) found = true;
+    }
+  }
+  if (found) *any = 1ull;  // every writer stores the same value
+}
+
 // global_landsliding (LCM.cpp:5529-5654).  The reference sweeps the
 // interior row-major (x = 2..imax-1 outer, y = 2..jmax-1 inner); a cell
 // reads its 8 neighbours and moves material into them in place, so two cells
@@ -912,13 +945,18 @@ __global__ __launch_bounds__(1024) void k_global_landslide(int imax, int jmax, i
                                                            double EF, double* __restrict__ elev,
                                                            const double* __restrict__ bedrock,
                                                            double* __restrict__ moves,
-                                                           size_t plane, int* __restrict__ sweeps) {
+                                                           size_t plane, int* __restrict__ sweeps,
+                                                           const unsigned long long* __restrict__ any) {
   // neighbour order of the reference's checks
   constexpr int ox[8] = {1, 0, -1, -1, -1, 0, 1, 1};
   constexpr int oy[8] = {1, 1, 1, 0, -1, -1, -1, 0};
   __shared__ int s_pos, s_neg, s_again;
   const int tid = threadIdx.x;
   const int t0 = 3 * 2 + 2, t1 = 3 * (imax - 1) + (jmax - 1);
+  if (*any == 0) {  // no cell can move: one sweep, total 0
+    if (tid == 0) *sweeps = 1;
+    return;
+  }
   int inc = 0;
   for (;;) {
     ++inc;
@@ -937,10 +975,15 @@ __global__ __launch_bounds__(1024) void k_global_landslide(int imax, int jmax, i
         double e = elev[c];
         const double bpa = bedrock[c] + active;
         const double wet = (e <= bpa) ? 10 * DX : factor;
+        // no other cell of this wavefront touches the 3x3 block: its
+        // elevations are read at once and written at each move
+        double enb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) enb[k] = elev[static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k])];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
           const size_t nb = static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k]);
-          const double en = elev[nb];
+          const double en = enb[k];
           const double drop = (ox[k] != 0 && oy[k] != 0) ? (e - en) / 1.41 : (e - en);
           double d = 0.0;
           if (drop > wet && en > -9999) {
@@ -1090,7 +1133,9 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
                                                                const uint64_t* __restrict__ mask,
                                                                double* __restrict__ elev,
                                                                const double* __restrict__ bedrock,
-                                                               Grain gr) {
+                                                               Grain gr,
+                                                               const unsigned long long* __restrict__ any) {
+  if (*any == 0) return;  // no cell can move
   constexpr int ox[8] = {1, 0, -1, -1, -1, 0, 1, 1};
   constexpr int oy[8] = {1, 1, 1, 0, -1, -1, -1, 0};
   const int xmax = max(imax - 1, 2);  // the rows list entries are taken as
@@ -1100,23 +1145,33 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
     const int yhi = min(jmax - 1, (t - 2) / 3);  // x >= 2
     for (int y = ylo + static_cast<int>(threadIdx.x); y <= yhi; y += blockDim.x) {
       const int x = t - 3 * y;
-      // entries taken as row x: x itself, row 1 (taken as 2), row imax (taken as imax-1)
-      int runs = mask_bit(mask, W, x, y) ? 1 : 0;
-      if (x == 2 && mask_bit(mask, W, 1, y)) ++runs;
-      if (imax >= 3 && x == imax - 1 && mask_bit(mask, W, imax, y)) ++runs;
+      // Everything the cell reads is loaded at once: no other cell of this
+      // wavefront touches its 3x3 block, so the values stay in registers;
+      // memory is written at each move, before the grain slide that reads it.
       const size_t c = static_cast<size_t>(x) * P + y;
+      const bool m0 = mask_bit(mask, W, x, y);
+      const bool m1 = x == 2 && mask_bit(mask, W, 1, y);
+      const bool m2 = imax >= 3 && x == imax - 1 && mask_bit(mask, W, imax, y);
+      double e = elev[c];
+      const double bpa = bedrock[c] + a.active;
+      double en[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) en[k] = elev[static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k])];
+      // entries taken as row x: x itself, row 1 (taken as 2), row imax (taken as imax-1)
+      const int runs = (m0 ? 1 : 0) + (m1 ? 1 : 0) + (m2 ? 1 : 0);
       for (int run = 0; run < runs; ++run) {
-        const double bpa = bedrock[c] + a.active;
-        const double wet = (elev[c] <= bpa) ? 10000.0 : factor;
+        const double wet = (e <= bpa) ? 10000.0 : factor;
+#pragma unroll
         for (int k = 0; k < 8; ++k) {
-          const size_t nb = static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k]);
-          const double e = elev[c], en = elev[nb];
-          const double drop = (ox[k] != 0 && oy[k] != 0) ? (e - en) / 1.41 : (e - en);
-          if (drop > wet && en > -9999) {
+          const double drop = (ox[k] != 0 && oy[k] != 0) ? (e - en[k]) / 1.41 : (e - en[k]);
+          if (drop > wet && en[k] > -9999) {
             double diff = drop - wet;
             if ((e - diff) < bpa) diff = (e - bpa);
-            elev[c] = e - diff;
-            elev[nb] = en + diff;
+            e -= diff;
+            en[k] += diff;
+            const size_t nb = static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k]);
+            elev[c] = e;
+            elev[nb] = en[k];
             slide_gs(gr, c, nb, diff, elev, bedrock, a);
           }
         }
@@ -1182,6 +1237,21 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
   return HC_OK;
 }
 
+namespace {
+
+// d_red[4] := whether any cell of rows 2..xmax, columns 2..jmax-1 passes
+// the landsliding move test (k_slide_candidates); needs xmax >= 2, jmax >= 3.
+int slide_candidates(hc_ctx* c, double factor, double wet_low, int xmax) {
+  HC_HIP(c, hipMemsetAsync(c->d_red + 4, 0, sizeof(unsigned long long), c->stream));
+  const int jmax = c->g.jmax;
+  const dim3 blk(256), grd(static_cast<unsigned>((jmax - 2 + 255) / 256), static_cast<unsigned>(xmax - 1));
+  hipLaunchKernelGGL(k_slide_candidates, grd, blk, 0, c->stream, xmax, jmax, c->P, factor, wet_low,
+                     c->prm.active, static_cast<const double*>(c->z), c->bedrock, c->d_red + 4);
+  return HC_OK;
+}
+
+}  // namespace
+
 int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps) {
   if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "global_landsliding: FP64 contexts only");
   if (c->g.imax < 3 || c->g.jmax < 3) {  // no interior cell
@@ -1194,10 +1264,11 @@ int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps) {
   if (!c->d_slide_moves) {
     HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_slide_moves), 8 * plane * sizeof(double)));
   }
+  if (int rc = slide_candidates(c, factor, 10 * c->g.DX, c->g.imax - 1)) return rc;
   hipLaunchKernelGGL(k_global_landslide, dim3(1), dim3(1024), 0, c->stream, c->g.imax,
                      c->g.jmax, c->P, c->g.DX, factor, c->prm.active, c->prm.ERODEFACTOR,
                      static_cast<double*>(c->z), c->bedrock, c->d_slide_moves, plane,
-                     reinterpret_cast<int*>(c->d_red + 3));
+                     reinterpret_cast<int*>(c->d_red + 3), c->d_red + 4);
   HC_HIP(c, hipGetLastError());
   int n = 0;
   HC_HIP(c, hipMemcpyAsync(c->h_pinned, c->d_red + 3, sizeof(int), hipMemcpyDeviceToHost,
@@ -1237,9 +1308,10 @@ int hc_in_channel_landsliding_impl(hc_ctx* c, double failureangle) {
   const double factor = std::tan((failureangle * (3.141592654 / 180))) * c->g.DX;
   const ErodeArgs a = erode_args(c);
   const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
+  if (int rc = slide_candidates(c, factor, 10000.0, std::max(c->g.imax - 1, 2))) return rc;
   hipLaunchKernelGGL(k_in_channel_landslide, dim3(1), dim3(1024), 0, c->stream, a, c->g.imax,
                      c->g.jmax, c->P, c->W, factor, c->mask, static_cast<double*>(c->z),
-                     c->bedrock, gr);
+                     c->bedrock, gr, c->d_red + 4);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

@@ -168,3 +168,55 @@ def test_in_channel_landsliding_vs_oracle(tmp_path, angle, nrows, ncols):
     assert np.array_equal(ctx.download("grain"), o.field("grain"))
     assert np.array_equal(ctx.download("strata").reshape(o.field("strata").shape), o.field("strata"))
     assert np.array_equal(ctx.download("hasgrain"), o.field_int("hasgrain"))
+
+
+@pytest.mark.parametrize("bump", [0.0, 3.0])
+def test_landsliding_gentle_dem(tmp_path, bump):
+    """Both landsliding sweeps on a gentle 1 m DEM (drops well under
+    tan(45) DX), where no cell passes the move test, so the device skips the
+    sweep (global_landsliding reports its one sweep, elevations stay as they
+    were), and on the same DEM with one cell raised 3 m, a single candidate
+    that starts the cascade: sweep counts, elevations and grain bitwise as
+    the oracle's."""
+    nrows, ncols = 36, 52
+    rng = np.random.Generator(np.random.PCG64(29))
+    dem = inputs.planar_dem(nrows, ncols, 1.0, 0.05, 0.1, 0.6, 1011, channel_depth=0.4,
+                            south_only=False)
+    dem[17, 23] += bump
+    props = inputs.dirichlet_props(nrows, ncols, rng)
+    water = np.where(rng.random((nrows, ncols)) < 0.4, 0.1, 0.0)
+    w = inputs.make_workload(str(tmp_path / "gd"), dem, 1.0, np.array([10.0, 0.0]), 15, 1,
+                             overrides={"hydro_model_only": "no", "active_layer_thickness": "0.3"},
+                             water_init=water, graindata=props, name="gd")
+    o = OracleModel.from_workload(w)
+    p = o.p
+    e0 = o.field("elev").copy()
+    ctx = hc.Context(p.imax, p.jmax, p.DX, hc_params_from(p), hc.HC_FP64, sediment=True)
+    for f in ("water_depth", "elev", "bedrock", "grain"):
+        ctx.upload(f, o.field(f))
+    ctx.upload("strata", o.field("strata"))
+    ctx.upload("hasgrain", o.field_int("hasgrain"))
+    sweeps = ctx.global_landsliding(45.0)
+    sweeps_o = o.call("global_landsliding", 45.0)
+    assert sweeps == sweeps_o
+    elev = ctx.download("elev")
+    assert np.array_equal(elev, o.field("elev"))
+    if bump == 0.0:
+        assert sweeps == 1 and np.array_equal(elev, e0)
+    else:
+        assert sweeps > 1
+    # the raised cell again, for in_channel_landsliding
+    e = o.field("elev")
+    e[18, 24] += bump
+    e1 = e.copy()
+    ctx.upload("elev", e)
+    o.call("scan_area")
+    ctx.scan_area()
+    assert np.array_equal(ctx.download("mask"), o.field_int("mask"))
+    ctx.in_channel_landsliding(45.0)
+    o.call("in_channel_landsliding", 45.0)
+    elev = ctx.download("elev")
+    assert np.array_equal(elev, o.field("elev"))
+    assert np.array_equal(ctx.download("grain"), o.field("grain"))
+    assert np.array_equal(ctx.download("strata").reshape(o.field("strata").shape), o.field("strata"))
+    assert np.array_equal(elev, e1) == (bump == 0.0)
