@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libhc.so")
-SOURCES = ["hc_capi.hip", "hc_hydro.hip", "hc_erode.hip", "hc_loop.hip", "hc_adjoint.hip"]
+SOURCES = ["hc_capi.hip", "hc_hydro.hip", "hc_erode.hip", "hc_loop.hip", "hc_adjoint.hip", "hc_state.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Strict IEEE FP64 (no contraction); packed FP32 instructions disabled.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",

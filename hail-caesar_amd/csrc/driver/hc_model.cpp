@@ -11,6 +11,7 @@
 #include <cctype>
 #include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -524,6 +525,66 @@ class CatchmentModel {
   int NR() const { return NR_; }
   int NC() const { return NC_; }
 
+  // ------------------------------------------------ checkpoint / restart
+  // An exact binary checkpoint: every scalar and array the run loop carries
+  // from one iteration to the next (clocks, TOPMODEL store, input points,
+  // drainage area, erosion and output accumulators, the series written so
+  // far), then the context's device state image (hc_state_save).  The
+  // reference restarts only from rasters at 6 significant figures and a
+  // Grainz file (LCM.cpp:331-373, 590-683) and does not restore qx, qy,
+  // vel_dir or its clocks.  A model created from the same inputs that
+  // restarts from the file continues bit for bit.
+  void checkpoint(const std::string& path) {
+    if (loop_started_)
+      throw Error("checkpoint: the device-resident loop holds the run's state on the device; "
+                  "checkpoint a host-stepped model");
+    size_t nimg = 0;
+    check(hc_state_bytes(ctx_, &nimg), "checkpoint");
+    std::vector<unsigned char> img(nimg);
+    check(hc_state_save(ctx_, img.data(), nimg), "checkpoint");
+    std::ofstream f(path, std::ios::binary | std::ios::trunc);
+    if (!f) throw Error("checkpoint: cannot open " + path);
+    f.write(kCkMagic, sizeof(kCkMagic));
+    CkWriter w{f};
+    visit_state(w);
+    const uint64_t n = nimg;
+    f.write(reinterpret_cast<const char*>(&n), sizeof(n));
+    f.write(reinterpret_cast<const char*>(img.data()), static_cast<std::streamsize>(nimg));
+    f.flush();
+    if (!f) throw Error("checkpoint: write to " + path + " failed");
+  }
+
+  void restart(const std::string& path) {
+    if (loop_started_)
+      throw Error("restart: the device-resident loop has started; restart a freshly created model");
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw Error("restart: cannot open " + path);
+    char magic[sizeof(kCkMagic)];
+    f.read(magic, sizeof(magic));
+    if (!f || std::memcmp(magic, kCkMagic, sizeof(magic)) != 0)
+      throw Error("restart: " + path + " is not a checkpoint of this version");
+    // the host state is kept aside: a file that fails its checks leaves the model as it was
+    CkReader r{f, path};
+    StateCopy saved = take_state();
+    try {
+      visit_state(r);
+      uint64_t n = 0;
+      r.raw(&n, sizeof(n));
+      size_t nimg = 0;
+      check(hc_state_bytes(ctx_, &nimg), "restart");
+      if (n != nimg) throw Error("restart: the device state in " + path + " does not fit this model");
+      std::vector<unsigned char> img(nimg);
+      r.raw(img.data(), nimg);
+      if (f.peek() != std::char_traits<char>::eof())
+        throw Error("restart: trailing bytes in " + path);
+      check(hc_state_load(ctx_, img.data(), nimg), "restart");
+    } catch (...) {
+      put_state(std::move(saved));
+      throw;
+    }
+    if (opt_.write_files) rewrite_series_file();
+  }
+
   bool scalar(const std::string& n, double* v) const {
     if (n == "time_step") *v = time_step_;
     else if (n == "cycle") *v = cycle_;
@@ -544,6 +605,140 @@ class CatchmentModel {
   }
 
  private:
+  static constexpr char kCkMagic[8] = {'H', 'C', 'C', 'K', 'P', 'T', '0', '1'};
+
+  struct CkWriter {
+    std::ofstream& f;
+    void raw(const void* p, size_t n) { f.write(static_cast<const char*>(p), static_cast<std::streamsize>(n)); }
+    void same(int v, const char*) { i(v); }
+    void i(int& v) { const int64_t x = v; raw(&x, sizeof(x)); }
+    void d(double& v) { raw(&v, sizeof(v)); }
+    template <class T>
+    void vec(std::vector<T>& v, bool) {
+      const uint64_t n = v.size();
+      raw(&n, sizeof(n));
+      if (n) raw(v.data(), n * sizeof(T));
+    }
+  };
+
+  struct CkReader {
+    std::ifstream& f;
+    const std::string& path;
+    void raw(void* p, size_t n) {
+      f.read(static_cast<char*>(p), static_cast<std::streamsize>(n));
+      if (!f) throw Error("restart: " + path + " is truncated");
+    }
+    int64_t get() {
+      int64_t x = 0;
+      raw(&x, sizeof(x));
+      return x;
+    }
+    void same(int v, const char* what) {
+      if (get() != v) throw Error(std::string("restart: the checkpoint's ") + what + " differs from this model's");
+    }
+    void i(int& v) {
+      const int64_t x = get();
+      if (x < INT_MIN || x > INT_MAX) throw Error("restart: corrupt checkpoint " + path);
+      v = static_cast<int>(x);
+    }
+    void d(double& v) { raw(&v, sizeof(v)); }
+    template <class T>
+    void vec(std::vector<T>& v, bool fixed) {
+      uint64_t n = 0;
+      raw(&n, sizeof(n));
+      if (fixed && n != v.size())
+        throw Error("restart: an array of the checkpoint does not match this model's grid");
+      if (n > (uint64_t{1} << 34) / sizeof(T)) throw Error("restart: corrupt checkpoint " + path);
+      v.resize(n);
+      if (n) raw(v.data(), n * sizeof(T));
+    }
+  };
+
+  // Everything the run loop carries between iterations, in file order.  The
+  // inputs (parameters, rain series, grid) are the model's own; the first
+  // entries check that they match the run that wrote the file.
+  template <class V>
+  void visit_state(V& v) {
+    v.same(imax_, "row count");
+    v.same(jmax_, "column count");
+    v.same(rfnum_, "rain cell count");
+    v.same(hydro_only_ ? 1 : 0, "erosion mode");
+    v.same(opt_.precision, "precision");
+    v.i(totalinputpoints_);
+    v.i(counter_);
+    v.i(hours_);
+    double* ds[] = {&time_step_, &cycle_, &maxdepth_, &temptot_, &waterinput_, &waterOut_, &iod_,
+                    &time_1_, &save_time_, &creep_time_, &creep_time2_, &previous_, &new_cycle_,
+                    &old_cycle_, &erode_mult_, &erode_call_, &globalsediq_, &sediQ_, &last_dt_,
+                    &baseflow_, &tx_, &tlastcalc_, &Qs_step_, &Qs_hour_, &Qs_over_, &Qs_last_,
+                    &Qw_newvol_, &Qw_oldvol_, &Qw_lastvol_, &Qw_stepvol_, &Qw_hourvol_, &Qw_hour_,
+                    &Qw_overvol_, &temptotal_, &old_sediq_, &Jw_newvol_, &Jw_oldvol_, &Jw_lastvol_,
+                    &Jw_stepvol_, &Jw_hourvol_, &Jw_hour_, &Jw_overvol_, &sand_out_};
+    for (double* d : ds) v.d(*d);
+    v.vec(elev_, true);
+    v.vec(area_, true);
+    v.vec(area_depth_, true);
+    v.vec(rfarea_, true);
+    v.vec(j_, false);
+    v.vec(jo_, false);
+    v.vec(j_mean_, false);
+    v.vec(old_j_mean_, false);
+    v.vec(new_j_mean_, false);
+    v.vec(nActual_, false);
+    v.vec(input_counter_, false);
+    v.vec(in_x_, false);
+    v.vec(in_y_, false);
+    v.vec(pts_x_, false);
+    v.vec(pts_y_, false);
+    v.vec(pts_a_, false);
+    v.vec(sum_grain_, false);
+    v.vec(old_sum_grain_, false);
+    v.vec(Qg_step_, false);
+    v.vec(Qg_hour_, false);
+    v.vec(Qg_over_, false);
+    v.vec(Qg_last_, false);
+    v.vec(series_, false);
+  }
+
+  // the host state visit_state covers, kept aside while a restart reads
+  struct StateCopy {
+    std::vector<unsigned char> bytes;
+  };
+  struct CkMem {
+    std::vector<unsigned char>& b;
+    size_t pos = 0;
+    bool out = true;
+    void raw(void* p, size_t n) {
+      if (out) {
+        const unsigned char* q = static_cast<const unsigned char*>(p);
+        b.insert(b.end(), q, q + n);
+      } else {
+        std::memcpy(p, b.data() + pos, n);
+        pos += n;
+      }
+    }
+    void same(int, const char*) {}
+    void i(int& v) { raw(&v, sizeof(v)); }
+    void d(double& v) { raw(&v, sizeof(v)); }
+    template <class T>
+    void vec(std::vector<T>& v, bool) {
+      uint64_t n = v.size();
+      raw(&n, sizeof(n));
+      if (!out) v.resize(n);
+      if (n) raw(v.data(), n * sizeof(T));
+    }
+  };
+  StateCopy take_state() {
+    StateCopy s;
+    CkMem m{s.bytes};
+    visit_state(m);
+    return s;
+  }
+  void put_state(StateCopy s) {
+    CkMem m{s.bytes, 0, false};
+    visit_state(m);
+  }
+
   size_t at(int x, int y) const { return static_cast<size_t>(x) * NC_ + y; }
   void check(int rc, const char* what) {
     if (rc != HC_OK) throw Error(std::string(what) + ": " + hc_last_error(ctx_));
@@ -897,17 +1092,27 @@ class CatchmentModel {
       tlastcalc_ = cycle_;
     }
   }
-  void append_series_row() {  // column formats of LCM.cpp:2427-2474
+  // one row of the time series in the column formats of LCM.cpp:2427-2474
+  static std::string series_line(const double* r) {
     std::ostringstream os;
-    const double* r = &series_[series_.size() - 14];
-    os << std::fixed << std::setprecision(0) << hours_;
+    os << std::fixed << std::setprecision(0) << r[0];
     os << " " << std::fixed << std::setprecision(6) << r[1];
     os << " " << std::fixed << std::setprecision(6) << r[2];
     os << " " << std::fixed << std::setprecision(6) << r[3];
     os << " " << std::fixed << std::setprecision(10) << r[4];
     for (int n = 0; n < 9; ++n) os << " " << std::fixed << std::setprecision(10) << r[5 + n];
+    return os.str();
+  }
+  void append_series_row() {
     std::ofstream f(write_path_ + "/" + write_fname_, std::ios_base::app | std::ios_base::out);
-    f << os.str() << std::endl;
+    f << series_line(&series_[series_.size() - 14]) << std::endl;
+  }
+  // after a restart: the file holds exactly the rows of the checkpoint (a
+  // killed run may have appended rows after its last checkpoint)
+  void rewrite_series_file() {
+    std::ofstream f(write_path_ + "/" + write_fname_, std::ios_base::trunc | std::ios_base::out);
+    if (!f) throw Error("restart: unable to write to " + write_path_ + "/" + write_fname_);
+    for (size_t k = 0; k + 14 <= series_.size(); k += 14) f << series_line(&series_[k]) << std::endl;
   }
 
   // ASCII raster without the padding ring, 6 significant digits
@@ -1152,6 +1357,28 @@ int hcm_set_graph(hcm_model* h, int use_graph) {
 }
 
 hc_ctx* hcm_context(hcm_model* h) { return (h && h->m) ? h->m->ctx() : nullptr; }
+
+int hcm_checkpoint(hcm_model* h, const char* path) {
+  if (!h || !h->m || !path) return HC_ERR_ARG;
+  try {
+    h->m->checkpoint(path);
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return HC_ERR_STATE;
+  }
+  return HC_OK;
+}
+
+int hcm_restart(hcm_model* h, const char* path) {
+  if (!h || !h->m || !path) return HC_ERR_ARG;
+  try {
+    h->m->restart(path);
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return HC_ERR_STATE;
+  }
+  return HC_OK;
+}
 
 int hcm_series(hcm_model* h, double* out, int max_rows) {
   if (!h || !h->m) return 0;

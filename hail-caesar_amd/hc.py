@@ -34,7 +34,8 @@ SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upl
            "hc_download", "hc_flow_route", "hc_depth_update", "hc_flow_depth",
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
            "hc_global_landsliding", "hc_creep", "hc_in_channel_landsliding",
-           "hc_synchronize", "hc_check_health", "hc_loop_begin", "hc_loop_run",
+           "hc_synchronize", "hc_check_health", "hc_state_bytes", "hc_state_save",
+           "hc_state_load", "hc_loop_begin", "hc_loop_run",
            "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
            "hc_set_tiles", "hc_bench_fused", "hc_ens_create", "hc_ens_destroy",
            "hc_ens_last_error", "hc_ens_run", "hc_ens_sync", "hc_ens_tune", "hc_ens_set_tiles",
@@ -43,7 +44,7 @@ SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upl
 MODEL_SYMBOLS = ["hcm_create", "hcm_create_arrays", "hcm_destroy", "hcm_last_error",
                  "hcm_iterate", "hcm_download", "hcm_field_count", "hcm_scalar",
                  "hcm_series", "hcm_set_graph", "hcm_context", "hcm_loop_begin",
-                 "hcm_loop_collect"]
+                 "hcm_loop_collect", "hcm_checkpoint", "hcm_restart"]
 RF_MAX = 17
 
 
@@ -141,6 +142,9 @@ def lib():
         L.hc_in_channel_landsliding.argtypes = [vp, d]
         L.hc_synchronize.argtypes = [vp]
         L.hc_check_health.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+        L.hc_state_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+        L.hc_state_save.argtypes = [vp, vp, ctypes.c_size_t]
+        L.hc_state_load.argtypes = [vp, vp, ctypes.c_size_t]
         L.hc_loop_begin.argtypes = [vp, ctypes.POINTER(hc_loop_init)]
         L.hc_loop_run.argtypes = [vp, i, i]
         L.hc_loop_sync.argtypes = [vp, ctypes.POINTER(hc_loop_state), vp, i, ctypes.POINTER(i)]
@@ -329,6 +333,21 @@ class Context:
         if rc not in (HC_OK, HC_ERR_UNSTABLE):
             self._check(rc)
         return n.value
+
+    # ---- exact state image
+    def state_save(self) -> bytes:
+        """The context's device state image (hc_state_save)."""
+        n = ctypes.c_size_t()
+        self._check(self.L.hc_state_bytes(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(self.L.hc_state_save(self.h, buf, n.value))
+        return buf.raw
+
+    def state_load(self, image: bytes) -> None:
+        """Load an image written by state_save on a context of the same
+        grid, precision and configuration (hc_state_load)."""
+        buf = ctypes.create_string_buffer(bytes(image), len(image))
+        self._check(self.L.hc_state_load(self.h, buf, len(image)))
 
     # ---- device-resident stepping
     def loop_begin(self, init: hc_loop_init) -> None:

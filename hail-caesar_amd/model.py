@@ -64,6 +64,8 @@ def _lib():
         L.hcm_context.restype = vp
         L.hcm_loop_begin.argtypes = [vp]
         L.hcm_loop_collect.argtypes = [vp, i, vp, vp, vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.hcm_checkpoint.argtypes = [vp, ctypes.c_char_p]
+        L.hcm_restart.argtypes = [vp, ctypes.c_char_p]
         L._hcm_ready = True
     return L
 
@@ -170,6 +172,19 @@ class Model:
         if rows:
             self.L.hcm_series(self.h, out.ctypes.data, rows)
         return out
+
+    def checkpoint(self, path: str) -> None:
+        """Exact binary checkpoint of a host-stepped run (hcm_checkpoint)."""
+        rc = self.L.hcm_checkpoint(self.h, os.fsencode(path))
+        if rc != 0:
+            raise hc.HCError(rc, self.L.hcm_last_error(self.h).decode())
+
+    def restart(self, path: str) -> None:
+        """Continue from a checkpoint written by a model of the same inputs
+        (hcm_restart); a failed restart leaves the model as it was."""
+        rc = self.L.hcm_restart(self.h, os.fsencode(path))
+        if rc != 0:
+            raise hc.HCError(rc, self.L.hcm_last_error(self.h).decode())
 
     def set_graph(self, on: bool) -> None:
         self.L.hcm_set_graph(self.h, 1 if on else 0)

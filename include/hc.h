@@ -214,6 +214,23 @@ int hc_synchronize(hc_ctx* ctx);
  * cells.  Returns HC_ERR_UNSTABLE when any are found (count in *bad). */
 int hc_check_health(hc_ctx* ctx, long long* bad);
 
+/* Exact binary state image of a host-stepped context.  Replaces the
+ * reference's lossy hot start: water depth re-read from a raster written at
+ * 6 significant figures (water_init_from_raster_on, LCM.cpp:331-373,
+ * 1258-1262; save_raster_data, LR.cpp:1216) and grain state from a Grainz
+ * file (read_in_graindata_from_file, LCM.cpp:590-683), with qx, qy, vel_dir
+ * and the clocks not restored.  The image holds every persistent device
+ * buffer in its stored layout and the values that decide how the next step
+ * computes, so a context of the same grid, precision and configuration that
+ * loads it continues bit for bit.  hc_state_bytes gives the image size;
+ * save and load take exactly that many bytes.  Both return HC_ERR_STATE on
+ * a context whose device-resident loop (hc_loop_begin) has started, and
+ * load returns HC_ERR_STATE for an image of another grid or configuration
+ * and HC_ERR_ARG for a foreign or truncated one. */
+int hc_state_bytes(hc_ctx* ctx, size_t* bytes);
+int hc_state_save(hc_ctx* ctx, void* host, size_t bytes);
+int hc_state_load(hc_ctx* ctx, const void* host, size_t bytes);
+
 /* ---- device-resident stepping ----------------------------------------- */
 
 /* Rain cells 1..HC_RF_MAX-1 (index 0 unused, as in the reference's vectors). */

@@ -79,6 +79,21 @@ int hcm_set_graph(hcm_model* m, int use_graph);
 /* The device context the model drives (for hc.h calls such as profiling). */
 struct hc_ctx* hcm_context(hcm_model* m);
 
+/* Exact binary checkpoint of a host-stepped run, written to `path`: every
+ * scalar and array the loop carries between iterations (clocks, TOPMODEL
+ * store, input points, drainage area, erosion and output accumulators, the
+ * series so far) and the device state image of hc_state_save.  Replaces the
+ * reference's lossy hot start from rasters and a Grainz file
+ * (water_init_from_raster_on, LCM.cpp:331-373; read_in_graindata_from_file,
+ * LCM.cpp:590-683), which restores neither qx, qy, vel_dir nor the clocks.
+ * hcm_restart loads it into a model created from the same inputs, which
+ * then continues bit for bit as the run that wrote it.  Both return
+ * HC_ERR_STATE with a message (hcm_last_error) for a device-resident loop
+ * that has started, an unreadable, foreign or truncated file, or one written
+ * by a model of another grid, rain cell count, mode or precision; a restart
+ * that fails leaves the model as it was. */
+int hcm_checkpoint(hcm_model* m, const char* path);
+int hcm_restart(hcm_model* m, const char* path);
 /* Rows written by output_data so far (14 columns: hours, Qw, Jw, sand_out,
  * Qs, Qg[1..9]); copies at most max_rows; returns the row count. */
 int hcm_series(hcm_model* m, double* out, int max_rows);
