@@ -274,8 +274,23 @@ class CatchmentModel {
     creep_time2_ = cycle_;
     time_1_ = cycle_;
   }
-  void initialise_rainfall_runoff() { topmodel_runoff(1.0); }  // LCM.cpp:2041-2053
-  void initialise_drainage_area() {                           // LCM.cpp:2055-2069
+  void initialise_rainfall_runoff() {  // LCM.cpp:2041-2053
+    if (complex_rain_) {
+      // the runoffGrid (LSDRainfallRunoff.cpp:96-104): j = jo = 1e-9, the means 0
+      const size_t n = static_cast<size_t>(NR_) * NC_;
+      rj_.assign(n, 0.000000001);
+      rjo_.assign(n, 0.000000001);
+      rj_mean_.assign(n, 0.0);
+      rold_.assign(n, 0.0);
+      rnew_.assign(n, 0.0);
+      topmodel_runoff_grid(1.0);
+      upload_runoff();
+      return;
+    }
+    topmodel_runoff(1.0);
+  }
+  void initialise_drainage_area() {  // LCM.cpp:2055-2069
+    if (complex_rain_) return;       // every cell takes its own runoff: no input points
     zero_and_calc_drainage_area();
     get_catchment_input_points();
   }
@@ -308,7 +323,8 @@ class CatchmentModel {
   void reach_water_and_sediment_input() {}  // reach mode is rejected at parse time
   void catchment_waterinputs() {             // LCM.cpp:2027-2039
     waterinput_ = 0;
-    catchment_water_input_and_hydrology(get_flow_timestep());
+    if (complex_rain_) catchment_water_input_grid(get_flow_timestep());
+    else catchment_water_input_and_hydrology(get_flow_timestep());
   }
   void flow_route() {  // LCM.cpp:3063-3247
     last_dt_ = get_flow_timestep();
@@ -406,7 +422,7 @@ class CatchmentModel {
     return !(get_cycle() / 60 < get_maxcycle());
   }
 
-  bool device_loop_enabled() const { return opt_.device_loop && hydro_only_; }
+  bool device_loop_enabled() const { return opt_.device_loop && hydro_only_ && !complex_rain_; }
   void set_graph(bool g) { opt_.use_graph = g ? 1 : 0; }
 
   // Device-resident stepping (hc_loop_*): n loop bodies with the scalar
@@ -582,6 +598,7 @@ class CatchmentModel {
       put_state(std::move(saved));
       throw;
     }
+    if (complex_rain_) upload_runoff();  // the device planes of the last TOPMODEL update
     if (opt_.write_files) rewrite_series_file();
   }
 
@@ -664,6 +681,7 @@ class CatchmentModel {
     v.same(rfnum_, "rain cell count");
     v.same(hydro_only_ ? 1 : 0, "erosion mode");
     v.same(opt_.precision, "precision");
+    v.same(complex_rain_ ? 1 : 0, "rainfall mode");
     v.i(totalinputpoints_);
     v.i(counter_);
     v.i(hours_);
@@ -698,6 +716,13 @@ class CatchmentModel {
     v.vec(Qg_over_, false);
     v.vec(Qg_last_, false);
     v.vec(series_, false);
+    v.d(runoff_t_);
+    v.i(runoff_jm_set_);
+    v.vec(rj_, false);
+    v.vec(rjo_, false);
+    v.vec(rj_mean_, false);
+    v.vec(rold_, false);
+    v.vec(rnew_, false);
   }
 
   // the host state visit_state covers, kept aside while a restart reads
@@ -823,7 +848,8 @@ class CatchmentModel {
         int idx = 1;
         while (std::getline(ss, tok, ',') && idx <= 9) d_[idx++] = std::atof(tok.c_str());
       }
-      else if ((k == "spatially_complex_rainfall_on" || k == "vegetation_on" || k == "groundwater_on" ||
+      else if (k == "spatially_complex_rainfall_on") complex_rain_ = yes();
+      else if ((k == "vegetation_on" || k == "groundwater_on" ||
                 k == "reach_mode" || k == "lateral_erosion_on") && yes())
         throw Error("option '" + k + "' is not provided by this build (hot path only)");
     }
@@ -978,6 +1004,95 @@ class CatchmentModel {
       get_catchment_input_points();
     }
   }
+  // ------------------------------------- spatially complex rainfall ----
+  // topmodel_runoff(cycle, runoff) (LCM.cpp:3610-3627): the rain grid of
+  // rainGrid::create (LSDRainfallRunoff.cpp:36-63) -- one rain cell: its
+  // value everywhere; otherwise cells 1..imax-1, 1..jmax-1 whose hydroindex
+  // is 1..rfnum-1 take that column (the last index takes none), all others 0
+  // -- held in double, then runoffGrid::calculate_runoff (LSDRainfallRunoff.
+  // cpp:175-241) on cells 1..imax-1, 1..jmax-1 above -9999, rain_factor
+  // passed as an int, a 60 s step.  Erosion never takes a catchment cell to
+  // -9999, so the DEM as loaded gives the same cell set.
+  double rain_grid(int ts, size_t c) const {
+    if (rfnum_ == 1) return static_cast<double>(rain_at(ts, 0));
+    const int rf = rfarea_[c];
+    if (rf >= 1 && rf < rfnum_) return static_cast<double>(rain_at(ts, rf - 1));
+    return 0.0;
+  }
+  void topmodel_runoff_grid(double cyc) {
+    const int ts = static_cast<int>(cyc / rain_data_time_step_);
+    const int rfi = static_cast<int>(rain_factor_);
+    for (int m = 1; m < imax_; ++m)
+      for (int n = 1; n < jmax_; ++n) {
+        const size_t c = at(m, n);
+        if (!(elev_[c] > -9999)) continue;
+        double local = 0;
+        const double lts = 60;
+        rold_[c] = rnew_[c];
+        rjo_[c] = rj_[c];
+        const double r = rain_grid(ts, c);
+        if (r > 0) local = rfi * ((r / 1000) / 3600);
+        if (local == 0) {
+          rj_[c] = rjo_[c] / (1 + ((rjo_[c] * lts) / M_));
+          rnew_[c] = M_ / lts * std::log(1 + ((rjo_[c] * lts) / M_));
+        }
+        if (local > 0) {
+          rj_[c] = local / (((local - rjo_[c]) / rjo_[c]) * std::exp((0 - local) * lts / M_) + 1);
+          rnew_[c] = (M_ / lts) * std::log(((local - rjo_[c]) + rjo_[c] * std::exp((local * lts) / M_)) / local);
+        }
+        if (rnew_[c] < 0) rnew_[c] = 0;
+      }
+  }
+  void upload_runoff() { check(hc_runoff_set(ctx_, rold_.data(), rnew_.data()), "runoff_set"); }
+  // catchment_water_input_and_hydrology(flow_timestep, runoff)
+  // (LCM.cpp:3544-3606).  The water goes in on the device (hc_runoff_add)
+  // from the planes of the last TOPMODEL update and the previous call's
+  // hydrograph time, which is the j_mean this call's sums read; before the
+  // first hydrograph the grid's j_mean is 0 and nothing goes in.
+  void catchment_water_input_grid(double fts) {
+    if (runoff_jm_set_) check(hc_runoff_add(ctx_, runoff_t_, fts, ERODEFACTOR_), "runoff_add");
+    for (int i = 1; i < imax_; ++i)
+      for (int j = 1; j < jmax_; ++j) waterinput_ += rj_mean_[at(i, j)] * DX_ * DX_;
+    for (int i = 1; i < imax_; ++i)
+      for (int j = 1; j < jmax_; ++j) {
+        double amt = rj_mean_[at(i, j)] * fts;
+        if (amt > ERODEFACTOR_) amt = ERODEFACTOR_;
+        waterinput_ += (amt / fts) * DX_ * DX_;
+      }
+    double njmax = 0;
+    for (int m = 1; m <= imax_; ++m)
+      for (int n = 1; n <= jmax_; ++n)
+        if (rnew_[at(m, n)] > njmax) njmax = rnew_[at(m, n)];
+    bool updated = false;
+    if (cycle_ >= time_1_) {
+      const unsigned cells = static_cast<unsigned>(jmax_) * static_cast<unsigned>(imax_);
+      do {
+        time_1_++;
+        topmodel_runoff_grid(time_1_);
+        updated = true;
+        if (time_step_ > max_time_step_ && njmax > (0.2 / (cells * DX_ * DX_))) {
+          cycle_ = time_1_ + (max_time_step_ / 60);
+          time_step_ = max_time_step_;
+        }
+      } while (time_1_ < cycle_);
+    }
+    runoff_t_ = time_1_ - cycle_;  // calchydrograph (LCM.cpp:3728-3739)
+    for (int m = 1; m <= imax_; ++m)
+      for (int n = 1; n <= jmax_; ++n) {
+        const size_t c = at(m, n);
+        rj_mean_[c] = rold_[c] + (((rnew_[c] - rold_[c]) / 2) * (2 - runoff_t_));
+      }
+    runoff_jm_set_ = 1;
+    if (updated) upload_runoff();
+  }
+  // output_data(temptotal, runoff)'s runoff volume (LCM.cpp:2487-2501,
+  // 2571-2583): each catchment cell's term added in turn
+  void add_grid_runoff_volume(double& vol, double minutes) const {
+    for (int i = 1; i <= imax_; ++i)
+      for (int j = 1; j <= jmax_; ++j)
+        if (elev_[at(i, j)] > no_data_value_) vol += (rj_mean_[at(i, j)] * DX_ * DX_) * (minutes * 60);
+  }
+
   void get_catchment_input_points() {  // LCM.cpp:3741-3767
     totalinputpoints_ = 0;
     for (int n = 1; n <= rfnum_; ++n) input_counter_[n] = 0;
@@ -1046,8 +1161,12 @@ class CatchmentModel {
   // --------------------------------------------------- time series output
   void output_data() {  // LCM.cpp:2310-2478
     Qw_newvol_ += temptotal_ * ((cycle_ - previous_) * 60);
-    for (int nn = 1; nn <= rfnum_; ++nn)
-      Jw_newvol_ += (j_mean_[nn] * DX_ * DX_ * nActual_[nn]) * ((cycle_ - previous_) * 60);
+    if (complex_rain_) {
+      add_grid_runoff_volume(Jw_newvol_, cycle_ - previous_);
+    } else {
+      for (int nn = 1; nn <= rfnum_; ++nn)
+        Jw_newvol_ += (j_mean_[nn] * DX_ * DX_ * nActual_[nn]) * ((cycle_ - previous_) * 60);
+    }
     const double osi = output_file_save_interval_;
     if ((new_cycle_ < old_cycle_) || (cycle_ - previous_ >= osi)) {
       while ((tx_ > previous_) && (cycle_ >= tx_)) {
@@ -1068,8 +1187,12 @@ class CatchmentModel {
         Qw_stepvol_ = Qw_newvol_ - Qw_oldvol_;
         Qw_hourvol_ = Qw_stepvol_ - Qw_overvol_ + Qw_lastvol_;
         Qw_hour_ = Qw_hourvol_ / (60 * osi);
-        for (int nn = 1; nn <= rfnum_; ++nn)
-          Jw_overvol_ += (j_mean_[nn] * DX_ * DX_ * nActual_[nn]) * ((cycle_ - tx_) * 60);
+        if (complex_rain_) {
+          add_grid_runoff_volume(Jw_overvol_, cycle_ - tx_);
+        } else {
+          for (int nn = 1; nn <= rfnum_; ++nn)
+            Jw_overvol_ += (j_mean_[nn] * DX_ * DX_ * nActual_[nn]) * ((cycle_ - tx_) * 60);
+        }
         Jw_stepvol_ = Jw_newvol_ - Jw_oldvol_;
         Jw_hourvol_ = Jw_stepvol_ - Jw_overvol_ + Jw_lastvol_;
         Jw_hour_ = Jw_hourvol_ / (60 * osi);
@@ -1194,6 +1317,12 @@ class CatchmentModel {
   std::vector<double> series_;
   std::vector<double> topmodel_table_;
   bool loop_started_ = false;
+  // spatially complex rainfall: the runoffGrid (j, jo, j_mean, old, new per
+  // cell), the last hydrograph time, and whether j_mean has been formed yet
+  bool complex_rain_ = false;
+  std::vector<double> rj_, rjo_, rj_mean_, rold_, rnew_;
+  double runoff_t_ = 0;
+  int runoff_jm_set_ = 0;
 };
 
 }  // namespace hcdrv

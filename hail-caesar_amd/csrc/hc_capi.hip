@@ -157,6 +157,27 @@ int read_scalar(hc_ctx* c, const void* dev, double* out, bool as_bits) {
   return HC_OK;
 }
 
+// Spatially complex rainfall's water input (LCM.cpp:3559-3573): every cell
+// 1..imax-1, 1..jmax-1 takes its runoff over the step, capped at the
+// erosion limit.  The runoff is the TOPMODEL interpolation of calchydrograph
+// (LCM.cpp:3728-3739) formed per cell from the planes of the last TOPMODEL
+// update, in the reference's operation order (no contraction here, so it is
+// the value the host forms for its sums).
+template <class T>
+__global__ __launch_bounds__(256) void k_runoff_add(int imax, int jmax, int P, double t, double fts,
+                                                     double cap, const double* __restrict__ jold,
+                                                     const double* __restrict__ jnew,
+                                                     T* __restrict__ h) {
+  const int y = 1 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int x = 1 + static_cast<int>(blockIdx.y);
+  if (y >= jmax || x >= imax) return;
+  const size_t c = static_cast<size_t>(x) * P + y;
+  const double jm = jold[c] + (((jnew[c] - jold[c]) / 2) * (2 - t));
+  double amt = jm * fts;
+  if (amt > cap) amt = cap;
+  h[c] = static_cast<T>(static_cast<double>(h[c]) + amt);
+}
+
 }  // namespace
 
 extern "C" {
@@ -254,7 +275,7 @@ int hc_destroy(hc_ctx* c) {
                   c->hasgrain, c->needgrain, c->bedrock, c->Tau, c->Vel, c->ss, c->sdir,
                   c->etot, c->etot3, c->gtot, c->d_red, c->d_partials, c->d_count, c->d_out,
                   c->d_pts, c->d_amt, c->d_ctl, c->d_rec, c->d_top, c->d_area, c->d_rfarea,
-                  c->d_pmask, c->d_dpart, c->d_slide_moves};
+                  c->d_pmask, c->d_dpart, c->d_slide_moves, c->d_runoff};
   for (auto& gexec : c->loop_graph)
     if (gexec) (void)hipGraphExecDestroy(gexec);
   for (auto e : c->prof_ev) (void)hipEventDestroy(e);
@@ -477,6 +498,36 @@ int hc_add_point_inputs(hc_ctx* c, const int* x, const int* y, const double* amt
   if (rc) return rc;
   // the host vectors die at return: the copies above are from pageable
   // memory and complete before hipMemcpyAsync returns
+  return HC_OK;
+}
+
+int hc_runoff_set(hc_ctx* c, const double* old_j_mean, const double* new_j_mean) {
+  if (!c || !old_j_mean || !new_j_mean) return HC_ERR_ARG;
+  if (!c->d_runoff) {
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_runoff), 2 * c->ncell * sizeof(double)));
+    HC_HIP(c, hipMemsetAsync(c->d_runoff, 0, 2 * c->ncell * sizeof(double), c->stream));
+  }
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  int rc = put_plane(c, c->d_runoff, false, old_j_mean, 1, 0);
+  if (rc == HC_OK) rc = put_plane(c, c->d_runoff + c->ncell, false, new_j_mean, 1, 0);
+  return rc;
+}
+
+int hc_runoff_add(hc_ctx* c, double time, double flow_timestep, double cap) {
+  if (!c) return HC_ERR_ARG;
+  if (!c->d_runoff) return hc_fail(c, HC_ERR_STATE, "runoff_add: no runoff planes (hc_runoff_set)");
+  if (c->g.imax < 2 || c->g.jmax < 2) return HC_OK;  // no cell 1..imax-1, 1..jmax-1
+  const dim3 blk(256), grd(static_cast<unsigned>((c->g.jmax - 1 + 255) / 256),
+                          static_cast<unsigned>(c->g.imax - 1));
+  if (c->prec == HC_FP64)
+    hipLaunchKernelGGL(k_runoff_add<double>, grd, blk, 0, c->stream, c->g.imax, c->g.jmax, c->P,
+                       time, flow_timestep, cap, c->d_runoff, c->d_runoff + c->ncell,
+                       static_cast<double*>(c->h[c->cur]));
+  else
+    hipLaunchKernelGGL(k_runoff_add<float>, grd, blk, 0, c->stream, c->g.imax, c->g.jmax, c->P,
+                       time, flow_timestep, cap, c->d_runoff, c->d_runoff + c->ncell,
+                       static_cast<float*>(c->h[c->cur]));
+  HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
 

@@ -215,6 +215,7 @@ struct hc_ctx {
                                         // [3] global_landsliding sweep count,
                                         // [4] landsliding: some cell can move
   double* d_slide_moves = nullptr;      // hillslope scratch: 8 planes (landslide moves, tempcreep)
+  double* d_runoff = nullptr;           // spatially complex rainfall: old, new j_mean planes
   double* d_partials = nullptr;         // per-block partial sums
   unsigned int* d_count = nullptr;      // last-block counters
   double* d_dpart = nullptr;            // fused-kernel edge drain: one slot per tile

@@ -34,7 +34,7 @@ SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upl
            "hc_download", "hc_flow_route", "hc_depth_update", "hc_flow_depth",
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
            "hc_global_landsliding", "hc_creep", "hc_in_channel_landsliding",
-           "hc_synchronize", "hc_check_health", "hc_state_bytes", "hc_state_save",
+           "hc_runoff_set", "hc_runoff_add", "hc_synchronize", "hc_check_health", "hc_state_bytes", "hc_state_save",
            "hc_state_load", "hc_loop_begin", "hc_loop_run",
            "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
            "hc_set_tiles", "hc_bench_fused", "hc_ens_create", "hc_ens_destroy",
@@ -143,6 +143,8 @@ def lib():
         L.hc_synchronize.argtypes = [vp]
         L.hc_check_health.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         L.hc_state_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+        L.hc_runoff_set.argtypes = [vp, vp, vp]
+        L.hc_runoff_add.argtypes = [vp, d, d, d]
         L.hc_state_save.argtypes = [vp, vp, ctypes.c_size_t]
         L.hc_state_load.argtypes = [vp, vp, ctypes.c_size_t]
         L.hc_loop_begin.argtypes = [vp, ctypes.POINTER(hc_loop_init)]
@@ -333,6 +335,20 @@ class Context:
         if rc not in (HC_OK, HC_ERR_UNSTABLE):
             self._check(rc)
         return n.value
+
+    # ---- spatially complex rainfall
+    def runoff_set(self, old_j_mean: np.ndarray, new_j_mean: np.ndarray) -> None:
+        """Upload the runoff grid's planes (hc_runoff_set)."""
+        shape = (self.imax + 2, self.jmax + 2)
+        a = np.ascontiguousarray(old_j_mean, dtype=np.float64)
+        b = np.ascontiguousarray(new_j_mean, dtype=np.float64)
+        if a.shape != shape or b.shape != shape:
+            raise ValueError("runoff planes must be (imax+2, jmax+2)")
+        self._check(self.L.hc_runoff_set(self.h, a.ctypes.data, b.ctypes.data))
+
+    def runoff_add(self, time: float, flow_timestep: float, cap: float) -> None:
+        """Add every cell's runoff over the step (hc_runoff_add)."""
+        self._check(self.L.hc_runoff_add(self.h, time, flow_timestep, cap))
 
     # ---- exact state image
     def state_save(self) -> bytes:

@@ -429,6 +429,24 @@ def erosion_1024(path: str, size: int = 1024, hours: int = 24, seed: int = 1002)
     return w
 
 
+def complex_rain(path: str, rfnum: int = 1, size: int = 40, seed: int = 1021,
+                 hours: int = 1) -> Workload:
+    """spatially_complex_rainfall_on (a per-cell TOPMODEL): size^2 at 5 m,
+    a 2 h triangular storm peaking at 30 mm/h in 10 min steps, 1 cm of water
+    to start; with rfnum > 1 a hydroindex of rfnum bands of rows, band k's
+    rain (1 + k/2) times the first's."""
+    dem = planar_dem(size, size, 5.0, 0.01, 5.0, 0.6, seed)
+    rain = triangular_storm(2, 30.0, 10)
+    hydro = None
+    if rfnum > 1:
+        hydro = np.where(dem > NODATA, 1 + (np.arange(size)[:, None] * rfnum) // size, -9999)
+        hydro = np.broadcast_to(hydro, dem.shape).astype(np.int32)
+        rain = np.stack([rain * (1.0 + 0.5 * k) for k in range(rfnum)], axis=1)
+    return make_workload(path, dem, 5.0, rain, 10, hours, hydroindex=hydro,
+                         water_init=np.where(dem > NODATA, 0.01, 0.0),
+                         overrides={"spatially_complex_rainfall_on": "yes"}, name="cplx")
+
+
 def gamma_rain(hours: float, step_min: float, shape: float, mean: float,
                rng: np.random.Generator, ncells: int = 1) -> np.ndarray:
     n = int(round(hours * 60 / step_min))

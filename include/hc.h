@@ -207,6 +207,19 @@ int hc_creep(hc_ctx* ctx, double time, double creep_rate);
  * sediment contexts. */
 int hc_in_channel_landsliding(hc_ctx* ctx, double failureangle);
 
+/* Spatially complex rainfall (spatially_complex_rainfall_on).  The
+ * reference keeps a per-cell TOPMODEL (runoffGrid, LSDRainfallRunoff.cpp:
+ * 175-241) and each step adds every cell's runoff to its water depth
+ * (catchment_water_input_and_hydrology with the runoff grid, LCM.cpp:
+ * 3559-3573).  hc_runoff_set uploads the grid's old and new j_mean planes
+ * (reference layout, (imax+2) x (jmax+2)) after a TOPMODEL update;
+ * hc_runoff_add adds to every cell 1..imax-1, 1..jmax-1 the runoff
+ * old + ((new - old)/2)(2 - time) (calchydrograph, LCM.cpp:3728-3739) times
+ * flow_timestep, capped at `cap` (ERODEFACTOR), formed in the reference's
+ * order.  hc_runoff_add before any hc_runoff_set returns HC_ERR_STATE. */
+int hc_runoff_set(hc_ctx* ctx, const double* old_j_mean, const double* new_j_mean);
+int hc_runoff_add(hc_ctx* ctx, double time, double flow_timestep, double cap);
+
 /* Block until every call enqueued on the context has finished. */
 int hc_synchronize(hc_ctx* ctx);
 

@@ -67,6 +67,9 @@ struct orc_model {
   std::vector<int> nActual, input_counter;
   std::vector<int> in_x, in_y;
   int totalinputpoints = 0;
+  // spatially complex rainfall: the runoffGrid's per-cell arrays
+  // (LSDRainfallRunoff.cpp:96-104: j = jo = 1e-9, the means 0)
+  std::vector<double> rj, rjo, rj_mean, rold, rnew;
   // scalars (LCM.hpp:456-548 defaults)
   double time_step = 1, cycle = 0, maxdepth = 10, temptot = 0, waterinput = 0,
          waterOut = 0, iod = 0, baseflow = 0.00000005, time_1 = 1,
@@ -283,6 +286,93 @@ struct orc_model {
       j_mean[n] = old_j_mean[n] + (((new_j_mean[n] - old_j_mean[n]) / 2) * (2 - t));
   }
 
+  // ------------------------------------- spatially complex rainfall ----
+  // topmodel_runoff(cycle, runoff) (LCM.cpp:3610-3627): the rain grid of
+  // rainGrid::create (LSDRainfallRunoff.cpp:36-63) -- with one rain cell the
+  // first column's value everywhere; otherwise cells 1..imax-1, 1..jmax-1
+  // whose hydroindex is 1..rfnum-1 take column index-1, all others 0 -- held
+  // in double; then runoffGrid::calculate_runoff (LSDRainfallRunoff.cpp:
+  // 175-241) on cells 1..imax-1, 1..jmax-1 above -9999, with rain_factor
+  // passed as an int and a 60 s step.
+  double rain_grid(int ts, long c) const {
+    if (p.rfnum == 1) return static_cast<double>(rain_at(ts, 0));
+    const int rf = rfarea[c];
+    if (rf >= 1 && rf < p.rfnum) return static_cast<double>(rain_at(ts, rf - 1));
+    return 0.0;
+  }
+  void topmodel_runoff_grid(double cyc) {
+    const int ts = static_cast<int>(cyc / p.rain_data_time_step);
+    const int rfi = static_cast<int>(rain_factor);
+    for (int m = 1; m < p.imax; ++m) {
+      for (int n = 1; n < p.jmax; ++n) {
+        const long c = at(m, n);
+        if (!(elev[c] > -9999)) continue;
+        double local = 0;
+        const double lts = 60;
+        rold[c] = rnew[c];
+        rjo[c] = rj[c];
+        const double r = rain_grid(ts, c);
+        if (r > 0) local = rfi * ((r / 1000) / 3600);
+        if (local == 0) {
+          rj[c] = rjo[c] / (1 + ((rjo[c] * lts) / p.M));
+          rnew[c] = p.M / lts * std::log(1 + ((rjo[c] * lts) / p.M));
+        }
+        if (local > 0) {
+          rj[c] = local / (((local - rjo[c]) / rjo[c]) * std::exp((0 - local) * lts / p.M) + 1);
+          rnew[c] = (p.M / lts) *
+                    std::log(((local - rjo[c]) + rjo[c] * std::exp((local * lts) / p.M)) / local);
+        }
+        if (rnew[c] < 0) rnew[c] = 0;
+      }
+    }
+  }
+  // calchydrograph(time, runoff) (LCM.cpp:3728-3739), cells 1..imax, 1..jmax
+  void calchydrograph_grid(double t) {
+    for (int m = 1; m <= p.imax; ++m)
+      for (int n = 1; n <= p.jmax; ++n) {
+        const long c = at(m, n);
+        rj_mean[c] = rold[c] + (((rnew[c] - rold[c]) / 2) * (2 - t));
+      }
+  }
+  // catchment_water_input_and_hydrology(flow_timestep, runoff)
+  // (LCM.cpp:3544-3606): every cell 1..imax-1, 1..jmax-1 takes its own
+  // runoff, capped at ERODEFACTOR; the step-size test uses the largest
+  // new_j_mean from before the TOPMODEL updates.
+  void catchment_water_input_grid(double fts) {
+    for (int i = 1; i < p.imax; ++i)
+      for (int jj = 1; jj < p.jmax; ++jj) waterinput += rj_mean[at(i, jj)] * p.DX * p.DX;
+    for (int i = 1; i < p.imax; ++i)
+      for (int jj = 1; jj < p.jmax; ++jj) {
+        double amt = rj_mean[at(i, jj)] * fts;
+        if (amt > p.ERODEFACTOR) amt = p.ERODEFACTOR;
+        waterinput += (amt / fts) * p.DX * p.DX;
+        h[at(i, jj)] += amt;
+      }
+    double njmax = 0;
+    for (int m = 1; m <= p.imax; ++m)
+      for (int n = 1; n <= p.jmax; ++n)
+        if (rnew[at(m, n)] > njmax) njmax = rnew[at(m, n)];
+    if (cycle >= time_1) {
+      const unsigned cells = static_cast<unsigned>(p.jmax) * static_cast<unsigned>(p.imax);
+      do {
+        time_1++;
+        topmodel_runoff_grid(time_1);
+        if (time_step > p.max_time_step && njmax > (0.2 / (cells * p.DX * p.DX))) {
+          cycle = time_1 + (p.max_time_step / 60);
+          time_step = p.max_time_step;
+        }
+      } while (time_1 < cycle);
+    }
+    calchydrograph_grid(time_1 - cycle);
+  }
+  // output_data(temptotal, runoff)'s runoff volume (LCM.cpp:2487-2501,
+  // 2571-2583): each catchment cell's term added in turn
+  void add_grid_runoff_volume(double& vol, double minutes) const {
+    for (int i = 1; i <= p.imax; ++i)
+      for (int jj = 1; jj <= p.jmax; ++jj)
+        if (elev[at(i, jj)] > p.no_data_value) vol += (rj_mean[at(i, jj)] * p.DX * p.DX) * (minutes * 60);
+  }
+
   // LCM.cpp:3741-3767
   int get_catchment_input_points() {
     totalinputpoints = 0;
@@ -412,8 +502,12 @@ struct orc_model {
   // LCM.cpp:2310-2478 (rows recorded instead of appended to a text file)
   void output_data() {
     Qw_newvol += temptotal * ((cycle - previous) * 60);
-    for (int nn = 1; nn <= p.rfnum; ++nn)
-      Jw_newvol += (j_mean[nn] * p.DX * p.DX * nActual[nn]) * ((cycle - previous) * 60);
+    if (p.spatially_complex_rainfall) {
+      add_grid_runoff_volume(Jw_newvol, cycle - previous);
+    } else {
+      for (int nn = 1; nn <= p.rfnum; ++nn)
+        Jw_newvol += (j_mean[nn] * p.DX * p.DX * nActual[nn]) * ((cycle - previous) * 60);
+    }
     const double osi = p.output_file_save_interval;
     if ((new_cycle < old_cycle) || (cycle - previous >= osi)) {
       while ((tx > previous) && (cycle >= tx)) {
@@ -446,8 +540,12 @@ struct orc_model {
         Qw_stepvol = Qw_newvol - Qw_oldvol;
         Qw_hourvol = Qw_stepvol - Qw_overvol + Qw_lastvol;
         Qw_hour = Qw_hourvol / (60 * osi);
-        for (int nn = 1; nn <= p.rfnum; ++nn)
-          Jw_overvol += (j_mean[nn] * p.DX * p.DX * nActual[nn]) * ((cycle - tx) * 60);
+        if (p.spatially_complex_rainfall) {
+          add_grid_runoff_volume(Jw_overvol, cycle - tx);
+        } else {
+          for (int nn = 1; nn <= p.rfnum; ++nn)
+            Jw_overvol += (j_mean[nn] * p.DX * p.DX * nActual[nn]) * ((cycle - tx) * 60);
+        }
         Jw_stepvol = Jw_newvol - Jw_oldvol;
         Jw_hourvol = Jw_stepvol - Jw_overvol + Jw_lastvol;
         Jw_hour = Jw_hourvol / (60 * osi);
@@ -1057,7 +1155,8 @@ struct orc_model {
     cycle += time_step / 60;
     new_cycle = std::fmod(cycle, p.output_file_save_interval);
     waterinput = 0;                                          // LCM.cpp:2027-2039
-    catchment_water_input_and_hydrology(get_flow_timestep());
+    if (p.spatially_complex_rainfall) catchment_water_input_grid(get_flow_timestep());
+    else catchment_water_input_and_hydrology(get_flow_timestep());
     if (dtf) *dtf = get_flow_timestep();
     flow_route();
     depth_update();
@@ -1104,6 +1203,13 @@ orc_model* orc_create(const orc_params* p) {
   m->new_j_mean.assign(rf + 1, 0.0);
   m->nActual.assign(rf + 1, 0);
   m->input_counter.assign(rf + 1, 0);
+  if (p->spatially_complex_rainfall) {
+    m->rj.assign(N, 0.000000001);
+    m->rjo.assign(N, 0.000000001);
+    m->rj_mean.assign(N, 0.0);
+    m->rold.assign(N, 0.0);
+    m->rnew.assign(N, 0.0);
+  }
   m->in_x.assign(static_cast<size_t>(p->imax) * p->jmax + 2, 0);
   m->in_y.assign(static_cast<size_t>(p->imax) * p->jmax + 2, 0);
   for (auto* v : {&m->sum_grain, &m->old_sum_grain, &m->Qg_step, &m->Qg_hour,
@@ -1138,6 +1244,8 @@ double* orc_field(orc_model* m, const char* name, long* len) {
   else if (n == "sl") v = &m->sl;
   else if (n == "su") v = &m->su;
   else if (n == "sd") v = &m->sd;
+  else if (n == "runoff_j_mean") v = &m->rj_mean;
+  else if (n == "runoff_new_j_mean") v = &m->rnew;
   if (!v || v->empty()) { *len = 0; return nullptr; }
   *len = static_cast<long>(v->size());
   return v->data();
@@ -1173,6 +1281,8 @@ int orc_scalar(orc_model* m, const char* name, double* value, int set) {
   ORC_D("time_1", time_1) ORC_D("previous", previous) ORC_D("erode_mult", erode_mult)
   ORC_D("erode_call", erode_call) ORC_D("globalsediq", globalsediq) ORC_D("sediQ", sediQ)
   ORC_D("new_cycle", new_cycle) ORC_D("old_cycle", old_cycle)
+  ORC_D("Qw_newvol", Qw_newvol) ORC_D("Jw_newvol", Jw_newvol) ORC_D("Jw_overvol", Jw_overvol)
+  ORC_D("Jw_hour", Jw_hour)
 #undef ORC_D
   if (n == "counter") {
     if (set) m->counter = static_cast<int>(*value); else *value = m->counter;
@@ -1208,6 +1318,12 @@ void orc_initialise(orc_model* m) {
     for (int j = 1; j < p.jmax; ++j)
       if (m->elev[m->at(i, j)] > -9999) m->nActual[m->rfarea[m->at(i, j)]]++;
   m->time_1 = m->cycle;  // set_time_counters, LCM.cpp:1862-1871
+  if (p.spatially_complex_rainfall) {
+    // initialise_rainfall_runoff and initialise_drainage_area (LCM.cpp:2041-2069):
+    // the per-cell TOPMODEL only; no drainage area or input points
+    m->topmodel_runoff_grid(1.0);
+    return;
+  }
   m->topmodel_runoff(1.0);
   m->drainage_area();
   m->get_catchment_input_points();

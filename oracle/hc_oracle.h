@@ -52,6 +52,8 @@ typedef struct orc_params {
   double vegTauCrit;
   int scan_interval;          /* 5 (main.cpp:129) */
   int nthreads;               /* OpenMP threads for the flow/depth loops */
+  int spatially_complex_rainfall; /* spatially_complex_rainfall_on (LCM.cpp:1355-1359) */
+  double no_data_value;       /* the DEM header's NODATA value (LCM.cpp:125) */
 } orc_params;
 
 typedef struct orc_model orc_model;

@@ -49,6 +49,7 @@ class orc_params(ctypes.Structure):
         ("chann_lateral_erosion", ctypes.c_double),
         ("veg_lat_restriction", ctypes.c_double), ("vegTauCrit", ctypes.c_double),
         ("scan_interval", ctypes.c_int), ("nthreads", ctypes.c_int),
+        ("spatially_complex_rainfall", ctypes.c_int), ("no_data_value", ctypes.c_double),
     ]
 
 
@@ -132,6 +133,8 @@ def params_from_dict(p: Dict[str, str], imax: int, jmax: int, dx: float,
     o.vegTauCrit = _atof(g("vegetation_crit_shear", "100"))
     o.scan_interval = 5
     o.nthreads = nthreads
+    o.spatially_complex_rainfall = 1 if g("spatially_complex_rainfall_on", "no") == "yes" else 0
+    o.no_data_value = -9999.0  # the synthetic DEMs' NODATA_value header
     return o
 
 

@@ -231,6 +231,9 @@ double* ref_field(void* h, const char* name, int* d0, int* d1) {
   if (n == "ss") return arr2(s.ss, d0, d1);
   if (n == "grain") return arr2(s.grain, d0, d1);
   if (n == "edge") return arr2(s.edge, d0, d1);
+  runoffGrid& ro = *static_cast<RefRun*>(h)->runoff;
+  if (n == "runoff_j_mean") return arr2(ro.j_mean_array, d0, d1);
+  if (n == "runoff_new_j_mean") return arr2(ro.new_j_mean_array, d0, d1);
   *d0 = *d1 = 0;
   return nullptr;
 }
@@ -311,6 +314,7 @@ int ref_scalar(void* h, const char* name, double* value, int set) {
   HC_D(froude_limit) HC_D(ERODEFACTOR) HC_D(active) HC_D(Csuspmax)
   HC_D(max_vel) HC_D(M) HC_D(globalsediq) HC_D(sediQ) HC_D(Qw_newvol)
   HC_D(Qw_hour) HC_D(tx) HC_D(new_cycle) HC_D(old_cycle) HC_D(temptotal)
+  HC_D(Jw_newvol) HC_D(Jw_overvol) HC_D(Jw_hour)
   HC_D(d1) HC_D(d2) HC_D(d3) HC_D(d4) HC_D(d5) HC_D(d6) HC_D(d7) HC_D(d8)
   HC_D(d9) HC_D(rain_data_time_step) HC_D(in_out_difference_allowed)
   HC_D(water_depth_erosion_threshold) HC_D(output_file_save_interval)

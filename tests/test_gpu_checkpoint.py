@@ -41,12 +41,15 @@ def _case(tmp_path, case):
     if case == "erosion":
         w = _erosion_workload(os.path.join(str(tmp_path), "ero"))
         return w, {}, HYDRO_FIELDS + SED_FIELDS, 160, 53
+    if case == "complex-rain":
+        w = inputs.complex_rain(os.path.join(str(tmp_path), "cr"), 3)
+        return w, {}, HYDRO_FIELDS, 700, 333
     w = inputs.flow_only_256(os.path.join(str(tmp_path), "flow"), size=48, hours=1)
     kw = {} if case == "fp64" else {"precision": hc.HC_FP32, "elev_fixed": case == "fp32-fixed"}
     return w, kw, HYDRO_FIELDS, 400, 137
 
 
-@pytest.mark.parametrize("case", ["fp64", "fp32", "fp32-fixed", "erosion"])
+@pytest.mark.parametrize("case", ["fp64", "fp32", "fp32-fixed", "erosion", "complex-rain"])
 def test_restart_continues_bit_identical(tmp_path, case):
     w, kw, fields, n, k = _case(tmp_path, case)
     whole = Model(w.path, w.params_file, **kw)

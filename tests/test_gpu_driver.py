@@ -197,3 +197,43 @@ def test_fp32_fully_wet_error(tmp_path):
     print(f"fp32 fully wet depth error vs fp64: {res}")
     assert res["exact_z"]["max"] < 1e-4
     assert res["fixed_z"]["max"] < 1e-4
+
+
+@pytest.mark.parametrize("rfnum", [1, 3])
+def test_driver_complex_rainfall_vs_oracle(tmp_path, rfnum):
+    """spatially_complex_rainfall_on through the driver: the per-cell
+    TOPMODEL on the host (host libm, as the reference), every cell's water
+    added on the device (hc_runoff_add); FP64 within 1e-10 of the oracle
+    (pinned bitwise to the compiled reference) over the whole run -- step
+    records, fields, the water input and the series -- and the FP32 form
+    (fixed-point elevations) against the FP64 form at the bar of the
+    rain-fed shallow flow above: 1e-4 or four times the FP64 run's own
+    sensitivity to 1e-8 m of DEM noise, volume to 1e-6."""
+    w = inputs.complex_rain(os.path.join(str(tmp_path), f"c{rfnum}"), rfnum)
+    o = OracleModel.from_workload(w)
+    ot, oc, od = o.iterate(10 ** 7)
+    m = Model(w.path, w.params_file)
+    gt, gc, gd = m.iterate(10 ** 7)
+    assert m.finished and len(gt) == len(ot) > 1000
+    _assert_rel(gc, oc, 1e-12, 0, "cycle")
+    _assert_rel(gd, od, 1e-10, 0, "dt")
+    _assert_rel(gt, ot, 1e-10, 1e-12, "temptot")
+    for f in ("water_depth", "qx", "qy"):
+        _assert_rel(m.download(f), o.field(f), 1e-10, 1e-13, f)
+    _assert_rel(m.series(), o.series(), 1e-10, 1e-12, "series")
+    _assert_rel(m.scalar("waterinput"), o.scalar("waterinput"), 1e-10, 0, "waterinput")
+    assert o.scalar("waterinput") > 0
+    b = OracleModel.from_workload(w, initialise=False)
+    e = b.field("elev")
+    land = e > -9999
+    e[land] += 1e-8 * np.random.Generator(np.random.PCG64(3)).standard_normal(land.sum())
+    b.L.orc_initialise(b.h)
+    b.iterate(10 ** 7)
+    floor = _errs(b.field("water_depth"), o.field("water_depth"))
+    f32 = Model(w.path, w.params_file, precision=hc.HC_FP32, elev_fixed=True)
+    f32.iterate(10 ** 7)
+    assert f32.finished
+    err = _errs(f32.download("water_depth"), o.field("water_depth"))
+    print(f"complex rain fp32 depth error {err}, floor {floor}")
+    assert err["max"] <= max(1e-4, 4 * floor["max"]) and err["l2"] <= max(1e-4, 4 * floor["l2"])
+    assert err["vol"] < 1e-6

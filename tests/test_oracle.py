@@ -77,6 +77,38 @@ def test_oracle_spatial_n_vs_compiled_reference(tmp_path):
         assert np.array_equal(r.field(f), o.field(f)), f
 
 
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("rfnum", [1, 3])
+def test_spatially_complex_rainfall_vs_compiled_reference(tmp_path, rfnum):
+    """The spatially complex rainfall path (per-cell TOPMODEL,
+    LSDRainfallRunoff.cpp:175-241; per-cell water input and output sums,
+    LCM.cpp:3544-3606, 2481-2583) of the oracle bitwise against the compiled
+    reference over a whole run: step records, fields, the runoff grid's
+    j_mean and new_j_mean, and the output series.  With a hydroindex the
+    last rain cell index takes no rain in the reference's rain grid
+    (rainGrid::create, LSDRainfallRunoff.cpp:49-60), which this pins."""
+    w = inputs.complex_rain(str(tmp_path / f"c{rfnum}"), rfnum)
+    r = RefModel(w.path, w.params_file, threads=1)
+    o = OracleModel.from_workload(w)
+    assert np.array_equal(r.field("runoff_new_j_mean"), o.field("runoff_new_j_mean"))
+    rt, rc, rd = r.iterate(10 ** 7)
+    ot, oc, od = o.iterate(10 ** 7)
+    assert len(rt) == len(ot) > 100
+    assert np.array_equal(rt, ot) and np.array_equal(rc, oc) and np.array_equal(rd, od)
+    for f in ("water_depth", "qx", "qy", "runoff_j_mean", "runoff_new_j_mean"):
+        assert np.array_equal(r.field(f), o.field(f)), f
+    for k in ("Qw_newvol", "Jw_newvol", "Jw_overvol", "Jw_hour", "waterinput", "time_1"):
+        assert r.scalar(k) == o.scalar(k), k
+    assert o.scalar("Jw_overvol") > 0 and len(o.series()) >= 1
+    assert o.field("runoff_new_j_mean").max() > 0
+    if rfnum > 1:  # the last rain cell's rows decay only (no rain reaches them)
+        jm = o.field("runoff_new_j_mean")[1:-1, 1:-1]
+        rows = np.arange(jm.shape[0])
+        top = jm[rows < jm.shape[0] // 3 - 1]
+        bottom = jm[rows > 2 * jm.shape[0] // 3 + 1][:-1, :-1]
+        assert bottom.max() < 1e-3 * top.max()
+
+
 def test_oracle_boscastle_known_good_answers(tmp_path):
     """The reference's own regression case (test/run_tests.sh +
     basic_kga_tests.py): 71 simulated hours, 88450 iterations."""
