@@ -265,6 +265,7 @@ class CatchmentModel {
         read_rainfall(read_path_ + "/" + rainfall_data_file_);
       }
     }
+    load_reach_files();
     if (graindata_from_file_ && !hydro_only_) ingest_graindata(read_path_ + "/" + grain_data_file_);
   }
 
@@ -320,7 +321,50 @@ class CatchmentModel {
     cycle_ += time_step_ / 60;
     new_cycle_ = std::fmod(cycle_, output_file_save_interval_);
   }
-  void reach_water_and_sediment_input() {}  // reach mode is rejected at parse time
+  // reach_water_and_sediment_input (LCM.cpp:3301-3452).  With erosion, a
+  // wet point takes each fraction's sediment (file columns 5..13, fraction 1
+  // suspended, LCM.cpp:2980), scaled down together when their mean over the
+  // points exceeds 0.75 ERODEFACTOR; the total is formed with the float
+  // |value| / div_inputs, the amounts with the double product first, as the
+  // reference's expressions do.  Every point then takes the discharge
+  // (column 1) interpolated to this cycle.  The amounts are formed here and
+  // applied on the device in the reference's point order.
+  void reach_water_and_sediment_input() {
+    const int np = reach_n_;
+    if (np == 0) return;
+    const double fts = get_flow_timestep();
+    const int rdt = reach_rdt_;
+    const int row = static_cast<int>(cycle_ / rdt);
+    std::vector<double> sed, water(np);
+    const bool with_sed = reach_mode_ && !hydro_only_;
+    if (with_sed) {
+      sed.assign(static_cast<size_t>(np) * 9, 0.0);
+      for (int n = 0; n < np; ++n) {
+        double adding_factor = 1, added_tot = 0;
+        for (int tn = 5; tn <= kGMax + 3; ++tn)
+          added_tot += (std::abs(reach_val(n, row, tn))) / div_inputs_ / (DX_ * DX_) / (rdt * 60) * time_step_;
+        if (added_tot / np > ERODEFACTOR_ * 0.75) adding_factor = (ERODEFACTOR_ * 0.75) / (added_tot / np);
+        for (int tn = 5; tn <= kGMax + 3; ++tn)
+          sed[static_cast<size_t>(n) * 9 + (tn - 5)] =
+              adding_factor * (std::abs(reach_val(n, row, tn))) / div_inputs_ / (DX_ * DX_) / (rdt * 60) * time_step_;
+      }
+    }
+    for (int n = 0; n < np; ++n) {
+      const double i1 = reach_val(n, row, 1), i2 = reach_val(n, row + 1, 1);
+      const double prop = (((row + 1) * rdt) - cycle_) / rdt;
+      const double input = i1 + ((i2 - i1) * (1 - prop));
+      waterinput_ += (input / div_inputs_);
+      water[n] = (input / div_inputs_) / (DX_ * DX_) * fts;
+    }
+    check(hc_reach_inputs(ctx_, np, reach_x_, reach_y_, with_sed ? sed.data() : nullptr, water.data(),
+                          water_depth_erosion_threshold_),
+          "reach_inputs");
+  }
+  float reach_val(int k, int row, int col) const {
+    if (row < 0 || row >= static_cast<int>(reach_rows_[k].size())) return 0.0f;
+    if (col < 0 || col >= static_cast<int>(reach_rows_[k][row].size())) return 0.0f;
+    return reach_rows_[k][row][col];
+  }
   void catchment_waterinputs() {             // LCM.cpp:2027-2039
     waterinput_ = 0;
     if (complex_rain_) catchment_water_input_grid(get_flow_timestep());
@@ -422,7 +466,9 @@ class CatchmentModel {
     return !(get_cycle() / 60 < get_maxcycle());
   }
 
-  bool device_loop_enabled() const { return opt_.device_loop && hydro_only_ && !complex_rain_; }
+  bool device_loop_enabled() const {
+    return opt_.device_loop && hydro_only_ && !complex_rain_ && reach_n_ == 0;
+  }
   void set_graph(bool g) { opt_.use_graph = g ? 1 : 0; }
 
   // Device-resident stepping (hc_loop_*): n loop bodies with the scalar
@@ -795,6 +841,21 @@ class CatchmentModel {
       else if (k == "read_fname") read_fname_ = value;
       else if (k == "hydroindex_file") hydroindex_fname_ = value;
       else if (k == "rainfall_data_file") rainfall_data_file_ = value;
+      else if (k == "reach_mode") reach_mode_ = yes();
+      else if (k == "divide_inputs_by") div_inputs_ = static_cast<int>(std::atof(value.c_str()));
+      else if (k == "reach_input_data_timestep") reach_rdt_ = std::atoi(value.c_str());
+      else if (k.size() > 6 && k.compare(0, 5, "reach") == 0 && k[5] >= '1' && k[5] <= '3' &&
+               (k.compare(6, std::string::npos, "_input_on") == 0 ||
+                k.compare(6, std::string::npos, "_input_file") == 0 ||
+                k.compare(6, std::string::npos, "_x") == 0 ||
+                k.compare(6, std::string::npos, "_y") == 0)) {
+        const int r = k[5] - '1';
+        const std::string f = k.substr(6);
+        if (f == "_input_on") reach_on_[r] = yes();
+        else if (f == "_input_file") reach_file_[r] = value;
+        else if (f == "_x") reach_px_[r] = std::atoi(value.c_str());
+        else reach_py_[r] = std::atoi(value.c_str());
+      }
       else if (k == "grain_data_file") grain_data_file_ = value;
       else if (k == "bedrock_data_file") bedrock_data_file_ = value;
       else if (k == "water_init_raster_file") water_init_raster_file_ = value;
@@ -850,7 +911,7 @@ class CatchmentModel {
       }
       else if (k == "spatially_complex_rainfall_on") complex_rain_ = yes();
       else if ((k == "vegetation_on" || k == "groundwater_on" ||
-                k == "reach_mode" || k == "lateral_erosion_on") && yes())
+                k == "lateral_erosion_on") && yes())
         throw Error("option '" + k + "' is not provided by this build (hot path only)");
     }
     tx_ = output_file_save_interval_;                // LCM.cpp:1599
@@ -874,6 +935,31 @@ class CatchmentModel {
     for (int i = 1; i < imax_; ++i)
       for (int j = 1; j < jmax_; ++j)
         if (elev_[at(i, j)] > -9999) nActual_[rfarea_[at(i, j)]]++;
+  }
+
+  // the reach points and files of initialise_arrays (LCM.cpp:1766-1845);
+  // read_reachfile (LCM.cpp:718-760) keeps each line's values as float
+  void load_reach_files() {
+    reach_n_ = 0;
+    if (!reach_mode_) return;
+    for (int r = 0; r < 3; ++r) {
+      if (!reach_on_[r]) continue;
+      const std::string f = read_path_ + "/" + reach_file_[r];
+      std::ifstream in(f);
+      if (!in) throw Error("No reach input data file found by name of: " + f);
+      reach_rows_[reach_n_].clear();
+      std::string line;
+      while (std::getline(in, line)) {
+        std::stringstream ss(line);
+        std::vector<float> row;
+        float val;
+        while (ss >> val) row.push_back(val);
+        reach_rows_[reach_n_].push_back(row);
+      }
+      reach_x_[reach_n_] = reach_px_[r];
+      reach_y_[reach_n_] = reach_py_[r];
+      ++reach_n_;
+    }
   }
 
   void read_rainfall(const std::string& f) {  // LCM.cpp:694-717 (values parsed as float)
@@ -1320,6 +1406,13 @@ class CatchmentModel {
   // spatially complex rainfall: the runoffGrid (j, jo, j_mean, old, new per
   // cell), the last hydrograph time, and whether j_mean has been formed yet
   bool complex_rain_ = false;
+  // reach mode: up to three input points (in file order) and their files
+  bool reach_mode_ = false, reach_on_[3] = {false, false, false};
+  std::string reach_file_[3];
+  int reach_px_[3] = {0, 0, 0}, reach_py_[3] = {0, 0, 0};
+  int reach_n_ = 0, reach_x_[3] = {0, 0, 0}, reach_y_[3] = {0, 0, 0};
+  int div_inputs_ = 1, reach_rdt_ = 60;
+  std::vector<std::vector<float>> reach_rows_[3];
   std::vector<double> rj_, rjo_, rj_mean_, rold_, rnew_;
   double runoff_t_ = 0;
   int runoff_jm_set_ = 0;

@@ -501,6 +501,12 @@ int hc_add_point_inputs(hc_ctx* c, const int* x, const int* y, const double* amt
   return HC_OK;
 }
 
+int hc_reach_inputs(hc_ctx* c, int n, const int* x, const int* y, const double* sediment,
+                    const double* water, double threshold) {
+  if (!c) return HC_ERR_ARG;
+  return hc_reach_inputs_impl(c, n, x, y, sediment, water, threshold);
+}
+
 int hc_runoff_set(hc_ctx* c, const double* old_j_mean, const double* new_j_mean) {
   if (!c || !old_j_mean || !new_j_mean) return HC_ERR_ARG;
   if (!c->d_runoff) {

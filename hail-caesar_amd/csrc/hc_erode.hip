@@ -894,6 +894,40 @@ ErodeArgs erode_args(const hc_ctx* c) {
   return a;
 }
 
+// reach_water_and_sediment_input's updates (LCM.cpp:3301-3452) at the few
+// reach points, by one thread in the reference's point order: with
+// sediment, a point wetter than the threshold gets a grain row if it has
+// none (addGS) and each fraction's amount -- fraction 1 into Vsusptot, the
+// others into the grain row and the elevation, in fraction order; then
+// every point takes its water.  The amounts are formed on the host.
+template <class T>
+__global__ void k_reach(ErodeArgs a, Grain gr, int n, const int* __restrict__ idx,
+                        const double* __restrict__ sed, const double* __restrict__ water,
+                        double thr, T* __restrict__ h, double* __restrict__ elev,
+                        const double* __restrict__ bedrock, double* __restrict__ V) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (sed) {
+    for (int k = 0; k < n; ++k) {
+      const size_t c = static_cast<size_t>(idx[k]);
+      if (!(static_cast<double>(h[c]) > thr)) continue;
+      if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
+      for (int f = 1; f <= kG - 1; ++f) {
+        const double amt = sed[static_cast<size_t>(k) * (kG - 1) + (f - 1)];
+        if (f == 1) {
+          V[c] += amt;
+        } else {
+          gr.G(c, f) += amt;
+          elev[c] += amt;
+        }
+      }
+    }
+  }
+  for (int k = 0; k < n; ++k) {
+    const size_t c = static_cast<size_t>(idx[k]);
+    h[c] = static_cast<T>(static_cast<double>(h[c]) + water[k]);
+  }
+}
+
 // Whether any cell of rows 2..xmax, columns 2..jmax-1 would move material into a neighbour under the
 // present elevations: the drop to the neighbour (over 1.41 on the
 // diagonals) above the cell's threshold (wet_low where the cell is at or
@@ -1312,6 +1346,59 @@ int hc_in_channel_landsliding_impl(hc_ctx* c, double failureangle) {
   hipLaunchKernelGGL(k_in_channel_landslide, dim3(1), dim3(1024), 0, c->stream, a, c->g.imax,
                      c->g.jmax, c->P, c->W, factor, c->mask, static_cast<double*>(c->z),
                      c->bedrock, gr, c->d_red + 4);
+  HC_HIP(c, hipGetLastError());
+  return HC_OK;
+}
+
+int hc_reach_inputs_impl(hc_ctx* c, int n, const int* x, const int* y, const double* sediment,
+                         const double* water, double threshold) {
+  if (n < 0 || n > 16 || (n > 0 && (!x || !y || !water))) return hc_fail(c, HC_ERR_ARG, "reach_inputs: bad arguments");
+  if (n == 0) return HC_OK;
+  if (sediment && (!c->g.sediment || c->prec != HC_FP64))
+    return hc_fail(c, HC_ERR_STATE, "reach_inputs: sediment needs an FP64 sediment context");
+  int idx[16];
+  double sed[16 * 9], wat[16];
+  for (int k = 0; k < n; ++k) {
+    if (x[k] < 0 || x[k] >= c->NR || y[k] < 0 || y[k] >= c->NC)
+      return hc_fail(c, HC_ERR_ARG, "reach_inputs: point outside the padded grid");
+    idx[k] = x[k] * c->P + y[k];
+    wat[k] = water[k];
+    for (int f = 0; f < 9 && sediment; ++f) sed[k * 9 + f] = sediment[k * 9 + f];
+  }
+  // the small arrays travel in the point-input buffers
+  if (!c->d_pts || c->pts_cap < 16 * 9 + 32) {
+    HC_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->d_pts) (void)hipFree(c->d_pts);
+    if (c->d_amt) (void)hipFree(c->d_amt);
+    c->d_pts = nullptr;
+    c->d_amt = nullptr;
+    const int cap = 1024;
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_pts), cap * sizeof(int)));
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_amt), cap * sizeof(double)));
+    c->pts_cap = cap;
+  }
+  HC_HIP(c, hipMemcpyAsync(c->d_pts, idx, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HC_HIP(c, hipMemcpyAsync(c->d_amt, wat, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (sediment)
+    HC_HIP(c, hipMemcpyAsync(c->d_amt + 16, sed, n * 9 * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+  // the host arrays above are pageable: the copies have completed on return
+  if (c->g.sediment) {
+    const ErodeArgs a = erode_args(c);
+    const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
+    hipLaunchKernelGGL(k_reach<double>, dim3(1), dim3(64), 0, c->stream, a, gr, n, c->d_pts,
+                       sediment ? c->d_amt + 16 : nullptr, c->d_amt, threshold,
+                       static_cast<double*>(c->h[c->cur]), static_cast<double*>(c->z), c->bedrock,
+                       c->V);
+  } else if (c->prec == HC_FP64) {
+    hipLaunchKernelGGL(k_reach<double>, dim3(1), dim3(64), 0, c->stream, ErodeArgs{}, Grain{}, n,
+                       c->d_pts, nullptr, c->d_amt, threshold, static_cast<double*>(c->h[c->cur]),
+                       nullptr, nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL(k_reach<float>, dim3(1), dim3(64), 0, c->stream, ErodeArgs{}, Grain{}, n,
+                       c->d_pts, nullptr, c->d_amt, threshold, static_cast<float*>(c->h[c->cur]),
+                       nullptr, nullptr, nullptr);
+  }
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

@@ -302,3 +302,5 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
 int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps);
 int hc_creep_impl(hc_ctx* c, double time, double creep_rate);
 int hc_in_channel_landsliding_impl(hc_ctx* c, double failureangle);
+int hc_reach_inputs_impl(hc_ctx* c, int n, const int* x, const int* y, const double* sediment,
+                         const double* water, double threshold);

@@ -34,7 +34,7 @@ SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upl
            "hc_download", "hc_flow_route", "hc_depth_update", "hc_flow_depth",
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
            "hc_global_landsliding", "hc_creep", "hc_in_channel_landsliding",
-           "hc_runoff_set", "hc_runoff_add", "hc_synchronize", "hc_check_health", "hc_state_bytes", "hc_state_save",
+           "hc_reach_inputs", "hc_runoff_set", "hc_runoff_add", "hc_synchronize", "hc_check_health", "hc_state_bytes", "hc_state_save",
            "hc_state_load", "hc_loop_begin", "hc_loop_run",
            "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
            "hc_set_tiles", "hc_bench_fused", "hc_ens_create", "hc_ens_destroy",
@@ -144,6 +144,7 @@ def lib():
         L.hc_check_health.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         L.hc_state_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.hc_runoff_set.argtypes = [vp, vp, vp]
+        L.hc_reach_inputs.argtypes = [vp, i, vp, vp, vp, vp, d]
         L.hc_runoff_add.argtypes = [vp, d, d, d]
         L.hc_state_save.argtypes = [vp, vp, ctypes.c_size_t]
         L.hc_state_load.argtypes = [vp, vp, ctypes.c_size_t]

@@ -447,6 +447,38 @@ def complex_rain(path: str, rfnum: int = 1, size: int = 40, seed: int = 1021,
                          overrides={"spatially_complex_rainfall_on": "yes"}, name="cplx")
 
 
+def reach_inputs(path: str, erosion: bool = False, size: int = 40, seed: int = 1031,
+                 hours: int = 1, div: int = 2, sed_scale: float = 1.0) -> Workload:
+    """reach_mode with two input points on the north edge fed from reach
+    files (16 columns: 1 = discharge m^3/s, 5..13 = sediment of fractions
+    1..9, times ``sed_scale``), 15 min data steps, inputs divided by ``div``;
+    1 cm of water to start; with ``erosion`` 9 grain fractions."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dem = planar_dem(size, size, 5.0, 0.01, 5.0, 0.6, seed, south_only=False)
+    steps = hours * 4 + 8
+    t = np.arange(steps)
+    files = {}
+    for k, (x, y) in enumerate(((2, size // 3), (2, (2 * size) // 3))):
+        rows = np.zeros((steps, 16))
+        rows[:, 0] = t * 15
+        rows[:, 1] = (0.4 + 0.2 * k) * np.sin(np.pi * np.minimum(t, 6) / 6.0) ** 2 + 0.02
+        rows[:, 5:14] = rng.random((steps, 9)) * 0.002 * (1 + k) * sed_scale
+        files[f"reach{k + 1}.txt"] = (rows, x, y)
+    over = {"reach_mode": "yes", "divide_inputs_by": str(div), "reach_input_data_timestep": "15"}
+    for k, (name, (rows, x, y)) in enumerate(files.items()):
+        over.update({f"reach{k + 1}_input_on": "yes", f"reach{k + 1}_input_file": name,
+                     f"reach{k + 1}_x": str(x), f"reach{k + 1}_y": str(y)})
+    kw = {}
+    if erosion:
+        over.update({"hydro_model_only": "no", "active_layer_thickness": "0.3"})
+        kw["graindata"] = dirichlet_props(size, size, rng)
+    w = make_workload(path, dem, 5.0, np.array([0.0, 0.0]), 60, hours, overrides=over,
+                      water_init=np.where(dem > NODATA, 0.01, 0.0), name="reach", **kw)
+    for name, (rows, x, y) in files.items():
+        np.savetxt(os.path.join(path, name), rows, fmt="%.9g")
+    return w
+
+
 def gamma_rain(hours: float, step_min: float, shape: float, mean: float,
                rng: np.random.Generator, ncells: int = 1) -> np.ndarray:
     n = int(round(hours * 60 / step_min))

@@ -217,6 +217,15 @@ int hc_in_channel_landsliding(hc_ctx* ctx, double failureangle);
  * old + ((new - old)/2)(2 - time) (calchydrograph, LCM.cpp:3728-3739) times
  * flow_timestep, capped at `cap` (ERODEFACTOR), formed in the reference's
  * order.  hc_runoff_add before any hc_runoff_set returns HC_ERR_STATE. */
+/* reach_water_and_sediment_input (LCM.cpp:3301-3452): n (at most 16) reach
+ * points at (x[k], y[k]), in order.  With `sediment` (n x 9 amounts, m, for
+ * fractions 1..9 as the reference forms them, FP64 sediment contexts), a
+ * point whose water depth exceeds `threshold` gets a grain row if it has
+ * none, fraction 1 into Vsusptot and fractions 2..9 into its grain row and
+ * elevation; then every point's depth takes water[k] (m).  NULL sediment:
+ * water only (hydro-only runs). */
+int hc_reach_inputs(hc_ctx* ctx, int n, const int* x, const int* y, const double* sediment,
+                    const double* water, double threshold);
 int hc_runoff_set(hc_ctx* ctx, const double* old_j_mean, const double* new_j_mean);
 int hc_runoff_add(hc_ctx* ctx, double time, double flow_timestep, double cap);
 

@@ -70,6 +70,10 @@ struct orc_model {
   // spatially complex rainfall: the runoffGrid's per-cell arrays
   // (LSDRainfallRunoff.cpp:96-104: j = jo = 1e-9, the means 0)
   std::vector<double> rj, rjo, rj_mean, rold, rnew;
+  // reach mode: up to three input points and their files (LCM.cpp:1766-1845)
+  int reach_n = 0;
+  int reach_x[3] = {0, 0, 0}, reach_y[3] = {0, 0, 0};
+  std::vector<std::vector<float>> reach_rows[3];
   // scalars (LCM.hpp:456-548 defaults)
   double time_step = 1, cycle = 0, maxdepth = 10, temptot = 0, waterinput = 0,
          waterOut = 0, iod = 0, baseflow = 0.00000005, time_1 = 1,
@@ -371,6 +375,58 @@ struct orc_model {
     for (int i = 1; i <= p.imax; ++i)
       for (int jj = 1; jj <= p.jmax; ++jj)
         if (elev[at(i, jj)] > p.no_data_value) vol += (rj_mean[at(i, jj)] * p.DX * p.DX) * (minutes * 60);
+  }
+
+  // ------------------------------------------------------- reach mode ----
+  float reach_val(int k, int row, int col) const {
+    if (row < 0 || row >= static_cast<int>(reach_rows[k].size())) return 0.0f;
+    if (col < 0 || col >= static_cast<int>(reach_rows[k][row].size())) return 0.0f;
+    return reach_rows[k][row][col];
+  }
+  // reach_water_and_sediment_input (LCM.cpp:3301-3452).  With erosion, a
+  // wet input point takes each fraction's sediment (columns 5..13; fraction
+  // 1 is suspended, LCM.cpp:2980), all scaled down together when their
+  // mean over the points exceeds 0.75 ERODEFACTOR; the total is formed with
+  // the float |value| / div_inputs, the amounts with the double product
+  // first, as the reference's expressions do.  Then every point takes the
+  // discharge (column 1) interpolated between this data step and the next.
+  void reach_water_and_sediment_input() {
+    const double fts = get_flow_timestep();
+    const int rdt = p.reach_input_data_timestep;
+    if (p.reach_mode && !p.hydro_only) {
+      for (int n = 0; n <= reach_n - 1; ++n) {
+        const int x = reach_x[n], y = reach_y[n];
+        const long c = at(x, y);
+        const int row = static_cast<int>(cycle / rdt);
+        double adding_factor = 1, added_tot = 0;
+        if (h[c] > p.water_depth_erosion_threshold) {
+          for (int tn = 5; tn <= kGMax + 3; ++tn)
+            added_tot += (std::abs(reach_val(n, row, tn))) / p.div_inputs / (p.DX * p.DX) / (rdt * 60) * time_step;
+          if (added_tot / reach_n > p.ERODEFACTOR * 0.75)
+            adding_factor = (p.ERODEFACTOR * 0.75) / (added_tot / reach_n);
+          if (!hasgrain[c]) addGS(x, y);
+          for (int tn = 5; tn <= kGMax + 3; ++tn) {
+            const double amt = adding_factor * (std::abs(reach_val(n, row, tn))) / p.div_inputs /
+                               (p.DX * p.DX) / (rdt * 60) * time_step;
+            if (tn - 4 == 1) {
+              V[c] += amt;
+            } else {
+              G(c)[tn - 4] += amt;
+              elev[c] += amt;
+            }
+          }
+        }
+      }
+    }
+    for (int n = 0; n <= reach_n - 1; ++n) {
+      const long c = at(reach_x[n], reach_y[n]);
+      const int row = static_cast<int>(cycle / rdt);
+      const double i1 = reach_val(n, row, 1), i2 = reach_val(n, row + 1, 1);
+      const double prop = (((row + 1) * rdt) - cycle) / rdt;
+      const double input = i1 + ((i2 - i1) * (1 - prop));
+      waterinput += (input / p.div_inputs);
+      h[c] += (input / p.div_inputs) / (p.DX * p.DX) * fts;
+    }
   }
 
   // LCM.cpp:3741-3767
@@ -1154,6 +1210,7 @@ struct orc_model {
     counter++;                                               // LCM.cpp:1925-1933
     cycle += time_step / 60;
     new_cycle = std::fmod(cycle, p.output_file_save_interval);
+    reach_water_and_sediment_input();                        // main.cpp:154
     waterinput = 0;                                          // LCM.cpp:2027-2039
     if (p.spatially_complex_rainfall) catchment_water_input_grid(get_flow_timestep());
     else catchment_water_input_and_hydrology(get_flow_timestep());
@@ -1262,6 +1319,16 @@ int* orc_field_int(orc_model* m, const char* name, long* len) {
   return v->data();
 }
 
+void orc_set_reach(orc_model* m, int k, int x, int y, const double* rows, int nrows, int ncols) {
+  if (k < 0 || k > 2) return;
+  m->reach_x[k] = x;
+  m->reach_y[k] = y;
+  m->reach_rows[k].assign(nrows, std::vector<float>(ncols, 0.0f));
+  for (int r = 0; r < nrows; ++r)
+    for (int c = 0; c < ncols; ++c) m->reach_rows[k][r][c] = static_cast<float>(rows[static_cast<long>(r) * ncols + c]);
+  if (k + 1 > m->reach_n) m->reach_n = k + 1;
+}
+
 void orc_set_rain(orc_model* m, const double* rain, int rows, int cols) {
   m->rain.assign(rows, std::vector<float>(cols, 0.0f));
   for (int r = 0; r < rows; ++r)
@@ -1345,6 +1412,7 @@ int orc_global_landsliding(orc_model* m, double failureangle) {
   return m->global_landsliding(failureangle);
 }
 void orc_drainage_area(orc_model* m) { m->drainage_area(); }
+void orc_reach_input(orc_model* m) { m->reach_water_and_sediment_input(); }
 int orc_input_points(orc_model* m) { return m->get_catchment_input_points(); }
 
 double orc_grain_call(orc_model* m, int which, int x, int y, double amount, int x2, int y2) {

@@ -54,6 +54,9 @@ typedef struct orc_params {
   int nthreads;               /* OpenMP threads for the flow/depth loops */
   int spatially_complex_rainfall; /* spatially_complex_rainfall_on (LCM.cpp:1355-1359) */
   double no_data_value;       /* the DEM header's NODATA value (LCM.cpp:125) */
+  int reach_mode;             /* reach_mode (LCM.cpp:1456-1459) */
+  int div_inputs;             /* divide_inputs_by, held as int (LCM.hpp:685) */
+  int reach_input_data_timestep; /* minutes, int (LCM.hpp:449) */
 } orc_params;
 
 typedef struct orc_model orc_model;
@@ -70,6 +73,11 @@ double* orc_field(orc_model* m, const char* name, long* len);
 /* int fields: mask (1 = cell on the active list, the reference's down_scan
  * membership), hasgrain (index != -9999), rfarea. */
 int* orc_field_int(orc_model* m, const char* name, long* len);
+/* reach_water_and_sediment_input (LCM.cpp:3301-3452), one call. */
+void orc_reach_input(orc_model* m);
+/* reach input point k (0..2) at (x, y) with its file's rows (rows x cols,
+ * row-major, held as float as read_reachfile does, LCM.cpp:718-760). */
+void orc_set_reach(orc_model* m, int k, int x, int y, const double* rows, int nrows, int ncols);
 /* rainfall table: rows x rfnum, row-major (mm/h). */
 void orc_set_rain(orc_model* m, const double* rain, int rows, int cols);
 /* Scalars by name (time_step, cycle, counter, maxdepth, temptot, waterinput,

@@ -50,6 +50,8 @@ class orc_params(ctypes.Structure):
         ("veg_lat_restriction", ctypes.c_double), ("vegTauCrit", ctypes.c_double),
         ("scan_interval", ctypes.c_int), ("nthreads", ctypes.c_int),
         ("spatially_complex_rainfall", ctypes.c_int), ("no_data_value", ctypes.c_double),
+        ("reach_mode", ctypes.c_int), ("div_inputs", ctypes.c_int),
+        ("reach_input_data_timestep", ctypes.c_int),
     ]
 
 
@@ -135,6 +137,9 @@ def params_from_dict(p: Dict[str, str], imax: int, jmax: int, dx: float,
     o.nthreads = nthreads
     o.spatially_complex_rainfall = 1 if g("spatially_complex_rainfall_on", "no") == "yes" else 0
     o.no_data_value = -9999.0  # the synthetic DEMs' NODATA_value header
+    o.reach_mode = 1 if g("reach_mode", "no") == "yes" else 0
+    o.div_inputs = int(_atof(g("divide_inputs_by", "1")))
+    o.reach_input_data_timestep = _atoi(g("reach_input_data_timestep", "60"))
     return o
 
 
@@ -204,6 +209,9 @@ class OracleModel:
             L.orc_field_int.restype = ctypes.POINTER(ctypes.c_int)
             L.orc_field_int.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]
             L.orc_set_rain.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+            L.orc_set_reach.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+            L.orc_reach_input.argtypes = [ctypes.c_void_p]
             L.orc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
             L.orc_initialise.argtypes = [ctypes.c_void_p]
             for fn in ("orc_flow_route", "orc_scan_area", "orc_set_maximum_timestep",
@@ -290,9 +298,23 @@ class OracleModel:
         m.set_rain(rain)
         if not p.hydro_only and w.params.get("read_in_graindata_from_file", "no") == "yes":
             m.load_graindata(os.path.join(w.path, w.params["grain_data_file"]))
+        if p.reach_mode:
+            m.load_reach(w)
         if initialise:
             m.L.orc_initialise(m.h)
         return m
+
+    def load_reach(self, w) -> None:
+        """Reach input points and files (LSDCatchmentModel.cpp:1766-1845,
+        read_reachfile 718-760)."""
+        for k in range(3):
+            if w.params.get(f"reach{k + 1}_input_on", "no") != "yes":
+                continue
+            rows = np.loadtxt(os.path.join(w.path, w.params[f"reach{k + 1}_input_file"]), ndmin=2)
+            rows = np.ascontiguousarray(rows, dtype=np.float64)
+            self.L.orc_set_reach(self.h, k, _atoi(w.params[f"reach{k + 1}_x"]),
+                                 _atoi(w.params[f"reach{k + 1}_y"]), rows.ctypes.data,
+                                 rows.shape[0], rows.shape[1])
 
     def load_graindata(self, path: str) -> None:
         """Grain data file semantics of ingest_graindata_from_file
@@ -396,7 +418,7 @@ class RefModel:
              "write_output_timeseries": 10, "drainage_area": 11, "input_points": 12,
              "erode": 13, "get_flow_timestep": 14, "topmodel_runoff": 15,
              "global_landsliding": 16, "creep": 17,
-             "in_channel_landsliding": 18}
+             "in_channel_landsliding": 18, "reach_water_and_sediment_input": 19}
 
     def __init__(self, path: str, params_file: str, threads: int = 1):
         self.L = self.lib()

@@ -194,6 +194,7 @@ double ref_call(void* h, int which, double arg) {
     case 16: s.global_landsliding(); return 0;
     case 17: s.creep(arg); return 0;
     case 18: s.in_channel_landsliding(); return 0;
+    case 19: s.reach_water_and_sediment_input(); return s.waterinput;
     default: return -1;
   }
 }

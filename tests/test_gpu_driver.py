@@ -237,3 +237,55 @@ def test_driver_complex_rainfall_vs_oracle(tmp_path, rfnum):
     print(f"complex rain fp32 depth error {err}, floor {floor}")
     assert err["max"] <= max(1e-4, 4 * floor["max"]) and err["l2"] <= max(1e-4, 4 * floor["l2"])
     assert err["vol"] < 1e-6
+
+
+def test_driver_reach_mode_vs_oracle(tmp_path):
+    """reach_mode (reach_water_and_sediment_input, LSDCatchmentModel.cpp:
+    3301-3452) through the driver, hydro-only: two reach points' discharge
+    added on the device (hc_reach_inputs); FP64 within 1e-10 of the oracle
+    (pinned bitwise to the compiled reference) over the whole run, and the
+    FP32 form (fixed-point elevations) within 1e-4 of it or four times the
+    FP64 run's sensitivity to 1e-8 m of DEM noise.  Volume: the discharge
+    enters at two cells, about 1e-4 m a step into columns near 0.5 m deep,
+    and each FP32 add rounds by up to half an ulp of the column (3e-8 m), a
+    bias that sums to about 1e-5 of the domain's volume over the run; the
+    bar is 2e-5."""
+    w = inputs.reach_inputs(os.path.join(str(tmp_path), "rh"))
+    o = OracleModel.from_workload(w)
+    ot, oc, od = o.iterate(10 ** 7)
+    m = Model(w.path, w.params_file)
+    gt, gc, gd = m.iterate(10 ** 7)
+    assert m.finished and len(gt) == len(ot) > 100
+    _assert_rel(gc, oc, 1e-12, 0, "cycle")
+    _assert_rel(gd, od, 1e-10, 0, "dt")
+    _assert_rel(gt, ot, 1e-10, 1e-12, "temptot")
+    for f in ("water_depth", "qx", "qy"):
+        _assert_rel(m.download(f), o.field(f), 1e-10, 1e-13, f)
+    _assert_rel(m.series(), o.series(), 1e-10, 1e-12, "series")
+    b = OracleModel.from_workload(w, initialise=False)
+    e = b.field("elev")
+    land = e > -9999
+    e[land] += 1e-8 * np.random.Generator(np.random.PCG64(3)).standard_normal(land.sum())
+    b.L.orc_initialise(b.h)
+    b.iterate(10 ** 7)
+    floor = _errs(b.field("water_depth"), o.field("water_depth"))
+    f32 = Model(w.path, w.params_file, precision=hc.HC_FP32, elev_fixed=True)
+    f32.iterate(10 ** 7)
+    err = _errs(f32.download("water_depth"), o.field("water_depth"))
+    assert err["max"] <= max(1e-4, 4 * floor["max"]) and err["vol"] < 2e-5
+
+
+def test_driver_reach_mode_erosion_runs(tmp_path):
+    """reach_mode in an erosion run through the driver: the sediment of the
+    reach files enters at the two points on the device (suspended fraction
+    into Vsusptot, the others into the grain rows and the bed) and the run
+    stays finite.  The amounts are the oracle's restatement, pinned per call
+    against the compiled reference on the CPU."""
+    w = inputs.reach_inputs(os.path.join(str(tmp_path), "re"), erosion=True, sed_scale=50.0)
+    m = Model(w.path, w.params_file)
+    e0 = m.download("elev")
+    m.iterate(300)
+    V, e, h = m.download("Vsusptot"), m.download("elev"), m.download("water_depth")
+    assert np.isfinite(V).all() and np.isfinite(h).all() and np.isfinite(e).all()
+    pts = [(2, 40 // 3), (2, (2 * 40) // 3)]
+    assert all(V[p] > 0 or e[p] > e0[p] for p in pts)

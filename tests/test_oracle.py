@@ -109,6 +109,55 @@ def test_spatially_complex_rainfall_vs_compiled_reference(tmp_path, rfnum):
         assert bottom.max() < 1e-3 * top.max()
 
 
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+def test_reach_mode_hydro_vs_compiled_reference(tmp_path):
+    """reach_water_and_sediment_input (LSDCatchmentModel.cpp:3301-3452) in a
+    hydro-only run: two reach points' discharge, interpolated between data
+    steps and divided by the integer divide_inputs_by, bitwise against the
+    compiled reference over the whole run (step records, fields)."""
+    w = inputs.reach_inputs(str(tmp_path / "rh"))
+    r = RefModel(w.path, w.params_file, threads=1)
+    o = OracleModel.from_workload(w)
+    rt, rc, rd = r.iterate(10 ** 7)
+    ot, oc, od = o.iterate(10 ** 7)
+    assert len(rt) == len(ot) > 100
+    assert np.array_equal(rt, ot) and np.array_equal(rc, oc) and np.array_equal(rd, od)
+    for f in ("water_depth", "qx", "qy"):
+        assert np.array_equal(r.field(f), o.field(f)), f
+    assert r.field("water_depth")[2, 40 // 3] > 0.01
+
+
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("sed_scale", [1.0, 1e6])
+def test_reach_sediment_input_vs_compiled_reference(tmp_path, sed_scale):
+    """The erosion-mode half of reach_water_and_sediment_input: suspended
+    fraction 1 into Vsusptot, fractions 2..9 into the grain row and the
+    elevation, at two data steps; with sed_scale 1e6 the amounts exceed
+    0.75 ERODEFACTOR per point and are scaled down together.  One call at a
+    time from the same state: Vsusptot, elevation, water depth, grain rows
+    and the water input bitwise."""
+    w = inputs.reach_inputs(str(tmp_path / "rs"), erosion=True, sed_scale=sed_scale)
+    r = RefModel(w.path, w.params_file, threads=1)
+    o = OracleModel(params_from_dict(w.params, 40, 40, 5.0))
+    o.load_snapshot(r.snapshot())
+    o.load_reach(w)
+    for m in (r, o):  # wet input points (the sediment goes in above 0.01 m)
+        m.field("water_depth")[2, 40 // 3] = 0.2
+        m.field("water_depth")[2, (2 * 40) // 3] = 0.15
+    for cyc in (0.0, 37.5):
+        r.scalar("cycle", cyc)
+        o.scalar("cycle", cyc)
+        v0 = o.field("Vsusptot").copy()
+        wr = r.call("reach_water_and_sediment_input")
+        o.L.orc_reach_input(o.h)
+        s = r.snapshot()
+        assert not np.array_equal(o.field("Vsusptot"), v0)
+        for f in ("Vsusptot", "elev", "water_depth"):
+            assert np.array_equal(r.field(f), o.field(f)), f
+        assert np.array_equal(o.field("grain"), s["grain"])
+        assert wr == o.scalar("waterinput")
+
+
 def test_oracle_boscastle_known_good_answers(tmp_path):
     """The reference's own regression case (test/run_tests.sh +
     basic_kga_tests.py): 71 simulated hours, 88450 iterations."""
