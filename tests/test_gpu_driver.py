@@ -245,11 +245,11 @@ def test_driver_reach_mode_vs_oracle(tmp_path):
     added on the device (hc_reach_inputs); FP64 within 1e-10 of the oracle
     (pinned bitwise to the compiled reference) over the whole run, and the
     FP32 form (fixed-point elevations) within 1e-4 of it or four times the
-    FP64 run's sensitivity to 1e-8 m of DEM noise.  Volume: the discharge
-    enters at two cells, about 1e-4 m a step into columns near 0.5 m deep,
-    and each FP32 add rounds by up to half an ulp of the column (3e-8 m), a
-    bias that sums to about 1e-5 of the domain's volume over the run; the
-    bar is 2e-5."""
+    FP64 run's sensitivity to 1e-8 m of DEM noise.  Volume: the FP32 run
+    ends 8e-6 away from the FP64 run's volume here (the FP64 run moves by
+    7e-8 under the DEM noise), a systematic difference not yet isolated --
+    rounding of the point adds is not it (an FP64 carry on them changed it by
+    under 1%); the bar is 1e-5."""
     w = inputs.reach_inputs(os.path.join(str(tmp_path), "rh"))
     o = OracleModel.from_workload(w)
     ot, oc, od = o.iterate(10 ** 7)
@@ -272,7 +272,8 @@ def test_driver_reach_mode_vs_oracle(tmp_path):
     f32 = Model(w.path, w.params_file, precision=hc.HC_FP32, elev_fixed=True)
     f32.iterate(10 ** 7)
     err = _errs(f32.download("water_depth"), o.field("water_depth"))
-    assert err["max"] <= max(1e-4, 4 * floor["max"]) and err["vol"] < 2e-5
+    print(f"reach fp32 depth error {err}, floor {floor}")
+    assert err["max"] <= max(1e-4, 4 * floor["max"]) and err["vol"] < 1e-5
 
 
 def test_driver_reach_mode_erosion_runs(tmp_path):
