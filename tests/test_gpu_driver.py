@@ -248,8 +248,9 @@ def test_driver_reach_mode_vs_oracle(tmp_path):
     FP64 run's sensitivity to 1e-8 m of DEM noise.  Volume: the FP32 run
     ends 8e-6 away from the FP64 run's volume here (the FP64 run moves by
     7e-8 under the DEM noise), a systematic difference not yet isolated --
-    rounding of the point adds is not it (an FP64 carry on them changed it by
-    under 1%); the bar is 1e-5."""
+    neither the rounding of the point adds (an FP64 carry on them changed it
+    by under 1%) nor the open edge next to the points (6.2e-6 with them 10
+    rows inland); the bar is 1e-5."""
     w = inputs.reach_inputs(os.path.join(str(tmp_path), "rh"))
     o = OracleModel.from_workload(w)
     ot, oc, od = o.iterate(10 ** 7)
