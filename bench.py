@@ -210,39 +210,25 @@ def run_ensemble(steps: int, warmup: int, n: int = 64):
 
 
 def run_adjoint(size: int = 512, steps: int = 2000, dt: float = 0.5, every: int = 100):
-    """The adjoint workload: size^2 cells at 5 m, fractal DEM (5 m relief, 1 %
-    slope to the south edge), log-normal n, a uniform 0.1 m start, 8 rain
-    blocks (2 x 4) under a varying rain rate, `steps` steps at a fixed dt;
-    the loss is the outlet-discharge misfit against a target made with n
-    perturbed by 10 % smooth noise.  Times one forward (loss) and one reverse
-    sweep (gradients w.r.t. n and the 8 multipliers, including the segment
-    recomputation from the checkpoints)."""
+    """The adjoint workload (inputs.adjoint_case): size^2 cells at 5 m,
+    fractal DEM (5 m relief, 1 % slope to the south edge), log-normal n, a
+    uniform 0.1 m start, 8 rain blocks (2 x 4) under a varying rain rate,
+    `steps` steps at a fixed dt; the loss is the outlet-discharge misfit
+    against a target made with n perturbed by 10 % smooth noise.  Times one
+    forward (loss) and one reverse sweep (gradients w.r.t. n and the 8
+    multipliers, including the segment recomputation from the checkpoints)."""
     from hail_caesar_amd import hc, inputs
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from pyoracle import pad
-    rng = np.random.Generator(np.random.PCG64(1005))
-    dem = inputs.planar_dem(size, size, 5.0, 0.01, 5.0, 0.6, 1005, channel_depth=2.0)
-    n = inputs.lognormal_mannings(size, size, 0.035, 0.3, rng)
-    xi = rng.standard_normal((size, size))
-    k = np.fft.fftfreq(size)
-    g = np.exp(-2 * (np.pi * 8.0) ** 2 * (k[:, None] ** 2 + k[None, :] ** 2))
-    sm = np.real(np.fft.ifft2(np.fft.fft2(xi) * g))
-    npert = n * (1 + 0.1 * sm / np.abs(sm).max())
-    block = -np.ones((size + 2, size + 2), np.int32)
-    r = (np.arange(size) * 2) // size
-    c = (np.arange(size) * 4) // size
-    block[1:-1, 1:-1] = r[:, None] * 4 + c[None, :]
-    t = np.arange(steps)
-    rain = (10.0 + 30.0 * np.sin(t / 300.0) ** 2) / 3.6e6
+    case = inputs.adjoint_case(size=size, steps=steps)
+    pad = inputs.pad
     p = hc.default_params(hflow_threshold=0.001, edgeslope=0.01, spatially_var_mannings=1)
-    ctx = hc.Context(size, size, 5.0, p, hc.HC_FP64)
-    ctx.upload("elev", pad(dem, -9999.0))
-    ctx.upload("water_depth", pad(np.where(dem > -9999, 0.1, 0.0), 0.0))
-    ctx.upload("mannings", pad(npert, 0.0))
-    adj = hc.Adjoint(ctx, steps, dt, block, rain, checkpoint_every=every)
+    ctx = hc.Context(size, size, case["DX"], p, hc.HC_FP64)
+    ctx.upload("elev", pad(case["dem"], -9999.0))
+    ctx.upload("water_depth", pad(case["water"], 0.0))
+    ctx.upload("mannings", pad(case["n_target"], 0.0))
+    adj = hc.Adjoint(ctx, steps, dt, case["block"], case["rain"], checkpoint_every=every)
     _, qt = adj.forward(np.ones(8))
     adj.set_target(qt)
-    ctx.upload("mannings", pad(n, 0.0))
+    ctx.upload("mannings", pad(case["n"], 0.0))
     adj.forward(np.ones(8))  # warm
     t0 = time.perf_counter()
     loss, _ = adj.forward(np.ones(8))

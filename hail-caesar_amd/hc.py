@@ -7,8 +7,10 @@ machine without it raises, and nothing here falls back to a CPU path.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -113,6 +115,62 @@ class HCError(RuntimeError):
 
 
 _lib = None
+# Every live handle object (contexts, adjoints, models, ensembles), closed by
+# _close_all at interpreter exit while the HIP runtime is still whole.
+_live = weakref.WeakSet()
+_exiting = False
+
+
+def track(obj) -> None:
+    """Register an object with a close() that frees device resources; lower
+    `_close_rank` is closed first (borrowers before what they borrow)."""
+    _live.add(obj)
+
+
+def _close_all() -> None:
+    """atexit: close every live handle, borrowers first (ensembles borrow
+    member contexts, adjoints borrow a context).  Registered after torch's
+    own exit handlers (its import comes first), so it runs before them; a
+    handle left open into the runtime's static teardown aborts the process
+    (std::bad_variant_access).  Later finalizers find their handles closed."""
+    global _exiting
+    for obj in sorted(list(_live), key=lambda o: getattr(o, "_close_rank", 9)):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    _exiting = True
+
+
+def hip_runtimes_mapped() -> set:
+    """Distinct HIP runtime files (libamdhip64) mapped into this process."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64" in line:
+                    paths.add(os.path.realpath(line.split()[-1]))
+    except OSError:
+        pass
+    return paths
+
+
+def _choose_runtime() -> None:
+    """Decide the HIP runtime libhc.so binds to, before it is loaded.
+
+    The torch wheel bundles its own HIP runtime (torch/lib/libamdhip64.so,
+    soname libamdhip64.so.7) and loads it by a name that does not match the
+    system runtime's file.  Loaded after libhc.so has initialised the system
+    runtime (/opt/rocm/lib), torch maps a second copy, and the two copies
+    tear each other down at exit (std::bad_variant_access, abort).  Loaded
+    first, its runtime satisfies libhc.so's libamdhip64.so.7 dependency and
+    the process has exactly one.  So wherever torch is importable it is
+    imported here, first; every Python caller (tests, benchmarks, the
+    autograd harness) then runs libhc.so on the same runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def lib():
@@ -120,7 +178,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with hail-caesar_amd/build.py")
+        _choose_runtime()
         L = ctypes.CDLL(LIB_PATH)
+        rt = hip_runtimes_mapped()
+        if len(rt) > 1:
+            raise HCError(HC_ERR_STATE, "two HIP runtimes are mapped into this process "
+                          f"({', '.join(sorted(rt))}); import torch before anything loads "
+                          "the system HIP runtime")
+        atexit.register(_close_all)
         vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
         L.hc_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(hc_grid), ctypes.POINTER(hc_params)]
         L.hc_destroy.argtypes = [vp]
@@ -244,12 +309,16 @@ class Context:
         self.shape = (imax + 2, jmax + 2)
         self.precision = precision
         self.sediment = sediment
+        track(self)
+
+    _close_rank = 3
 
     def close(self):
         if getattr(self, "_borrowed", False):
             return
         if getattr(self, "h", None) and self.h.value:
-            self.L.hc_destroy(self.h)
+            if not _exiting:
+                self.L.hc_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):
@@ -428,10 +497,14 @@ class Adjoint:
         if rc != HC_OK:
             raise HCError(rc, "hc_adj_create failed")
         self.h = h
+        track(self)
+
+    _close_rank = 1
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
-            self.L.hc_adj_destroy(self.h)
+            if not _exiting:
+                self.L.hc_adj_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

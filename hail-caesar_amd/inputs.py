@@ -581,3 +581,51 @@ def ensemble_members(n: int = 64, lo: int = 128, hi: int = 1024, hours: int = 6,
                          "water_init_raster_file": "water_init.asc"})
         out.append((arrays, params_text_dict(over, rain_step_min=15, run_hours=hours)))
     return out
+
+
+def pad(a: np.ndarray, fill: float) -> np.ndarray:
+    """An interior (imax x jmax) array on the reference's padded layout
+    ((imax+2) x (jmax+2), a one-cell ring of `fill`)."""
+    out = np.full((a.shape[0] + 2, a.shape[1] + 2), fill, dtype=a.dtype)
+    out[1:-1, 1:-1] = a
+    return out
+
+
+def smooth_noise(rng: np.random.Generator, shape, sigma: float) -> np.ndarray:
+    """White noise convolved with a Gaussian of `sigma` cells (periodic FFT
+    convolution), scaled to unit max."""
+    xi = rng.standard_normal(shape)
+    ky = np.fft.fftfreq(shape[0])[:, None]
+    kx = np.fft.fftfreq(shape[1])[None, :]
+    g = np.exp(-2 * (np.pi * sigma) ** 2 * (kx ** 2 + ky ** 2))
+    s = np.real(np.fft.ifft2(np.fft.fft2(xi) * g))
+    return s / np.abs(s).max()
+
+
+def adjoint_case(size: int = 512, steps: int = 2000, seed: int = 1005, dx: float = 5.0,
+                 channel: float = 0.0,
+                 relief: float = 5.0, slope: float = 0.01, depth: float = 0.1,
+                 median_n: float = 0.035, sigma_n: float = 0.3, sigma_pert: float = 8.0):
+    """The adjoint workload: size^2 cells at dx, a fractal DEM (`relief` m,
+    roughness 0.6) on a `slope` to the south edge (with `channel` > 0, a
+    channel carved that deep down the middle column to the south edge),
+    log-normal Manning n, a uniform `depth` m start, 8 rain blocks (2 rows x
+    4 columns of the grid) under a rain rate of (10 + 30 sin^2(t/300)) mm/h
+    per step t, and the target's n field: n (1 + 0.1 s), s Gaussian-smoothed
+    noise (sigma_pert cells) scaled to unit max.  Returns a dict of interior
+    arrays (dem, n, n_target, water), the padded block map (-1 off the grid)
+    and the per-step rain rates in m/s."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dem = planar_dem(size, size, dx, slope, relief, 0.6, seed, channel_depth=channel)
+    # n is held at FP32 precision (the gradient is with respect to FP32 n)
+    n = lognormal_mannings(size, size, median_n, sigma_n, rng).astype(np.float32).astype(np.float64)
+    npert = n * (1 + 0.1 * smooth_noise(rng, (size, size), sigma_pert))
+    block = -np.ones((size + 2, size + 2), np.int32)
+    r = (np.arange(size) * 2) // size
+    c = (np.arange(size) * 4) // size
+    block[1:-1, 1:-1] = r[:, None] * 4 + c[None, :]
+    t = np.arange(steps)
+    rain = (10.0 + 30.0 * np.sin(t / 300.0) ** 2) / 3.6e6
+    water = np.where(dem > NODATA, depth, 0.0)
+    return {"dem": dem, "n": n, "n_target": npert, "water": water, "block": block,
+            "rain": rain, "DX": dx}

@@ -123,10 +123,14 @@ class Model:
             raise hc.HCError(rc, msg)
         self.imax = int(self.scalar("imax"))
         self.jmax = int(self.scalar("jmax"))
+        hc.track(self)
+
+    _close_rank = 2
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
-            self.L.hcm_destroy(self.h)
+            if not hc._exiting:
+                self.L.hcm_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):
@@ -214,10 +218,14 @@ class Ensemble:
         if rc != 0:
             raise hc.HCError(rc, "hc_ens_create failed")
         self.e = e
+        hc.track(self)
+
+    _close_rank = 0
 
     def close(self):
         if getattr(self, "e", None) and self.e.value:
-            self.L.hc_ens_destroy(self.e)
+            if not hc._exiting:
+                self.L.hc_ens_destroy(self.e)
             self.e = ctypes.c_void_p()
 
     def __del__(self):

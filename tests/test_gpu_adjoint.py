@@ -191,3 +191,127 @@ def test_autograd_function(case):
     gn, gm = adj.backward()
     assert np.array_equal(nt.grad.numpy(), gn)
     assert np.array_equal(mt.grad.numpy(), gm)
+
+
+# ---- the adjoint workload at its stated shape: 512^2 at 5 m, 2000 steps at
+# dt = 0.5 s, checkpoints every 100 steps (inputs.adjoint_case)
+BIG, BIG_STEPS = 512, 2000
+
+
+def _big(channel):
+    c = inputs.adjoint_case(size=BIG, steps=BIG_STEPS, channel=channel)
+    p = hc.default_params(hflow_threshold=0.001, edgeslope=0.01, spatially_var_mannings=1)
+    ctx = hc.Context(BIG, BIG, c["DX"], p, hc.HC_FP64)
+    ctx.upload("elev", pad(c["dem"], -9999.0))
+    ctx.upload("water_depth", pad(c["water"], 0.0))
+    ctx.upload("mannings", pad(c["n_target"], 0.0))
+    return c, ctx
+
+
+def _big_adjoint(c, ctx, nsteps):
+    adj = hc.Adjoint(ctx, nsteps, 0.5, c["block"], c["rain"][:nsteps], checkpoint_every=100)
+    _, qt = adj.forward(np.ones(8))
+    adj.set_target(qt)
+    n = pad(c["n"], 0.0)
+    ctx.upload("mannings", n)
+    return adj, n
+
+
+def _fd_blocks(adj, eps):
+    out = []
+    for b in range(8):
+        m = np.ones(8)
+        m[b] += eps
+        lp, _ = adj.forward(m)
+        m[b] -= 2 * eps
+        lm, _ = adj.forward(m)
+        out.append((lp - lm) / (2 * eps))
+    return np.array(out)
+
+
+def test_adjoint_workload_vs_central_differences():
+    """The 512^2 / 2000-step workload on its fractal DEM: the gradients with
+    respect to all 8 rain-block multipliers and to n along a smooth relative
+    direction agree with central differences at eps = 1e-4, all taken at the
+    multipliers 1."""
+    c, ctx = _big(channel=0.0)
+    adj, n = _big_adjoint(c, ctx, BIG_STEPS)
+    loss, q = adj.forward(np.ones(8))
+    gn, gm = adj.backward()
+    fd = _fd_blocks(adj, 1e-4)
+    v = pad(inputs.smooth_noise(np.random.Generator(np.random.PCG64(9)), (BIG, BIG), 8.0), 0.0)
+    eps = 1e-4
+    ctx.upload("mannings", n * (1 + eps * v))
+    lp, _ = adj.forward(np.ones(8))
+    ctx.upload("mannings", n * (1 - eps * v))
+    lm, _ = adj.forward(np.ones(8))
+    ctx.upload("mannings", n)
+    fdn = (lp - lm) / (2 * eps)
+    an = float(np.sum(gn * n * v))
+    print(f"loss {loss:.10e}")
+    for b in range(8):
+        print(f"block {b}: adjoint {gm[b]:.10e} fd {fd[b]:.10e} rel {abs(gm[b]-fd[b])/abs(fd[b]):.2e}")
+    print(f"n: adjoint {an:.10e} fd {fdn:.10e} rel {abs(an-fdn)/abs(fdn):.2e}")
+    # measured: blocks within 2.9e-4 (the O(eps) curvature of the central
+    # difference and a few faces crossing a limiter), n within 7e-5
+    assert loss > 0
+    assert np.all(np.abs(gm - fd) <= 1e-3 * np.abs(fd))
+    assert abs(an - fdn) <= 1e-3 * abs(fdn)
+    adj.close()
+    ctx.close()
+
+
+def test_adjoint_workload_forward_matches_oracle():
+    """The same 2000-step forward against the oracle driven call by call (the
+    rain added to each block's cells, flow_route, depth_update, water_flux_out
+    at dt = 0.5 s, scan_area every 5th step): discharge series to 1e-10."""
+    import os
+    c, ctx = _big(channel=0.0)
+    adj, n = _big_adjoint(c, ctx, BIG_STEPS)
+    _, q = adj.forward(np.ones(8))
+    params = inputs.params_text_dict({"spatially_variable_mannings_on": "yes",
+                                      "hflow_threshold": "0.001", "slope_on_edge_cell": "0.01"})
+    p = params_from_dict(params, BIG, BIG, c["DX"], nthreads=min(16, os.cpu_count() or 1))
+    o = OracleModel(p)
+    elev = pad(c["dem"], -9999.0)
+    o.field("elev")[...] = elev
+    o.field("mannings")[...] = n
+    h = o.field("water_depth")
+    h[...] = pad(c["water"], 0.0)
+    o.call("scan_area")
+    block, rain = c["block"], c["rain"]
+    live = (block >= 0) & (elev > -9999)
+    qo = np.zeros(BIG_STEPS)
+    for s in range(BIG_STEPS):
+        h[live] = h[live] + rain[s] * 0.5 * 1.0
+        o.scalar("time_step", 0.5)
+        o.scalar("maxdepth", 0.1)
+        o.call("flow_route")
+        o.call("depth_update")
+        qo[s] = o.call("water_flux_out")
+        if (s + 1) % 5 == 0:
+            o.call("scan_area")
+    np.testing.assert_allclose(q, qo, rtol=1e-10, atol=1e-12 * qo.max())
+    adj.close()
+    ctx.close()
+
+
+def test_adjoint_exact_in_a_chaotic_channel():
+    """The same workload with a 2 m channel carved down the middle: there the
+    fixed-dt scheme is chaotic (a 1e-9 change of one rain multiplier grows by
+    about 2 % per step to metres of depth), so the loss is not smooth at the
+    scale of eps = 1e-4 and the tangent is large.  After 1000 steps the
+    adjoint still equals the central difference taken inside the smooth
+    piece (eps = 1e-8) for every block: the reverse sweep is the exact
+    derivative of the forward it differentiates."""
+    c, ctx = _big(channel=2.0)
+    adj, n = _big_adjoint(c, ctx, 1000)
+    adj.forward(np.ones(8))
+    gn, gm = adj.backward()
+    fd8 = _fd_blocks(adj, 1e-8)
+    fd4 = _fd_blocks(adj, 1e-4)
+    for b in range(8):
+        print(f"block {b}: adjoint {gm[b]:.8e} fd(1e-8) {fd8[b]:.8e} fd(1e-4) {fd4[b]:.8e}")
+    assert np.all(np.abs(gm - fd8) <= 1e-3 * np.abs(gm) + 0.02)
+    adj.close()
+    ctx.close()
