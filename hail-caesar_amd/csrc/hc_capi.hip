@@ -492,9 +492,14 @@ int hc_add_point_inputs(hc_ctx* c, const int* x, const int* y, const double* amt
       return hc_fail(c, HC_ERR_ARG, "add_point_inputs: point outside the padded grid");
     idx[k] = x[k] * c->P + y[k];
   }
+  std::vector<int> sorted(idx);
+  std::sort(sorted.begin(), sorted.end());
+  const bool repeats = std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end();
   HC_HIP(c, hipMemcpyAsync(c->d_pts, idx.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HC_HIP(c, hipMemcpyAsync(c->d_amt, amt, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  int rc = hc_launch_points(c, n);
+  // distinct cells: one thread per point; repeated cells: in list order
+  int rc = repeats ? hc_launch_points_ordered(c, n, c->d_pts, nullptr, nullptr, c->d_amt)
+                   : hc_launch_points(c, n);
   if (rc) return rc;
   // the host vectors die at return: the copies above are from pageable
   // memory and complete before hipMemcpyAsync returns

@@ -830,6 +830,44 @@ __global__ void k_points(int n, const int* __restrict__ idx, const double* __res
   }
 }
 
+// Point inputs where a cell may appear more than once: the reference adds
+// the amounts one after another in list order (LCM.cpp:3463-3477).  Every
+// point scans the whole list through LDS; the first occurrence of a cell adds
+// its own amount and then those of the later points on that cell, in list
+// order, and writes the cell once; later occurrences write nothing.  Cells
+// come from idx (x*P+y) or, when idx is null, from the x and y lists.  Costs
+// n/256 LDS passes per block: for short lists and lists with repeats.
+template <typename T>
+__global__ __launch_bounds__(256) void k_points_ordered(int n, int P, const int* __restrict__ idx,
+                                                        const int* __restrict__ xs,
+                                                        const int* __restrict__ ys,
+                                                        const double* __restrict__ amt,
+                                                        T* __restrict__ h) {
+  __shared__ int s_cell[256];
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const bool live = k < n;
+  const int my = live ? (idx ? idx[k] : xs[k] * P + ys[k]) : -1;
+  bool first = live;
+  T v = T(0);
+  if (live) v = static_cast<T>(static_cast<double>(h[my]) + amt[k]);
+  for (int base = 0; base < n; base += 256) {
+    const int j = base + static_cast<int>(threadIdx.x);
+    __syncthreads();
+    s_cell[threadIdx.x] = (j < n) ? (idx ? idx[j] : xs[j] * P + ys[j]) : -2;
+    __syncthreads();
+    const int m = min(256, n - base);
+    for (int t = 0; t < m; ++t) {
+      if (s_cell[t] != my) continue;  // cells are >= 0: -1 and -2 never match
+      const int jj = base + t;
+      if (jj < k)
+        first = false;
+      else if (jj > k && first)
+        v = static_cast<T>(static_cast<double>(v) + amt[jj]);
+    }
+  }
+  if (first) h[my] = v;
+}
+
 // ------------------------------------------------------------- health
 template <typename T>
 __global__ __launch_bounds__(256) void k_health(int imax, int jmax, int P, const T* __restrict__ h,
@@ -1207,6 +1245,20 @@ int hc_launch_points(hc_ctx* c, int n) {
   else
     hipLaunchKernelGGL(k_points<float>, dim3(nb), dim3(256), 0, c->stream, n, c->d_pts,
                        c->d_amt, (float*)c->h[c->cur]);
+  HC_HIP(c, hipGetLastError());
+  return HC_OK;
+}
+
+int hc_launch_points_ordered(hc_ctx* c, int n, const int* idx, const int* xs, const int* ys,
+                             const double* amt) {
+  if (n <= 0) return HC_OK;
+  const int nb = (n + 255) / 256;
+  if (c->prec == HC_FP64)
+    hipLaunchKernelGGL(k_points_ordered<double>, dim3(nb), dim3(256), 0, c->stream, n, c->P, idx,
+                       xs, ys, amt, (double*)c->h[c->cur]);
+  else
+    hipLaunchKernelGGL(k_points_ordered<float>, dim3(nb), dim3(256), 0, c->stream, n, c->P, idx,
+                       xs, ys, amt, (float*)c->h[c->cur]);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

@@ -295,6 +295,10 @@ int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double*
 int hc_launch_scan_ptr(hc_ctx* c, const void* h, uint64_t* mask);
 int hc_launch_flux_out(hc_ctx* c, double ts);
 int hc_launch_points(hc_ctx* c, int n);
+// Point inputs applied in list order where cells repeat; cells from idx
+// (x*P+y), or from xs/ys when idx is null.
+int hc_launch_points_ordered(hc_ctx* c, int n, const int* idx, const int* xs, const int* ys,
+                             const double* amt);
 int hc_launch_health(hc_ctx* c);
 int hc_launch_fill(hc_ctx* c, void* p, size_t n, double v, bool is_float);
 // erosion (hc_erode.hip)

@@ -164,10 +164,13 @@ int hc_scan_area(hc_ctx* ctx);
 int hc_water_flux_out(hc_ctx* ctx, double time_step, double* temptot);
 
 /* Input-point injection of catchment_water_input_and_hydrology
- * (LCM.cpp:3463-3477): water_depth[x[k]][y[k]] += amt[k].  Points must be
- * distinct cells. */
+ * (LCM.cpp:3463-3477): water_depth[x[k]][y[k]] += amt[k] for k = 0..n-1.  A
+ * cell listed more than once takes each of its amounts in turn, in list
+ * order, as the reference's loop does.  Points must lie in the padded grid
+ * (0 <= x <= imax+1, 0 <= y <= jmax+1). */
 int hc_add_point_inputs(hc_ctx* ctx, const int* x, const int* y,
                         const double* amt, int n);
+
 
 /* LSDCatchmentModel::erode (LCM.cpp:4211-4845) with its grain helpers
  * (LCM.cpp:3874-4171).  *time_step is read and updated exactly as the
