@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -178,6 +179,31 @@ __global__ __launch_bounds__(256) void k_runoff_add(int imax, int jmax, int P, d
   h[c] = static_cast<T>(static_cast<double>(h[c]) + amt);
 }
 
+// Staging of hc_add_point_inputs_noalloc: the x list, the y list (int32) and
+// the amounts (double) of up to HC_NOALLOC_MAX_POINTS points.  Each device the
+// library runs on holds its own copy until the process ends.  In host code
+// `scratch` names a host-side stand-in: only the address resolved by
+// hipGetSymbolAddress (scratch_on_device) may reach a kernel or a copy.
+constexpr size_t kScratchBytes = static_cast<size_t>(HC_NOALLOC_MAX_POINTS) * 16;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kScratchDevices] = {};
+
+// This device's address of `scratch`, resolved once per device with the
+// device current (the caller makes it current).  Null on failure.
+unsigned char* scratch_on_device(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_dev[device] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_dev[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -262,6 +288,9 @@ int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
     return fail(rc);
   if (hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned), 8 * sizeof(double)) != hipSuccess)
     return fail(HC_ERR_HIP);
+  // the device current here is c->device
+  c->scratch = scratch_on_device(c->device);
+  if (!c->scratch) return fail(HC_ERR_HIP);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(HC_ERR_HIP);
   *out = c;
   return HC_OK;
@@ -504,6 +533,26 @@ int hc_add_point_inputs(hc_ctx* c, const int* x, const int* y, const double* amt
   // the host vectors die at return: the copies above are from pageable
   // memory and complete before hipMemcpyAsync returns
   return HC_OK;
+}
+
+int hc_add_point_inputs_noalloc(hc_ctx* c, const int* x, const int* y, const double* amt, int n) {
+  if (!c || n < 0 || (n > 0 && (!x || !y || !amt))) return HC_ERR_ARG;
+  if (n > HC_NOALLOC_MAX_POINTS)
+    return hc_fail(c, HC_ERR_ARG,
+                   "add_point_inputs_noalloc: more than HC_NOALLOC_MAX_POINTS (16384) points");
+  if (n == 0) return HC_OK;
+  if (!c->scratch) return hc_fail(c, HC_ERR_STATE, "add_point_inputs_noalloc: no staging array");
+  for (int k = 0; k < n; ++k)
+    if (x[k] < 0 || x[k] >= c->NR || y[k] < 0 || y[k] >= c->NC)
+      return hc_fail(c, HC_ERR_ARG, "add_point_inputs_noalloc: point outside the padded grid");
+  int* xs = reinterpret_cast<int*>(c->scratch);
+  int* ys = xs + HC_NOALLOC_MAX_POINTS;
+  double* am = reinterpret_cast<double*>(c->scratch + 8 * static_cast<size_t>(HC_NOALLOC_MAX_POINTS));
+  // pageable host sources: each copy has consumed the caller's array when it returns
+  HC_HIP(c, hipMemcpyAsync(xs, x, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HC_HIP(c, hipMemcpyAsync(ys, y, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HC_HIP(c, hipMemcpyAsync(am, amt, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return hc_launch_points_ordered(c, n, nullptr, xs, ys, am);
 }
 
 int hc_reach_inputs(hc_ctx* c, int n, const int* x, const int* y, const double* sediment,

@@ -240,6 +240,7 @@ struct hc_ctx {
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev;      // start/stop pairs around fused launches
   int prof_n = 0;
+  unsigned char* scratch = nullptr;     // this device's static staging array (hc_capi.hip)
   int* d_pts = nullptr;                 // point input cells (x*P+y)
   double* d_amt = nullptr;              // point input amounts
   int pts_cap = 0;

@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "libhc.so")
 HC_OK, HC_ERR_ARG, HC_ERR_HIP, HC_ERR_STATE, HC_ERR_UNSTABLE, HC_ERR_HALTED = 0, -1, -2, -3, -4, -5
 HC_FP64, HC_FP32 = 0, 1
 HC_WILCOCK, HC_EINSTEIN = 1, 2
+HC_NOALLOC_MAX_POINTS = 16384
 
 FIELDS = {
     "water_depth": 0, "elev": 1, "qx": 2, "qy": 3, "mannings": 4, "Vsusptot": 5,
@@ -34,7 +35,8 @@ INT_FIELDS = {"mask", "hasgrain"}
 # exported symbols (must match include/hc.h)
 SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upload",
            "hc_download", "hc_flow_route", "hc_depth_update", "hc_flow_depth",
-           "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs", "hc_erode",
+           "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs",
+           "hc_add_point_inputs_noalloc", "hc_erode",
            "hc_global_landsliding", "hc_creep", "hc_in_channel_landsliding",
            "hc_reach_inputs", "hc_runoff_set", "hc_runoff_add", "hc_synchronize", "hc_check_health", "hc_state_bytes", "hc_state_save",
            "hc_state_load", "hc_loop_begin", "hc_loop_run",
@@ -201,6 +203,7 @@ def lib():
         L.hc_scan_area.argtypes = [vp]
         L.hc_water_flux_out.argtypes = [vp, d, ctypes.POINTER(d)]
         L.hc_add_point_inputs.argtypes = [vp, vp, vp, vp, i]
+        L.hc_add_point_inputs_noalloc.argtypes = [vp, vp, vp, vp, i]
         L.hc_erode.argtypes = [vp, d, ctypes.POINTER(d), ctypes.POINTER(d), vp]
         L.hc_global_landsliding.argtypes = [vp, d, ctypes.POINTER(i)]
         L.hc_creep.argtypes = [vp, d, d]
@@ -373,6 +376,16 @@ class Context:
         y = np.ascontiguousarray(ys, dtype=np.int32)
         a = np.ascontiguousarray(amts, dtype=np.float64)
         self._check(self.L.hc_add_point_inputs(self.h, x.ctypes.data, y.ctypes.data, a.ctypes.data, len(x)))
+
+    def add_point_inputs_noalloc(self, xs: Sequence[int], ys: Sequence[int],
+                                 amts: Sequence[float]) -> None:
+        """hc_add_point_inputs_noalloc: the same adds, staged in the device's
+        static array (at most HC_NOALLOC_MAX_POINTS points)."""
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        y = np.ascontiguousarray(ys, dtype=np.int32)
+        a = np.ascontiguousarray(amts, dtype=np.float64)
+        self._check(self.L.hc_add_point_inputs_noalloc(self.h, x.ctypes.data, y.ctypes.data,
+                                                        a.ctypes.data, len(x)))
 
     def erode(self, mult: float, time_step: float):
         ts = ctypes.c_double(time_step)

@@ -171,6 +171,21 @@ int hc_water_flux_out(hc_ctx* ctx, double time_step, double* temptot);
 int hc_add_point_inputs(hc_ctx* ctx, const int* x, const int* y,
                         const double* amt, int n);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hc_add_point_inputs without any allocation.  Where
+ * hc_add_point_inputs stages the points in context buffers that it grows on
+ * demand, this stages them in a static device array of HC_NOALLOC_MAX_POINTS
+ * entries (256 KiB, one per device, held until the process ends); more points
+ * return HC_ERR_ARG.  Results are those of hc_add_point_inputs, repeated
+ * cells included.  Every context on one device shares that device's array
+ * and the copy into it is asynchronous: calls on one context are ordered by
+ * its stream, but calls on two contexts of one device race unless the caller
+ * orders them (hc_synchronize on the first context before calling on the
+ * second). */
+#define HC_NOALLOC_MAX_POINTS 16384
+int hc_add_point_inputs_noalloc(hc_ctx* ctx, const int* x, const int* y,
+                                const double* amt, int n);
 
 /* LSDCatchmentModel::erode (LCM.cpp:4211-4845) with its grain helpers
  * (LCM.cpp:3874-4171).  *time_step is read and updated exactly as the
