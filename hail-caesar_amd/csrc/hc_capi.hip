@@ -521,9 +521,18 @@ int hc_add_point_inputs(hc_ctx* c, const int* x, const int* y, const double* amt
       return hc_fail(c, HC_ERR_ARG, "add_point_inputs: point outside the padded grid");
     idx[k] = x[k] * c->P + y[k];
   }
-  std::vector<int> sorted(idx);
-  std::sort(sorted.begin(), sorted.end());
-  const bool repeats = std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end();
+  // repeated cells: test-and-set one bit per padded cell, then clear the
+  // bits set (O(n), the bitmap kept by the context)
+  if (c->pt_bits.size() != static_cast<size_t>(c->NR) * c->W)
+    c->pt_bits.assign(static_cast<size_t>(c->NR) * c->W, 0);
+  bool repeats = false;
+  for (int k = 0; k < n; ++k) {
+    uint64_t& w = c->pt_bits[static_cast<size_t>(x[k]) * c->W + (y[k] >> 6)];
+    const uint64_t b = 1ull << (y[k] & 63);
+    repeats = repeats || (w & b);
+    w |= b;
+  }
+  for (int k = 0; k < n; ++k) c->pt_bits[static_cast<size_t>(x[k]) * c->W + (y[k] >> 6)] = 0;
   HC_HIP(c, hipMemcpyAsync(c->d_pts, idx.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HC_HIP(c, hipMemcpyAsync(c->d_amt, amt, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   // distinct cells: one thread per point; repeated cells: in list order

@@ -244,6 +244,7 @@ struct hc_ctx {
   int* d_pts = nullptr;                 // point input cells (x*P+y)
   double* d_amt = nullptr;              // point input amounts
   int pts_cap = 0;
+  std::vector<uint64_t> pt_bits;        // host: repeated-point test, one bit per padded cell
 
   std::string err;
 };

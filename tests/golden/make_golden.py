@@ -10,6 +10,8 @@ Runs only where oracle/_ref/libhcref.so was built from the reference sources
   flow_run_64.npz             a whole 2 h 64^2 flow-only run: per-step
                               outlet discharge, clock and flow timestep, final
                               h/qx/qy
+  config1_run_256.npz         the same for the 256^2 configuration (final
+                              fields subsampled, with whole-field sums)
 The reference runs single-threaded (its OpenMP loops race on shared members
 with spatial n and in erode).
 
@@ -80,6 +82,30 @@ def full_run(w, name):
     print("wrote", name, len(tt), "steps")
 
 
+def config1_run(w, name, stride=4):
+    """The plain flow-only configuration at its stated size (256^2, 2 h):
+    per-step outlet discharge, clock and flow timestep over the whole run,
+    and the final h/qx/qy on every `stride`-th row and column of the padded
+    layout with whole-field sums (the full fields would be 1.5 MB).  The DEM
+    is regenerated from its seed by the tests; its SHA-256 is kept to check
+    that.  Uniform n and no erosion, so the reference's OpenMP loops have no
+    race and it runs on 8 threads."""
+    import hashlib
+    r = RefModel(w.path, w.params_file, threads=8)
+    tt, cy, dt = r.iterate(10 ** 7)
+    out = {"params_json": json.dumps(w.params), "DX": w.dx, "stride": stride,
+           "dem_sha256": hashlib.sha256(np.ascontiguousarray(w.dem).tobytes()).hexdigest(),
+           "temptot": tt, "cycle": cy, "dtflow": dt}
+    for f in ("water_depth", "qx", "qy"):
+        a = r.field(f).copy()
+        out[f + "_sub"] = a[::stride, ::stride]
+        out[f + "_sum"] = float(a.sum())
+        out[f + "_abssum"] = float(np.abs(a).sum())
+        out[f + "_max"] = float(a.max())
+    np.savez_compressed(os.path.join(HERE, name), **out)
+    print("wrote", name, len(tt), "steps")
+
+
 def erode_case(name, n, steps, law, ncalls=2):
     """A wet, steep grain-bearing state (all four edges open) after `steps`
     reference iterations in erosion mode, then `ncalls` erode() calls."""
@@ -113,6 +139,11 @@ def erode_case(name, n, steps, law, ncalls=2):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "config1":
+        with tempfile.TemporaryDirectory() as tmp:
+            config1_run(inputs.flow_only_256(os.path.join(tmp, "d"), size=256, hours=2),
+                        "config1_run_256.npz")
+        return
     if not RefModel.available():
         sys.exit("compiled reference not built (make -C oracle ref)")
     with tempfile.TemporaryDirectory() as tmp:
@@ -126,6 +157,8 @@ def main():
         per_call(w2, 300, "flow_step_spatial_n_48.npz")
         w3 = inputs.flow_only_256(os.path.join(tmp, "c"), size=64, hours=2)
         full_run(w3, "flow_run_64.npz")
+        w4 = inputs.flow_only_256(os.path.join(tmp, "d"), size=256, hours=2)
+        config1_run(w4, "config1_run_256.npz")
     erode_case("erode_wilcock_32.npz", 32, 60, "wilcock")
     erode_case("erode_einstein_24.npz", 24, 40, "einstein")
 

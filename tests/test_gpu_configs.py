@@ -1,0 +1,184 @@
+"""GPU: the benchmark configurations at their stated shapes.
+
+  * the plain flow-only configuration, 256^2 at 5 m for 2 simulated hours,
+    host-stepped and device-resident FP64, against the compiled reference's
+    own whole run (tests/golden/config1_run_256.npz, written by
+    make_golden.py): per-step outlet discharge, clock and flow timestep, and
+    the final h/qx/qy, to 1e-10;
+  * the large single grid, 4096^2 at 2 m, fully wet: the FP64 device loop
+    against the oracle for 40 loop bodies (the truncated oracle: the full 12 h
+    is 1.6e12 cell-steps), to 1e-10; then the FP32 storage form against the
+    FP64 form after 200 bodies, beside the FP64 form's own response to 1e-8 m
+    of DEM noise;
+  * the batched ensemble, 64 catchments of 128..1024 cells a side, stepped
+    together: its 4 smallest members against the oracle run on each alone,
+    FP64 to 1e-10, FP32 (fixed-point elevations) within the depth bar.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import hail_caesar_amd.inputs as inputs
+from hail_caesar_amd import hc
+from hail_caesar_amd.model import Ensemble, Model
+from pyoracle import OracleModel, pad, params_from_dict
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _oracle(arrays, params):
+    """The oracle on an in-memory workload (the arrays Model takes)."""
+    dem = arrays["dem"]
+    p = params_from_dict(params, dem.shape[0], dem.shape[1], float(arrays["DX"]), nthreads=THREADS)
+    o = OracleModel(p)
+    o.field("elev")[...] = pad(dem, -9999.0)
+    if arrays.get("mannings") is not None and p.spatially_var_mannings:
+        o.field("mannings")[1:-1, 1:-1] = arrays["mannings"]
+    if arrays.get("water_init") is not None:
+        o.field("water_depth")[1:-1, 1:-1] = arrays["water_init"]
+    o.set_rain(np.asarray(arrays["rain"], dtype=np.float64))
+    o.L.orc_initialise(o.h)
+    return o
+
+
+def _errs(a, b):
+    a, b = a[1:-1, 1:-1], b[1:-1, 1:-1]
+    d = np.abs(a - b)
+    s = float(np.abs(b).max())
+    return {"max": float(d.max() / s), "l2": float(np.sqrt((d ** 2).sum() / (b ** 2).sum())),
+            "frac_1e-4": float((d > 1e-4 * s).mean()), "vol": float(abs(a.sum() - b.sum()) / b.sum())}
+
+
+# ------------------------------------------------------------------ 256^2, 2 h
+@pytest.mark.parametrize("device_loop", [False, True])
+def test_config1_against_the_compiled_reference(tmp_path, device_loop):
+    g = np.load(os.path.join(GOLDEN, "config1_run_256.npz"))
+    w = inputs.flow_only_256(str(tmp_path / "c1"), size=256, hours=2)
+    assert hashlib.sha256(np.ascontiguousarray(w.dem).tobytes()).hexdigest() == str(g["dem_sha256"])
+    m = Model(w.path, w.params_file, device_loop=device_loop, use_graph=device_loop)
+    parts = []
+    while True:
+        parts.append(m.iterate(5000))
+        if m.finished:
+            break
+    tt, cy, dt = (np.concatenate([p[k] for p in parts]) for k in range(3))
+    assert len(tt) == len(g["temptot"]) == 2606
+    np.testing.assert_allclose(cy, g["cycle"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(dt, g["dtflow"], rtol=1e-10, atol=0)
+    np.testing.assert_allclose(tt, g["temptot"], rtol=1e-10, atol=1e-12 * g["temptot"].max())
+    s = int(g["stride"])
+    for f in ("water_depth", "qx", "qy"):
+        a = m.download(f)
+        np.testing.assert_allclose(a[::s, ::s], g[f + "_sub"], rtol=1e-10, atol=1e-13, err_msg=f)
+        for k, v in (("sum", a.sum()), ("abssum", np.abs(a).sum()), ("max", a.max())):
+            assert abs(v - float(g[f"{f}_{k}"])) <= 1e-10 * abs(float(g[f"{f}_{k}"])), (f, k)
+    m.close()
+
+
+# --------------------------------------------------------------- 4096^2, 2 m
+@pytest.fixture(scope="module")
+def grid4096():
+    return inputs.grid4096_arrays()
+
+
+def test_config3_fp64_against_the_oracle(grid4096):
+    """40 loop bodies from the fully wet state (the active mask is first
+    built at counter 5, so 36 of them route), every cell active."""
+    arrays, params = grid4096
+    m = Model(None, None, precision=hc.HC_FP64, arrays=arrays, params=params, device_loop=True,
+              use_graph=True)
+    tt, cy, dt = m.iterate(40)
+    o = _oracle(arrays, params)
+    ot, oc, od = o.iterate(40)
+    assert len(tt) == len(ot) == 40
+    np.testing.assert_allclose(cy, oc, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(dt, od, rtol=1e-10, atol=0)
+    np.testing.assert_allclose(tt, ot, rtol=1e-10, atol=1e-12 * max(ot.max(), 1.0))
+    for f in ("water_depth", "qx", "qy"):
+        a, b = m.download(f), o.field(f)
+        np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-13, err_msg=f)
+    assert m.download("mask")[1:-1, 1:-1].all()
+    m.close()
+
+
+def test_config3_fp32_against_fp64(grid4096):
+    """The FP32 storage form (fixed-point elevations) after 200 loop bodies
+    against the FP64 form, beside the FP64 form started from the DEM plus
+    1e-8 m of noise: the scheme's limiters (outflow > h/4 resets the flux to
+    h/5; the Froude cap) are discontinuous, so any perturbation flips them on
+    a few cells near a threshold.  Bars: depth max-norm (relative to the
+    largest depth) within the FP64 noise floor's or 1e-4; depth L2 and the
+    share of cells off by more than 1e-4 within the floor's or 1e-5; total
+    water volume to 1e-7."""
+    arrays, params = grid4096
+    steps = 200
+    runs = {}
+    noisy = dict(arrays)
+    dem = arrays["dem"]
+    live = dem > inputs.NODATA
+    noisy["dem"] = dem + np.where(live, 1e-8 * np.random.Generator(np.random.PCG64(3)).standard_normal(dem.shape), 0.0)
+    for name, prec, arr in (("fp64", hc.HC_FP64, arrays), ("fp32", hc.HC_FP32, arrays),
+                            ("floor", hc.HC_FP64, noisy)):
+        m = Model(None, None, precision=prec, arrays=arr, params=params, device_loop=True,
+                  use_graph=True, elev_fixed=prec == hc.HC_FP32)
+        m.iterate(steps)
+        runs[name] = {f: m.download(f) for f in ("water_depth", "qx", "qy")}
+        m.close()
+    res = {f: _errs(runs["fp32"][f], runs["fp64"][f]) for f in ("water_depth", "qx", "qy")}
+    floor = {f: _errs(runs["floor"][f], runs["fp64"][f]) for f in ("water_depth", "qx", "qy")}
+    print(f"fp32 vs fp64 after {steps} bodies: {res}\nfp64 with 1e-8 m DEM noise: {floor}")
+    # measured: FP32 max 1.75e-4, L2 3.2e-6, 6.6e-7 of cells over 1e-4,
+    # volume 2.3e-8; the FP64 noise floor max 2.7e-4, L2 3.4e-6, 7.7e-7
+    h, hf = res["water_depth"], floor["water_depth"]
+    assert h["max"] <= max(1e-4, hf["max"])
+    assert h["l2"] <= max(1e-5, hf["l2"])
+    assert h["frac_1e-4"] <= max(1e-5, hf["frac_1e-4"])
+    assert h["vol"] <= 1e-7
+
+
+# ------------------------------------------------------- 64-member ensemble
+@pytest.fixture(scope="module")
+def members():
+    return inputs.ensemble_members(64, hours=6, wet_depth=0.1)
+
+
+@pytest.mark.parametrize("prec", [hc.HC_FP64, hc.HC_FP32])
+def test_config4_smallest_members_against_the_oracle(members, prec):
+    """All 64 members stepped together for 200 ensemble steps (each with its
+    own adaptive dt); the 4 smallest against the oracle run alone for the
+    same loop bodies.  FP64: fields and series to 1e-10.  FP32 with
+    fixed-point elevations: depth within 1e-4 of the largest depth (max-norm)
+    and discharge series within 1e-4 relative."""
+    steps = 200
+    fp32 = prec == hc.HC_FP32
+    ms = [Model(None, None, precision=prec, arrays=a, params=p, device_loop=True, elev_fixed=fp32)
+          for a, p in members]
+    ens = Ensemble(ms)
+    res = ens.iterate(steps, use_graph=True)
+    sizes = [a["dem"].shape[0] for a, _ in members]
+    for k in np.argsort(sizes, kind="stable")[:4]:
+        arrays, params = members[k]
+        o = _oracle(arrays, params)
+        ot, oc, od = o.iterate(steps)
+        tt, cy, dt = res[k]
+        assert len(tt) == len(ot) == steps, k
+        if not fp32:
+            np.testing.assert_allclose(cy, oc, rtol=1e-12, atol=0)
+            np.testing.assert_allclose(dt, od, rtol=1e-10, atol=0)
+            np.testing.assert_allclose(tt, ot, rtol=1e-10, atol=1e-12 * max(ot.max(), 1.0))
+            for f in ("water_depth", "qx", "qy"):
+                np.testing.assert_allclose(ms[k].download(f), o.field(f), rtol=1e-10, atol=1e-13,
+                                           err_msg=f"member {k} {f}")
+        else:
+            e = _errs(ms[k].download("water_depth"), o.field("water_depth"))
+            print(f"member {k} ({sizes[k]}^2) fp32 depth error: {e}")
+            assert e["max"] <= 1e-4, (k, e)
+            np.testing.assert_allclose(tt, ot, rtol=1e-4, atol=1e-4 * max(ot.max(), 1e-6))
+    ens.close()
+    for m in ms:
+        m.close()

@@ -256,6 +256,27 @@ def test_oracle_full_run_golden():
         assert np.array_equal(o.field(f), g[f]), f
 
 
+def test_oracle_config1_run_vs_reference_golden(tmp_path):
+    """The plain flow-only configuration at its stated size (256^2, 2 h, 2606
+    steps) run by the compiled reference on 8 threads: the oracle reproduces
+    every step's outlet discharge, clock and flow timestep, and the final
+    fields, bit for bit."""
+    import hashlib
+    import hail_caesar_amd.inputs as inputs
+    g = _golden("config1_run_256.npz")
+    w = inputs.flow_only_256(str(tmp_path / "c1"), size=256, hours=2)
+    assert hashlib.sha256(np.ascontiguousarray(w.dem).tobytes()).hexdigest() == str(g["dem_sha256"])
+    o = OracleModel.from_workload(w, nthreads=8)
+    tt, cy, dt = o.iterate(10 ** 7)
+    assert np.array_equal(tt, g["temptot"]) and np.array_equal(cy, g["cycle"])
+    assert np.array_equal(dt, g["dtflow"])
+    s = int(g["stride"])
+    for f in ("water_depth", "qx", "qy"):
+        a = o.field(f)
+        assert np.array_equal(a[::s, ::s], g[f + "_sub"]), f
+        assert float(a.sum()) == float(g[f + "_sum"]) and float(a.max()) == float(g[f + "_max"]), f
+
+
 # ------------------------------------------------------------ closed forms
 def _flat_oracle(nrows, ncols, dx=5.0, **over):
     p = dict(inputs.BASE_PARAMS)
