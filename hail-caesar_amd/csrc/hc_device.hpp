@@ -33,6 +33,20 @@ __device__ __forceinline__ void st_elem(T* base, uint32_t i, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<uint32_t>(sizeof(T))) = v;
 }
 
+// The same with the nontemporal hint: data streamed through once per launch
+// (the fused kernel's outputs, and the inputs no neighbouring tile reads
+// again), kept from displacing the rows that are read twice.
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* base, uint32_t i) {
+  return __builtin_nontemporal_load(reinterpret_cast<const T*>(
+      reinterpret_cast<const char*>(base) + i * static_cast<uint32_t>(sizeof(T))));
+}
+template <typename T>
+__device__ __forceinline__ void st_nt(T* base, uint32_t i, T v) {
+  __builtin_nontemporal_store(
+      v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<uint32_t>(sizeof(T))));
+}
+
 // A value materialised in a register at this point of the program (the
 // compiler may not recompute it at, or sink it into, each use after here).
 __device__ __forceinline__ float pin_value(float v) {

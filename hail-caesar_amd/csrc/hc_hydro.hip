@@ -256,6 +256,10 @@ struct FusedPtrs {
   T* qx1;
   T* qy1;
   unsigned long long* red;
+  // device loop: one maximum depth per wavefront (slot 4 * block + wavefront),
+  // reduced into red by the scan kernel that follows; null: a block maximum
+  // and one conditional atomic on red per block
+  double* wmax;
   hc::InjectArgs inj;  // rain input at the input points (device loop only)
 };
 
@@ -466,15 +470,15 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     const uint32_t o = off(k, r0);
     hh[k] = ld_elem(h0, o);
     zt[k] = ld_elem(zz, o);
-    if (k >= 1) {
-      qxo[k] = ld_elem(qx0, o);
-      qyo[k] = ld_elem(qy0, o);
+    if (k >= 1) {  // the fluxes' only re-read is the tile below's first row
+      qxo[k] = ld_nt(qx0, o);
+      qyo[k] = ld_nt(qy0, o);
     }
   };
   auto load_n = [&](int k0, int r0) {
     if (a.spatial_n) {
 #pragma unroll
-      for (int k = k0; k < R + 2; ++k) nn[k] = ld_elem(nman, off(k, r0));
+      for (int k = k0; k < R + 2; ++k) nn[k] = ld_nt(nman, off(k, r0));
     } else {
 #pragma unroll
       for (int k = k0; k < R + 2; ++k) nn[k] = nu;
@@ -650,9 +654,9 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       }
       if (owned && xo < NR) {
         const uint32_t o = off(k - 1, r0);
-        st_elem(h1, o, hn);
-        st_elem(qx1, o, qxNp);
-        st_elem(qy1, o, qyNp);
+        st_nt(h1, o, hn);
+        st_nt(qx1, o, qxNp);
+        st_nt(qy1, o, qyNp);
       }
     }
     qxNp = qxN;
@@ -717,7 +721,13 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       for (int k = 2; k < R + 2; ++k) row(win, k, r0);
     }
   }
-  block_max_pos(red, owned && active_wave ? static_cast<double>(md) : 0.0);
+  const double mine = owned && active_wave ? static_cast<double>(md) : 0.0;
+  if (p.wmax) {  // uniform
+    const double wm = wave_max(mine);
+    if (lane == 0) p.wmax[4 * blockIdx.x + wv] = wm;
+  } else {
+    block_max_pos(red, mine);
+  }
   if (drains) {  // uniform per block
     __shared__ double s_dw[4];
     dacc = wave_sum(dacc);
@@ -782,6 +792,7 @@ __global__ __launch_bounds__(256) void k_ens_fused(const EnsMember* __restrict__
     p.qx1 = table_ptr<T>(M.qx[g ^ 1]);
     p.qy1 = table_ptr<T>(M.qy[g ^ 1]);
     p.red = table_ptr<unsigned long long>(M.red);
+    p.wmax = nullptr;
     p.inj.pmask = table_ptr<const uint64_t>(M.pmask);
     p.inj.rf_uniform = M.rf_uniform;
     p.inj.rfarea = table_ptr<const int>(M.rfarea);
@@ -998,7 +1009,7 @@ void hc_fused_geometry(const hc_ctx* c, int R, int sw, int* nbx, int* nby) {
 }
 
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* haltp, bool swap,
-                         const DrainArgs* drain, const hc::InjectArgs* inject) {
+                         const DrainArgs* drain, const hc::InjectArgs* inject, double* wmax) {
   if (c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
   if (!hc::fused_offsets_fit(c->NR, c->P))
@@ -1009,6 +1020,8 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
   int nbx, nby;
   hc_fused_geometry(c, R * ns, hc_fused_stride(c), &nbx, &nby);
   const int nblocks = nbx * nby;
+  if (wmax && static_cast<size_t>(4) * nblocks > c->wmax_cap)
+    return hc_fail(c, HC_ERR_STATE, "flow_depth: wavefront maxima buffer too small");
   const DrainArgs dr = drain ? *drain : DrainArgs{};
   if (dr.ts && static_cast<size_t>(hc::EdgeSlots(nbx, R * ns, c->g.imax, c->g.jmax,
                                                   hc_fused_stride(c)).count(nby)) >
@@ -1036,6 +1049,7 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     p.qx1 = (T*)c->qx[o];
     p.qy1 = (T*)c->qy[o];
     p.red = c->d_red;
+    p.wmax = wmax;
     p.inj = inject ? *inject : hc::InjectArgs{};
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, ns, p,
                        dtp, haltp, dr);
@@ -1177,6 +1191,7 @@ int hc_launch_fused64_ptrs(hc_ctx* c, double dt, const double* h0, const double*
   p.qx1 = qx1;
   p.qy1 = qy1;
   p.red = c->d_red;
+  p.wmax = nullptr;
   p.inj = hc::InjectArgs{};
   const DrainArgs dr{};
   auto go = [&](auto kern) {

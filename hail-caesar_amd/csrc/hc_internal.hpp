@@ -220,6 +220,8 @@ struct hc_ctx {
   unsigned int* d_count = nullptr;      // last-block counters
   double* d_dpart = nullptr;            // fused-kernel edge drain: one slot per tile
   size_t dpart_cap = 0;
+  double* d_wmax = nullptr;             // device loop: fused kernel's per-wavefront maxdepth
+  size_t wmax_cap = 0;
   double* d_out = nullptr;              // [0] temptot
   double* h_pinned = nullptr;           // pinned host mirror (8 doubles)
   // device-resident loop
@@ -275,7 +277,7 @@ int hc_launch_depth_update(hc_ctx* c, double dt);
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
                          const int* haltp = nullptr, bool swap = true,
                          const hc::DrainArgs* drain = nullptr,
-                         const hc::InjectArgs* inject = nullptr);
+                         const hc::InjectArgs* inject = nullptr, double* wmax = nullptr);
 // Geometry of the fused kernel's launch for the context's tile rows.
 // Columns per wavefront of this context's single-grid fused kernel.
 int hc_fused_stride(const hc_ctx* c);

@@ -265,12 +265,20 @@ __global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restri
                                                    uint64_t* __restrict__ pmask,
                                                    const double* __restrict__ partials,
                                                    int nby, int nbx, int R, int sw,
-                                                   LoopRecord* __restrict__ rec) {
+                                                   LoopRecord* __restrict__ rec,
+                                                   const double* __restrict__ wmax, int nwmax,
+                                                   unsigned long long* __restrict__ red) {
   // the control kernel alone sets halt, so the fused kernel of this
   // iteration ran exactly when it is still clear here
   const bool live = !ctl->halt && !ctl->err;
   const bool scan = live && (ctl->counter % a.scan_interval) == 0;
   const bool rebuild = live && ctl->pts_dirty;
+  if (live) {  // uniform: the fused kernel's per-wavefront maxima into maxdepth
+    double m = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nwmax; i += gridDim.x * 256)
+      m = fmax(m, wmax[i]);
+    block_max_pos(red, m);
+  }
   if (live && blockIdx.x == 0)
     finish_drain(ctl, rec, partials, nby, nbx, R, a.imax, a.jmax, sw);
   if (!scan && !rebuild) return;  // uniform over the grid
@@ -371,7 +379,8 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   inj.rfarea = c->d_rfarea;
   inj.amt = c->d_ctl->amt;
   inj.rf_uniform = c->rf_uniform;
-  int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt, true, &dr, &inj);
+  int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt, true, &dr, &inj,
+                                c->d_wmax);
   if (rc) return rc;
   if (prof) {
     HC_HIP(c, hipEventRecord(c->prof_ev[2 * k_in_run + 1], c->stream));
@@ -384,14 +393,15 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   int nbx, nby;
   const int sw = hc_fused_stride(c);
   hc_fused_geometry(c, R, sw, &nbx, &nby);
+  const int nwmax = 4 * nbx * nby;
   if (c->prec == HC_FP64)
     hipLaunchKernelGGL(k_loop_scan<double>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
                        (const double*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
-                       c->d_dpart, nby, nbx, R, sw, c->d_rec);
+                       c->d_dpart, nby, nbx, R, sw, c->d_rec, c->d_wmax, nwmax, c->d_red);
   else
     hipLaunchKernelGGL(k_loop_scan<float>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
                        (const float*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
-                       c->d_dpart, nby, nbx, R, sw, c->d_rec);
+                       c->d_dpart, nby, nbx, R, sw, c->d_rec, c->d_wmax, nwmax, c->d_red);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
@@ -458,6 +468,18 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
       c->d_dpart = nullptr;
       HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_dpart), need * sizeof(double)));
       c->dpart_cap = need;
+    }
+    // per-wavefront maxima of the single-grid fused kernel: 4 per block at
+    // the most blocks any tiling makes (4 rows per block)
+    int nbx, nby;
+    hc_fused_geometry(c, 4, hc_fused_stride(c), &nbx, &nby);
+    const size_t wneed = static_cast<size_t>(4) * nbx * nby;
+    if (wneed > c->wmax_cap) {
+      if (c->d_wmax) (void)hipFree(c->d_wmax);
+      c->d_wmax = nullptr;
+      hc_drop_loop_graphs(c);
+      HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_wmax), wneed * sizeof(double)));
+      c->wmax_cap = wneed;
     }
   }
   if (c->d_top) (void)hipFree(c->d_top);
