@@ -89,7 +89,7 @@ def run_gpu(precision: str, arrays, params, steps: int, warmup: int, use_graph: 
         cands = [tuple(tiles)]
     else:
         rows, strip, tune_us = ctx.tune_tiles(0.05)
-        cands = [(rows, strip), (4, 1), (8, 1), (4, 2), (8, 2), (16, 1)]
+        cands = [(rows, strip), (4, 2), (8, 1), (4, 4), (8, 2), (16, 1)]
     loop_us = {}
     m.set_graph(False)
     for rs in dict.fromkeys(cands):

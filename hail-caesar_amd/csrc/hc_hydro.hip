@@ -18,6 +18,8 @@
 #include <cmath>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "hc_device.hpp"
 #include "hc_internal.hpp"
 
@@ -1009,7 +1011,8 @@ void hc_fused_geometry(const hc_ctx* c, int R, int sw, int* nbx, int* nby) {
 }
 
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* haltp, bool swap,
-                         const DrainArgs* drain, const hc::InjectArgs* inject, double* wmax) {
+                         const DrainArgs* drain, const hc::InjectArgs* inject, double* wmax,
+                         hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
   if (!hc::fused_offsets_fit(c->NR, c->P))
@@ -1051,8 +1054,12 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     p.red = c->d_red;
     p.wmax = wmax;
     p.inj = inject ? *inject : hc::InjectArgs{};
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, ns, p,
-                       dtp, haltp, dr);
+    if (ev_start)  // the events time the kernel's own execution (dispatch packet timestamps)
+      hipExtLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, ev_start, ev_stop, 0, a,
+                            dt, nbx, nblocks, ns, p, dtp, haltp, dr);
+    else
+      hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, a, dt, nbx, nblocks, ns, p,
+                         dtp, haltp, dr);
   };
   double* td = nullptr;
   float* tf = nullptr;

@@ -277,7 +277,8 @@ int hc_launch_depth_update(hc_ctx* c, double dt);
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
                          const int* haltp = nullptr, bool swap = true,
                          const hc::DrainArgs* drain = nullptr,
-                         const hc::InjectArgs* inject = nullptr, double* wmax = nullptr);
+                         const hc::InjectArgs* inject = nullptr, double* wmax = nullptr,
+                         hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // Geometry of the fused kernel's launch for the context's tile rows.
 // Columns per wavefront of this context's single-grid fused kernel.
 int hc_fused_stride(const hc_ctx* c);

@@ -369,7 +369,6 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
                      c->d_pmask, c->d_red, c->d_rec);
   HC_HIP(c, hipGetLastError());
   const bool prof = c->prof_on && k_in_run >= 0 && 2 * k_in_run + 1 < static_cast<int>(c->prof_ev.size());
-  if (prof) HC_HIP(c, hipEventRecord(c->prof_ev[2 * k_in_run], c->stream));
   hc::DrainArgs dr;
   dr.ts = &c->d_ctl->time_step;
   dr.thr = c->prm.water_depth_erosion_threshold;
@@ -380,12 +379,10 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   inj.amt = c->d_ctl->amt;
   inj.rf_uniform = c->rf_uniform;
   int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt, true, &dr, &inj,
-                                c->d_wmax);
+                                c->d_wmax, prof ? c->prof_ev[2 * k_in_run] : nullptr,
+                                prof ? c->prof_ev[2 * k_in_run + 1] : nullptr);
   if (rc) return rc;
-  if (prof) {
-    HC_HIP(c, hipEventRecord(c->prof_ev[2 * k_in_run + 1], c->stream));
-    c->prof_n = k_in_run + 1;
-  }
+  if (prof) c->prof_n = k_in_run + 1;
   const int o = c->cur;  // flow_depth flipped the generation
   const int items = hcdev::scan_items(c->g.imax, c->W, static_cast<int>(c->esz));
   const int nb = std::max(1, std::min((items + 3) / 4, 1024));

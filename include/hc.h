@@ -342,9 +342,11 @@ int hc_set_tiles(hc_ctx* ctx, int rows, int strip);
 int hc_bench_fused(hc_ctx* ctx, double dt_flow, int reps, double* us);
 
 /* Kernel timing for the benchmark: with profiling on, hc_loop_run (without
- * graphs) brackets every fused flow_route+depth_update launch with HIP events
- * on the context stream; hc_loop_kernel_time returns their summed duration
- * (ms) and count for the last run. */
+ * graphs) launches every fused flow_route+depth_update kernel with a start
+ * and a stop HIP event on the context stream, taken from the kernel's own
+ * dispatch (hipExtLaunchKernel), so they time its execution alone;
+ * hc_loop_kernel_time returns their summed duration (ms) and count for the
+ * last run. */
 int hc_loop_profile(hc_ctx* ctx, int on);
 int hc_loop_kernel_time(hc_ctx* ctx, double* total_ms, int* launches);
 
