@@ -429,6 +429,31 @@ def erosion_1024(path: str, size: int = 1024, hours: int = 24, seed: int = 1002)
     return w
 
 
+def erosion_run(path: str, size: int = 48, hours: int = 1, seed: int = 1002,
+                failure_angle: float = 8.0, law: str = "wilcock") -> Workload:
+    """A whole flow + erosion run small enough to compare step by step: size^2
+    at 10 m on a 2 % slope with 20 m of fractal relief and a 1 m channel, all
+    four edges open, 9 grain fractions (d1 = 0.0005 m) in seeded Dirichlet
+    surface proportions, a 1 m active layer, a uniform 0.1 m start, a
+    triangular storm peaking at 40 mm/h over the run, and a low slope failure
+    angle so that the hillslope processes move material (creep and global
+    landsliding fire on the first iteration, in-channel landsliding every 10
+    iterations).  The grain sizes go to the reference by its harness, to the
+    oracle by extra['grain_d'] and to this build's driver by hc_grain_sizes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dem = planar_dem(size, size, 10.0, 0.02, 20.0, 0.6, seed, channel_depth=1.0,
+                     south_only=False)
+    props = dirichlet_props(size, size, rng)
+    w = make_workload(path, dem, 10.0, triangular_storm(hours, 40.0, 15), 15, hours,
+                      overrides={"hydro_model_only": "no", "active_layer_thickness": "1",
+                                 "topmodel_m_value": "0.0005", "transport_law": law,
+                                 "slope_failure_thresh": repr(failure_angle),
+                                 "hc_grain_sizes": ",".join(repr(d) for d in EROSION_GRAIN_D)},
+                      water_init=np.where(dem > NODATA, 0.1, 0.0), graindata=props, name="erun")
+    w.extra["grain_d"] = EROSION_GRAIN_D
+    return w
+
+
 def complex_rain(path: str, rfnum: int = 1, size: int = 40, seed: int = 1021,
                  hours: int = 1) -> Workload:
     """spatially_complex_rainfall_on (a per-cell TOPMODEL): size^2 at 5 m,

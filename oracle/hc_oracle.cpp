@@ -76,7 +76,7 @@ struct orc_model {
   std::vector<std::vector<float>> reach_rows[3];
   // scalars (LCM.hpp:456-548 defaults)
   double time_step = 1, cycle = 0, maxdepth = 10, temptot = 0, waterinput = 0,
-         waterOut = 0, iod = 0, baseflow = 0.00000005, time_1 = 1,
+         waterOut = 0, iod = 0, baseflow = 0.00000005, time_1 = 1, creep_time = 1, creep_time2 = 1,
          previous = 0, new_cycle = 0, old_cycle = 0, erode_mult = 1,
          erode_call = 0, globalsediq = 0, sediQ = 0, rain_factor = 1;
   int counter = 0;
@@ -1220,6 +1220,17 @@ struct orc_model {
     if ((counter % p.scan_interval) == 0) scan_area();      // LCM.cpp:2019-2025
     if (!p.hydro_only) call_erosion();
     water_flux_out();
+    if (!p.hydro_only) {  // main.cpp:185-187 with main.cpp:127-132's intervals
+      if ((counter % 10) == 0) in_channel_landsliding(p.failureangle);  // LCM.cpp:1968-1974
+      if (cycle > creep_time) {                                          // LCM.cpp:1976-1984
+        creep_time += 14400;
+        creep(0.028, p.creep_rate);
+      }
+      if (cycle > creep_time2) {                                         // LCM.cpp:1986-1996
+        creep_time2 += 1440;
+        global_landsliding(p.failureangle);
+      }
+    }
     temptotal = temptot;                                     // LCM.cpp:1952-1966
     output_data();
     if (tt) *tt = temptot;
@@ -1346,6 +1357,7 @@ int orc_scalar(orc_model* m, const char* name, double* value, int set) {
   ORC_D("temptot", temptot) ORC_D("waterinput", waterinput) ORC_D("waterOut", waterOut)
   ORC_D("input_output_difference", iod) ORC_D("baseflow", baseflow)
   ORC_D("time_1", time_1) ORC_D("previous", previous) ORC_D("erode_mult", erode_mult)
+  ORC_D("creep_time", creep_time) ORC_D("creep_time2", creep_time2)
   ORC_D("erode_call", erode_call) ORC_D("globalsediq", globalsediq) ORC_D("sediQ", sediQ)
   ORC_D("new_cycle", new_cycle) ORC_D("old_cycle", old_cycle)
   ORC_D("Qw_newvol", Qw_newvol) ORC_D("Jw_newvol", Jw_newvol) ORC_D("Jw_overvol", Jw_overvol)
@@ -1385,6 +1397,8 @@ void orc_initialise(orc_model* m) {
     for (int j = 1; j < p.jmax; ++j)
       if (m->elev[m->at(i, j)] > -9999) m->nActual[m->rfarea[m->at(i, j)]]++;
   m->time_1 = m->cycle;  // set_time_counters, LCM.cpp:1862-1871
+  m->creep_time = m->cycle;
+  m->creep_time2 = m->cycle;
   if (p.spatially_complex_rainfall) {
     // initialise_rainfall_runoff and initialise_drainage_area (LCM.cpp:2041-2069):
     // the per-cell TOPMODEL only; no drainage area or input points

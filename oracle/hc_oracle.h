@@ -57,6 +57,8 @@ typedef struct orc_params {
   int reach_mode;             /* reach_mode (LCM.cpp:1456-1459) */
   int div_inputs;             /* divide_inputs_by, held as int (LCM.hpp:685) */
   int reach_input_data_timestep; /* minutes, int (LCM.hpp:449) */
+  double creep_rate;          /* creep_rate, CREEP_RATE (LCM.hpp:499, LCM.cpp:1547-1551) */
+  double failureangle;        /* slope_failure_thresh (LCM.hpp:470, LCM.cpp:1553-1558) */
 } orc_params;
 
 typedef struct orc_model orc_model;

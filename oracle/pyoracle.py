@@ -52,6 +52,7 @@ class orc_params(ctypes.Structure):
         ("spatially_complex_rainfall", ctypes.c_int), ("no_data_value", ctypes.c_double),
         ("reach_mode", ctypes.c_int), ("div_inputs", ctypes.c_int),
         ("reach_input_data_timestep", ctypes.c_int),
+        ("creep_rate", ctypes.c_double), ("failureangle", ctypes.c_double),
     ]
 
 
@@ -140,6 +141,8 @@ def params_from_dict(p: Dict[str, str], imax: int, jmax: int, dx: float,
     o.reach_mode = 1 if g("reach_mode", "no") == "yes" else 0
     o.div_inputs = int(_atof(g("divide_inputs_by", "1")))
     o.reach_input_data_timestep = _atoi(g("reach_input_data_timestep", "60"))
+    o.creep_rate = _atof(g("creep_rate", "0.0025"))
+    o.failureangle = _atof(g("slope_failure_thresh", "45"))
     return o
 
 

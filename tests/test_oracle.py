@@ -416,6 +416,34 @@ def test_oracle_erode_golden(name):
 
 
 @pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("law", ["wilcock", "einstein"])
+def test_oracle_erosion_run_vs_reference(tmp_path, law):
+    """A whole flow + erosion run (48^2, 1 simulated hour, about 2600
+    iterations): the loop body of src/main.cpp:139-204 with call_erosion and
+    the three hillslope processes on their triggers (LCM.cpp:1968-1996:
+    in-channel landsliding every 10 iterations, creep and global landsliding
+    on the first iteration, where they move material: up to 2.4 m at the 8
+    degree failure angle).  The compiled reference runs single-threaded (its
+    erosion loops race); every step's outlet discharge, clock and flow
+    timestep and the final state are bitwise equal."""
+    w = inputs.erosion_run(str(tmp_path / "er"), law=law)
+    r = RefModel(w.path, w.params_file, threads=1)
+    r.scalar("d1", inputs.EROSION_GRAIN_D[0])
+    rt, rc, rd = r.iterate(10 ** 7)
+    o = OracleModel.from_workload(w, nthreads=4)
+    ot, oc, od = o.iterate(10 ** 7)
+    assert len(rt) == len(ot) > 2000
+    assert np.array_equal(ot, rt) and np.array_equal(oc, rc) and np.array_equal(od, rd)
+    s = r.snapshot()
+    for f in ("elev", "water_depth", "qx", "qy", "Vsusptot", "Tau", "Vel", "ss", "grain"):
+        assert np.array_equal(o.field(f), s[f]), f
+    assert np.array_equal(o.field("strata").reshape(s["strata"].shape), s["strata"])
+    assert np.array_equal(o.field_int("hasgrain"), s["hasgrain"])
+    assert o.scalar("globalsediq") == s["globalsediq"]
+    assert np.abs(s["elev"][1:-1, 1:-1] - w.dem).max() > 1.0  # the landslides moved material
+
+
+@pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
 def test_oracle_grain_helpers_vs_reference(tmp_path):
     """d50, sand_fraction, sort_active, addGS and slide_GS one call at a time
     on the reference's own grain state (LSDCatchmentModel.cpp:3874-4171)."""
