@@ -258,36 +258,75 @@ def run_adjoint(size: int = 512, steps: int = 2000, dt: float = 0.5, every: int 
 
 
 def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50):
-    """Flow + erosion (FP64, host-stepped as the reference's loop): size^2 at
-    10 m, fractal relief 20 m on a 1 % slope, 9 grain fractions (d = 0.0005
-    .. 0.128 m) in the reference's default surface proportions, Wilcock-Crowe,
-    1 m active layer, a 40 mm/h storm, from a uniform 0.2 m start so that
-    flow and entrainment are active everywhere.  Times whole loop bodies
-    (flow_route, depth_update, scan, erode at the reference's erosion
-    interval, water_flux_out, output)."""
+    """Flow + erosion (host-stepped as the reference's loop): size^2 at 10 m,
+    fractal relief 20 m on a 1 % slope with a carved channel, 9 grain
+    fractions (d = 0.0005 .. 0.128 m) in seeded Dirichlet surface proportions
+    per cell, Wilcock-Crowe, 1 m active layer, a 6 h triangular storm peaking
+    at 40 mm/h, from a uniform 0.2 m start so that flow and entrainment are
+    active everywhere.  Times whole loop bodies (flow_route, depth_update,
+    scan, erode at the reference's erosion interval, the hillslope processes
+    on their triggers, water_flux_out, output) in the FP64 form and in the FP32
+    storage form, then reports the FP32 form's per-field error against the FP64
+    form after the same loop bodies."""
     from hail_caesar_amd import hc, inputs
     from hail_caesar_amd.model import Model
     dem = inputs.planar_dem(size, size, 10.0, 0.01, 20.0, 0.6, 1002)
+    props = inputs.dirichlet_props(size, size, np.random.Generator(np.random.PCG64(1002)))
     arrays = {"dem": dem, "water_init": np.where(dem > inputs.NODATA, 0.2, 0.0),
               "rain": np.vstack([inputs.triangular_storm(6, 40.0, 15)[:, None],
-                                 np.zeros((200, 1))]).astype(np.float32), "DX": 10.0}
+                                 np.zeros((200, 1))]).astype(np.float32), "DX": 10.0,
+              "graindata": inputs.graindata_array(props, 1.0)}
     params = inputs.params_text_dict({
         "hydro_model_only": "no", "active_layer_thickness": "1", "transport_law": "wilcock",
         "water_init_from_raster_on": "yes", "water_init_raster_file": "water_init.asc",
+        "read_in_graindata_from_file": "yes", "grain_data_file": "grain.dat",
         "hc_grain_sizes": ",".join(repr(d) for d in inputs.EROSION_GRAIN_D)},
         rain_step_min=15, run_hours=24)
-    m = Model(None, None, precision=hc.HC_FP64, arrays=arrays, params=params)
-    m.iterate(warmup)
-    m.synchronize()
-    t = time.perf_counter()
-    m.iterate(steps)
-    m.synchronize()
-    wall = time.perf_counter() - t
+    del props
     cells = size * size
-    return {"workload": f"{size}^2 at 10 m, 9 fractions, Wilcock-Crowe, wet start", "dtype": "f64",
-            "steps": steps, "value": round(cells * steps / wall, 1), "unit": "cell-steps/s",
-            "ms_per_step": round(1000 * wall / steps, 4),
-            "sim_minutes": round(m.scalar("cycle"), 3)}
+    out = {"workload": f"{size}^2 at 10 m, 9 fractions (seeded Dirichlet proportions), "
+                       "Wilcock-Crowe, 1 m active layer, 40 mm/h storm, wet start (0.2 m)",
+           "steps": steps, "unit": "cell-steps/s"}
+    fields = {}
+    for name, prec in (("fp64", hc.HC_FP64), ("fp32", hc.HC_FP32)):
+        m = Model(None, None, precision=prec, arrays=arrays, params=params)
+        m.iterate(warmup)
+        m.synchronize()
+        t = time.perf_counter()
+        m.iterate(steps)
+        m.synchronize()
+        wall = time.perf_counter() - t
+        out[name] = {"value": round(cells * steps / wall, 1), "ms_per_step": round(1000 * wall / steps, 4),
+                     "sim_minutes": round(m.scalar("cycle"), 3)}
+        fields[name] = {f: m.download(f) for f in ("elev", "water_depth", "qx", "qy", "Vsusptot", "grain")}
+        m.close()
+        print(f"[bench] erosion {name}: {out[name]['value']:.4e} cell-steps/s, "
+              f"{out[name]['ms_per_step']} ms/step", file=sys.stderr, flush=True)
+    # the FP64 form's own response to 1e-8 m of DEM noise, after the same bodies
+    noisy = dict(arrays)
+    noise = 1e-8 * np.random.Generator(np.random.PCG64(3)).standard_normal(dem.shape)
+    noisy["dem"] = np.where(dem > inputs.NODATA, dem + noise, dem)
+    m = Model(None, None, precision=hc.HC_FP64, arrays=noisy, params=params)
+    m.iterate(warmup + steps)
+    fields["floor"] = {f: m.download(f) for f in ("elev", "water_depth", "qx", "qy", "Vsusptot", "grain")}
+    m.close()
+    out["value"] = out["fp64"]["value"]
+    out["dtype"] = "f64"
+    b = fields["fp64"]
+    change = float(np.abs(b["elev"][1:-1, 1:-1] - dem).max())
+
+    def errs(a):
+        e = {"elev": {"max_abs": float(np.abs(a["elev"] - b["elev"])[1:-1, 1:-1].max()),
+                      "max_elevation_change": change}}
+        for f in ("water_depth", "qx", "qy", "Vsusptot", "grain"):
+            e[f] = _field_errors(a[f] if a[f].ndim == 2 else a[f].sum(axis=2),
+                                 b[f] if b[f].ndim == 2 else b[f].sum(axis=2))
+        return e
+    out["fp32_error"] = {"after_loop_bodies": warmup + steps, "fields": errs(fields["fp32"]),
+                         "fp64_floor": errs(fields["floor"]),
+                         "note": "grain: the cells' summed active-layer grain; fp64_floor: the FP64 "
+                                 "form from the DEM plus 1e-8 m of noise, against the FP64 form"}
+    return out
 
 
 def accuracy_run(size: int, hours: int):
@@ -503,8 +542,6 @@ def main():
               file=sys.stderr, flush=True)
     if not args.no_erosion and args.only is None:
         out["erosion"] = run_erosion()
-        print(f"[bench] erosion: {out['erosion']['value']:.4e} cell-steps/s, "
-              f"{out['erosion']['ms_per_step']} ms/step", file=sys.stderr, flush=True)
     if not args.no_adjoint and args.only is None:
         out["adjoint"] = run_adjoint()
         print(f"[bench] adjoint: forward {out['adjoint']['forward_ms_per_step']} ms/step, reverse "

@@ -194,7 +194,9 @@ class CatchmentModel {
     p.chann_lateral_erosion = chann_lateral_erosion_;
     p.veg_lat_restriction = veg_lat_restriction_;
     p.flat_dz = opt_.flat_dz;
-    p.elev_fixed = opt_.elev_fixed;
+    // erosion moves elevations every erode call: they stay FP64 in both
+    // precisions, so the fixed-point form applies to flow-only runs
+    p.elev_fixed = hydro_only_ ? opt_.elev_fixed : 0;
     check(hc_create(&ctx_, &g, &p), "hc_create");
   }
 
@@ -266,7 +268,10 @@ class CatchmentModel {
       }
     }
     load_reach_files();
-    if (graindata_from_file_ && !hydro_only_) ingest_graindata(read_path_ + "/" + grain_data_file_);
+    if (graindata_from_file_ && !hydro_only_) {
+      if (arrays_) ingest_graindata_arrays();
+      else ingest_graindata(read_path_ + "/" + grain_data_file_);
+    }
   }
 
   void set_time_counters() {  // LCM.cpp:1862-1871
@@ -1001,6 +1006,25 @@ class CatchmentModel {
           if (col < cols.size()) strata[c * 90 + z * 9 + k] = cols[col];
         }
     }
+    upload(HC_FIELD_GRAIN, grain.data());
+    upload(HC_FIELD_STRATA, strata.data());
+    upload(HC_FIELD_HASGRAIN, has.data());
+  }
+
+  // the grain data file's values given in memory, one row per cell
+  void ingest_graindata_arrays() {
+    if (!arrays_->graindata) throw Error("in-memory grain data missing");
+    const size_t n = elev_.size();
+    std::vector<double> grain(n * 11, 0.0), strata(n * 90, 0.0);
+    std::vector<int32_t> has(n, 0);
+    for (int i = 0; i < imax_; ++i)
+      for (int j = 0; j < jmax_; ++j) {
+        const double* v = arrays_->graindata + (static_cast<size_t>(i) * jmax_ + j) * 101;
+        const size_t c = at(i + 1, j + 1);
+        has[c] = 1;
+        for (int k = 0; k <= kGMax; ++k) grain[c * 11 + k] = v[k];
+        for (int k = 0; k < 90; ++k) strata[c * 90 + k] = v[11 + k];
+      }
     upload(HC_FIELD_GRAIN, grain.data());
     upload(HC_FIELD_STRATA, strata.data());
     upload(HC_FIELD_HASGRAIN, has.data());

@@ -67,6 +67,19 @@ int put_plane(hc_ctx* c, void* dev, bool dev_float, const double* host, int hstr
   return HC_OK;
 }
 
+// Plane k of a sediment buffer (sesz-byte elements).
+void* splane(const hc_ctx* c, void* base, size_t k) {
+  return static_cast<char*>(base) + k * c->ncell * c->sesz;
+}
+
+// Zeroed sediment buffer of `planes` planes of sesz-byte elements.
+int salloc(hc_ctx* c, void** p, size_t planes) {
+  const size_t bytes = planes * c->ncell * c->sesz;
+  HC_HIP(c, hipMalloc(p, bytes));
+  HC_HIP(c, hipMemsetAsync(*p, 0, bytes, c->stream));
+  return HC_OK;
+}
+
 // Fixed-point elevations: q = round((z - base) * 2^shift) with base the
 // middle of the valid range and the largest shift (<= 40) that keeps |q| <=
 // 2^30 - 1, so differences of two valid cells fit in an int32; no-data
@@ -213,13 +226,14 @@ int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
   *out = nullptr;
   if (grid->imax < 1 || grid->jmax < 1 || !(grid->DX > 0)) return HC_ERR_ARG;
   if (grid->precision != HC_FP64 && grid->precision != HC_FP32) return HC_ERR_ARG;
-  if (grid->sediment && grid->precision != HC_FP64) return HC_ERR_ARG;
+  if (grid->sediment && grid->precision == HC_FP32 && params->elev_fixed) return HC_ERR_ARG;
   hc_ctx* c = new (std::nothrow) hc_ctx;
   if (!c) return HC_ERR_ARG;
   c->g = *grid;
   c->prm = *params;
   c->prec = grid->precision;
   c->esz = (c->prec == HC_FP64) ? 8 : 4;
+  c->sesz = c->esz;
   c->NR = grid->imax + 2;
   c->NC = grid->jmax + 2;
   c->P = ((c->NC + 63) / 64) * 64;
@@ -249,11 +263,13 @@ int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
         return fail(rc);
     }
   }
-  if (c->prec == HC_FP64) {
+  if (c->prec == HC_FP64 || grid->sediment) {  // FP64 elevations (both sediment forms)
     if ((rc = dalloc(c, reinterpret_cast<double**>(&c->z), n))) return fail(rc);
-    if (params->spatially_var_mannings &&
-        (rc = dalloc(c, reinterpret_cast<double**>(&c->nman), n)))
-      return fail(rc);
+    if (params->spatially_var_mannings) {
+      rc = (c->prec == HC_FP64) ? dalloc(c, reinterpret_cast<double**>(&c->nman), n)
+                                : dalloc(c, reinterpret_cast<float**>(&c->nman), n);
+      if (rc) return fail(rc);
+    }
     if ((rc = hc_launch_fill(c, c->z, n, -9999.0, false))) return fail(rc);
   } else {
     if ((rc = dalloc(c, reinterpret_cast<float**>(&c->z), n))) return fail(rc);
@@ -272,14 +288,14 @@ int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
   }
   if ((rc = dalloc(c, &c->mask, static_cast<size_t>(c->NR) * c->W))) return fail(rc);
   if (grid->sediment) {
-    if ((rc = dalloc(c, &c->V, n)) || (rc = dalloc(c, &c->qxs, n)) ||
-        (rc = dalloc(c, &c->qys, n)) || (rc = dalloc(c, &c->veldir, 4 * n)) ||
-        (rc = dalloc(c, &c->grain, 11 * n)) || (rc = dalloc(c, &c->strata, 90 * n)) ||
+    if ((rc = salloc(c, &c->V, 1)) || (rc = salloc(c, &c->qxs, 1)) ||
+        (rc = salloc(c, &c->qys, 1)) || (rc = salloc(c, &c->veldir, 4)) ||
+        (rc = salloc(c, &c->grain, 11)) || (rc = salloc(c, &c->strata, 90)) ||
         (rc = dalloc(c, &c->hasgrain, n)) || (rc = dalloc(c, &c->needgrain, n)) ||
-        (rc = dalloc(c, &c->bedrock, n)) || (rc = dalloc(c, &c->Tau, n)) ||
-        (rc = dalloc(c, &c->Vel, n)) || (rc = dalloc(c, &c->ss, n)) ||
-        (rc = dalloc(c, &c->sdir, 40 * n)) || (rc = dalloc(c, &c->etot, n)) ||
-        (rc = dalloc(c, &c->etot3, n)) || (rc = dalloc(c, &c->gtot, 20)))
+        (rc = dalloc(c, &c->bedrock, n)) || (rc = salloc(c, &c->Tau, 1)) ||
+        (rc = salloc(c, &c->Vel, 1)) || (rc = salloc(c, &c->ss, 1)) ||
+        (rc = salloc(c, &c->sdir, 40)) || (rc = salloc(c, &c->etot, 1)) ||
+        (rc = salloc(c, &c->etot3, 1)) || (rc = dalloc(c, &c->gtot, 20)))
       return fail(rc);
     if ((rc = hc_launch_fill(c, c->bedrock, n, -9999.0, false))) return fail(rc);
   }
@@ -332,50 +348,51 @@ int hc_upload(hc_ctx* c, int field, const void* host, size_t count) {
   HC_HIP(c, hipStreamSynchronize(c->stream));
   const double* d = static_cast<const double*>(host);
   const bool f32 = (c->prec == HC_FP32);
+  const bool sf = (c->sesz == 4);  // sediment planes in float
   switch (field) {
     case HC_FIELD_WATER_DEPTH: return put_plane(c, c->h[c->cur], f32, d, 1, 0);
     case HC_FIELD_QX: return put_plane(c, c->qx[c->cur], f32, d, 1, 0);
     case HC_FIELD_QY: return put_plane(c, c->qy[c->cur], f32, d, 1, 0);
     case HC_FIELD_ELEV: {
-      int rc = put_plane(c, c->z, f32, d, 1, 0);
-      if (rc == HC_OK && f32) rc = put_plane(c, c->z64, false, d, 1, 0);
+      int rc = put_plane(c, c->z, f32 && !c->g.sediment, d, 1, 0);
+      if (rc == HC_OK && c->z64) rc = put_plane(c, c->z64, false, d, 1, 0);
       if (rc == HC_OK && c->zq) rc = put_fixed_elev(c, d);
       return rc;
     }
     case HC_FIELD_MANNINGS:
       if (!c->nman) return hc_fail(c, HC_ERR_STATE, "upload: context has uniform Manning n");
       return put_plane(c, c->nman, f32, d, 1, 0);
-    case HC_FIELD_VSUSP: return put_plane(c, c->V, false, d, 1, 0);
-    case HC_FIELD_QXS: return put_plane(c, c->qxs, false, d, 1, 0);
-    case HC_FIELD_QYS: return put_plane(c, c->qys, false, d, 1, 0);
+    case HC_FIELD_VSUSP: return put_plane(c, c->V, sf, d, 1, 0);
+    case HC_FIELD_QXS: return put_plane(c, c->qxs, sf, d, 1, 0);
+    case HC_FIELD_QYS: return put_plane(c, c->qys, sf, d, 1, 0);
     case HC_FIELD_BEDROCK: return put_plane(c, c->bedrock, false, d, 1, 0);
-    case HC_FIELD_TAU: return put_plane(c, c->Tau, false, d, 1, 0);
-    case HC_FIELD_VEL: return put_plane(c, c->Vel, false, d, 1, 0);
-    case HC_FIELD_SS: return put_plane(c, c->ss, false, d, 1, 0);
+    case HC_FIELD_TAU: return put_plane(c, c->Tau, sf, d, 1, 0);
+    case HC_FIELD_VEL: return put_plane(c, c->Vel, sf, d, 1, 0);
+    case HC_FIELD_SS: return put_plane(c, c->ss, sf, d, 1, 0);
     case HC_FIELD_VEL_DIR: {
       const int dirs[4] = {1, 3, 5, 7};
       for (int k = 0; k < 4; ++k) {
-        int rc = put_plane(c, c->veldir + k * c->ncell, false, d, 9, dirs[k]);
+        int rc = put_plane(c, splane(c, c->veldir, k), sf, d, 9, dirs[k]);
         if (rc) return rc;
       }
       return HC_OK;
     }
     case HC_FIELD_GRAIN:
       for (int k = 0; k < 11; ++k) {
-        int rc = put_plane(c, c->grain + k * c->ncell, false, d, 11, k);
+        int rc = put_plane(c, splane(c, c->grain, k), sf, d, 11, k);
         if (rc) return rc;
       }
       return HC_OK;
     case HC_FIELD_STRATA:
       for (int k = 0; k < 90; ++k) {
-        int rc = put_plane(c, c->strata + k * c->ncell, false, d, 90, k);
+        int rc = put_plane(c, splane(c, c->strata, k), sf, d, 90, k);
         if (rc) return rc;
       }
       return HC_OK;
     case HC_FIELD_SU: case HC_FIELD_SR: case HC_FIELD_SD: case HC_FIELD_SL: {
       const int dir = (field == HC_FIELD_SU) ? 0 : (field == HC_FIELD_SR) ? 1 : (field == HC_FIELD_SD) ? 2 : 3;
       for (int k = 0; k < 10; ++k) {
-        int rc = put_plane(c, c->sdir + (dir * 10 + k) * c->ncell, false, d, 10, k);
+        int rc = put_plane(c, splane(c, c->sdir, dir * 10 + k), sf, d, 10, k);
         if (rc) return rc;
       }
       return HC_OK;
@@ -406,46 +423,47 @@ int hc_download(hc_ctx* c, int field, void* host, size_t count) {
   HC_HIP(c, hipStreamSynchronize(c->stream));
   double* d = static_cast<double*>(host);
   const bool f32 = (c->prec == HC_FP32);
+  const bool sf = (c->sesz == 4);  // sediment planes in float
   switch (field) {
     case HC_FIELD_WATER_DEPTH: return get_plane(c, c->h[c->cur], f32, d, 1, 0);
     case HC_FIELD_QX: return get_plane(c, c->qx[c->cur], f32, d, 1, 0);
     case HC_FIELD_QY: return get_plane(c, c->qy[c->cur], f32, d, 1, 0);
-    case HC_FIELD_ELEV: return get_plane(c, f32 ? static_cast<void*>(c->z64) : c->z, false, d, 1, 0);
+    case HC_FIELD_ELEV: return get_plane(c, c->z64 ? static_cast<void*>(c->z64) : c->z, false, d, 1, 0);
     case HC_FIELD_MANNINGS:
       if (!c->nman) return hc_fail(c, HC_ERR_STATE, "download: context has uniform Manning n");
       return get_plane(c, c->nman, f32, d, 1, 0);
-    case HC_FIELD_VSUSP: return get_plane(c, c->V, false, d, 1, 0);
-    case HC_FIELD_QXS: return get_plane(c, c->qxs, false, d, 1, 0);
-    case HC_FIELD_QYS: return get_plane(c, c->qys, false, d, 1, 0);
+    case HC_FIELD_VSUSP: return get_plane(c, c->V, sf, d, 1, 0);
+    case HC_FIELD_QXS: return get_plane(c, c->qxs, sf, d, 1, 0);
+    case HC_FIELD_QYS: return get_plane(c, c->qys, sf, d, 1, 0);
     case HC_FIELD_BEDROCK: return get_plane(c, c->bedrock, false, d, 1, 0);
-    case HC_FIELD_TAU: return get_plane(c, c->Tau, false, d, 1, 0);
-    case HC_FIELD_VEL: return get_plane(c, c->Vel, false, d, 1, 0);
-    case HC_FIELD_SS: return get_plane(c, c->ss, false, d, 1, 0);
+    case HC_FIELD_TAU: return get_plane(c, c->Tau, sf, d, 1, 0);
+    case HC_FIELD_VEL: return get_plane(c, c->Vel, sf, d, 1, 0);
+    case HC_FIELD_SS: return get_plane(c, c->ss, sf, d, 1, 0);
     case HC_FIELD_VEL_DIR: {
       std::fill(d, d + count, 0.0);
       const int dirs[4] = {1, 3, 5, 7};
       for (int k = 0; k < 4; ++k) {
-        int rc = get_plane(c, c->veldir + k * c->ncell, false, d, 9, dirs[k]);
+        int rc = get_plane(c, splane(c, c->veldir, k), sf, d, 9, dirs[k]);
         if (rc) return rc;
       }
       return HC_OK;
     }
     case HC_FIELD_GRAIN:
       for (int k = 0; k < 11; ++k) {
-        int rc = get_plane(c, c->grain + k * c->ncell, false, d, 11, k);
+        int rc = get_plane(c, splane(c, c->grain, k), sf, d, 11, k);
         if (rc) return rc;
       }
       return HC_OK;
     case HC_FIELD_STRATA:
       for (int k = 0; k < 90; ++k) {
-        int rc = get_plane(c, c->strata + k * c->ncell, false, d, 90, k);
+        int rc = get_plane(c, splane(c, c->strata, k), sf, d, 90, k);
         if (rc) return rc;
       }
       return HC_OK;
     case HC_FIELD_SU: case HC_FIELD_SR: case HC_FIELD_SD: case HC_FIELD_SL: {
       const int dir = (field == HC_FIELD_SU) ? 0 : (field == HC_FIELD_SR) ? 1 : (field == HC_FIELD_SD) ? 2 : 3;
       for (int k = 0; k < 10; ++k) {
-        int rc = get_plane(c, c->sdir + (dir * 10 + k) * c->ncell, false, d, 10, k);
+        int rc = get_plane(c, splane(c, c->sdir, dir * 10 + k), sf, d, 10, k);
         if (rc) return rc;
       }
       return HC_OK;

@@ -1,6 +1,8 @@
 // Erosion and deposition ("erodepo"): LSDCatchmentModel::erode
 // (LSDCatchmentModel.cpp:4211-4845) with its grain helpers sort_active,
-// addGS, sand_fraction, d50 and slide_GS (3874-4171), FP64.
+// addGS, sand_fraction, d50 and slide_GS (3874-4171).  FP64 arithmetic in
+// the reference's operation order; the sediment planes are stored in FP64 or,
+// in the FP32 storage form, in FP32 (elevations and bedrock stay FP64).
 //
 // Layout: dense per-cell grain (11 planes), strata (10 layers x 9 fractions
 // = 90 planes, plane z*9+n), a has-grain flag standing for index != -9999,
@@ -43,20 +45,26 @@ namespace {
 
 constexpr int kG = 10;  // G_MAX
 
-struct Grain {
-  double* g;       // 11 planes
-  double* s;       // 90 planes
+// E: storage of the grain planes and every other sediment plane (double,
+// or float in the FP32 storage form); all arithmetic is FP64, rounded to E
+// where a value is stored.
+template <typename E>
+struct GrainT {
+  E* g;            // 11 planes
+  E* s;            // 90 planes
   int32_t* has;
   size_t nc;       // plane stride (cells)
-  __device__ double& G(size_t c, int n) const { return g[n * nc + c]; }
-  __device__ double& S(size_t c, int z, int n) const { return s[(z * 9 + n) * nc + c]; }
+  __device__ E& G(size_t c, int n) const { return g[n * nc + c]; }
+  __device__ E& S(size_t c, int z, int n) const { return s[(z * 9 + n) * nc + c]; }
 };
 
 // LCM.cpp:3874-3947
-__device__ void sort_active(const Grain& gr, size_t c, const ErodeArgs& a);
+template <typename E>
+__device__ void sort_active(const GrainT<E>& gr, size_t c, const ErodeArgs& a);
 
 // LCM.cpp:3949-3983 (sort_active afterwards, as addGS does)
-__device__ void add_gs(const Grain& gr, size_t c, double elev, double bedrock, const ErodeArgs& a) {
+template <typename E>
+__device__ void add_gs(const GrainT<E>& gr, size_t c, double elev, double bedrock, const ErodeArgs& a) {
   gr.has[c] = 1;
   gr.G(c, 0) = 0;
   for (int n = 1; n <= kG - 1; ++n) gr.G(c, n) = a.active * a.dprop[n];
@@ -68,7 +76,8 @@ __device__ void add_gs(const Grain& gr, size_t c, double elev, double bedrock, c
   sort_active(gr, c, a);
 }
 
-__device__ void sort_active(const Grain& gr, size_t c, const ErodeArgs& a) {
+template <typename E>
+__device__ void sort_active(const GrainT<E>& gr, size_t c, const ErodeArgs& a) {
   double total = 0.0;
   for (int n = 0; n <= kG; ++n) total += gr.G(c, n);
   if (total > (a.active * 1.5)) {
@@ -94,14 +103,16 @@ __device__ void sort_active(const Grain& gr, size_t c, const ErodeArgs& a) {
 }
 
 // sort_active as called from outside addGS: allocates first if needed
-__device__ void sort_active_alloc(const Grain& gr, size_t c, const double* elev,
+template <typename E>
+__device__ void sort_active_alloc(const GrainT<E>& gr, size_t c, const double* elev,
                                   const double* bedrock, const ErodeArgs& a) {
   if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
   sort_active(gr, c, a);
 }
 
 // LCM.cpp:3985-4009
-__device__ double sand_fraction(const Grain& gr, size_t c) {
+template <typename E>
+__device__ double sand_fraction(const GrainT<E>& gr, size_t c) {
   double at = 0, st = 0;
   for (int n = 1; n <= kG; ++n) at += (gr.G(c, n));
   for (int n = 1; n <= 2; ++n) st += (gr.G(c, n));
@@ -110,7 +121,8 @@ __device__ double sand_fraction(const Grain& gr, size_t c) {
 }
 
 // LCM.cpp:4038-4075
-__device__ double d50(const Grain& gr, size_t c, const ErodeArgs& a) {
+template <typename E>
+__device__ double d50(const GrainT<E>& gr, size_t c, const ErodeArgs& a) {
   double at = 0, mx = 0, mn = 0;
   double cum[12];
   cum[0] = 0;
@@ -131,7 +143,8 @@ __device__ double d50(const Grain& gr, size_t c, const ErodeArgs& a) {
 }
 
 // LCM.cpp:4077-4171, material moves from `src` to `dst`
-__device__ void slide_gs(const Grain& gr, size_t src, size_t dst, double amount, const double* elev,
+template <typename E>
+__device__ void slide_gs(const GrainT<E>& gr, size_t src, size_t dst, double amount, const double* elev,
                          const double* bedrock, const ErodeArgs& a) {
   double total = 0;
   if (gr.has[src] && gr.has[dst]) {
@@ -186,17 +199,18 @@ __device__ __forceinline__ int dyp(int p) { return p == 1 ? -1 : (p == 5 ? 1 : 0
 __device__ __forceinline__ int sdir_of(int p) { return (p - 1) / 2; }
 
 // -------------------------------------------------------- entrainment
+template <typename E>
 __global__ __launch_bounds__(256) void k_entrain(ErodeArgs a, double time_step, double mult,
-                                                 const double* __restrict__ h,
+                                                 const E* __restrict__ h,
                                                  const double* __restrict__ elev,
                                                  double* __restrict__ bedrock,
-                                                 const double* __restrict__ nman,
+                                                 const E* __restrict__ nman,
                                                  const uint64_t* __restrict__ mask,
-                                                 const double* __restrict__ V,
-                                                 const double* __restrict__ vd,
-                                                 double* __restrict__ Vel, double* __restrict__ Tau,
-                                                 double* __restrict__ ss, double* __restrict__ sdir,
-                                                 Grain gr, int32_t* __restrict__ need,
+                                                 const E* __restrict__ V,
+                                                 const E* __restrict__ vd,
+                                                 E* __restrict__ Vel, E* __restrict__ Tau,
+                                                 E* __restrict__ ss, E* __restrict__ sdir,
+                                                 GrainT<E> gr, int32_t* __restrict__ need,
                                                  unsigned long long* __restrict__ red) {
   const int y = blockIdx.x * 64 + threadIdx.x;
   const int x = blockIdx.y * blockDim.y + threadIdx.y;
@@ -215,7 +229,7 @@ __global__ __launch_bounds__(256) void k_entrain(ErodeArgs a, double time_step, 
       double veltot = 0, vel = 0, qtot = 0, tau = 0, slopetot = 0;
       double tempdir[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       double temp_dist[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-      const double mann = a.spatial_n ? nman[c] : a.mannings;
+      const double mann = a.spatial_n ? static_cast<double>(nman[c]) : a.mannings;
       if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
       for (int p = 1; p <= 7; p += 2) {
         const size_t c2 = c + static_cast<long>(dxp(p)) * P + dyp(p);
@@ -323,8 +337,9 @@ __global__ __launch_bounds__(256) void k_entrain(ErodeArgs a, double time_step, 
 }
 
 // addGS for the neighbours the entrainment pass flagged
+template <typename E>
 __global__ __launch_bounds__(256) void k_alloc(ErodeArgs a, const double* __restrict__ elev,
-                                               const double* __restrict__ bedrock, Grain gr,
+                                               const double* __restrict__ bedrock, GrainT<E> gr,
                                                int32_t* __restrict__ need, size_t n) {
   for (size_t c = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; c < n;
        c += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -332,31 +347,6 @@ __global__ __launch_bounds__(256) void k_alloc(ErodeArgs a, const double* __rest
       if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
       need[c] = 0;
     }
-  }
-}
-
-// Lateral slide from neighbour `cn` into cell `c` (LCM.cpp:4619-4664 and
-// 4689-4742; veg and edge are zero in this build).  As in the reference, once
-// the first amount is positive the slide is carried out even when the bedrock
-// or domain-edge test has set it back to zero: slide_GS with a zero amount
-// still clamps negative fractions and re-sorts both active layers.
-__device__ __forceinline__ void lateral_slide(const ErodeArgs& a, const Grain& gr, double mult,
-                                              double time_step, const double* h, double* elev,
-                                              const double* bedrock, double tau_c, double e3,
-                                              size_t cn, size_t c, bool at_edge,
-                                              double& elev_update) {
-  double amt = 0;
-  if (h[cn] < a.wdet)
-    amt = mult * a.lateral_constant * tau_c * 0.0 * time_step / a.DX;
-  else
-    amt = a.chann_lateral * e3 * (elev[cn] - elev[c]) / a.DX * 0.1;
-  if (amt > 0) {
-    amt *= 1 - (0.0 * (1 - a.veg_lat));
-    if ((elev[cn] - amt) < bedrock[cn] || at_edge) amt = 0;
-    if (amt > a.ERODEFACTOR * 0.1) amt = a.ERODEFACTOR * 0.1;
-    elev_update += amt;
-    elev[cn] -= amt;
-    slide_gs(gr, cn, c, amt, elev, bedrock, a);
   }
 }
 
@@ -410,7 +400,8 @@ struct CellRaw {
   size_t c;
   int64_t has;
 };
-__device__ __forceinline__ void cell_fetch(CellRaw& r, const Grain& gr, size_t c, const double* h,
+template <typename E>
+__device__ __forceinline__ void cell_fetch(CellRaw& r, const GrainT<E>& gr, size_t c, const E* h,
                                            const double* elev, const double* bedrock, int lane) {
   r.c = c;
   r.gv = (lane <= kG) ? gr.g[static_cast<size_t>(lane) * gr.nc + c] : 0.0;
@@ -429,7 +420,8 @@ __device__ __forceinline__ void cell_take(CellW& w, const CellRaw& r) {
 }
 
 // cell state back to memory (fraction n by lane n, the rest by lane 0)
-__device__ __forceinline__ void cell_store(const CellW& w, const Grain& gr, double* elev,
+template <typename E>
+__device__ __forceinline__ void cell_store(const CellW& w, const GrainT<E>& gr, double* elev,
                                            int lane) {
   if (lane <= kG) gr.g[static_cast<size_t>(lane) * gr.nc + w.c] = w.g;
   if (lane == 0) {
@@ -439,12 +431,14 @@ __device__ __forceinline__ void cell_store(const CellW& w, const Grain& gr, doub
 }
 
 // strata of a cell into the lanes (slot 0: element lane, slot 1: lane + 64)
-__device__ __forceinline__ void strata_load(const Grain& gr, size_t c, int lane, double& s0,
+template <typename E>
+__device__ __forceinline__ void strata_load(const GrainT<E>& gr, size_t c, int lane, double& s0,
                                             double& s1) {
   s0 = gr.s[static_cast<size_t>(lane) * gr.nc + c];
   s1 = (lane + 64 < 90) ? gr.s[static_cast<size_t>(lane + 64) * gr.nc + c] : 0.0;
 }
-__device__ __forceinline__ void strata_store(const Grain& gr, size_t c, int lane, double s0,
+template <typename E>
+__device__ __forceinline__ void strata_store(const GrainT<E>& gr, size_t c, int lane, double s0,
                                              double s1) {
   gr.s[static_cast<size_t>(lane) * gr.nc + c] = s0;
   if (lane + 64 < 90) gr.s[static_cast<size_t>(lane + 64) * gr.nc + c] = s1;
@@ -488,7 +482,8 @@ __device__ __forceinline__ LaneG make_lane(const ErodeArgs& a, int lane) {
 }
 
 // sort_active (LCM.cpp:3874-3947) on a register cell
-__device__ __forceinline__ void sort_active_w(const Grain& gr, CellW& w, const ErodeArgs& a,
+template <typename E>
+__device__ __forceinline__ void sort_active_w(const GrainT<E>& gr, CellW& w, const ErodeArgs& a,
                                               const LaneG& L) {
   const double total = lane_sum(w.g, 0, kG);
   if (total > (a.active * 1.5)) {
@@ -519,7 +514,8 @@ __device__ __forceinline__ void sort_active_w(const Grain& gr, CellW& w, const E
 }
 
 // addGS (LCM.cpp:3949-3983) on a register cell
-__device__ __forceinline__ void add_gs_w(const Grain& gr, CellW& w, const ErodeArgs& a,
+template <typename E>
+__device__ __forceinline__ void add_gs_w(const GrainT<E>& gr, CellW& w, const ErodeArgs& a,
                                          const LaneG& L) {
   w.has = 1;
   w.g = L.frac ? a.active * L.dprop : 0.0;  // fractions 0 and 10 start empty
@@ -532,7 +528,8 @@ __device__ __forceinline__ void add_gs_w(const Grain& gr, CellW& w, const ErodeA
   sort_active_w(gr, w, a, L);
 }
 
-__device__ __forceinline__ void sort_active_alloc_w(const Grain& gr, CellW& w, const ErodeArgs& a,
+template <typename E>
+__device__ __forceinline__ void sort_active_alloc_w(const GrainT<E>& gr, CellW& w, const ErodeArgs& a,
                                                     const LaneG& L) {
   if (!w.has) add_gs_w(gr, w, a, L);
   sort_active_w(gr, w, a, L);
@@ -540,7 +537,8 @@ __device__ __forceinline__ void sort_active_alloc_w(const Grain& gr, CellW& w, c
 
 // slide_GS (LCM.cpp:4077-4171) between register cells: `src` to `dst`, one
 // fraction per lane (lanes 1..9)
-__device__ __forceinline__ void slide_gs_w(const Grain& gr, CellW& src, CellW& dst, double amount,
+template <typename E>
+__device__ __forceinline__ void slide_gs_w(const GrainT<E>& gr, CellW& src, CellW& dst, double amount,
                                            const ErodeArgs& a, const LaneG& L) {
   const bool f = L.frac != 0;
   // sum over n of G(src, n) where positive, in n order (adding 0.0 elsewhere
@@ -586,8 +584,12 @@ __device__ __forceinline__ void slide_gs_w(const Grain& gr, CellW& src, CellW& d
 }
 
 // one lateral slide from neighbour `nb` into cell `w` (LCM.cpp:4619-4664,
-// 4689-4742; veg and edge are zero in this build): as lateral_slide above
-__device__ __forceinline__ void lateral_slide_w(const ErodeArgs& a, const Grain& gr, double mult,
+// 4689-4742; veg and edge are zero in this build).  As in the reference, once
+// the first amount is positive the slide is carried out even when the bedrock
+// or domain-edge test has set it back to zero: slide_GS with a zero amount
+// still clamps negative fractions and re-sorts both active layers.
+template <typename E>
+__device__ __forceinline__ void lateral_slide_w(const ErodeArgs& a, const GrainT<E>& gr, double mult,
                                                 double time_step, double tau_c, double e3,
                                                 CellW& nb, CellW& w, bool at_edge,
                                                 double& elev_update, const LaneG& L) {
@@ -612,17 +614,18 @@ __device__ __forceinline__ void lateral_slide_w(const ErodeArgs& a, const Grain&
 // 2 + l/8; terms 0-3 its own su, sr, sd, sl, terms 4-7 the neighbours' su of
 // y+1, sd of y-1, sl of x+1, sr of x-1), summed in the reference's order by
 // lanes 8k and 8k+4.
+template <typename E>
 __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, double mult,
-                                                const double* __restrict__ h,
+                                                const E* __restrict__ h,
                                                 double* __restrict__ elev,
                                                 const double* __restrict__ bedrock,
                                                 const uint64_t* __restrict__ mask,
-                                                double* __restrict__ V,
-                                                const double* __restrict__ Tau,
-                                                const double* __restrict__ ss,
-                                                const double* __restrict__ sdir, Grain gr,
-                                                double* __restrict__ etot,
-                                                double* __restrict__ etot3) {
+                                                E* __restrict__ V,
+                                                const E* __restrict__ Tau,
+                                                const E* __restrict__ ss,
+                                                const E* __restrict__ sdir, GrainT<E> gr,
+                                                E* __restrict__ etot,
+                                                E* __restrict__ etot3) {
   const int y = 2 + blockIdx.x;
   if (y >= a.jmax || a.imax < 3) return;
   const int lane = threadIdx.x;
@@ -732,13 +735,14 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
 
 // --------------------------------------------------------- y-lateral slide
 // One wavefront per row x, walking y = 2 .. jmax-1 (LCM.cpp:4684-4747).
+template <typename E>
 __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step, double mult,
-                                                  const double* __restrict__ h,
+                                                  const E* __restrict__ h,
                                                   double* __restrict__ elev,
                                                   const double* __restrict__ bedrock,
                                                   const uint64_t* __restrict__ mask,
-                                                  const double* __restrict__ Tau, Grain gr,
-                                                  const double* __restrict__ etot3) {
+                                                  const E* __restrict__ Tau, GrainT<E> gr,
+                                                  const E* __restrict__ etot3) {
   const int x = 1 + blockIdx.x;
   if (x > a.imax || a.jmax < 3) return;
   const int lane = threadIdx.x;
@@ -795,9 +799,10 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
 }
 
 // ------------------------------------------------------------ edge outputs
-__global__ __launch_bounds__(256) void k_edges(ErodeArgs a, const double* __restrict__ h,
-                                               double* __restrict__ V,
-                                               const double* __restrict__ sdir, size_t nc,
+template <typename E>
+__global__ __launch_bounds__(256) void k_edges(ErodeArgs a, const E* __restrict__ h,
+                                               E* __restrict__ V,
+                                               const E* __restrict__ sdir, size_t nc,
                                                float* __restrict__ gtot) {
   // The reference's two loops visit the edge cells in a fixed order (entry
   // e: y or x = 2 + e/2, the far edge for even e, the near edge for odd e),
@@ -901,10 +906,10 @@ ErodeArgs erode_args(const hc_ctx* c) {
 // others into the grain row and the elevation, in fraction order; then
 // every point takes its water.  The amounts are formed on the host.
 template <class T>
-__global__ void k_reach(ErodeArgs a, Grain gr, int n, const int* __restrict__ idx,
+__global__ void k_reach(ErodeArgs a, GrainT<T> gr, int n, const int* __restrict__ idx,
                         const double* __restrict__ sed, const double* __restrict__ water,
                         double thr, T* __restrict__ h, double* __restrict__ elev,
-                        const double* __restrict__ bedrock, double* __restrict__ V) {
+                        const double* __restrict__ bedrock, T* __restrict__ V) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (sed) {
     for (int k = 0; k < n; ++k) {
@@ -1068,11 +1073,12 @@ __device__ constexpr int kCreepOy[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 // (a slide touches the cell and one neighbour, so sources interact within
 // Chebyshev distance 2), one block, a barrier per wavefront.  Runs before
 // the elevations change (the slides' grain allocation reads them).
+template <typename E>
 __global__ __launch_bounds__(1024) void k_creep_grain(ErodeArgs a, int imax, int jmax, int P,
                                                       double DX, double rate, double time,
                                                       const double* __restrict__ elev,
                                                       const double* __restrict__ bedrock,
-                                                      Grain gr) {
+                                                      GrainT<E> gr) {
   const int t0 = 3 * 2 + 2, t1 = 3 * (imax - 1) + (jmax - 1);
   for (int t = t0; t <= t1; ++t) {
     const int xlo = max(2, (t - (jmax - 1) + 2) / 3);
@@ -1162,12 +1168,13 @@ __global__ __launch_bounds__(256) void k_creep_add(int imax, int jmax, int P,
 // per wavefront.  A cell runs once for each list entry taken as its row (its
 // own, and row 1's for row 2, row imax's for row imax-1), consecutively as
 // in the reference.
+template <typename E>
 __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int imax, int jmax,
                                                                int P, int W, double factor,
                                                                const uint64_t* __restrict__ mask,
                                                                double* __restrict__ elev,
                                                                const double* __restrict__ bedrock,
-                                                               Grain gr,
+                                                               GrainT<E> gr,
                                                                const unsigned long long* __restrict__ any) {
   if (*any == 0) return;  // no cell can move
   constexpr int ox[8] = {1, 0, -1, -1, -1, 0, 1, 1};
@@ -1217,13 +1224,26 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
 
 }  // namespace
 
-int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot_out) {
-  if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "erode: FP64 contexts only");
+namespace {
+
+template <typename E>
+GrainT<E> grain_of(hc_ctx* c) {
+  return GrainT<E>{static_cast<E*>(c->grain), static_cast<E*>(c->strata), c->hasgrain, c->ncell};
+}
+
+template <typename E>
+int erode_t(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot_out) {
   const ErodeArgs a = erode_args(c);
-  const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
-  const double* h = static_cast<const double*>(c->h[c->cur]);
+  const GrainT<E> gr = grain_of<E>(c);
+  const E* h = static_cast<const E*>(c->h[c->cur]);
   double* elev = static_cast<double*>(c->z);
-  const double* nman = static_cast<const double*>(c->nman);
+  const E* nman = static_cast<const E*>(c->nman);
+  E* V = static_cast<E*>(c->V);
+  E* Tau = static_cast<E*>(c->Tau);
+  E* ss = static_cast<E*>(c->ss);
+  E* sdir = static_cast<E*>(c->sdir);
+  E* etot = static_cast<E*>(c->etot);
+  E* etot3 = static_cast<E*>(c->etot3);
   const dim3 blk(64, 4);
   const dim3 cells(static_cast<unsigned>(c->P / 64), static_cast<unsigned>((c->NR + 3) / 4));
   double ts = *time_step * 1.5;  // LCM.cpp:4219
@@ -1231,11 +1251,11 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
   for (int pass = 0;; ++pass) {
     if (pass > 10000) return hc_fail(c, HC_ERR_UNSTABLE, "erode: timestep loop did not converge");
     HC_HIP(c, hipMemsetAsync(c->d_red + 1, 0, sizeof(unsigned long long), c->stream));
-    hipLaunchKernelGGL(k_entrain, cells, blk, 0, c->stream, a, ts, mult, h, elev, c->bedrock, nman,
-                       c->mask, c->V, c->veldir, c->Vel, c->Tau, c->ss, c->sdir, gr, c->needgrain,
-                       c->d_red + 1);
+    hipLaunchKernelGGL(k_entrain<E>, cells, blk, 0, c->stream, a, ts, mult, h, elev, c->bedrock,
+                       nman, c->mask, static_cast<const E*>(V), static_cast<const E*>(c->veldir),
+                       static_cast<E*>(c->Vel), Tau, ss, sdir, gr, c->needgrain, c->d_red + 1);
     HC_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_alloc, dim3(1024), dim3(256), 0, c->stream, a, elev, c->bedrock, gr,
+    hipLaunchKernelGGL(k_alloc<E>, dim3(1024), dim3(256), 0, c->stream, a, elev, c->bedrock, gr,
                        c->needgrain, c->ncell);
     HC_HIP(c, hipGetLastError());
     HC_HIP(c, hipMemcpyAsync(c->h_pinned, c->d_red + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1244,22 +1264,24 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
     if (tb > a.ERODEFACTOR) ts *= (a.ERODEFACTOR / tb) * 0.5;  // LCM.cpp:4550-4553
     if (!(tb > a.ERODEFACTOR)) break;
   }
-  HC_HIP(c, hipMemsetAsync(c->etot, 0, c->ncell * sizeof(double), c->stream));
-  HC_HIP(c, hipMemsetAsync(c->etot3, 0, c->ncell * sizeof(double), c->stream));
+  HC_HIP(c, hipMemsetAsync(etot, 0, c->ncell * sizeof(E), c->stream));
+  HC_HIP(c, hipMemsetAsync(etot3, 0, c->ncell * sizeof(E), c->stream));
   const int ncol = std::max(0, c->g.jmax - 2);
   // The deposition sweep runs each column top to bottom and the y-slide each
   // row left to right (the reference's in-place order): one wavefront per
   // column / row, its lanes sharing the per-cell loads and strata moves.
   if (ncol > 0) {
-    hipLaunchKernelGGL(k_deposit, dim3(ncol), dim3(64), 0, c->stream, a, ts, mult, h, elev,
-                       c->bedrock, c->mask, c->V, c->Tau, c->ss, c->sdir, gr, c->etot, c->etot3);
+    hipLaunchKernelGGL(k_deposit<E>, dim3(ncol), dim3(64), 0, c->stream, a, ts, mult, h, elev,
+                       c->bedrock, c->mask, V, static_cast<const E*>(Tau),
+                       static_cast<const E*>(ss), static_cast<const E*>(sdir), gr, etot, etot3);
     HC_HIP(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_lateral_y, dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
-                     c->bedrock, c->mask, c->Tau, gr, c->etot3);
+  hipLaunchKernelGGL(k_lateral_y<E>, dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
+                     c->bedrock, c->mask, static_cast<const E*>(Tau), gr,
+                     static_cast<const E*>(etot3));
   HC_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(k_edges, dim3(1), dim3(256), 0, c->stream, a, h, c->V, c->sdir, c->ncell,
-                     c->gtot);
+  hipLaunchKernelGGL(k_edges<E>, dim3(1), dim3(256), 0, c->stream, a, h, V,
+                     static_cast<const E*>(sdir), c->ncell, c->gtot);
   HC_HIP(c, hipGetLastError());
   float g[20];
   HC_HIP(c, hipMemcpyAsync(g, c->gtot, sizeof(float) * 10, hipMemcpyDeviceToHost, c->stream));
@@ -1269,6 +1291,13 @@ int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, f
   if (gtot_out)
     for (int n = 0; n < 10; ++n) gtot_out[n] = g[n];
   return HC_OK;
+}
+
+}  // namespace
+
+int hc_erode_impl(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* gtot_out) {
+  return c->sesz == 4 ? erode_t<float>(c, mult, time_step, tempbmax, gtot_out)
+                      : erode_t<double>(c, mult, time_step, tempbmax, gtot_out);
 }
 
 namespace {
@@ -1287,7 +1316,6 @@ int slide_candidates(hc_ctx* c, double factor, double wet_low, int xmax) {
 }  // namespace
 
 int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps) {
-  if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "global_landsliding: FP64 contexts only");
   if (c->g.imax < 3 || c->g.jmax < 3) {  // no interior cell
     if (sweeps) *sweeps = 1;
     return HC_OK;
@@ -1314,18 +1342,21 @@ int hc_global_landsliding_impl(hc_ctx* c, double failureangle, int* sweeps) {
 }
 
 int hc_creep_impl(hc_ctx* c, double time, double creep_rate) {
-  if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "creep: FP64 contexts only");
   const size_t plane = c->ncell;
   if (!c->d_slide_moves) {
     HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_slide_moves), 8 * plane * sizeof(double)));
   }
   const ErodeArgs a = erode_args(c);
-  const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
   double* elev = static_cast<double*>(c->z);
   const int imax = c->g.imax, jmax = c->g.jmax;
-  if (imax >= 3 && jmax >= 3)
-    hipLaunchKernelGGL(k_creep_grain, dim3(1), dim3(1024), 0, c->stream, a, imax, jmax, c->P,
-                       c->g.DX, creep_rate, time, elev, c->bedrock, gr);
+  if (imax >= 3 && jmax >= 3) {
+    if (c->sesz == 4)
+      hipLaunchKernelGGL(k_creep_grain<float>, dim3(1), dim3(1024), 0, c->stream, a, imax, jmax,
+                         c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<float>(c));
+    else
+      hipLaunchKernelGGL(k_creep_grain<double>, dim3(1), dim3(1024), 0, c->stream, a, imax, jmax,
+                         c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<double>(c));
+  }
   const dim3 blk(256), grd(static_cast<unsigned>((jmax + 255) / 256), static_cast<unsigned>(imax));
   hipLaunchKernelGGL(k_creep_sum, grd, blk, 0, c->stream, imax, jmax, c->P, c->g.DX, creep_rate,
                      time, elev, c->bedrock, c->d_slide_moves);
@@ -1336,16 +1367,19 @@ int hc_creep_impl(hc_ctx* c, double time, double creep_rate) {
 }
 
 int hc_in_channel_landsliding_impl(hc_ctx* c, double failureangle) {
-  if (c->prec != HC_FP64) return hc_fail(c, HC_ERR_STATE, "in_channel_landsliding: FP64 contexts only");
   if (c->g.imax < 2 || c->g.jmax < 3) return HC_OK;  // no column 2..jmax-1 or row 2
   // the reference's factor, from the host's libm as in the reference
   const double factor = std::tan((failureangle * (3.141592654 / 180))) * c->g.DX;
   const ErodeArgs a = erode_args(c);
-  const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
   if (int rc = slide_candidates(c, factor, 10000.0, std::max(c->g.imax - 1, 2))) return rc;
-  hipLaunchKernelGGL(k_in_channel_landslide, dim3(1), dim3(1024), 0, c->stream, a, c->g.imax,
-                     c->g.jmax, c->P, c->W, factor, c->mask, static_cast<double*>(c->z),
-                     c->bedrock, gr, c->d_red + 4);
+  if (c->sesz == 4)
+    hipLaunchKernelGGL(k_in_channel_landslide<float>, dim3(1), dim3(1024), 0, c->stream, a,
+                       c->g.imax, c->g.jmax, c->P, c->W, factor, c->mask,
+                       static_cast<double*>(c->z), c->bedrock, grain_of<float>(c), c->d_red + 4);
+  else
+    hipLaunchKernelGGL(k_in_channel_landslide<double>, dim3(1), dim3(1024), 0, c->stream, a,
+                       c->g.imax, c->g.jmax, c->P, c->W, factor, c->mask,
+                       static_cast<double*>(c->z), c->bedrock, grain_of<double>(c), c->d_red + 4);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
@@ -1354,8 +1388,8 @@ int hc_reach_inputs_impl(hc_ctx* c, int n, const int* x, const int* y, const dou
                          const double* water, double threshold) {
   if (n < 0 || n > 16 || (n > 0 && (!x || !y || !water))) return hc_fail(c, HC_ERR_ARG, "reach_inputs: bad arguments");
   if (n == 0) return HC_OK;
-  if (sediment && (!c->g.sediment || c->prec != HC_FP64))
-    return hc_fail(c, HC_ERR_STATE, "reach_inputs: sediment needs an FP64 sediment context");
+  if (sediment && !c->g.sediment)
+    return hc_fail(c, HC_ERR_STATE, "reach_inputs: sediment needs a sediment context");
   int idx[16];
   double sed[16 * 9], wat[16];
   for (int k = 0; k < n; ++k) {
@@ -1383,22 +1417,19 @@ int hc_reach_inputs_impl(hc_ctx* c, int n, const int* x, const int* y, const dou
     HC_HIP(c, hipMemcpyAsync(c->d_amt + 16, sed, n * 9 * sizeof(double), hipMemcpyHostToDevice,
                              c->stream));
   // the host arrays above are pageable: the copies have completed on return
-  if (c->g.sediment) {
-    const ErodeArgs a = erode_args(c);
-    const Grain gr{c->grain, c->strata, c->hasgrain, c->ncell};
-    hipLaunchKernelGGL(k_reach<double>, dim3(1), dim3(64), 0, c->stream, a, gr, n, c->d_pts,
-                       sediment ? c->d_amt + 16 : nullptr, c->d_amt, threshold,
-                       static_cast<double*>(c->h[c->cur]), static_cast<double*>(c->z), c->bedrock,
-                       c->V);
-  } else if (c->prec == HC_FP64) {
-    hipLaunchKernelGGL(k_reach<double>, dim3(1), dim3(64), 0, c->stream, ErodeArgs{}, Grain{}, n,
-                       c->d_pts, nullptr, c->d_amt, threshold, static_cast<double*>(c->h[c->cur]),
-                       nullptr, nullptr, nullptr);
-  } else {
-    hipLaunchKernelGGL(k_reach<float>, dim3(1), dim3(64), 0, c->stream, ErodeArgs{}, Grain{}, n,
-                       c->d_pts, nullptr, c->d_amt, threshold, static_cast<float*>(c->h[c->cur]),
-                       nullptr, nullptr, nullptr);
-  }
+  const ErodeArgs a = c->g.sediment ? erode_args(c) : ErodeArgs{};
+  const double* sedp = (c->g.sediment && sediment) ? c->d_amt + 16 : nullptr;
+  double* zp = c->g.sediment ? static_cast<double*>(c->z) : nullptr;
+  if (c->prec == HC_FP64)
+    hipLaunchKernelGGL(k_reach<double>, dim3(1), dim3(64), 0, c->stream, a,
+                       c->g.sediment ? grain_of<double>(c) : GrainT<double>{}, n, c->d_pts, sedp,
+                       c->d_amt, threshold, static_cast<double*>(c->h[c->cur]), zp, c->bedrock,
+                       static_cast<double*>(c->V));
+  else
+    hipLaunchKernelGGL(k_reach<float>, dim3(1), dim3(64), 0, c->stream, a,
+                       c->g.sediment ? grain_of<float>(c) : GrainT<float>{}, n, c->d_pts, sedp,
+                       c->d_amt, threshold, static_cast<float*>(c->h[c->cur]), zp, c->bedrock,
+                       static_cast<float*>(c->V));
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

@@ -38,11 +38,11 @@ namespace {
 // the face (hn, zn).  `edge` is -1 for x<=2 / y<=2, +1 for x==imax /
 // y==jmax, else 0 (the `<= 2` override is applied last in the reference).
 // Leaves q (and qs) unchanged unless the face condition holds.
-template <bool SED>
+template <bool SED, typename VT = double>
 __device__ __forceinline__ void face64(double& q, double& qs, double hc, double hn, double zc,
                                        double zn, double n, double dt, const HydroArgs& a,
                                        int edge, bool xface, double Vc, double Vn,
-                                       double* vel_pos, double* vel_neg) {
+                                       VT* vel_pos, VT* vel_neg) {
   if (!((hc > 0 || hn > 0) && zn > -9999)) return;
   const double eta_c = zc + hc;
   const double eta_n = zn + hn;
@@ -122,18 +122,21 @@ __device__ __forceinline__ int edge_code(int i, int imaxlike) {
 
 // ------------------------------------------------------------ flow_route
 // In-place per-cell map (each face is written by exactly one cell).
-template <bool SED>
+// T: storage of depths, fluxes, n and the sediment planes (double, or float
+// in the FP32 storage form of erosion runs); arithmetic in FP64 throughout,
+// rounded once at each store.
+template <bool SED, typename T>
 __global__ __launch_bounds__(256) void k_flow_route64(HydroArgs a, double dt,
-                                                      const double* __restrict__ h,
+                                                      const T* __restrict__ h,
                                                       const double* __restrict__ z,
-                                                      double* __restrict__ qx,
-                                                      double* __restrict__ qy,
-                                                      const double* __restrict__ nman,
+                                                      T* __restrict__ qx,
+                                                      T* __restrict__ qy,
+                                                      const T* __restrict__ nman,
                                                       const uint64_t* __restrict__ mask,
-                                                      double* __restrict__ qxs,
-                                                      double* __restrict__ qys,
-                                                      const double* __restrict__ V,
-                                                      double* __restrict__ vd, size_t plane) {
+                                                      T* __restrict__ qxs,
+                                                      T* __restrict__ qys,
+                                                      const T* __restrict__ V,
+                                                      T* __restrict__ vd, size_t plane) {
   const int y = blockIdx.x * 64 + threadIdx.x;
   const int x = blockIdx.y * blockDim.y + threadIdx.y;
   if (x < 1 || x > a.imax || y < 1 || y > a.jmax) return;
@@ -142,7 +145,7 @@ __global__ __launch_bounds__(256) void k_flow_route64(HydroArgs a, double dt,
   const size_t c = static_cast<size_t>(x) * P + y;
   const double zc = z[c];
   if (!(zc > -9999)) return;
-  const double n = a.spatial_n ? nman[c] : a.mannings;
+  const double n = a.spatial_n ? static_cast<double>(nman[c]) : a.mannings;
   const double hc = h[c];
   double qv, qsv = 0;
   // x face (neighbour x-1)
@@ -150,35 +153,35 @@ __global__ __launch_bounds__(256) void k_flow_route64(HydroArgs a, double dt,
     const size_t cn = c - P;
     qv = qx[c];
     if (SED) qsv = qxs[c];
-    face64<SED>(qv, qsv, hc, h[cn], zc, z[cn], n, dt, a, edge_code(x, a.imax), true,
-                SED ? V[c] : 0.0, SED ? V[cn] : 0.0, SED ? &vd[3 * plane + c] : nullptr,
-                SED ? &vd[1 * plane + cn] : nullptr);
-    qx[c] = qv;
-    if (SED) qxs[c] = qsv;
+    face64<SED, T>(qv, qsv, hc, h[cn], zc, z[cn], n, dt, a, edge_code(x, a.imax), true,
+                   SED ? static_cast<double>(V[c]) : 0.0, SED ? static_cast<double>(V[cn]) : 0.0,
+                   SED ? &vd[3 * plane + c] : nullptr, SED ? &vd[1 * plane + cn] : nullptr);
+    qx[c] = static_cast<T>(qv);
+    if (SED) qxs[c] = static_cast<T>(qsv);
   }
   // y face (neighbour y-1)
   {
     const size_t cn = c - 1;
     qv = qy[c];
     if (SED) qsv = qys[c];
-    face64<SED>(qv, qsv, hc, h[cn], zc, z[cn], n, dt, a, edge_code(y, a.jmax), false,
-                SED ? V[c] : 0.0, SED ? V[cn] : 0.0, SED ? &vd[0 * plane + c] : nullptr,
-                SED ? &vd[2 * plane + cn] : nullptr);
-    qy[c] = qv;
-    if (SED) qys[c] = qsv;
+    face64<SED, T>(qv, qsv, hc, h[cn], zc, z[cn], n, dt, a, edge_code(y, a.jmax), false,
+                   SED ? static_cast<double>(V[c]) : 0.0, SED ? static_cast<double>(V[cn]) : 0.0,
+                   SED ? &vd[0 * plane + c] : nullptr, SED ? &vd[2 * plane + cn] : nullptr);
+    qy[c] = static_cast<T>(qv);
+    if (SED) qys[c] = static_cast<T>(qsv);
   }
 }
 
 // ---------------------------------------------------------- depth_update
-template <bool SED>
-__global__ __launch_bounds__(256) void k_depth64(HydroArgs a, double dt, double* __restrict__ h,
+template <bool SED, typename T>
+__global__ __launch_bounds__(256) void k_depth64(HydroArgs a, double dt, T* __restrict__ h,
                                                  const double* __restrict__ z,
-                                                 const double* __restrict__ qx,
-                                                 const double* __restrict__ qy,
+                                                 const T* __restrict__ qx,
+                                                 const T* __restrict__ qy,
                                                  const uint64_t* __restrict__ mask,
-                                                 double* __restrict__ V,
-                                                 const double* __restrict__ qxs,
-                                                 const double* __restrict__ qys,
+                                                 T* __restrict__ V,
+                                                 const T* __restrict__ qxs,
+                                                 const T* __restrict__ qys,
                                                  unsigned long long* red) {
   const int y = blockIdx.x * 64 + threadIdx.x;
   const int x = blockIdx.y * blockDim.y + threadIdx.y;
@@ -186,13 +189,18 @@ __global__ __launch_bounds__(256) void k_depth64(HydroArgs a, double dt, double*
   if (x >= 1 && x <= a.imax && y >= 1 && y <= a.jmax && mask_bit(mask, a.mwords, x, y)) {
     const size_t P = a.pitch;
     const size_t c = static_cast<size_t>(x) * P + y;
-    double hv = h[c] + dt * (qx[c + P] - qx[c] + qy[c + 1] - qy[c]) / a.DX;
-    if (SED) V[c] = V[c] + dt * (qxs[c + P] - qxs[c] + qys[c + 1] - qys[c]) / a.DX;
+    const double qxc = qx[c], qxp = qx[c + P], qyc = qy[c], qyp = qy[c + 1];
+    double hv = static_cast<double>(h[c]) + dt * (qxp - qxc + qyp - qyc) / a.DX;
+    if (SED) {
+      const double sxc = qxs[c], sxp = qxs[c + P], syc = qys[c], syp = qys[c + 1];
+      V[c] = static_cast<T>(static_cast<double>(V[c]) + dt * (sxp - sxc + syp - syc) / a.DX);
+    }
+    hv = static_cast<double>(static_cast<T>(hv));  // the stored depth
     if (hv > 0) {
       if (z[c] == -9999) hv = 0;
       if (hv > md) md = hv;
     }
-    h[c] = hv;
+    h[c] = static_cast<T>(hv);
   }
   block_max_pos(red, md);
 }
@@ -436,7 +444,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     if constexpr (MODE == 0) {
       if (m && zc > Z(-9999)) {
         double dummy = 0;
-        face64<false>(qN, dummy, hcell, hl, zc, zl, ncell, dtd, a, code, false, 0, 0, nullptr,
+        face64<false, double>(qN, dummy, hcell, hl, zc, zl, ncell, dtd, a, code, false, 0, 0, nullptr,
                       nullptr);
       }
     } else {
@@ -593,9 +601,9 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     if constexpr (MODE == 0) {
       if (m && zt[k] > Z(-9999)) {  // both faces in one branch
         double dummy = 0;
-        face64<false>(qxN, dummy, hh[k], hh[k - 1], zt[k], zt[k - 1], nn[k], dtd, a,
+        face64<false, double>(qxN, dummy, hh[k], hh[k - 1], zt[k], zt[k - 1], nn[k], dtd, a,
                       edge_code(xr, a.imax), true, 0, 0, nullptr, nullptr);
-        face64<false>(qyN, dummy, hh[k], hl, zt[k], zl, nn[k], dtd, a, ycode, false, 0, 0,
+        face64<false, double>(qyN, dummy, hh[k], hl, zt[k], zl, nn[k], dtd, a, ycode, false, 0, 0,
                       nullptr, nullptr);
       }
     } else {
@@ -927,43 +935,55 @@ dim3 cell_grid(const hc_ctx* c) {
 }  // namespace
 
 
-int hc_launch_flow_route(hc_ctx* c, double dt) {
-  if (c->prec != HC_FP64)
-    return hc_fail(c, HC_ERR_STATE, "flow_route: the separate kernels are FP64-only; use hc_flow_depth");
-  const HydroArgs a = hydro_args(c);
-  const dim3 blk(64, 4);
+// The separate flow_route and depth_update: FP64 contexts, and the FP32
+// storage form of erosion runs (FP64 elevations; float depths, fluxes and
+// sediment planes, FP64 arithmetic).
+template <typename T>
+void launch_flow_route(hc_ctx* c, const HydroArgs& a, double dt) {
   const int k = c->cur;
-  if (c->g.sediment) {
-    hipLaunchKernelGGL(k_flow_route64<true>, cell_grid(c), blk, 0, c->stream, a, dt,
-                       (const double*)c->h[k], (const double*)c->z, (double*)c->qx[k],
-                       (double*)c->qy[k], (const double*)c->nman, c->mask, c->qxs, c->qys,
-                       (const double*)c->V, c->veldir, c->ncell);
-  } else {
-    hipLaunchKernelGGL(k_flow_route64<false>, cell_grid(c), blk, 0, c->stream, a, dt,
-                       (const double*)c->h[k], (const double*)c->z, (double*)c->qx[k],
-                       (double*)c->qy[k], (const double*)c->nman, c->mask, nullptr, nullptr,
-                       nullptr, nullptr, c->ncell);
-  }
+  const dim3 blk(64, 4);
+  if (c->g.sediment)
+    hipLaunchKernelGGL((k_flow_route64<true, T>), cell_grid(c), blk, 0, c->stream, a, dt,
+                       (const T*)c->h[k], (const double*)c->z, (T*)c->qx[k], (T*)c->qy[k],
+                       (const T*)c->nman, c->mask, (T*)c->qxs, (T*)c->qys, (const T*)c->V,
+                       (T*)c->veldir, c->ncell);
+  else
+    hipLaunchKernelGGL((k_flow_route64<false, T>), cell_grid(c), blk, 0, c->stream, a, dt,
+                       (const T*)c->h[k], (const double*)c->z, (T*)c->qx[k], (T*)c->qy[k],
+                       (const T*)c->nman, c->mask, nullptr, nullptr, nullptr, nullptr, c->ncell);
+}
+
+template <typename T>
+void launch_depth_update(hc_ctx* c, const HydroArgs& a, double dt) {
+  const int k = c->cur;
+  const dim3 blk(64, 4);
+  if (c->g.sediment)
+    hipLaunchKernelGGL((k_depth64<true, T>), cell_grid(c), blk, 0, c->stream, a, dt, (T*)c->h[k],
+                       (const double*)c->z, (const T*)c->qx[k], (const T*)c->qy[k], c->mask,
+                       (T*)c->V, (const T*)c->qxs, (const T*)c->qys, c->d_red);
+  else
+    hipLaunchKernelGGL((k_depth64<false, T>), cell_grid(c), blk, 0, c->stream, a, dt, (T*)c->h[k],
+                       (const double*)c->z, (const T*)c->qx[k], (const T*)c->qy[k], c->mask,
+                       nullptr, nullptr, nullptr, c->d_red);
+}
+
+int hc_launch_flow_route(hc_ctx* c, double dt) {
+  if (c->prec != HC_FP64 && !c->g.sediment)
+    return hc_fail(c, HC_ERR_STATE, "flow_route: FP32 hydro-only contexts step with hc_flow_depth");
+  const HydroArgs a = hydro_args(c);
+  if (c->prec == HC_FP64) launch_flow_route<double>(c, a, dt);
+  else launch_flow_route<float>(c, a, dt);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
 
 int hc_launch_depth_update(hc_ctx* c, double dt) {
-  if (c->prec != HC_FP64)
-    return hc_fail(c, HC_ERR_STATE, "depth_update: the separate kernels are FP64-only; use hc_flow_depth");
+  if (c->prec != HC_FP64 && !c->g.sediment)
+    return hc_fail(c, HC_ERR_STATE, "depth_update: FP32 hydro-only contexts step with hc_flow_depth");
   const HydroArgs a = hydro_args(c);
-  const dim3 blk(64, 4);
-  const int k = c->cur;
   HC_HIP(c, hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream));
-  if (c->g.sediment) {
-    hipLaunchKernelGGL(k_depth64<true>, cell_grid(c), blk, 0, c->stream, a, dt, (double*)c->h[k],
-                       (const double*)c->z, (const double*)c->qx[k], (const double*)c->qy[k],
-                       c->mask, c->V, (const double*)c->qxs, (const double*)c->qys, c->d_red);
-  } else {
-    hipLaunchKernelGGL(k_depth64<false>, cell_grid(c), blk, 0, c->stream, a, dt, (double*)c->h[k],
-                       (const double*)c->z, (const double*)c->qx[k], (const double*)c->qy[k],
-                       c->mask, nullptr, nullptr, nullptr, c->d_red);
-  }
+  if (c->prec == HC_FP64) launch_depth_update<double>(c, a, dt);
+  else launch_depth_update<float>(c, a, dt);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

@@ -192,22 +192,27 @@ struct hc_ctx {
   void* nman = nullptr;       // Manning n (state type), spatial only
   uint64_t* mask = nullptr;
 
-  // sediment state (FP64)
-  double* V = nullptr;        // Vsusptot
-  double* qxs = nullptr;
-  double* qys = nullptr;
-  double* veldir = nullptr;   // 4 planes: directions 1,3,5,7
-  double* grain = nullptr;    // 11 planes
-  double* strata = nullptr;   // 90 planes (layer-major: z*9+n)
+  // sediment state.  The planes below hold sesz-byte elements: double in an
+  // FP64 context, float in an FP32 one (the FP32 storage form; every kernel
+  // computes in FP64 and rounds once where it stores).  Elevations (z) and
+  // bedrock stay FP64 in both: erosion moves a cell by as little as 1e-9 m
+  // per call, below FP32's spacing of elevations (7.6e-6 m at 100 m).
+  size_t sesz = 8;
+  void* V = nullptr;          // Vsusptot
+  void* qxs = nullptr;
+  void* qys = nullptr;
+  void* veldir = nullptr;     // 4 planes: directions 1,3,5,7
+  void* grain = nullptr;      // 11 planes
+  void* strata = nullptr;     // 90 planes (layer-major: z*9+n)
   int32_t* hasgrain = nullptr;
   int32_t* needgrain = nullptr;
   double* bedrock = nullptr;
-  double* Tau = nullptr;
-  double* Vel = nullptr;
-  double* ss = nullptr;
-  double* sdir = nullptr;     // 4 directions x 10 planes: su, sr, sd, sl
-  double* etot = nullptr;     // erodetot
-  double* etot3 = nullptr;    // erodetot3
+  void* Tau = nullptr;
+  void* Vel = nullptr;
+  void* ss = nullptr;
+  void* sdir = nullptr;       // 4 directions x 10 planes: su, sr, sd, sl
+  void* etot = nullptr;       // erodetot
+  void* etot3 = nullptr;      // erodetot3
   float* gtot = nullptr;      // 20 floats
 
   // reductions and small device scalars

@@ -48,27 +48,28 @@ std::vector<Buf> state_buffers(hc_ctx* c) {
     b.push_back({c->qx[k], n * e});
     b.push_back({c->qy[k], n * e});
   }
-  b.push_back({c->z, n * e});
+  b.push_back({c->z, n * (c->g.sediment ? sizeof(double) : e)});  // FP64 with sediment
   if (c->z64) b.push_back({c->z64, n * sizeof(double)});
   if (c->zq) b.push_back({c->zq, n * sizeof(int32_t)});
   if (c->nman) b.push_back({c->nman, n * e});
   b.push_back({c->mask, static_cast<size_t>(c->NR) * c->W * sizeof(uint64_t)});
   if (c->g.sediment) {
-    b.push_back({c->V, n * sizeof(double)});
-    b.push_back({c->qxs, n * sizeof(double)});
-    b.push_back({c->qys, n * sizeof(double)});
-    b.push_back({c->veldir, 4 * n * sizeof(double)});
-    b.push_back({c->grain, 11 * n * sizeof(double)});
-    b.push_back({c->strata, 90 * n * sizeof(double)});
+    const size_t s = c->sesz;  // sediment planes: the context's storage type
+    b.push_back({c->V, n * s});
+    b.push_back({c->qxs, n * s});
+    b.push_back({c->qys, n * s});
+    b.push_back({c->veldir, 4 * n * s});
+    b.push_back({c->grain, 11 * n * s});
+    b.push_back({c->strata, 90 * n * s});
     b.push_back({c->hasgrain, n * sizeof(int32_t)});
     b.push_back({c->needgrain, n * sizeof(int32_t)});
     b.push_back({c->bedrock, n * sizeof(double)});
-    b.push_back({c->Tau, n * sizeof(double)});
-    b.push_back({c->Vel, n * sizeof(double)});
-    b.push_back({c->ss, n * sizeof(double)});
-    b.push_back({c->sdir, 40 * n * sizeof(double)});
-    b.push_back({c->etot, n * sizeof(double)});
-    b.push_back({c->etot3, n * sizeof(double)});
+    b.push_back({c->Tau, n * s});
+    b.push_back({c->Vel, n * s});
+    b.push_back({c->ss, n * s});
+    b.push_back({c->sdir, 40 * n * s});
+    b.push_back({c->etot, n * s});
+    b.push_back({c->etot3, n * s});
     b.push_back({c->gtot, 20 * sizeof(float)});
   }
   return b;

@@ -273,6 +273,18 @@ def write_graindata(path: str, props: np.ndarray, active: float,
         f.write("\n".join(lines) + "\n")
 
 
+def graindata_array(props: np.ndarray, active: float) -> np.ndarray:
+    """The values of write_graindata's lines without their x y index columns:
+    (nrows, ncols, 101) = grain[0..10] (active*props for fractions 1..9) and
+    strata[z=0..9][n=0..8] (active*props in every layer), for in-memory runs."""
+    nrows, ncols, nf = props.shape
+    assert nf == 9
+    out = np.zeros((nrows, ncols, 101))
+    out[:, :, 1:10] = active * props
+    out[:, :, 11:] = np.tile(active * props, (1, 1, 10))
+    return out
+
+
 # Parameter defaults shared by every synthetic workload.  Keys and meanings
 # follow LSDCatchmentModel::initialise_variables (LSDCatchmentModel.cpp:842-1610).
 BASE_PARAMS: Dict[str, str] = {
@@ -430,7 +442,8 @@ def erosion_1024(path: str, size: int = 1024, hours: int = 24, seed: int = 1002)
 
 
 def erosion_run(path: str, size: int = 48, hours: int = 1, seed: int = 1002,
-                failure_angle: float = 8.0, law: str = "wilcock") -> Workload:
+                failure_angle: float = 8.0, law: str = "wilcock",
+                dem_noise: float = 0.0) -> Workload:
     """A whole flow + erosion run small enough to compare step by step: size^2
     at 10 m on a 2 % slope with 20 m of fractal relief and a 1 m channel, all
     four edges open, 9 grain fractions (d1 = 0.0005 m) in seeded Dirichlet
@@ -439,10 +452,15 @@ def erosion_run(path: str, size: int = 48, hours: int = 1, seed: int = 1002,
     angle so that the hillslope processes move material (creep and global
     landsliding fire on the first iteration, in-channel landsliding every 10
     iterations).  The grain sizes go to the reference by its harness, to the
-    oracle by extra['grain_d'] and to this build's driver by hc_grain_sizes."""
+    oracle by extra['grain_d'] and to this build's driver by hc_grain_sizes.
+    dem_noise adds that many metres of seeded Gaussian noise to the DEM (a
+    perturbation for sensitivity floors)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     dem = planar_dem(size, size, 10.0, 0.02, 20.0, 0.6, seed, channel_depth=1.0,
                      south_only=False)
+    if dem_noise:
+        noise = np.random.Generator(np.random.PCG64(seed + 77)).standard_normal(dem.shape)
+        dem = np.where(dem > NODATA, dem + dem_noise * noise, dem)
     props = dirichlet_props(size, size, rng)
     w = make_workload(path, dem, 10.0, triangular_storm(hours, 40.0, 15), 15, hours,
                       overrides={"hydro_model_only": "no", "active_layer_thickness": "1",

@@ -39,7 +39,8 @@ class hcm_arrays(ctypes.Structure):
                 ("xll", ctypes.c_double), ("yll", ctypes.c_double), ("dem", ctypes.c_void_p),
                 ("mannings", ctypes.c_void_p), ("water_init", ctypes.c_void_p),
                 ("hydroindex", ctypes.c_void_p), ("rain", ctypes.c_void_p),
-                ("rain_rows", ctypes.c_int), ("rain_cols", ctypes.c_int)]
+                ("rain_rows", ctypes.c_int), ("rain_cols", ctypes.c_int),
+                ("graindata", ctypes.c_void_p)]
 
 
 def _lib():
@@ -80,7 +81,8 @@ class Model:
         """Files (path + params_file, as the reference's CLI) or, with
         ``arrays`` and ``params``, in-memory inputs: arrays holds 'dem' and
         optionally 'mannings', 'water_init', 'hydroindex', 'rain' (rows x
-        cells, mm/h) plus 'DX'."""
+        cells, mm/h), 'graindata' (nrows x ncols x 101: grain[0..10] and
+        strata[10][9] per cell, the grain data file's values) plus 'DX'."""
         self.L = _lib()
         opt = hcm_options(precision, 1 if fused else 0, 1, 1 if write_files else 0, flat_dz,
                           1 if device_loop else 0, 1 if use_graph else 0, 1 if elev_fixed else 0)
@@ -112,7 +114,7 @@ class Model:
             ar = hcm_arrays(dem.shape[0], dem.shape[1], float(arrays["DX"]), 0.0, 0.0, dem.ctypes.data,
                             ptr("mannings", np.float64), ptr("water_init", np.float64),
                             ptr("hydroindex", np.int32), rain.ctypes.data if rain is not None else None,
-                            rr, rc_)
+                            rr, rc_, ptr("graindata", np.float64))
             text = "".join(f"{k}: {v}\n" for k, v in params.items())
             rc = self.L.hcm_create_arrays(ctypes.byref(h), text.encode(), ctypes.byref(ar), ctypes.byref(opt))
         self.h = h

@@ -50,6 +50,10 @@ typedef struct hcm_arrays {
   const int* hydroindex;      /* hydroindex_file */
   const float* rain;          /* rainfall_data_file: rain_rows x rain_cols, mm/h */
   int rain_rows, rain_cols;
+  /* grain_data_file: per cell (nrows x ncols, row-major) the 101 values a
+   * line of that file holds after its x y index columns, grain[0..10] then
+   * strata[z=0..9][n=0..8] (LCM.cpp:611-683); every cell has a grain row */
+  const double* graindata;
 } hcm_arrays;
 int hcm_create_arrays(hcm_model** m, const char* params_text, const hcm_arrays* arrays,
                       const hcm_options* opt);

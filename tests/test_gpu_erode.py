@@ -268,3 +268,53 @@ def test_erosion_run_vs_oracle(tmp_path, law):
     np.testing.assert_allclose(m.download("strata").reshape(sb.shape), sb, rtol=1e-10,
                                atol=1e-12 * np.abs(sb).max())
     assert np.abs(m.download("elev")[1:-1, 1:-1] - w.dem).max() > 1.0
+
+
+def _erosion_fields(w, prec):
+    from hail_caesar_amd.model import Model
+    m = Model(w.path, w.params_file, precision=prec)
+    n = 0
+    while True:
+        n += len(m.iterate(5000)[0])
+        if m.finished:
+            break
+    out = {f: m.download(f) for f in ("elev", "water_depth", "qx", "qy", "Vsusptot", "grain")}
+    out["steps"] = n
+    m.close()
+    return out
+
+
+def _erosion_errs(a, b, dem):
+    e = {}
+    for f in ("water_depth", "qx", "qy", "Vsusptot", "grain"):
+        e[f] = float(np.abs(a[f] - b[f]).max() / max(np.abs(b[f]).max(), 1e-300))
+    # elevations: relative to the largest elevation change of the run
+    change = np.abs(b["elev"][1:-1, 1:-1] - dem).max()
+    e["elev"] = float(np.abs(a["elev"] - b["elev"])[1:-1, 1:-1].max() / change)
+    hv = b["water_depth"][1:-1, 1:-1].sum()
+    e["volume"] = float(abs(a["water_depth"][1:-1, 1:-1].sum() - hv) / hv)
+    gs = b["grain"][1:-1, 1:-1].sum()
+    e["grain_total"] = float(abs(a["grain"][1:-1, 1:-1].sum() - gs) / gs)
+    return e
+
+
+def test_fp32_erosion_form_against_fp64(tmp_path):
+    """The FP32 storage form of an erosion run (depths, fluxes, suspended
+    sediment, grain, strata and the other sediment planes in FP32, FP64
+    arithmetic, FP64 elevations) against the FP64 form over the same whole
+    run (inputs.erosion_run at 128^2, 1 h), beside the FP64 form's own
+    response to 1e-8 m of DEM noise: the erosion thresholds (tempbmax against
+    ERODEFACTOR, the active-layer sorts, the Froude and depth caps of the
+    flow) are discontinuous, so any perturbation flips them on a few cells
+    and the run departs from there.  Errors are per field relative to its
+    largest value (elevations: to the run's largest elevation change)."""
+    w = inputs.erosion_run(str(tmp_path / "er"), size=128)
+    wn = inputs.erosion_run(str(tmp_path / "ern"), size=128, dem_noise=1e-8)
+    b = _erosion_fields(w, hc.HC_FP64)
+    a = _erosion_fields(w, hc.HC_FP32)
+    f = _erosion_fields(wn, hc.HC_FP64)
+    err, floor = _erosion_errs(a, b, w.dem), _erosion_errs(f, b, w.dem)
+    print(f"steps fp64 {b['steps']} fp32 {a['steps']} floor {f['steps']}\nfp32: {err}\nfloor: {floor}")
+    assert abs(a["steps"] - b["steps"]) <= 0.01 * b["steps"]
+    for k in err:
+        assert err[k] <= max(4 * floor[k], 1e-3 if k in ("volume", "grain_total") else 1e-2), k
