@@ -7,6 +7,9 @@
 // input points, interval time series) stays here in FP64 exactly as the
 // reference computes it; every raster-wide method goes through the device
 // entry points of include/hc.h.
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cctype>
 #include <climits>
@@ -605,20 +608,57 @@ class CatchmentModel {
     if (loop_started_)
       throw Error("checkpoint: the device-resident loop holds the run's state on the device; "
                   "checkpoint a host-stepped model");
+    // a state with non-finite depths is not saved over a good one (the
+    // device check also counts negative depths, which rounding can leave in
+    // a sound run: only non-finite ones refuse)
+    long long bad = 0;
+    if (hc_check_health(ctx_, &bad) != HC_OK) {
+      std::vector<double> hv(hc_field_count(ctx_, HC_FIELD_WATER_DEPTH));
+      check(hc_download(ctx_, HC_FIELD_WATER_DEPTH, hv.data(), hv.size()), "checkpoint");
+      size_t nonfinite = 0;
+      for (double v : hv) nonfinite += std::isfinite(v) ? 0 : 1;
+      if (nonfinite)
+        throw Error("checkpoint: the state has " + std::to_string(nonfinite) +
+                    " non-finite water depths; not written");
+    }
     size_t nimg = 0;
     check(hc_state_bytes(ctx_, &nimg), "checkpoint");
     std::vector<unsigned char> img(nimg);
     check(hc_state_save(ctx_, img.data(), nimg), "checkpoint");
-    std::ofstream f(path, std::ios::binary | std::ios::trunc);
-    if (!f) throw Error("checkpoint: cannot open " + path);
-    f.write(kCkMagic, sizeof(kCkMagic));
-    CkWriter w{f};
-    visit_state(w);
-    const uint64_t n = nimg;
-    f.write(reinterpret_cast<const char*>(&n), sizeof(n));
-    f.write(reinterpret_cast<const char*>(img.data()), static_cast<std::streamsize>(nimg));
-    f.flush();
-    if (!f) throw Error("checkpoint: write to " + path + " failed");
+    // written beside the target and renamed over it once complete and on
+    // disk: a write cut short leaves the previous checkpoint as it was
+    const std::string tmp = path + ".tmp";
+    {
+      std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+      if (!f) throw Error("checkpoint: cannot open " + tmp);
+      f.write(kCkMagic, sizeof(kCkMagic));
+      CkWriter w{f};
+      visit_state(w);
+      const uint64_t n = nimg;
+      f.write(reinterpret_cast<const char*>(&n), sizeof(n));
+      f.write(reinterpret_cast<const char*>(img.data()), static_cast<std::streamsize>(nimg));
+      f.flush();
+      if (!f) {
+        std::remove(tmp.c_str());
+        throw Error("checkpoint: write to " + tmp + " failed");
+      }
+    }
+    if (!sync_file(tmp)) {
+      std::remove(tmp.c_str());
+      throw Error("checkpoint: cannot flush " + tmp + " to disk");
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) {
+      std::remove(tmp.c_str());
+      throw Error("checkpoint: cannot replace " + path);
+    }
+  }
+
+  static bool sync_file(const std::string& p) {
+    const int fd = ::open(p.c_str(), O_RDONLY);
+    if (fd < 0) return false;
+    const bool ok = ::fsync(fd) == 0;
+    ::close(fd);
+    return ok;
   }
 
   void restart(const std::string& path) {
@@ -650,6 +690,7 @@ class CatchmentModel {
       throw;
     }
     if (complex_rain_) upload_runoff();  // the device planes of the last TOPMODEL update
+    points_dirty_ = true;                // the restored list goes to the device again
     if (opt_.write_files) rewrite_series_file();
   }
 
@@ -1068,21 +1109,26 @@ class CatchmentModel {
   }
   void catchment_water_input_and_hydrology(double fts) {  // LCM.cpp:3455-3541
     for (int i = 1; i <= rfnum_; ++i) waterinput_ += j_mean_[i] * nActual_[i] * DX_ * DX_;
-    pts_x_.clear();
-    pts_y_.clear();
-    pts_a_.clear();
-    for (int z = 1; z <= totalinputpoints_; ++z) {
-      const int i = in_x_[z], j = in_y_[z];
-      const int rf = rfarea_[at(i, j)];
+    // the point list lives on the device and is uploaded when it changes; a
+    // point's amount depends only on its rain cell, so each step sends one
+    // amount per rain cell (the expression and cap of LCM.cpp:3466-3474)
+    if (points_dirty_) {
+      pts_x_.assign(in_x_.begin() + 1, in_x_.begin() + 1 + totalinputpoints_);
+      pts_y_.assign(in_y_.begin() + 1, in_y_.begin() + 1 + totalinputpoints_);
+      pts_rf_.resize(totalinputpoints_);
+      for (int z = 0; z < totalinputpoints_; ++z) pts_rf_[z] = rfarea_[at(pts_x_[z], pts_y_[z])];
+      check(hc_points_set(ctx_, pts_x_.data(), pts_y_.data(), pts_rf_.data(),
+                          static_cast<int>(pts_x_.size())),
+            "points_set");
+      points_dirty_ = false;
+    }
+    pts_a_.assign(rfnum_ + 1, 0.0);
+    for (int rf = 0; rf <= rfnum_; ++rf) {
       double amt = (j_mean_[rf] * nActual_[rf]) / (input_counter_[rf]) * fts;
       if (amt > ERODEFACTOR_) amt = ERODEFACTOR_;
-      pts_x_.push_back(i);
-      pts_y_.push_back(j);
-      pts_a_.push_back(amt);
+      pts_a_[rf] = amt;
     }
-    check(hc_add_point_inputs(ctx_, pts_x_.data(), pts_y_.data(), pts_a_.data(),
-                              static_cast<int>(pts_x_.size())),
-          "add_point_inputs");
+    check(hc_points_add(ctx_, pts_a_.data(), rfnum_ + 1), "points_add");
     if (cycle_ >= time_1_) {
       do {
         time_1_++;
@@ -1217,6 +1263,7 @@ class CatchmentModel {
         }
       }
     if (totalinputpoints_ == 0) totalinputpoints_ = 1;
+    points_dirty_ = true;
   }
   // LCM.cpp:2071-2270.  The elevation sort is std::sort over
   // (imax+2)(jmax+2) pairs exactly as the reference builds them, so ties
@@ -1410,8 +1457,9 @@ class CatchmentModel {
   std::vector<std::vector<float>> rain_;
   std::vector<double> j_, jo_, j_mean_, old_j_mean_, new_j_mean_;
   std::vector<int> nActual_, input_counter_, in_x_, in_y_;
-  std::vector<int> pts_x_, pts_y_;
+  std::vector<int> pts_x_, pts_y_, pts_rf_;
   std::vector<double> pts_a_;
+  bool points_dirty_ = true;  // the input-point list changed since hc_points_set
   int totalinputpoints_ = 0, counter_ = 0, hours_ = 0;
   double time_step_ = 1, cycle_ = 0, maxdepth_ = 10, temptot_ = 0, waterinput_ = 0,
          waterOut_ = 0, iod_ = 0, time_1_ = 1, save_time_ = 0, creep_time_ = 1, creep_time2_ = 1,

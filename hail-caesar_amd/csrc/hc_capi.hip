@@ -320,7 +320,8 @@ int hc_destroy(hc_ctx* c) {
                   c->hasgrain, c->needgrain, c->bedrock, c->Tau, c->Vel, c->ss, c->sdir,
                   c->etot, c->etot3, c->gtot, c->d_red, c->d_partials, c->d_count, c->d_out,
                   c->d_pts, c->d_amt, c->d_ctl, c->d_rec, c->d_top, c->d_area, c->d_rfarea,
-                  c->d_pmask, c->d_dpart, c->d_wmax, c->d_slide_moves, c->d_runoff};
+                  c->d_pmask, c->d_dpart, c->d_wmax, c->d_slide_moves, c->d_runoff,
+                  c->d_plist, c->d_pamt};
   for (auto& gexec : c->loop_graph)
     if (gexec) (void)hipGraphExecDestroy(gexec);
   for (auto e : c->prof_ev) (void)hipEventDestroy(e);
@@ -560,6 +561,57 @@ int hc_add_point_inputs(hc_ctx* c, const int* x, const int* y, const double* amt
   // the host vectors die at return: the copies above are from pageable
   // memory and complete before hipMemcpyAsync returns
   return HC_OK;
+}
+
+int hc_points_set(hc_ctx* c, const int* x, const int* y, const int* rain_cell, int n) {
+  if (!c || n < 0 || (n > 0 && (!x || !y || !rain_cell))) return HC_ERR_ARG;
+  if (c->pt_bits.size() != static_cast<size_t>(c->NR) * c->W)
+    c->pt_bits.assign(static_cast<size_t>(c->NR) * c->W, 0);
+  std::vector<int> buf(2 * static_cast<size_t>(n));
+  int bad = 0;  // 1: outside the grid, 2: repeated cell, 3: rain cell out of range
+  for (int k = 0; k < n; ++k) {
+    if (x[k] < 0 || x[k] >= c->NR || y[k] < 0 || y[k] >= c->NC) { bad = 1; break; }
+    if (rain_cell[k] < 0 || rain_cell[k] >= HC_RF_MAX) { bad = 3; break; }
+    uint64_t& w = c->pt_bits[static_cast<size_t>(x[k]) * c->W + (y[k] >> 6)];
+    const uint64_t b = 1ull << (y[k] & 63);
+    if (w & b) { bad = 2; break; }
+    w |= b;
+    buf[k] = x[k] * c->P + y[k];
+    buf[n + k] = rain_cell[k];
+  }
+  for (int k = 0; k < n; ++k)
+    if (x[k] >= 0 && x[k] < c->NR && y[k] >= 0 && y[k] < c->NC)
+      c->pt_bits[static_cast<size_t>(x[k]) * c->W + (y[k] >> 6)] = 0;
+  if (bad == 1) return hc_fail(c, HC_ERR_ARG, "points_set: point outside the padded grid");
+  if (bad == 2) return hc_fail(c, HC_ERR_ARG, "points_set: a cell is listed twice");
+  if (bad == 3) return hc_fail(c, HC_ERR_ARG, "points_set: rain cell out of range");
+  if (n > c->plist_cap || !c->d_pamt) {
+    HC_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->d_plist) (void)hipFree(c->d_plist);
+    c->d_plist = nullptr;
+    const int cap = std::max(n, 1024);
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_plist), 2 * static_cast<size_t>(cap) * sizeof(int)));
+    if (!c->d_pamt) HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_pamt), HC_RF_MAX * sizeof(double)));
+    c->plist_cap = cap;
+  }
+  if (n > 0) {
+    HC_HIP(c, hipMemcpyAsync(c->d_plist, buf.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HC_HIP(c, hipMemcpyAsync(c->d_plist + c->plist_cap, buf.data() + n, n * sizeof(int),
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  c->plist_n = n;
+  return HC_OK;
+}
+
+int hc_points_add(hc_ctx* c, const double* amt, int nrf) {
+  if (!c || !amt || nrf < 1 || nrf > HC_RF_MAX) return HC_ERR_ARG;
+  if (c->plist_n < 0) return hc_fail(c, HC_ERR_STATE, "points_add: no point list (hc_points_set)");
+  if (c->plist_n == 0) return HC_OK;
+  double a[HC_RF_MAX] = {0};
+  for (int k = 0; k < nrf; ++k) a[k] = amt[k];
+  // pageable source: the copy has consumed `a` when it returns
+  HC_HIP(c, hipMemcpyAsync(c->d_pamt, a, sizeof(a), hipMemcpyHostToDevice, c->stream));
+  return hc_launch_points_rc(c, c->plist_n, c->d_plist, c->d_plist + c->plist_cap, c->d_pamt);
 }
 
 int hc_add_point_inputs_noalloc(hc_ctx* c, const int* x, const int* y, const double* amt, int n) {

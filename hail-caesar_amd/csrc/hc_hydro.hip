@@ -851,6 +851,17 @@ __global__ void k_points(int n, const int* __restrict__ idx, const double* __res
   }
 }
 
+// Point inputs of the persistent list: the amount by the point's rain cell.
+template <typename T>
+__global__ void k_points_rc(int n, const int* __restrict__ idx, const int* __restrict__ rc,
+                            const double* __restrict__ amt, T* __restrict__ h) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    T* p = &h[idx[k]];
+    *p = static_cast<T>(static_cast<double>(*p) + amt[rc[k]]);
+  }
+}
+
 // Point inputs where a cell may appear more than once: the reference adds
 // the amounts one after another in list order (LCM.cpp:3463-3477).  Every
 // point scans the whole list through LDS; the first occurrence of a cell adds
@@ -1301,6 +1312,19 @@ int hc_launch_points_ordered(hc_ctx* c, int n, const int* idx, const int* xs, co
   else
     hipLaunchKernelGGL(k_points_ordered<float>, dim3(nb), dim3(256), 0, c->stream, n, c->P, idx,
                        xs, ys, amt, (float*)c->h[c->cur]);
+  HC_HIP(c, hipGetLastError());
+  return HC_OK;
+}
+
+int hc_launch_points_rc(hc_ctx* c, int n, const int* idx, const int* rc, const double* amt) {
+  if (n <= 0) return HC_OK;
+  const int nb = (n + 255) / 256;
+  if (c->prec == HC_FP64)
+    hipLaunchKernelGGL(k_points_rc<double>, dim3(nb), dim3(256), 0, c->stream, n, idx, rc, amt,
+                       (double*)c->h[c->cur]);
+  else
+    hipLaunchKernelGGL(k_points_rc<float>, dim3(nb), dim3(256), 0, c->stream, n, idx, rc, amt,
+                       (float*)c->h[c->cur]);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

@@ -252,6 +252,9 @@ struct hc_ctx {
   double* d_amt = nullptr;              // point input amounts
   int pts_cap = 0;
   std::vector<uint64_t> pt_bits;        // host: repeated-point test, one bit per padded cell
+  int* d_plist = nullptr;               // persistent input points: cells (x*P+y), then rain cells
+  double* d_pamt = nullptr;             // their per-rain-cell amounts (HC_RF_MAX)
+  int plist_n = -1, plist_cap = 0;      // -1: no list set
 
   std::string err;
 };
@@ -309,6 +312,8 @@ int hc_launch_points(hc_ctx* c, int n);
 // (x*P+y), or from xs/ys when idx is null.
 int hc_launch_points_ordered(hc_ctx* c, int n, const int* idx, const int* xs, const int* ys,
                              const double* amt);
+// h[idx[k]] += amt[rc[k]] for the persistent point list (distinct cells).
+int hc_launch_points_rc(hc_ctx* c, int n, const int* idx, const int* rc, const double* amt);
 int hc_launch_health(hc_ctx* c);
 int hc_launch_fill(hc_ctx* c, void* p, size_t n, double v, bool is_float);
 // erosion (hc_erode.hip)

@@ -36,7 +36,7 @@ INT_FIELDS = {"mask", "hasgrain"}
 SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upload",
            "hc_download", "hc_flow_route", "hc_depth_update", "hc_flow_depth",
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs",
-           "hc_add_point_inputs_noalloc", "hc_erode",
+           "hc_add_point_inputs_noalloc", "hc_points_set", "hc_points_add", "hc_erode",
            "hc_global_landsliding", "hc_creep", "hc_in_channel_landsliding",
            "hc_reach_inputs", "hc_runoff_set", "hc_runoff_add", "hc_synchronize", "hc_check_health", "hc_state_bytes", "hc_state_save",
            "hc_state_load", "hc_loop_begin", "hc_loop_run",
@@ -204,6 +204,8 @@ def lib():
         L.hc_water_flux_out.argtypes = [vp, d, ctypes.POINTER(d)]
         L.hc_add_point_inputs.argtypes = [vp, vp, vp, vp, i]
         L.hc_add_point_inputs_noalloc.argtypes = [vp, vp, vp, vp, i]
+        L.hc_points_set.argtypes = [vp, vp, vp, vp, i]
+        L.hc_points_add.argtypes = [vp, vp, i]
         L.hc_erode.argtypes = [vp, d, ctypes.POINTER(d), ctypes.POINTER(d), vp]
         L.hc_global_landsliding.argtypes = [vp, d, ctypes.POINTER(i)]
         L.hc_creep.argtypes = [vp, d, d]
@@ -386,6 +388,18 @@ class Context:
         a = np.ascontiguousarray(amts, dtype=np.float64)
         self._check(self.L.hc_add_point_inputs_noalloc(self.h, x.ctypes.data, y.ctypes.data,
                                                         a.ctypes.data, len(x)))
+
+    def points_set(self, xs: Sequence[int], ys: Sequence[int], rain_cell: Sequence[int]) -> None:
+        """hc_points_set: the persistent input-point list and each point's rain cell."""
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        y = np.ascontiguousarray(ys, dtype=np.int32)
+        r = np.ascontiguousarray(rain_cell, dtype=np.int32)
+        self._check(self.L.hc_points_set(self.h, x.ctypes.data, y.ctypes.data, r.ctypes.data, len(x)))
+
+    def points_add(self, amt_by_rain_cell: Sequence[float]) -> None:
+        """hc_points_add: every listed point takes its rain cell's amount."""
+        a = np.ascontiguousarray(amt_by_rain_cell, dtype=np.float64)
+        self._check(self.L.hc_points_add(self.h, a.ctypes.data, len(a)))
 
     def erode(self, mult: float, time_step: float):
         ts = ctypes.c_double(time_step)

@@ -171,6 +171,19 @@ int hc_water_flux_out(hc_ctx* ctx, double time_step, double* temptot);
 int hc_add_point_inputs(hc_ctx* ctx, const int* x, const int* y,
                         const double* amt, int n);
 
+/* The same injection with the point list kept on the device, for loops
+ * that inject every step: the reference's amount at a point depends only on
+ * the point's rain cell, (j_mean[rf] * nActualGridCells[rf]) /
+ * catchment_input_counter[rf] * flow_timestep capped at ERODEFACTOR
+ * (LCM.cpp:3463-3477).  hc_points_set uploads the list (cells x[k], y[k]
+ * in the padded grid, which must be distinct, and each point's rain cell
+ * rain_cell[k] in 0..HC_RF_MAX-1) when it changes (get_catchment_input_points,
+ * LCM.cpp:3741-3767); hc_points_add then adds amt[rain_cell[k]] to every
+ * point's depth.  hc_points_add before any hc_points_set returns
+ * HC_ERR_STATE; n = 0 sets an empty list. */
+int hc_points_set(hc_ctx* ctx, const int* x, const int* y, const int* rain_cell, int n);
+int hc_points_add(hc_ctx* ctx, const double* amt, int nrf);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): hc_add_point_inputs without any allocation.  Where

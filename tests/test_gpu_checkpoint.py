@@ -38,9 +38,10 @@ def _erosion_workload(path, size=48, seed=1013):
 
 
 def _case(tmp_path, case):
-    if case == "erosion":
+    if case in ("erosion", "erosion-fp32"):
         w = _erosion_workload(os.path.join(str(tmp_path), "ero"))
-        return w, {}, HYDRO_FIELDS + SED_FIELDS, 160, 53
+        kw = {"precision": hc.HC_FP32} if case == "erosion-fp32" else {}
+        return w, kw, HYDRO_FIELDS + SED_FIELDS, 160, 53
     if case == "complex-rain":
         w = inputs.complex_rain(os.path.join(str(tmp_path), "cr"), 3)
         return w, {}, HYDRO_FIELDS, 700, 333
@@ -49,7 +50,8 @@ def _case(tmp_path, case):
     return w, kw, HYDRO_FIELDS, 400, 137
 
 
-@pytest.mark.parametrize("case", ["fp64", "fp32", "fp32-fixed", "erosion", "complex-rain"])
+@pytest.mark.parametrize("case", ["fp64", "fp32", "fp32-fixed", "erosion", "erosion-fp32",
+                                  "complex-rain"])
 def test_restart_continues_bit_identical(tmp_path, case):
     w, kw, fields, n, k = _case(tmp_path, case)
     whole = Model(w.path, w.params_file, **kw)
@@ -73,7 +75,7 @@ def test_restart_continues_bit_identical(tmp_path, case):
     assert np.array_equal(cont.series(), whole.series())
     for s in SCALARS:
         assert cont.scalar(s) == whole.scalar(s), s
-    if case == "erosion":  # the bed moved on both sides of the checkpoint
+    if case.startswith("erosion"):  # the bed moved on both sides of the checkpoint
         e0 = Model(w.path, w.params_file, **kw).download("elev")
         assert not np.array_equal(e0, ek) and not np.array_equal(ek, whole.download("elev"))
 
@@ -186,3 +188,29 @@ def test_cli_time_slices_continue_bit_identical(tmp_path):
         with open(os.path.join(whole, "out", name), "rb") as fa, \
                 open(os.path.join(sliced, "out", name), "rb") as fb:
             assert fa.read() == fb.read(), name
+
+
+def test_checkpoint_write_is_atomic(tmp_path):
+    """A checkpoint is written beside its target and renamed over it when
+    complete: a write cut short (a truncated `.tmp` left behind) does not
+    touch the previous checkpoint, which still restarts bit for bit, and the
+    next checkpoint replaces both."""
+    w, kw, fields, n, k = _case(tmp_path, "fp64")
+    a = Model(w.path, w.params_file, **kw)
+    a.iterate(k)
+    ck = os.path.join(str(tmp_path), "run.ckpt")
+    a.checkpoint(ck)
+    good = open(ck, "rb").read()
+    with open(ck + ".tmp", "wb") as f:  # what a slice killed mid-write leaves
+        f.write(good[: len(good) // 3])
+    assert open(ck, "rb").read() == good
+    b = Model(w.path, w.params_file, **kw)
+    b.restart(ck)
+    tb, _, _ = b.iterate(20)
+    ta, _, _ = a.iterate(20)
+    assert np.array_equal(ta, tb)
+    a.checkpoint(ck)  # replaces the stale .tmp and the old checkpoint
+    assert not os.path.exists(ck + ".tmp")
+    c = Model(w.path, w.params_file, **kw)
+    c.restart(ck)
+    assert c.scalar("counter") == a.scalar("counter")

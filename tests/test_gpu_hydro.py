@@ -10,6 +10,7 @@ runs are held to the 1e-10 relative bar.
 import numpy as np
 import pytest
 
+from hail_caesar_amd import hc
 from hcutil import rel_err, to_context, wet_flow_state
 
 pytestmark = pytest.mark.gpu
@@ -126,6 +127,34 @@ def test_point_inputs_repeated_cells(tmp_path, precision):
         h[x, y] = dt(np.float64(h[x, y]) + a)
     ctx.add_point_inputs(xs, ys, amts)
     assert np.array_equal(ctx.download("water_depth"), h.astype(np.float64))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_persistent_point_list(tmp_path, precision):
+    """hc_points_set + hc_points_add (the list kept on the device, amounts by
+    rain cell) give the bits of hc_add_point_inputs with each point's amount
+    written out, over three steps with a list change between them; a repeated
+    cell, an out-of-grid point and an add before any list are refused."""
+    w, o = wet_flow_state(tmp_path, size=64, steps=0)
+    a = to_context(o, precision=precision)
+    b = to_context(o, precision=precision)
+    rng = np.random.Generator(np.random.PCG64(5))
+    with pytest.raises(hc.HCError):
+        a.points_add([0.0, 1.0])
+    for step in range(3):
+        if step != 1:  # a new list on steps 0 and 2
+            flat = rng.choice(66 * 66, size=1500, replace=False)
+            xs, ys = flat // 66, flat % 66
+            rc = rng.integers(0, 4, size=len(xs))
+            a.points_set(xs, ys, rc)
+        amt = rng.uniform(0, 1e-3, size=4)
+        a.points_add(amt)
+        b.add_point_inputs(xs, ys, amt[rc])
+        assert np.array_equal(a.download("water_depth"), b.download("water_depth")), step
+    with pytest.raises(hc.HCError):
+        a.points_set([3, 3], [4, 4], [1, 1])
+    with pytest.raises(hc.HCError):
+        a.points_set([70], [1], [1])
 
 
 def test_multistep_fused_vs_oracle(tmp_path):
