@@ -11,6 +11,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <limits>
 #include <cctype>
 #include <climits>
 #include <cmath>
@@ -40,6 +41,14 @@ constexpr int kDY8[9] = {0, -1, -1, 0, 1, 1, 1, 0, -1};
 struct Error : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+// A run stopped by a failure test (or a device loop halted early): the
+// C-ABI code to return and, for a device-resident run, the bodies that ran.
+struct Halted : Error {
+  int code, done;
+  Halted(const std::string& m, int c, int d = -1) : Error(m), code(c), done(d) {}
+};
+// The timestep floor when hcm_options.dt_floor is 0.
+constexpr double kDefaultDtFloor = 1e-4;
 
 // x86-64 behaviour of static_cast<int>(double) on out-of-range values
 // (reached at LSDCatchmentModel.cpp:2283 when erode returns 0).
@@ -451,9 +460,33 @@ class CatchmentModel {
   int get_maxcycle() const { return maxcycle_; }
   bool is_hydro_only() const { return hydro_only_; }
 
+  // The timestep floor in force (0: none).
+  double dt_floor() const {
+    if (opt_.dt_floor < 0) return 0.0;
+    return opt_.dt_floor > 0 ? opt_.dt_floor : kDefaultDtFloor;
+  }
+  // Failure tests before a body, on the previous body's maximum depth: the
+  // depth kernels report +inf for a non-finite or large negative depth, and
+  // the flow timestep of the next body is the Courant limit at that maximum
+  // (set_maximum_timestep holds time_step at or above it; the TOPMODEL
+  // catch-up lowers it to max_time_step at the least).  Nothing of the body
+  // has run when either throws.  The device loop makes the same tests.
+  void check_health() const {
+    if (!(maxdepth_ < std::numeric_limits<double>::infinity()))
+      throw Halted("unstable: a water depth went non-finite or below -1e-4 m (counter " +
+                       std::to_string(counter_) + ")", HC_ERR_UNSTABLE);
+    const double floor = std::min(dt_floor(), static_cast<double>(max_time_step_));
+    const double md = maxdepth_ <= 0.1 ? 0.1 : maxdepth_;
+    if (floor > 0 && courant_number_ * (DX_ / std::sqrt(kGravity * md)) < floor)
+      throw Halted("unstable: the flow timestep fell below the floor of " +
+                       std::to_string(floor) + " s (maximum depth " + std::to_string(maxdepth_) +
+                       " m, counter " + std::to_string(counter_) + ")", HC_ERR_UNSTABLE);
+  }
+
   // One loop body of main.cpp:139-202; returns true when main.cpp:204 ends
   // the run.
   bool iterate() {
+    check_health();
     set_loop_cycle();
     set_inputoutput_diff();
     set_maximum_timestep();
@@ -494,7 +527,9 @@ class CatchmentModel {
     std::vector<hc_loop_record> rec(static_cast<size_t>(std::max(n, 1)));
     hc_loop_state st{};
     int nrec = 0;
-    check(hc_loop_sync(ctx_, &st, rec.data(), n, &nrec), "loop_sync");
+    const int src = hc_loop_sync(ctx_, &st, rec.data(), n, &nrec);
+    if (src != HC_OK && src != HC_ERR_UNSTABLE && src != HC_ERR_HALTED) check(src, "loop_sync");
+    const std::string halt_msg = src != HC_OK ? hc_last_error(ctx_) : "";
     const int k = std::min(nrec, n);
     for (int i = 0; i < k; ++i) {
       const hc_loop_record& r = rec[i];
@@ -524,6 +559,8 @@ class CatchmentModel {
       new_j_mean_[m] = st.new_j_mean[m];
     }
     *fin = st.finished != 0;
+    // a halted loop: the bodies that ran are replayed above, then the cause
+    if (src != HC_OK) throw Halted(halt_msg, src, k);
     return k;
   }
 
@@ -584,6 +621,7 @@ class CatchmentModel {
     in.npoints = totalinputpoints_;
     in.first_x = in_x_[1];
     in.first_y = in_y_[1];
+    in.dt_floor = std::min(dt_floor(), static_cast<double>(std::max(0, max_time_step_)));
     check(hc_loop_begin(ctx_, &in), "loop_begin");
     loop_started_ = true;
   }
@@ -1510,6 +1548,7 @@ static hcm_options default_options(const hcm_options* opt) {
   o.device_loop = 0;
   o.use_graph = 0;
   o.elev_fixed = 0;
+  o.dt_floor = 0.0;
   if (opt) o = *opt;
   return o;
 }
@@ -1590,6 +1629,11 @@ int hcm_iterate(hcm_model* h, int n, double* tt, double* cy, double* dtf, int* d
         if (dtf) dtf[k] = h->m->last_dt();
       }
     }
+  } catch (const hcdrv::Halted& e) {
+    h->err = e.what();
+    if (done) *done = e.done >= 0 ? e.done : k;
+    if (finished) *finished = 0;
+    return e.code;
   } catch (const std::exception& e) {
     h->err = e.what();
     if (done) *done = k;
@@ -1618,6 +1662,11 @@ int hcm_loop_collect(hcm_model* h, int n, double* tt, double* cy, double* dtf, i
   int k = 0;
   try {
     k = h->m->collect_device(n, tt, cy, dtf, &fin);
+  } catch (const hcdrv::Halted& e) {
+    h->err = e.what();
+    if (done) *done = e.done;
+    if (finished) *finished = 0;
+    return e.code;
   } catch (const std::exception& e) {
     h->err = e.what();
     return HC_ERR_HIP;

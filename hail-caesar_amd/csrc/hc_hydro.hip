@@ -200,6 +200,7 @@ __global__ __launch_bounds__(256) void k_depth64(HydroArgs a, double dt, T* __re
       if (z[c] == -9999) hv = 0;
       if (hv > md) md = hv;
     }
+    if (!(hv >= hc::kUnstableDepth)) md = INFINITY;  // NaN, -inf or a large negative
     h[c] = static_cast<T>(hv);
   }
   block_max_pos(red, md);
@@ -390,6 +391,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   };
   const T nu = static_cast<T>(a.mannings);
   T md = 0;
+  bool sick = false;  // an updated depth is non-finite or below kUnstableDepth
   // water_flux_out (LCM.cpp:3005-3047) on this tile's edge cells, after
   // depth_update and before the store; maxdepth is taken before the drain
   const bool drains = drain.ts != nullptr && fused_edge_tile(bx, by, H, a.imax, a.jmax, SW);
@@ -646,6 +648,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
             if (zt[k - 1] == Z(-9999)) hn = T(0);
             if (hn > md) md = hn;
           }
+          sick |= !(hn >= T(hc::kUnstableDepth));
         }
       } else {
         const float t = __builtin_fmaf(dtDX, (qxN - qxNp) + (qyRp - qyNp), hh[k - 1]);
@@ -653,6 +656,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
         const bool zbad = z_nodata(zt[k - 1]);
         hn = mu ? ((pos && zbad) ? 0.0f : t) : hh[k - 1];
         md = fmaxf(md, (mu && pos && !zbad) ? t : 0.0f);
+        sick |= mu && !(t >= static_cast<float>(hc::kUnstableDepth));
       }
       if (drains && owned && xo >= 1 && xo <= a.imax && c >= 1 && c <= a.jmax &&
           (xo == 1 || xo == a.imax || c == 1 || c == a.jmax)) {
@@ -731,7 +735,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       for (int k = 2; k < R + 2; ++k) row(win, k, r0);
     }
   }
-  const double mine = owned && active_wave ? static_cast<double>(md) : 0.0;
+  const double mine = owned && active_wave ? (sick ? INFINITY : static_cast<double>(md)) : 0.0;
   if (p.wmax) {  // uniform
     const double wm = wave_max(mine);
     if (lane == 0) p.wmax[4 * blockIdx.x + wv] = wm;

@@ -22,6 +22,12 @@ namespace hc {
 
 constexpr double kGravity = 9.81;
 constexpr double kNoData = -9999.0;
+// A depth update that leaves a cell non-finite or below this depth (m) marks
+// the step unstable: the reported maximum depth becomes +inf.  The outflow
+// limiter (no face drains more than a quarter of its cell) keeps a sound
+// update at or above zero up to rounding, a few ulp of the depth, which stays
+// far above -1e-4 m for any depth below 100 m even in FP32.
+constexpr double kUnstableDepth = -1e-4;
 
 // Hydro kernel arguments (all passed by value).
 struct HydroArgs {
@@ -56,15 +62,21 @@ struct LoopCtl {
   double previous, old_cycle, new_cycle, time_1, baseflow, dt_flow;
   double j_mean[kRfMax], old_j_mean[kRfMax], new_j_mean[kRfMax], amt[kRfMax];
   int input_counter[kRfMax], nActual[kRfMax];
-  int counter, halt, err, npts, pts_dirty, first_new, last_first, iter;
+  int counter, halt, err, npts, pts_dirty, first_new, last_first, iter;  // err: kLoopErr*
   int halt_pending;  // the end-of-run test passed: halt before the next body
 };
 
 static_assert(kRfMax == HC_RF_MAX, "rain cell capacity");
 using LoopRecord = hc_loop_record;
 
+// LoopCtl::err values: the loop halts and hc_loop_sync reports the cause.
+constexpr int kLoopErrTable = 1;     // the TOPMODEL table ran out (HC_ERR_HALTED)
+constexpr int kLoopErrDepth = 2;     // a non-finite or negative depth (HC_ERR_UNSTABLE)
+constexpr int kLoopErrDtFloor = 3;   // the flow timestep fell below the floor (HC_ERR_UNSTABLE)
+
 struct LoopArgs {
   double courant, DX, in_out_allowed, osi, ERODEFACTOR, MIN_Q, MIN_Q_MAXVAL;
+  double dt_floor;  // stop (unstable) before a body whose flow timestep is below this
   int max_time_step, maxcycle, rfnum, imax, jmax, pitch, ntop, scan_interval;
   int mwords;  // words per row of the active and input-point bit masks
 };

@@ -62,6 +62,24 @@ __device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restr
       stop = true;
     }
     if (!stop) {
+      // health of the previous body, tested before anything of this one
+      // moves: its depth kernel reports +inf for a non-finite or large
+      // negative depth.  This body's flow timestep is min(time_step, the
+      // Courant limit at that maximum); set_maximum_timestep keeps time_step
+      // at or above the limit and the TOPMODEL catch-up lowers it to no less
+      // than max_time_step, which hc_loop_begin holds at or above the floor,
+      // so the timestep falls below the floor exactly when the limit does.
+      const double md = __longlong_as_double(static_cast<long long>(red[0]));
+      if (!(md < INFINITY))
+        C.err = hc::kLoopErrDepth;
+      else if (a.dt_floor > 0 && cfl(a, md <= 0.1 ? 0.1 : md) < a.dt_floor)
+        C.err = hc::kLoopErrDtFloor;
+      if (C.err) {
+        C.halt = 1;
+        stop = true;
+      }
+    }
+    if (!stop) {
       // a list rebuilt by the previous iteration: an empty one keeps a single
       // stale point, in_x[1]/in_y[1] of the last non-empty list (LCM.cpp:3764)
       if (C.pts_dirty) {
@@ -103,7 +121,7 @@ __device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restr
           C.time_1++;
           const int t = static_cast<int>(C.time_1);
           if (t < 0 || t >= a.ntop) {
-            C.err = 1;
+            C.err = hc::kLoopErrTable;
             C.halt = 1;
             break;
           }
@@ -428,6 +446,8 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
     return hc_fail(c, HC_ERR_ARG, "loop: bad input point count");
   if (in->first_x < 0 || in->first_x >= c->NR || in->first_y < 0 || in->first_y >= c->NC)
     return hc_fail(c, HC_ERR_ARG, "loop: first input point outside the padded grid");
+  if (!(in->dt_floor >= 0) || in->dt_floor > in->max_time_step)
+    return hc_fail(c, HC_ERR_ARG, "loop: dt_floor must lie in [0, max_time_step]");
   HC_HIP(c, hipStreamSynchronize(c->stream));
   LoopArgs& a = c->largs;
   a.courant = in->courant_number;
@@ -437,6 +457,7 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
   a.ERODEFACTOR = in->ERODEFACTOR;
   a.MIN_Q = in->MIN_Q;
   a.MIN_Q_MAXVAL = in->MIN_Q_MAXVAL;
+  a.dt_floor = in->dt_floor;
   a.max_time_step = in->max_time_step;
   a.maxcycle = in->maxcycle;
   a.rfnum = in->rfnum;
@@ -651,7 +672,15 @@ int hc_loop_sync(hc_ctx* c, hc_loop_state* st, hc_loop_record* rec, int max_reco
   if (rec && nr > 0)
     HC_HIP(c, hipMemcpy(rec, c->d_rec, sizeof(LoopRecord) * nr, hipMemcpyDeviceToHost));
   if (nrec) *nrec = C.iter;
-  if (C.err) return hc_fail(c, HC_ERR_HALTED, "loop: TOPMODEL table exhausted");
+  if (C.err == hc::kLoopErrTable) return hc_fail(c, HC_ERR_HALTED, "loop: TOPMODEL table exhausted");
+  if (C.err == hc::kLoopErrDepth)
+    return hc_fail(c, HC_ERR_UNSTABLE, "loop: non-finite or negative water depth after iteration " +
+                                           std::to_string(C.iter) + " of the call");
+  if (C.err == hc::kLoopErrDtFloor)
+    return hc_fail(c, HC_ERR_UNSTABLE, "loop: flow timestep below the floor (" +
+                                           std::to_string(c->largs.dt_floor) + " s) after iteration " +
+                                           std::to_string(C.iter) + " of the call");
+  if (C.err) return hc_fail(c, HC_ERR_HALTED, "loop: halted");
   return HC_OK;
 }
 

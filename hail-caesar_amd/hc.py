@@ -87,7 +87,7 @@ class hc_loop_init(ctypes.Structure):
         ("input_counter", ctypes.c_int * RF_MAX), ("topmodel", ctypes.c_void_p),
         ("ntop", ctypes.c_int), ("area", ctypes.c_void_p), ("rfarea", ctypes.c_void_p),
         ("in_x", ctypes.c_void_p), ("in_y", ctypes.c_void_p), ("npoints", ctypes.c_int),
-        ("first_x", ctypes.c_int), ("first_y", ctypes.c_int),
+        ("first_x", ctypes.c_int), ("first_y", ctypes.c_int), ("dt_floor", ctypes.c_double),
     ]
 
 
@@ -474,8 +474,12 @@ class Context:
         rec = np.zeros(max(1, max_records), dtype=RECORD_DTYPE)
         n = ctypes.c_int()
         rc = self.L.hc_loop_sync(self.h, ctypes.byref(st), rec.ctypes.data, max_records, ctypes.byref(n))
-        self._check(rc)
-        return st, rec[:min(n.value, max_records)]
+        rec = rec[:min(n.value, max_records)]
+        if rc != HC_OK:  # a halted loop still reports its state and records
+            err = HCError(rc, self.L.hc_last_error(self.h).decode())
+            err.state, err.records = st, rec
+            raise err
+        return st, rec
 
     def loop_profile(self, on: bool) -> None:
         self._check(self.L.hc_loop_profile(self.h, 1 if on else 0))

@@ -31,7 +31,7 @@ class hcm_options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("fused", ctypes.c_int), ("quiet", ctypes.c_int),
                 ("write_files", ctypes.c_int), ("flat_dz", ctypes.c_double),
                 ("device_loop", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("elev_fixed", ctypes.c_int)]
+                ("elev_fixed", ctypes.c_int), ("dt_floor", ctypes.c_double)]
 
 
 class hcm_arrays(ctypes.Structure):
@@ -77,15 +77,21 @@ class Model:
     def __init__(self, path: Optional[str], params_file: Optional[str], precision: int = hc.HC_FP64,
                  fused: bool = True, write_files: bool = False, flat_dz: float = 0.0,
                  device_loop: bool = False, use_graph: bool = False, arrays: Optional[dict] = None,
-                 params: Optional[dict] = None, elev_fixed: bool = False):
+                 params: Optional[dict] = None, elev_fixed: bool = False, dt_floor: float = 0.0):
         """Files (path + params_file, as the reference's CLI) or, with
         ``arrays`` and ``params``, in-memory inputs: arrays holds 'dem' and
         optionally 'mannings', 'water_init', 'hydroindex', 'rain' (rows x
         cells, mm/h), 'graindata' (nrows x ncols x 101: grain[0..10] and
-        strata[10][9] per cell, the grain data file's values) plus 'DX'."""
+        strata[10][9] per cell, the grain data file's values) plus 'DX'.
+
+        dt_floor: a body whose flow timestep would fall below it (seconds)
+        stops the run with HC_ERR_UNSTABLE, as does a depth gone non-finite
+        or below -1e-4 m; 0 selects 1e-4 s, a negative value turns the
+        timestep test off."""
         self.L = _lib()
         opt = hcm_options(precision, 1 if fused else 0, 1, 1 if write_files else 0, flat_dz,
-                          1 if device_loop else 0, 1 if use_graph else 0, 1 if elev_fixed else 0)
+                          1 if device_loop else 0, 1 if use_graph else 0, 1 if elev_fixed else 0,
+                          dt_floor)
         h = ctypes.c_void_p()
         self.workdir = None
         if arrays is None:
@@ -150,9 +156,11 @@ class Model:
         with _in_dir(self.workdir):
             rc = self.L.hcm_iterate(self.h, n, tt.ctypes.data, cy.ctypes.data, dt.ctypes.data,
                                     ctypes.byref(done), ctypes.byref(fin))
-        if rc != 0:
-            raise hc.HCError(rc, self.L.hcm_last_error(self.h).decode())
         k = done.value
+        if rc != 0:  # a stopped run keeps the records of the bodies that ran
+            err = hc.HCError(rc, self.L.hcm_last_error(self.h).decode())
+            err.results = (tt[:k], cy[:k], dt[:k])
+            raise err
         self.finished = bool(fin.value)
         return tt[:k], cy[:k], dt[:k]
 
@@ -248,17 +256,26 @@ class Ensemble:
     def collect(self, steps: int):
         """Per member (temptot, cycle, dt_flow) of the last run's steps."""
         out = []
-        for m in self.models:
+        first = None  # the first member that stopped on a failure
+        for i, m in enumerate(self.models):
             tt, cy, dt = np.zeros(steps), np.zeros(steps), np.zeros(steps)
             done, fin = ctypes.c_int(), ctypes.c_int()
             with _in_dir(m.workdir):
                 rc = self.L.hcm_loop_collect(m.h, steps, tt.ctypes.data, cy.ctypes.data,
                                              dt.ctypes.data, ctypes.byref(done), ctypes.byref(fin))
-            if rc != 0:
-                raise hc.HCError(rc, self.L.hcm_last_error(m.h).decode())
-            m.finished = bool(fin.value)
             k = done.value
+            if rc != 0:
+                if rc not in (hc.HC_ERR_UNSTABLE, hc.HC_ERR_HALTED):
+                    raise hc.HCError(rc, self.L.hcm_last_error(m.h).decode())
+                if first is None:
+                    first = hc.HCError(rc, f"member {i}: " + self.L.hcm_last_error(m.h).decode())
+                    first.member = i
+            else:
+                m.finished = bool(fin.value)
             out.append((tt[:k], cy[:k], dt[:k]))
+        if first is not None:  # every member's records are collected first
+            first.results = out
+            raise first
         return out
 
     def iterate(self, steps: int, use_graph: bool = True):

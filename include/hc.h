@@ -311,6 +311,10 @@ typedef struct hc_loop_init {
   const int* in_y;
   int npoints;            /* totalinputpoints (an empty list is 1 stale point) */
   int first_x, first_y;   /* in_x[1], in_y[1] kept for that case */
+  /* Failure detection (the reference has none): a body whose flow timestep
+   * would fall below dt_floor seconds is not run and the loop halts with
+   * HC_ERR_UNSTABLE.  0 disables the test; at most max_time_step. */
+  double dt_floor;
 } hc_loop_init;
 
 /* One record per iteration for output_data's interval series. */
@@ -321,6 +325,9 @@ typedef struct hc_loop_record {
 
 typedef struct hc_loop_state {
   double time_step, cycle, maxdepth, temptot, waterinput, waterOut, time_1, baseflow;
+  /* error: 0, or why the loop halted: 1 the TOPMODEL table ran out, 2 a
+   * depth went non-finite or below -1e-4 m (maxdepth is then +inf), 3 the
+   * flow timestep fell below dt_floor */
   int counter, finished, npoints, error;
   double j_mean[HC_RF_MAX], old_j_mean[HC_RF_MAX], new_j_mean[HC_RF_MAX];
 } hc_loop_state;
@@ -338,7 +345,13 @@ int hc_loop_begin(hc_ctx* ctx, const hc_loop_init* init);
 int hc_loop_run(hc_ctx* ctx, int n, int use_graph);
 
 /* Wait for the enqueued iterations and read the scalar state and up to
- * max_records per-iteration records of the last hc_loop_run. */
+ * max_records per-iteration records of the last hc_loop_run.  A loop that
+ * halted on a failure returns HC_ERR_UNSTABLE (a non-finite or large
+ * negative depth: the state holds the body that produced it; or a flow
+ * timestep below dt_floor: the state holds the last body run) or
+ * HC_ERR_HALTED (TOPMODEL table exhausted), with the state and the records of
+ * the bodies that ran filled in; later calls of hc_loop_run are no-ops until
+ * hc_loop_begin starts over. */
 int hc_loop_sync(hc_ctx* ctx, hc_loop_state* state, hc_loop_record* records,
                  int max_records, int* nrecords);
 

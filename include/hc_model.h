@@ -30,6 +30,13 @@ typedef struct hcm_options {
                      rasters are not written in this mode) */
   int use_graph;  /* with device_loop: replay captured HIP graphs */
   int elev_fixed; /* FP32 form: see hc_params.elev_fixed */
+  /* Failure detection, which the reference does not have: a loop body
+   * whose flow timestep would fall below dt_floor seconds is not run and
+   * hcm_iterate returns HC_ERR_UNSTABLE, as it does after a body that leaves
+   * a depth non-finite or below -1e-4 m.  0 selects 1e-4 s (a Courant
+   * limit that small needs a depth above 4e6 m at 1 m cells); < 0 turns the
+   * timestep test off (the depth test stays). */
+  double dt_floor;
 } hcm_options;
 
 /* Constructor + initialise_model_domain_extents + initialise_arrays +
@@ -64,7 +71,8 @@ const char* hcm_last_error(const hcm_model* m);
  * condition (main.cpp:204) ends the run.  Optional per-iteration records:
  * temptot after water_flux_out, cycle (minutes) and the flow timestep.
  * *done receives the number of iterations performed; *finished 1 when the
- * run is over. */
+ * run is over.  HC_ERR_UNSTABLE: the run stopped on a failure (see
+ * hcm_options.dt_floor); *done and the records cover the bodies that ran. */
 int hcm_iterate(hcm_model* m, int n, double* temptot, double* cycle, double* dtflow,
                 int* done, int* finished);
 
