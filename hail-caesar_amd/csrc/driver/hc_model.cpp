@@ -952,6 +952,8 @@ class CatchmentModel {
       else if (k == "write_elev_file") write_elev_file_ = yes();
       else if (k == "waterdepth_outfile_name") waterdepth_fname_ = value;
       else if (k == "write_waterdepth_file") write_waterd_file_ = yes();
+      else if (k == "grainsize_file") grainsize_fname_ = value;
+      else if (k == "write_grainsize_file") write_grainsz_file_ = yes();
       else if (k == "read_in_graindata_from_file") graindata_from_file_ = yes();
       else if (k == "bedrock_layer_on") bedrock_layer_on_ = yes();
       else if (k == "transport_law") {
@@ -1065,7 +1067,9 @@ class CatchmentModel {
     if (!in) throw Error("No grain data file found by name of: " + f);
     const size_t n = elev_.size();
     std::vector<double> grain(n * 11, 0.0), strata(n * 90, 0.0);
-    std::vector<int32_t> has(n, 0);
+    // index[x1][y1] = grain_array_tot, one number per line (LCM.cpp:638-656)
+    std::vector<int32_t> index(n, -9999);
+    long long tot = 0;
     std::string line;
     while (std::getline(in, line)) {
       std::stringstream ss(line);
@@ -1073,11 +1077,13 @@ class CatchmentModel {
       double val;
       while (ss >> val) cols.push_back(val);
       if (cols.size() < 3) continue;
+      ++tot;
       int x1 = static_cast<int>(cols[0]), y1 = static_cast<int>(cols[1]);
       if (x1 > imax_) x1 = imax_;
       if (y1 > jmax_) y1 = jmax_;
+      if (x1 < 0 || y1 < 0) throw Error("grain data file: negative cell coordinate");
       const size_t c = at(x1, y1);
-      has[c] = 1;
+      index[c] = static_cast<int32_t>(tot);
       for (int k = 0; k <= kGMax && 3 + k < static_cast<int>(cols.size()); ++k) grain[c * 11 + k] = cols[3 + k];
       for (int z = 0; z < 10; ++z)
         for (int k = 0; k < 9; ++k) {
@@ -1087,7 +1093,7 @@ class CatchmentModel {
     }
     upload(HC_FIELD_GRAIN, grain.data());
     upload(HC_FIELD_STRATA, strata.data());
-    upload(HC_FIELD_HASGRAIN, has.data());
+    check(hc_grain_index_set(ctx_, index.data(), n, tot), "grain index");
   }
 
   // the grain data file's values given in memory, one row per cell
@@ -1459,6 +1465,36 @@ class CatchmentModel {
       check(hc_download(ctx_, HC_FIELD_ELEV, e.data(), e.size()), "download elev");
       write_asc(write_path_ + "/" + elev_fname_ + suffix, e);
     }
+    if (write_grainsz_file_) write_grainz(write_path_ + "/" + grainsize_fname_ + suffix);
+  }
+  // The Grainz file (LSDGrainMatrix::write_grainMatrix_to_ascii_file,
+  // LGM.cpp:59-118, called from LCM.cpp:2711-2723): for every cell with a
+  // grain row in row-major order, "x y index", the 11 surface fractions and
+  // the 10 x 9 strata, each value followed by a space, at the stream's
+  // default 6 significant digits.  index is the reference's row number
+  // (hc_grain_index_get).  A hydro-only run has no rows: an empty file.
+  void write_grainz(const std::string& fname) {
+    if (dem_write_extension_ != "asc")
+      throw Error("Grainz output: the only extension is asc; dem_write_extension is " +
+                  dem_write_extension_);
+    std::ofstream out(fname);
+    if (!out) throw Error("unable to write to " + fname);
+    if (hydro_only_) return;
+    const size_t n = elev_.size();
+    std::vector<int32_t> idx(n);
+    check(hc_grain_index_get(ctx_, idx.data(), n, nullptr), "grain index");
+    std::vector<double> g(n * 11), st(n * 90);
+    check(hc_download(ctx_, HC_FIELD_GRAIN, g.data(), g.size()), "download grain");
+    check(hc_download(ctx_, HC_FIELD_STRATA, st.data(), st.size()), "download strata");
+    for (int i = 1; i <= imax_; ++i)
+      for (int j = 1; j <= jmax_; ++j) {
+        const size_t c = at(i, j);
+        if (idx[c] == -9999) continue;
+        out << i << " " << j << " " << idx[c] << " ";
+        for (int k = 0; k <= kGMax; ++k) out << g[c * 11 + k] << " ";
+        for (int k = 0; k < 90; ++k) out << st[c * 90 + k] << " ";
+        out << std::endl;
+      }
   }
 
   hcm_options opt_;
@@ -1468,7 +1504,7 @@ class CatchmentModel {
   std::string dem_read_extension_ = "asc", dem_write_extension_ = "asc", write_path_, read_path_;
   std::string write_fname_ = "catchment.dat", read_fname_, hydroindex_fname_, rainfall_data_file_,
               grain_data_file_, bedrock_data_file_, water_init_raster_file_, mannings_fname_,
-              elev_fname_, waterdepth_fname_;
+              elev_fname_, waterdepth_fname_, grainsize_fname_;
   int imax_ = 0, jmax_ = 0, NR_ = 0, NC_ = 0;
   double xll_ = 0, yll_ = 0, DX_ = 5.0, no_data_value_ = -9999.9;
   // parameters (LSDCatchmentModel.hpp:437-568 defaults)
@@ -1488,7 +1524,7 @@ class CatchmentModel {
        spatially_var_rainfall_ = false, spatially_var_mannings_ = false,
        graindata_from_file_ = false, bedrock_layer_on_ = false, wilcock_ = false,
        einstein_ = false, write_elev_file_ = false, write_waterd_file_ = false,
-       debug_print_cycle_ = false;
+       write_grainsz_file_ = false, debug_print_cycle_ = false;
   // state
   std::vector<double> elev_, area_, area_depth_;
   std::vector<int> rfarea_;

@@ -147,6 +147,34 @@ int put_int_plane(hc_ctx* c, int32_t* dev, const int32_t* host) {
   return HC_OK;
 }
 
+// Grain rows from the reference's row numbers (index[x][y] of the padded
+// grid, -9999 for none): the has-grain flags, and creation keys of epoch 0
+// that keep each row's number.  rows_total is grain_array_tot, the number
+// the next new row follows.
+int put_row_index(hc_ctx* c, const int32_t* index, long long rows_total) {
+  std::vector<int32_t> has(c->ncell, 0);
+  std::vector<uint64_t> key(c->ncell, hc::kRowNone);
+  std::vector<int32_t> seen;
+  for (int x = 0; x < c->NR; ++x)
+    for (int y = 0; y < c->NC; ++y) {
+      const int32_t v = index[static_cast<size_t>(x) * c->NC + y];
+      if (v == -9999) continue;
+      if (v < 1 || v > rows_total)
+        return hc_fail(c, HC_ERR_ARG, "grain index: a row number outside 1..rows_total");
+      const size_t d = static_cast<size_t>(x) * c->P + y;
+      has[d] = 1;
+      key[d] = static_cast<uint64_t>(v);
+      seen.push_back(v);
+    }
+  std::sort(seen.begin(), seen.end());
+  if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+    return hc_fail(c, HC_ERR_ARG, "grain index: two cells share a row number");
+  HC_HIP(c, hipMemcpy(c->hasgrain, has.data(), c->ncell * sizeof(int32_t), hipMemcpyHostToDevice));
+  HC_HIP(c, hipMemcpy(c->rowkey, key.data(), c->ncell * sizeof(uint64_t), hipMemcpyHostToDevice));
+  c->rows_base = rows_total;
+  return HC_OK;
+}
+
 int get_int_plane(hc_ctx* c, const int32_t* dev, int32_t* host) {
   std::vector<int32_t> tmp(c->ncell);
   HC_HIP(c, hipMemcpy(tmp.data(), dev, c->ncell * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -227,6 +255,10 @@ int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
   if (grid->imax < 1 || grid->jmax < 1 || !(grid->DX > 0)) return HC_ERR_ARG;
   if (grid->precision != HC_FP64 && grid->precision != HC_FP32) return HC_ERR_ARG;
   if (grid->sediment && grid->precision == HC_FP32 && params->elev_fixed) return HC_ERR_ARG;
+  // grain-row creation keys hold a cell coordinate in 15 bits
+  if (grid->sediment && (grid->imax + 1 >= (1 << hc::kRowMinorBits) ||
+                         grid->jmax + 1 >= (1 << hc::kRowMinorBits)))
+    return HC_ERR_ARG;
   hc_ctx* c = new (std::nothrow) hc_ctx;
   if (!c) return HC_ERR_ARG;
   c->g = *grid;
@@ -292,12 +324,16 @@ int hc_create(hc_ctx** out, const hc_grid* grid, const hc_params* params) {
         (rc = salloc(c, &c->qys, 1)) || (rc = salloc(c, &c->veldir, 4)) ||
         (rc = salloc(c, &c->grain, 11)) || (rc = salloc(c, &c->strata, 90)) ||
         (rc = dalloc(c, &c->hasgrain, n)) || (rc = dalloc(c, &c->needgrain, n)) ||
+        (rc = dalloc(c, &c->rowkey, n)) ||
         (rc = dalloc(c, &c->bedrock, n)) || (rc = salloc(c, &c->Tau, 1)) ||
         (rc = salloc(c, &c->Vel, 1)) || (rc = salloc(c, &c->ss, 1)) ||
         (rc = salloc(c, &c->sdir, 40)) || (rc = salloc(c, &c->etot, 1)) ||
         (rc = salloc(c, &c->etot3, 1)) || (rc = dalloc(c, &c->gtot, 20)))
       return fail(rc);
     if ((rc = hc_launch_fill(c, c->bedrock, n, -9999.0, false))) return fail(rc);
+    // no cell has a grain row yet
+    if (hipMemsetAsync(c->rowkey, 0xff, n * sizeof(uint64_t), c->stream) != hipSuccess)
+      return fail(HC_ERR_HIP);
   }
   if ((rc = dalloc(c, &c->d_red, 8)) || (rc = dalloc(c, &c->d_partials, 1024)) ||
       (rc = dalloc(c, &c->d_count, 16)) || (rc = dalloc(c, &c->d_out, 8)))
@@ -317,7 +353,7 @@ int hc_destroy(hc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->h[0], c->h[1], c->qx[0], c->qx[1], c->qy[0], c->qy[1], c->z, c->z64, c->zq,
                   c->nman, c->mask, c->V, c->qxs, c->qys, c->veldir, c->grain, c->strata,
-                  c->hasgrain, c->needgrain, c->bedrock, c->Tau, c->Vel, c->ss, c->sdir,
+                  c->hasgrain, c->needgrain, c->rowkey, c->bedrock, c->Tau, c->Vel, c->ss, c->sdir,
                   c->etot, c->etot3, c->gtot, c->d_red, c->d_partials, c->d_count, c->d_out,
                   c->d_pts, c->d_amt, c->d_ctl, c->d_rec, c->d_top, c->d_area, c->d_rfarea,
                   c->d_pmask, c->d_dpart, c->d_wmax, c->d_slide_moves, c->d_runoff,
@@ -398,8 +434,16 @@ int hc_upload(hc_ctx* c, int field, const void* host, size_t count) {
       }
       return HC_OK;
     }
-    case HC_FIELD_HASGRAIN:
-      return put_int_plane(c, c->hasgrain, static_cast<const int32_t*>(host));
+    case HC_FIELD_HASGRAIN: {
+      // rows numbered as a grain data file of one line per flagged cell in
+      // row-major order would number them (hc_grain_index_set gives others)
+      const int32_t* f = static_cast<const int32_t*>(host);
+      std::vector<int32_t> idx(static_cast<size_t>(c->NR) * c->NC, -9999);
+      long long n = 0;
+      for (size_t k = 0; k < idx.size(); ++k)
+        if (f[k]) idx[k] = static_cast<int32_t>(++n);
+      return put_row_index(c, idx.data(), n);
+    }
     case HC_FIELD_MASK: {
       const int32_t* m = static_cast<const int32_t*>(host);
       std::vector<uint64_t> bits(static_cast<size_t>(c->NR) * c->W, 0);
@@ -699,6 +743,50 @@ int hc_in_channel_landsliding(hc_ctx* c, double failureangle) {
 int hc_synchronize(hc_ctx* c) {
   if (!c) return HC_ERR_ARG;
   HC_HIP(c, hipStreamSynchronize(c->stream));
+  return HC_OK;
+}
+
+int hc_grain_index_set(hc_ctx* c, const int32_t* index, size_t count, long long rows_total) {
+  if (!c || !index) return HC_ERR_ARG;
+  if (!c->g.sediment) return hc_fail(c, HC_ERR_STATE, "grain index: not a sediment context");
+  if (count != static_cast<size_t>(c->NR) * c->NC || rows_total < 0 || rows_total > INT32_MAX)
+    return hc_fail(c, HC_ERR_ARG, "grain index: count is not (imax+2)*(jmax+2) or bad rows_total");
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  return put_row_index(c, index, rows_total);
+}
+
+int hc_grain_index_get(hc_ctx* c, int32_t* index, size_t count, long long* rows_total) {
+  if (!c || !index) return HC_ERR_ARG;
+  if (!c->g.sediment) return hc_fail(c, HC_ERR_STATE, "grain index: not a sediment context");
+  if (count != static_cast<size_t>(c->NR) * c->NC)
+    return hc_fail(c, HC_ERR_ARG, "grain index: count is not (imax+2)*(jmax+2)");
+  HC_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<uint64_t> key(c->ncell);
+  std::vector<int32_t> has(c->ncell);
+  HC_HIP(c, hipMemcpy(key.data(), c->rowkey, c->ncell * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  HC_HIP(c, hipMemcpy(has.data(), c->hasgrain, c->ncell * sizeof(int32_t), hipMemcpyDeviceToHost));
+  // rows of the initial data keep their numbers; later rows follow in key order
+  std::vector<std::pair<uint64_t, size_t>> made;
+  for (int x = 0; x < c->NR; ++x)
+    for (int y = 0; y < c->NC; ++y) {
+      const size_t d = static_cast<size_t>(x) * c->P + y, h = static_cast<size_t>(x) * c->NC + y;
+      const uint64_t k = key[d];
+      if (!has[d]) {
+        index[h] = -9999;
+      } else if (k == hc::kRowNone) {
+        return hc_fail(c, HC_ERR_STATE, "grain index: a grain row of unknown creation order");
+      } else if (k < hc::row_epoch_base(1)) {
+        index[h] = static_cast<int32_t>(k);
+      } else {
+        made.emplace_back(k, h);
+      }
+    }
+  std::sort(made.begin(), made.end());
+  if (c->rows_base + static_cast<long long>(made.size()) > INT32_MAX)
+    return hc_fail(c, HC_ERR_STATE, "grain index: more rows than an int holds");
+  for (size_t r = 0; r < made.size(); ++r)
+    index[made[r].second] = static_cast<int32_t>(c->rows_base + 1 + static_cast<long long>(r));
+  if (rows_total) *rows_total = c->rows_base + static_cast<long long>(made.size());
   return HC_OK;
 }
 

@@ -5,8 +5,15 @@
 // in the FP32 storage form, in FP32 (elevations and bedrock stay FP64).
 //
 // Layout: dense per-cell grain (11 planes), strata (10 layers x 9 fractions
-// = 90 planes, plane z*9+n), a has-grain flag standing for index != -9999,
-// and the directional bedload su/sr/sd/sl as 4 x 10 planes.
+// = 90 planes, plane z*9+n), a has-grain flag standing for index != -9999
+// with a creation key per cell that recovers the reference's row numbers
+// (hc::row_key), and the directional bedload su/sr/sd/sl as 4 x 10 planes.
+// Where a row is made, the kernel records the key of the reference's loop
+// position that makes it: the entrainment pass by atomic minimum over the
+// cells that would make it (the first in the reference's order does), the
+// serial hillslope sweeps and the reach points as they make it, and the
+// deposition and y-slide sweeps, which only make the row of the cell they
+// visit, by a stamp pass after the sweep.
 //
 // Phases and their parallel structure:
 //   entrainment (4236-4565)   per-cell map over the active mask, y < jmax;
@@ -54,6 +61,7 @@ struct GrainT {
   E* s;            // 90 planes
   int32_t* has;
   size_t nc;       // plane stride (cells)
+  uint64_t* key;   // creation key of each cell's row (hc::kRowNone: none)
   __device__ E& G(size_t c, int n) const { return g[n * nc + c]; }
   __device__ E& S(size_t c, int z, int n) const { return s[(z * 9 + n) * nc + c]; }
 };
@@ -145,7 +153,7 @@ __device__ double d50(const GrainT<E>& gr, size_t c, const ErodeArgs& a) {
 // LCM.cpp:4077-4171, material moves from `src` to `dst`
 template <typename E>
 __device__ void slide_gs(const GrainT<E>& gr, size_t src, size_t dst, double amount, const double* elev,
-                         const double* bedrock, const ErodeArgs& a) {
+                         const double* bedrock, const ErodeArgs& a, uint64_t dst_key) {
   double total = 0;
   if (gr.has[src] && gr.has[dst]) {
     for (int n = 1; n <= kG - 1; ++n)
@@ -173,6 +181,7 @@ __device__ void slide_gs(const GrainT<E>& gr, size_t src, size_t dst, double amo
   }
   if (gr.has[src] && !gr.has[dst]) {
     add_gs(gr, dst, elev[dst], bedrock[dst], a);
+    gr.key[dst] = dst_key;
     if (amount > a.active) {
       for (int n = 1; n <= kG - 1; ++n) gr.G(dst, n) += amount * a.dprop[n];
       amount = a.active;
@@ -211,7 +220,8 @@ __global__ __launch_bounds__(256) void k_entrain(ErodeArgs a, double time_step, 
                                                  E* __restrict__ Vel, E* __restrict__ Tau,
                                                  E* __restrict__ ss, E* __restrict__ sdir,
                                                  GrainT<E> gr, int32_t* __restrict__ need,
-                                                 unsigned long long* __restrict__ red) {
+                                                 unsigned long long* __restrict__ red,
+                                                 uint32_t ep) {
   const int y = blockIdx.x * 64 + threadIdx.x;
   const int x = blockIdx.y * blockDim.y + threadIdx.y;
   double tbmax = 0;
@@ -230,7 +240,11 @@ __global__ __launch_bounds__(256) void k_entrain(ErodeArgs a, double time_step, 
       double tempdir[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       double temp_dist[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       const double mann = a.spatial_n ? static_cast<double>(nman[c]) : a.mannings;
-      if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
+      if (!gr.has[c]) {  // LCM.cpp:4290, at loop position (y, x)
+        add_gs(gr, c, elev[c], bedrock[c], a);
+        atomicMin(reinterpret_cast<unsigned long long*>(&gr.key[c]),
+                  static_cast<unsigned long long>(hc::row_key(ep, y, x, 0)));
+      }
       for (int p = 1; p <= 7; p += 2) {
         const size_t c2 = c + static_cast<long>(dxp(p)) * P + dyp(p);
         if (h[c2] > a.wdet) {
@@ -316,7 +330,14 @@ __global__ __launch_bounds__(256) void k_entrain(ErodeArgs a, double time_step, 
           for (int p = 1; p <= 7; p += 2) {
             const size_t c2 = c + static_cast<long>(dxp(p)) * P + dyp(p);
             if (h[c2] > a.wdet) {
-              if (!gr.has[c2]) need[c2] = 1;
+              // LCM.cpp:4506: a neighbour without a row when the pass began
+              // (its key is none, or of this pass) gets one; the earliest
+              // position that asks for it numbers it
+              if (gr.key[c2] >= hc::row_epoch_base(ep)) {
+                need[c2] = 1;
+                atomicMin(reinterpret_cast<unsigned long long*>(&gr.key[c2]),
+                          static_cast<unsigned long long>(hc::row_key(ep, y, x, 1 + (p - 1) / 2)));
+              }
               double factor = 0;
               if (vd[sdir_of(p) * nc + c] > 0) factor += 0.75 * tempdir[p] / veltot;
               for (int n = 1; n <= kG - 1; ++n) {
@@ -348,6 +369,19 @@ __global__ __launch_bounds__(256) void k_alloc(ErodeArgs a, const double* __rest
       need[c] = 0;
     }
   }
+}
+
+// Key of every row a sweep made (has a row, no key yet): the sweeps that use
+// this make only the row of the cell they visit, at the reference's loop
+// position (y, x).
+__global__ __launch_bounds__(256) void k_row_stamp(int imax, int jmax, int P,
+                                                   const int32_t* __restrict__ has,
+                                                   uint64_t* __restrict__ key, uint32_t ep) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = blockIdx.y;
+  if (y > jmax + 1 || x > imax + 1) return;
+  const size_t c = static_cast<size_t>(x) * P + y;
+  if (has[c] && key[c] == hc::kRowNone) key[c] = hc::row_key(ep, y, x, 0);
 }
 
 // ------------------------------------------ register-window sweeps
@@ -909,13 +943,16 @@ template <class T>
 __global__ void k_reach(ErodeArgs a, GrainT<T> gr, int n, const int* __restrict__ idx,
                         const double* __restrict__ sed, const double* __restrict__ water,
                         double thr, T* __restrict__ h, double* __restrict__ elev,
-                        const double* __restrict__ bedrock, T* __restrict__ V) {
+                        const double* __restrict__ bedrock, T* __restrict__ V, uint32_t ep) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (sed) {
     for (int k = 0; k < n; ++k) {
       const size_t c = static_cast<size_t>(idx[k]);
       if (!(static_cast<double>(h[c]) > thr)) continue;
-      if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
+      if (!gr.has[c]) {  // LCM.cpp:3333, point k of the list
+        add_gs(gr, c, elev[c], bedrock[c], a);
+        gr.key[c] = hc::row_key(ep, static_cast<uint32_t>(k) >> 15, k & 0x7fff, 0);
+      }
       for (int f = 1; f <= kG - 1; ++f) {
         const double amt = sed[static_cast<size_t>(k) * (kG - 1) + (f - 1)];
         if (f == 1) {
@@ -1078,7 +1115,7 @@ __global__ __launch_bounds__(1024) void k_creep_grain(ErodeArgs a, int imax, int
                                                       double DX, double rate, double time,
                                                       const double* __restrict__ elev,
                                                       const double* __restrict__ bedrock,
-                                                      GrainT<E> gr) {
+                                                      GrainT<E> gr, uint32_t ep) {
   const int t0 = 3 * 2 + 2, t1 = 3 * (imax - 1) + (jmax - 1);
   for (int t = t0; t <= t1; ++t) {
     const int xlo = max(2, (t - (jmax - 1) + 2) / 3);
@@ -1096,8 +1133,9 @@ __global__ __launch_bounds__(1024) void k_creep_grain(ErodeArgs a, int imax, int
           double temp = ((es - en) / D) * rate * time / D;
           if ((es - temp) < bedrock[c]) temp = es - bedrock[c];
           if (temp < 0) temp = 0;
-          if (gr.has[c])
-            slide_gs(gr, c, static_cast<size_t>(static_cast<long>(c) + off), temp, elev, bedrock, a);
+          if (gr.has[c])  // a row made here is numbered at (x, y), direction k
+            slide_gs(gr, c, static_cast<size_t>(static_cast<long>(c) + off), temp, elev, bedrock, a,
+                     hc::row_key(ep, x, t - 3 * x, k));
         }
       }
     }
@@ -1175,7 +1213,8 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
                                                                double* __restrict__ elev,
                                                                const double* __restrict__ bedrock,
                                                                GrainT<E> gr,
-                                                               const unsigned long long* __restrict__ any) {
+                                                               const unsigned long long* __restrict__ any,
+                                                               uint32_t ep) {
   if (*any == 0) return;  // no cell can move
   constexpr int ox[8] = {1, 0, -1, -1, -1, 0, 1, 1};
   constexpr int oy[8] = {1, 1, 1, 0, -1, -1, -1, 0};
@@ -1200,6 +1239,9 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
       for (int k = 0; k < 8; ++k) en[k] = elev[static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k])];
       // entries taken as row x: x itself, row 1 (taken as 2), row imax (taken as imax-1)
       const int runs = (m0 ? 1 : 0) + (m1 ? 1 : 0) + (m2 ? 1 : 0);
+      // the reference visits the entries in row order (1, 2, ..., imax):
+      // the first of this cell's consecutive entries numbers its new rows
+      const int first = m1 ? 1 : (m0 ? x : imax);
       for (int run = 0; run < runs; ++run) {
         const double wet = (e <= bpa) ? 10000.0 : factor;
 #pragma unroll
@@ -1213,7 +1255,7 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
             const size_t nb = static_cast<size_t>(static_cast<long>(c) + ox[k] * P + oy[k]);
             elev[c] = e;
             elev[nb] = en[k];
-            slide_gs(gr, c, nb, diff, elev, bedrock, a);
+            slide_gs(gr, c, nb, diff, elev, bedrock, a, hc::row_key(ep, y, first, run * 8 + k));
           }
         }
       }
@@ -1226,9 +1268,29 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
 
 namespace {
 
+// The epoch of the next launch that can make grain rows.
+int next_row_epoch(hc_ctx* c, uint32_t* ep) {
+  if (c->row_epoch >= (1u << (64 - hc::kRowEpochShift)) - 1)
+    return hc_fail(c, HC_ERR_STATE, "grain rows: creation epochs exhausted");
+  *ep = ++c->row_epoch;
+  return HC_OK;
+}
+
+// k_row_stamp over the padded grid with a fresh epoch.
+int stamp_rows(hc_ctx* c) {
+  uint32_t ep;
+  if (int rc = next_row_epoch(c, &ep)) return rc;
+  const dim3 grd(static_cast<unsigned>((c->g.jmax + 2 + 255) / 256), static_cast<unsigned>(c->NR));
+  hipLaunchKernelGGL(k_row_stamp, grd, dim3(256), 0, c->stream, c->g.imax, c->g.jmax, c->P,
+                     c->hasgrain, c->rowkey, ep);
+  HC_HIP(c, hipGetLastError());
+  return HC_OK;
+}
+
 template <typename E>
 GrainT<E> grain_of(hc_ctx* c) {
-  return GrainT<E>{static_cast<E*>(c->grain), static_cast<E*>(c->strata), c->hasgrain, c->ncell};
+  return GrainT<E>{static_cast<E*>(c->grain), static_cast<E*>(c->strata), c->hasgrain, c->ncell,
+                   c->rowkey};
 }
 
 template <typename E>
@@ -1251,9 +1313,11 @@ int erode_t(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* 
   for (int pass = 0;; ++pass) {
     if (pass > 10000) return hc_fail(c, HC_ERR_UNSTABLE, "erode: timestep loop did not converge");
     HC_HIP(c, hipMemsetAsync(c->d_red + 1, 0, sizeof(unsigned long long), c->stream));
+    uint32_t ep;
+    if (int rc = next_row_epoch(c, &ep)) return rc;
     hipLaunchKernelGGL(k_entrain<E>, cells, blk, 0, c->stream, a, ts, mult, h, elev, c->bedrock,
                        nman, c->mask, static_cast<const E*>(V), static_cast<const E*>(c->veldir),
-                       static_cast<E*>(c->Vel), Tau, ss, sdir, gr, c->needgrain, c->d_red + 1);
+                       static_cast<E*>(c->Vel), Tau, ss, sdir, gr, c->needgrain, c->d_red + 1, ep);
     HC_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_alloc<E>, dim3(1024), dim3(256), 0, c->stream, a, elev, c->bedrock, gr,
                        c->needgrain, c->ncell);
@@ -1275,11 +1339,13 @@ int erode_t(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* 
                        c->bedrock, c->mask, V, static_cast<const E*>(Tau),
                        static_cast<const E*>(ss), static_cast<const E*>(sdir), gr, etot, etot3);
     HC_HIP(c, hipGetLastError());
+    if (int rc = stamp_rows(c)) return rc;
   }
   hipLaunchKernelGGL(k_lateral_y<E>, dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
                      c->bedrock, c->mask, static_cast<const E*>(Tau), gr,
                      static_cast<const E*>(etot3));
   HC_HIP(c, hipGetLastError());
+  if (int rc = stamp_rows(c)) return rc;
   hipLaunchKernelGGL(k_edges<E>, dim3(1), dim3(256), 0, c->stream, a, h, V,
                      static_cast<const E*>(sdir), c->ncell, c->gtot);
   HC_HIP(c, hipGetLastError());
@@ -1350,12 +1416,14 @@ int hc_creep_impl(hc_ctx* c, double time, double creep_rate) {
   double* elev = static_cast<double*>(c->z);
   const int imax = c->g.imax, jmax = c->g.jmax;
   if (imax >= 3 && jmax >= 3) {
+    uint32_t ep;
+    if (int rc = next_row_epoch(c, &ep)) return rc;
     if (c->sesz == 4)
       hipLaunchKernelGGL(k_creep_grain<float>, dim3(1), dim3(1024), 0, c->stream, a, imax, jmax,
-                         c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<float>(c));
+                         c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<float>(c), ep);
     else
       hipLaunchKernelGGL(k_creep_grain<double>, dim3(1), dim3(1024), 0, c->stream, a, imax, jmax,
-                         c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<double>(c));
+                         c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<double>(c), ep);
   }
   const dim3 blk(256), grd(static_cast<unsigned>((jmax + 255) / 256), static_cast<unsigned>(imax));
   hipLaunchKernelGGL(k_creep_sum, grd, blk, 0, c->stream, imax, jmax, c->P, c->g.DX, creep_rate,
@@ -1372,14 +1440,18 @@ int hc_in_channel_landsliding_impl(hc_ctx* c, double failureangle) {
   const double factor = std::tan((failureangle * (3.141592654 / 180))) * c->g.DX;
   const ErodeArgs a = erode_args(c);
   if (int rc = slide_candidates(c, factor, 10000.0, std::max(c->g.imax - 1, 2))) return rc;
+  uint32_t ep;
+  if (int rc = next_row_epoch(c, &ep)) return rc;
   if (c->sesz == 4)
     hipLaunchKernelGGL(k_in_channel_landslide<float>, dim3(1), dim3(1024), 0, c->stream, a,
                        c->g.imax, c->g.jmax, c->P, c->W, factor, c->mask,
-                       static_cast<double*>(c->z), c->bedrock, grain_of<float>(c), c->d_red + 4);
+                       static_cast<double*>(c->z), c->bedrock, grain_of<float>(c), c->d_red + 4,
+                       ep);
   else
     hipLaunchKernelGGL(k_in_channel_landslide<double>, dim3(1), dim3(1024), 0, c->stream, a,
                        c->g.imax, c->g.jmax, c->P, c->W, factor, c->mask,
-                       static_cast<double*>(c->z), c->bedrock, grain_of<double>(c), c->d_red + 4);
+                       static_cast<double*>(c->z), c->bedrock, grain_of<double>(c), c->d_red + 4,
+                       ep);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
@@ -1420,16 +1492,19 @@ int hc_reach_inputs_impl(hc_ctx* c, int n, const int* x, const int* y, const dou
   const ErodeArgs a = c->g.sediment ? erode_args(c) : ErodeArgs{};
   const double* sedp = (c->g.sediment && sediment) ? c->d_amt + 16 : nullptr;
   double* zp = c->g.sediment ? static_cast<double*>(c->z) : nullptr;
+  uint32_t ep = 0;
+  if (sedp)
+    if (int rc = next_row_epoch(c, &ep)) return rc;
   if (c->prec == HC_FP64)
     hipLaunchKernelGGL(k_reach<double>, dim3(1), dim3(64), 0, c->stream, a,
                        c->g.sediment ? grain_of<double>(c) : GrainT<double>{}, n, c->d_pts, sedp,
                        c->d_amt, threshold, static_cast<double*>(c->h[c->cur]), zp, c->bedrock,
-                       static_cast<double*>(c->V));
+                       static_cast<double*>(c->V), ep);
   else
     hipLaunchKernelGGL(k_reach<float>, dim3(1), dim3(64), 0, c->stream, a,
                        c->g.sediment ? grain_of<float>(c) : GrainT<float>{}, n, c->d_pts, sedp,
                        c->d_amt, threshold, static_cast<float*>(c->h[c->cur]), zp, c->bedrock,
-                       static_cast<float*>(c->V));
+                       static_cast<float*>(c->V), ep);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }

@@ -128,6 +128,30 @@ struct EnsMember {
   int W;                // mask words per row
 };
 
+// Creation order of grain rows.  The reference numbers a cell's grain row
+// (index[x][y]) by when it was made: rows of the grain data file by their
+// line (LCM.cpp:611-683), every later row by the next grain_array_tot in
+// addGS (LCM.cpp:3949-3955), which its single-threaded loops call in a fixed
+// order.  Each cell here keeps a key that sorts its row into that order:
+// epoch 0 = a row of the initial data (the low bits are its line number);
+// epoch e > 0 = made by the e-th kernel launch that can make rows, ordered
+// within the launch by the position in the reference's loop (major, minor
+// loop indices) and a sub-step (neighbour direction, repeat).  Rows made in
+// one launch that the reference makes in a different order get distinct
+// keys in that order.  Grids up to 32767 cells a side.
+constexpr uint64_t kRowNone = ~0ull;
+constexpr int kRowSubBits = 5, kRowMinorBits = 15, kRowMajorBits = 15;
+constexpr int kRowEpochShift = kRowSubBits + kRowMinorBits + kRowMajorBits;  // 35
+__host__ __device__ constexpr uint64_t row_key(uint32_t epoch, uint32_t major, uint32_t minor,
+                                               uint32_t sub) {
+  return (static_cast<uint64_t>(epoch) << kRowEpochShift) |
+         (static_cast<uint64_t>(major) << (kRowSubBits + kRowMinorBits)) |
+         (static_cast<uint64_t>(minor) << kRowSubBits) | sub;
+}
+__host__ __device__ constexpr uint64_t row_epoch_base(uint32_t epoch) {
+  return static_cast<uint64_t>(epoch) << kRowEpochShift;
+}
+
 // No-data value of the fixed-point elevations.
 constexpr int32_t kZqNodata = INT32_MIN;
 
@@ -218,6 +242,13 @@ struct hc_ctx {
   void* strata = nullptr;     // 90 planes (layer-major: z*9+n)
   int32_t* hasgrain = nullptr;
   int32_t* needgrain = nullptr;
+  // Grain-row creation order, for the reference's row numbering (see
+  // hc::row_key): a key per cell (kRowNone without a row), the last epoch
+  // handed to a kernel that can create rows, and the rows the initial grain
+  // data held (grain_array_tot after ingest_graindata_from_file).
+  uint64_t* rowkey = nullptr;
+  uint32_t row_epoch = 0;
+  long long rows_base = 0;
   double* bedrock = nullptr;
   void* Tau = nullptr;
   void* Vel = nullptr;

@@ -24,7 +24,7 @@
 
 namespace {
 
-constexpr char kMagic[8] = {'H', 'C', 'S', 'T', 'A', 'T', 'E', '1'};
+constexpr char kMagic[8] = {'H', 'C', 'S', 'T', 'A', 'T', 'E', '2'};
 
 struct StateHeader {
   char magic[8];
@@ -32,6 +32,8 @@ struct StateHeader {
   int32_t cur, fused_rows, fused_strip, zq_shift;
   double DX, zq_base;
   uint64_t red[8];       // d_red: maxdepth / tempbmax bits, health and sweep counts
+  int64_t rows_base;     // grain rows of the initial data
+  uint32_t row_epoch, pad0;
   uint64_t payload;      // bytes after the header
 };
 
@@ -63,6 +65,7 @@ std::vector<Buf> state_buffers(hc_ctx* c) {
     b.push_back({c->strata, 90 * n * s});
     b.push_back({c->hasgrain, n * sizeof(int32_t)});
     b.push_back({c->needgrain, n * sizeof(int32_t)});
+    b.push_back({c->rowkey, n * sizeof(uint64_t)});
     b.push_back({c->bedrock, n * sizeof(double)});
     b.push_back({c->Tau, n * s});
     b.push_back({c->Vel, n * s});
@@ -99,6 +102,8 @@ StateHeader header_of(const hc_ctx* c) {
   h.zq_shift = c->zq_shift;
   h.DX = c->g.DX;
   h.zq_base = c->zq_base;
+  h.rows_base = c->rows_base;
+  h.row_epoch = c->row_epoch;
   return h;
 }
 
@@ -173,6 +178,8 @@ int hc_state_load(hc_ctx* c, const void* host, size_t bytes) {
   c->cur = h.cur;
   c->zq_shift = h.zq_shift;
   c->zq_base = h.zq_base;
+  c->rows_base = h.rows_base;
+  c->row_epoch = h.row_epoch;
   return HC_OK;
 }
 

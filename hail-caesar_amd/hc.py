@@ -38,6 +38,7 @@ SYMBOLS = ["hc_create", "hc_destroy", "hc_last_error", "hc_field_count", "hc_upl
            "hc_scan_area", "hc_water_flux_out", "hc_add_point_inputs",
            "hc_add_point_inputs_noalloc", "hc_points_set", "hc_points_add", "hc_erode",
            "hc_global_landsliding", "hc_creep", "hc_in_channel_landsliding",
+           "hc_grain_index_set", "hc_grain_index_get",
            "hc_reach_inputs", "hc_runoff_set", "hc_runoff_add", "hc_synchronize", "hc_check_health", "hc_state_bytes", "hc_state_save",
            "hc_state_load", "hc_loop_begin", "hc_loop_run",
            "hc_loop_sync", "hc_loop_profile", "hc_loop_kernel_time", "hc_tune_tiles",
@@ -210,6 +211,8 @@ def lib():
         L.hc_global_landsliding.argtypes = [vp, d, ctypes.POINTER(i)]
         L.hc_creep.argtypes = [vp, d, d]
         L.hc_in_channel_landsliding.argtypes = [vp, d]
+        L.hc_grain_index_set.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_longlong]
+        L.hc_grain_index_get.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong)]
         L.hc_synchronize.argtypes = [vp]
         L.hc_check_health.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
         L.hc_state_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
@@ -400,6 +403,21 @@ class Context:
         """hc_points_add: every listed point takes its rain cell's amount."""
         a = np.ascontiguousarray(amt_by_rain_cell, dtype=np.float64)
         self._check(self.L.hc_points_add(self.h, a.ctypes.data, len(a)))
+
+    def grain_index_set(self, index: np.ndarray, rows_total: int) -> None:
+        """hc_grain_index_set: the reference's row numbers of the initial
+        grain rows ((imax+2) x (jmax+2), -9999 = none) and grain_array_tot."""
+        a = np.ascontiguousarray(index, dtype=np.int32)
+        assert a.shape == (self.imax + 2, self.jmax + 2)
+        self._check(self.L.hc_grain_index_set(self.h, a.ctypes.data, a.size, int(rows_total)))
+
+    def grain_index_get(self):
+        """hc_grain_index_get: (index, grain_array_tot), the reference's row
+        numbers of every cell's grain row."""
+        a = np.zeros((self.imax + 2, self.jmax + 2), dtype=np.int32)
+        tot = ctypes.c_longlong()
+        self._check(self.L.hc_grain_index_get(self.h, a.ctypes.data, a.size, ctypes.byref(tot)))
+        return a, tot.value
 
     def erode(self, mult: float, time_step: float):
         ts = ctypes.c_double(time_step)

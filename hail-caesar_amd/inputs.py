@@ -250,25 +250,29 @@ def write_rain(path: str, rates_mm_h: np.ndarray, pad_rows: int = 200) -> None:
 
 
 def write_graindata(path: str, props: np.ndarray, active: float,
-                    strata_props: Optional[np.ndarray] = None) -> None:
+                    strata_props: Optional[np.ndarray] = None,
+                    cells: Optional[np.ndarray] = None) -> None:
     """Grain data file: one line per cell, ``x y idx grain[0..10]
     strata[z=0..9][n=0..8]`` with 1-based padded coordinates x=row, y=col
     (LSDCatchmentModel.cpp:611-683).  ``props`` is (nrows, ncols, 9) surface
     proportions; grain[n] = active*props[n-1] for n=1..9, grain[0]=grain[10]=0.
-    Strata layers get the same proportions times ``active``."""
+    Strata layers get the same proportions times ``active``.  ``cells`` (k x 2,
+    0-based) lists the cells that get a line, in line order; default every
+    cell in row-major order."""
     nrows, ncols, nf = props.shape
     assert nf == 9
     if strata_props is None:
         strata_props = props
+    if cells is None:
+        cells = np.argwhere(np.ones((nrows, ncols), dtype=bool))
     lines: List[str] = []
     idx = 0
-    for i in range(nrows):
-        for j in range(ncols):
-            idx += 1
-            g = [0.0] + list(active * props[i, j]) + [0.0]
-            st = list(np.tile(active * strata_props[i, j], 10))
-            vals = [str(i + 1), str(j + 1), str(idx)] + ["%.17g" % v for v in g + st]
-            lines.append(" ".join(vals))
+    for i, j in np.asarray(cells):
+        idx += 1
+        g = [0.0] + list(active * props[i, j]) + [0.0]
+        st = list(np.tile(active * strata_props[i, j], 10))
+        vals = [str(i + 1), str(j + 1), str(idx)] + ["%.17g" % v for v in g + st]
+        lines.append(" ".join(vals))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -361,6 +365,7 @@ def make_workload(path: str, dem: np.ndarray, dx: float, rain_mm_h: np.ndarray,
                   water_init: Optional[np.ndarray] = None,
                   hydroindex: Optional[np.ndarray] = None,
                   graindata: Optional[np.ndarray] = None,
+                  grain_cells: Optional[np.ndarray] = None,
                   name: str = "run") -> Workload:
     os.makedirs(path, exist_ok=True)
     p = dict(BASE_PARAMS)
@@ -389,7 +394,7 @@ def make_workload(path: str, dem: np.ndarray, dx: float, rain_mm_h: np.ndarray,
         p["num_unique_rain_cells"] = str(int(hydroindex.max()))
     if graindata is not None:
         active = float((overrides or {}).get("active_layer_thickness", p["active_layer_thickness"]))
-        write_graindata(os.path.join(path, "grain.dat"), graindata, active)
+        write_graindata(os.path.join(path, "grain.dat"), graindata, active, cells=grain_cells)
         p["read_in_graindata_from_file"] = "yes"
         p["grain_data_file"] = "grain.dat"
     if overrides:
@@ -443,7 +448,8 @@ def erosion_1024(path: str, size: int = 1024, hours: int = 24, seed: int = 1002)
 
 def erosion_run(path: str, size: int = 48, hours: int = 1, seed: int = 1002,
                 failure_angle: float = 8.0, law: str = "wilcock",
-                dem_noise: float = 0.0) -> Workload:
+                dem_noise: float = 0.0, grain_cover: float = 1.0,
+                extra_params: Optional[Dict[str, str]] = None) -> Workload:
     """A whole flow + erosion run small enough to compare step by step: size^2
     at 10 m on a 2 % slope with 20 m of fractal relief and a 1 m channel, all
     four edges open, 9 grain fractions (d1 = 0.0005 m) in seeded Dirichlet
@@ -454,7 +460,10 @@ def erosion_run(path: str, size: int = 48, hours: int = 1, seed: int = 1002,
     iterations).  The grain sizes go to the reference by its harness, to the
     oracle by extra['grain_d'] and to this build's driver by hc_grain_sizes.
     dem_noise adds that many metres of seeded Gaussian noise to the DEM (a
-    perturbation for sensitivity floors)."""
+    perturbation for sensitivity floors).  grain_cover < 1 lists only that
+    share of the cells (seeded) in the grain data file, in a seeded shuffled
+    line order: the run then makes grain rows (addGS) as it goes.
+    extra_params are added to the parameter file."""
     rng = np.random.Generator(np.random.PCG64(seed))
     dem = planar_dem(size, size, 10.0, 0.02, 20.0, 0.6, seed, channel_depth=1.0,
                      south_only=False)
@@ -462,12 +471,19 @@ def erosion_run(path: str, size: int = 48, hours: int = 1, seed: int = 1002,
         noise = np.random.Generator(np.random.PCG64(seed + 77)).standard_normal(dem.shape)
         dem = np.where(dem > NODATA, dem + dem_noise * noise, dem)
     props = dirichlet_props(size, size, rng)
+    cells = None
+    if grain_cover < 1.0:
+        crng = np.random.Generator(np.random.PCG64(seed + 55))
+        cells = np.argwhere(crng.random((size, size)) < grain_cover)
+        cells = cells[crng.permutation(len(cells))]
+    over = {"hydro_model_only": "no", "active_layer_thickness": "1",
+            "topmodel_m_value": "0.0005", "transport_law": law,
+            "slope_failure_thresh": repr(failure_angle),
+            "hc_grain_sizes": ",".join(repr(d) for d in EROSION_GRAIN_D)}
+    over.update(extra_params or {})
     w = make_workload(path, dem, 10.0, triangular_storm(hours, 40.0, 15), 15, hours,
-                      overrides={"hydro_model_only": "no", "active_layer_thickness": "1",
-                                 "topmodel_m_value": "0.0005", "transport_law": law,
-                                 "slope_failure_thresh": repr(failure_angle),
-                                 "hc_grain_sizes": ",".join(repr(d) for d in EROSION_GRAIN_D)},
-                      water_init=np.where(dem > NODATA, 0.1, 0.0), graindata=props, name="erun")
+                      overrides=over, water_init=np.where(dem > NODATA, 0.1, 0.0),
+                      graindata=props, grain_cells=cells, name="erun")
     w.extra["grain_d"] = EROSION_GRAIN_D
     return w
 

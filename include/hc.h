@@ -238,6 +238,25 @@ int hc_creep(hc_ctx* ctx, double time, double creep_rate);
  * sediment contexts. */
 int hc_in_channel_landsliding(hc_ctx* ctx, double failureangle);
 
+/* The reference's grain-row numbers, index[x][y] (LCM.hpp; -9999 = no row),
+ * over the padded grid ((imax+2) x (jmax+2) int32).  The reference numbers
+ * the rows of its grain data file by line (ingest_graindata_from_file,
+ * LCM.cpp:611-683) and every later row by the next grain_array_tot when
+ * addGS makes it (LCM.cpp:3949-3955); its Grainz output (LGM.cpp:59-118)
+ * prints that number on every line.  Grain rows here are per cell; each
+ * keeps the position in the reference's single-threaded loops where it was
+ * made, so the numbers come out as the reference's.
+ *   set: the initial rows (also sets the has-grain flags) and rows_total,
+ *        the reference's grain_array_tot after ingest (the line count, which
+ *        duplicate lines make larger than the number of rows).  Uploading
+ *        HC_FIELD_HASGRAIN numbers the flagged cells 1..n in row-major order.
+ *   get: every row's number; rows made since are numbered rows_total+1,...
+ *        in creation order; *rows_total (may be NULL) receives the new
+ *        grain_array_tot.
+ * Sediment contexts only. */
+int hc_grain_index_set(hc_ctx* ctx, const int32_t* index, size_t count, long long rows_total);
+int hc_grain_index_get(hc_ctx* ctx, int32_t* index, size_t count, long long* rows_total);
+
 /* Spatially complex rainfall (spatially_complex_rainfall_on).  The
  * reference keeps a per-cell TOPMODEL (runoffGrid, LSDRainfallRunoff.cpp:
  * 175-241) and each step adds every cell's runoff to its water depth

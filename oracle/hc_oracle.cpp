@@ -61,6 +61,9 @@ struct orc_model {
   std::vector<double> strata;   // [cell][10][9]
   std::vector<double> sr, sl, su, sd;  // [cell][10]
   std::vector<int> mask, hasgrain, rfarea;
+  // the reference's row number of each cell's grain row (index[x][y],
+  // -9999 = none): file rows by line, later rows by grain_array_tot in addGS
+  std::vector<int> gindex;
   // rain-runoff state (LCM.cpp:1779-1788, 3649-3724)
   std::vector<std::vector<float>> rain;
   std::vector<double> j, jo, j_mean, old_j_mean, new_j_mean;
@@ -668,6 +671,7 @@ struct orc_model {
     const long c = at(x, y);
     grain_array_tot++;
     hasgrain[c] = 1;
+    gindex[c] = grain_array_tot;  // LCM.cpp:3954-3955
     double* g = G(c);
     g[0] = 0;
     for (int n = 1; n <= kGMax - 1; ++n) g[n] = p.active * p.dprop[n];
@@ -1262,6 +1266,7 @@ orc_model* orc_create(const orc_params* p) {
   }
   m->mask.assign(N, 0);
   m->hasgrain.assign(N, 0);
+  m->gindex.assign(N, -9999);
   m->rfarea.assign(N, 1);
   const int rf = p->rfnum;
   m->j.assign(rf + 1, 0.000000001);
@@ -1324,6 +1329,7 @@ int* orc_field_int(orc_model* m, const char* name, long* len) {
   std::vector<int>* v = nullptr;
   if (n == "mask") v = &m->mask;
   else if (n == "hasgrain") v = &m->hasgrain;
+  else if (n == "index") v = &m->gindex;
   else if (n == "rfarea") v = &m->rfarea;
   if (!v) { *len = 0; return nullptr; }
   *len = static_cast<long>(v->size());

@@ -325,6 +325,7 @@ class OracleModel:
         g = self.field("grain")
         st = self.field("strata").reshape(self.shape + (10, 9))
         hg = self.field_int("hasgrain")
+        gi = self.field_int("index")
         imax, jmax = self.p.imax, self.p.jmax
         data = np.loadtxt(path, ndmin=2)
         tot = 0
@@ -333,6 +334,7 @@ class OracleModel:
             x1 = min(int(row[0]), imax)
             y1 = min(int(row[1]), jmax)
             hg[x1, y1] = 1
+            gi[x1, y1] = tot  # index[x1][y1] = grain_array_tot (LCM.cpp:641-656)
             g[x1, y1, :] = row[3:14]
             st[x1, y1] = row[14:104].reshape(10, 9)
         self.scalar("grain_array_tot", float(tot))
@@ -351,6 +353,9 @@ class OracleModel:
         self.field("grain")[...] = snap["grain"]
         self.field("strata")[...] = snap["strata"].reshape(self.shape + (90,))
         self.field_int("hasgrain")[...] = snap["hasgrain"]
+        if "index" in snap:
+            self.field_int("index")[...] = snap["index"]
+            self.scalar("grain_array_tot", float(snap["grain_array_tot"]))
         for k in ("time_step", "maxdepth", "erode_mult", "globalsediq", "counter", "cycle"):
             self.scalar(k, float(snap[k]))
 
@@ -508,6 +513,8 @@ class RefModel:
         out["grain"] = grain
         out["strata"] = strata
         out["hasgrain"] = has.astype(np.int32)
+        out["index"] = idx.copy()
+        out["grain_array_tot"] = self.scalar("grain_array_tot")
         for k in ("time_step", "maxdepth", "erode_mult", "globalsediq", "counter", "cycle"):
             out[k] = self.scalar(k)
         out["sum_grain"] = self.vec("sum_grain").copy()

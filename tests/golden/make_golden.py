@@ -12,6 +12,12 @@ Runs only where oracle/_ref/libhcref.so was built from the reference sources
                               h/qx/qy
   config1_run_256.npz         the same for the 256^2 configuration (final
                               fields subsampled, with whole-field sums)
+  grainz_run_48.npz           a whole 48^2 erosion run whose grain data file
+                              lists half the cells in shuffled order: the
+                              final grain-row numbers (index) and
+                              grain_array_tot, and of every Grainz file the
+                              reference wrote (LGM.cpp:59-118) its name, its
+                              x y index columns and its first 40 lines
 The reference runs single-threaded (its OpenMP loops race on shared members
 with spatial n and in erode).
 
@@ -106,6 +112,36 @@ def config1_run(w, name, stride=4):
     print("wrote", name, len(tt), "steps")
 
 
+GRAINZ_PARAMS = {"write_grainsize_file": "yes", "grainsize_file": "grainz",
+                 "raster_output_interval": "30", "dem_write_extension": "asc"}
+
+
+def grainz_run(name):
+    """The erosion-run workload with half the cells in its grain data file
+    (inputs.erosion_run(grain_cover=0.5)) and Grainz output every 30
+    simulated minutes, run to its end."""
+    with tempfile.TemporaryDirectory() as tmp:
+        w = inputs.erosion_run(os.path.join(tmp, "gz"), grain_cover=0.5, extra_params=GRAINZ_PARAMS)
+        r = RefModel(w.path, w.params_file, threads=1)
+        r.scalar("d1", inputs.EROSION_GRAIN_D[0])
+        tt, _, _ = r.iterate(10 ** 7)
+        out = {"index": r.field_int("index").copy(),
+               "grain_array_tot": r.scalar("grain_array_tot"), "steps": len(tt)}
+        odir = os.path.join(w.path, "out")
+        files = sorted(f for f in os.listdir(odir) if f.startswith("grainz"))
+        out["files"] = np.array(files)
+        for k, f in enumerate(files):
+            with open(os.path.join(odir, f)) as fh:
+                lines = fh.read().split("\n")
+            assert lines[-1] == ""
+            lines = lines[:-1]
+            out[f"cols{k}"] = np.array([[int(t) for t in ln.split(" ")[:3]] for ln in lines],
+                                       dtype=np.int32).reshape(-1, 3)
+            out[f"head{k}"] = np.array(lines[:40])
+        np.savez_compressed(os.path.join(HERE, name), **out)
+        print("wrote", name, len(tt), "steps,", files)
+
+
 def erode_case(name, n, steps, law, ncalls=2):
     """A wet, steep grain-bearing state (all four edges open) after `steps`
     reference iterations in erosion mode, then `ncalls` erode() calls."""
@@ -139,6 +175,9 @@ def erode_case(name, n, steps, law, ncalls=2):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "grainz":
+        grainz_run("grainz_run_48.npz")
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "config1":
         with tempfile.TemporaryDirectory() as tmp:
             config1_run(inputs.flow_only_256(os.path.join(tmp, "d"), size=256, hours=2),
@@ -161,6 +200,7 @@ def main():
         config1_run(w4, "config1_run_256.npz")
     erode_case("erode_wilcock_32.npz", 32, 60, "wilcock")
     erode_case("erode_einstein_24.npz", 24, 40, "einstein")
+    grainz_run("grainz_run_48.npz")
 
 
 if __name__ == "__main__":

@@ -38,6 +38,9 @@ def _erosion_workload(path, size=48, seed=1013):
 
 
 def _case(tmp_path, case):
+    if case == "erosion-partial":  # grain rows made during the run on both sides
+        w = inputs.erosion_run(os.path.join(str(tmp_path), "erp"), grain_cover=0.5)
+        return w, {}, HYDRO_FIELDS + SED_FIELDS, 160, 53
     if case in ("erosion", "erosion-fp32"):
         w = _erosion_workload(os.path.join(str(tmp_path), "ero"))
         kw = {"precision": hc.HC_FP32} if case == "erosion-fp32" else {}
@@ -51,7 +54,7 @@ def _case(tmp_path, case):
 
 
 @pytest.mark.parametrize("case", ["fp64", "fp32", "fp32-fixed", "erosion", "erosion-fp32",
-                                  "complex-rain"])
+                                  "erosion-partial", "complex-rain"])
 def test_restart_continues_bit_identical(tmp_path, case):
     w, kw, fields, n, k = _case(tmp_path, case)
     whole = Model(w.path, w.params_file, **kw)
@@ -75,6 +78,10 @@ def test_restart_continues_bit_identical(tmp_path, case):
     assert np.array_equal(cont.series(), whole.series())
     for s in SCALARS:
         assert cont.scalar(s) == whole.scalar(s), s
+    if case.startswith("erosion"):  # grain-row numbers continue as they would have
+        ia, na = whole.context().grain_index_get()
+        ic, nc = cont.context().grain_index_get()
+        assert np.array_equal(ic, ia) and nc == na
     if case.startswith("erosion"):  # the bed moved on both sides of the checkpoint
         e0 = Model(w.path, w.params_file, **kw).download("elev")
         assert not np.array_equal(e0, ek) and not np.array_equal(ek, whole.download("elev"))

@@ -270,6 +270,79 @@ def test_erosion_run_vs_oracle(tmp_path, law):
     assert np.abs(m.download("elev")[1:-1, 1:-1] - w.dem).max() > 1.0
 
 
+GRAINZ_PARAMS = {"write_grainsize_file": "yes", "grainsize_file": "grainz",
+                 "raster_output_interval": "30", "dem_write_extension": "asc"}
+
+
+def _close6(a: str, b: str) -> bool:
+    """Two numbers printed at 6 significant digits agree to their last digit
+    (values within 1e-10 can round apart across a digit boundary)."""
+    x, y = float(a), float(b)
+    return x == y or abs(x - y) <= 1.01e-5 * max(abs(x), abs(y))
+
+
+def test_grainz_output_vs_reference(tmp_path):
+    """The Grainz writer (LGM.cpp:59-118) on a whole 48^2 erosion run whose
+    grain data file lists half the cells in shuffled order, so that about
+    1150 grain rows are made during the run, on the device, by the
+    entrainment pass and its neighbours, deposition, the slides, creep and
+    in-channel landsliding.  Against the compiled reference's own output
+    (tests/golden/grainz_run_48.npz): the same three files, every line's
+    x y index columns equal (the reference's row numbers, from the creation
+    order), the first 40 lines token for token with the values to their
+    sixth digit; hc_grain_index_get equals the reference's index and
+    grain_array_tot.  Against the oracle on the same run: the series and
+    fields to 1e-10 as the full-cover run, and every value of the final file
+    to its sixth digit."""
+    import os
+    from hail_caesar_amd.model import Model
+    g = np.load(os.path.join(GOLDEN, "grainz_run_48.npz"))
+    w = inputs.erosion_run(str(tmp_path / "gz"), grain_cover=0.5, extra_params=GRAINZ_PARAMS)
+    o = OracleModel.from_workload(w, nthreads=min(16, os.cpu_count() or 1))
+    ot, oc, od = o.iterate(10 ** 7)
+    m = Model(w.path, w.params_file, write_files=True)
+    parts = []
+    while True:
+        parts.append(m.iterate(5000))
+        if m.finished:
+            break
+    gt, gc, gd = (np.concatenate([p[k] for p in parts]) for k in range(3))
+    assert len(gt) == len(ot) == int(g["steps"])
+    np.testing.assert_allclose(gc, oc, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(gd, od, rtol=1e-10, atol=0)
+    for f in ("elev", "water_depth", "grain"):
+        a, b = m.download(f), o.field(f)
+        np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12 * np.abs(b).max(), err_msg=f)
+    idx, tot = m.context().grain_index_get()
+    assert np.array_equal(idx, g["index"])
+    assert tot == int(g["grain_array_tot"])
+    files = sorted(f for f in os.listdir(os.path.join(w.path, "out")) if f.startswith("grainz"))
+    assert files == list(g["files"])
+    for k, name in enumerate(files):
+        with open(os.path.join(w.path, "out", name)) as fh:
+            text = fh.read()
+        assert text.endswith("\n")
+        lines = text.split("\n")[:-1]
+        cols = np.array([[int(t) for t in ln.split(" ")[:3]] for ln in lines], dtype=np.int32)
+        assert np.array_equal(cols.reshape(-1, 3), g[f"cols{k}"]), name
+        same = 0
+        for mine, ref in zip(lines, g[f"head{k}"]):
+            ta, tb = mine.split(" "), str(ref).split(" ")
+            assert len(ta) == len(tb) == 3 + 11 + 90 + 1 and ta[-1] == tb[-1] == "", name
+            assert ta[:3] == tb[:3]
+            assert all(_close6(x, y) for x, y in zip(ta[3:-1], tb[3:-1])), (name, mine, ref)
+            same += mine == str(ref)
+        print(f"{name}: {len(lines)} lines; {same} of the first 40 byte-identical")
+    # the final file's values against the oracle's grain rows
+    og, ost = o.field("grain"), o.field("strata")
+    for ln in lines:
+        t = ln.split(" ")
+        x, y = int(t[0]), int(t[1])
+        want = list(og[x, y]) + list(ost[x, y].ravel())
+        assert all(_close6(v, "%g" % wv) for v, wv in zip(t[3:-1], want)), ln
+    m.close()
+
+
 def _erosion_fields(w, prec):
     from hail_caesar_amd.model import Model
     m = Model(w.path, w.params_file, precision=prec)

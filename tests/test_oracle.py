@@ -415,9 +415,41 @@ def test_oracle_erode_golden(name):
     assert o.scalar("globalsediq") == float(g["out_globalsediq"])
 
 
+GRAINZ_PARAMS = {"write_grainsize_file": "yes", "grainsize_file": "grainz",
+                 "raster_output_interval": "30", "dem_write_extension": "asc"}
+
+
+def test_oracle_grain_row_numbers_vs_golden(tmp_path):
+    """Grain-row numbers (index[x][y]) after a whole erosion run whose grain
+    data file lists half the cells in shuffled line order, so that about
+    1150 rows are made by addGS during the run (entrainment and its
+    neighbours, deposition, the slides, creep and in-channel landsliding):
+    the oracle's numbers and grain_array_tot equal the compiled reference's
+    (tests/golden/grainz_run_48.npz).  The golden Grainz files' x y index
+    columns are the cells with a row in row-major order with those numbers
+    (the final file) -- the listing LGM.cpp:59-118 prints."""
+    g = _golden("grainz_run_48.npz")
+    w = inputs.erosion_run(str(tmp_path / "gz"), grain_cover=0.5, extra_params=GRAINZ_PARAMS)
+    o = OracleModel.from_workload(w, nthreads=4)
+    tt, _, _ = o.iterate(10 ** 7)
+    assert len(tt) == int(g["steps"])
+    idx = o.field_int("index")
+    gi = g["index"]
+    assert np.array_equal(idx, gi)
+    assert o.scalar("grain_array_tot") == float(g["grain_array_tot"])
+    with open(os.path.join(w.path, "grain.dat")) as f:
+        lines = sum(1 for _ in f)
+    made = int((gi > lines).sum())  # rows 1..lines came from the file
+    assert int((gi > 0).sum()) == lines + made == int(g["grain_array_tot"]) and made > 1000
+    last = len(g["files"]) - 1
+    cells = np.argwhere(gi[1:-1, 1:-1] != -9999) + 1
+    want = np.column_stack([cells, gi[cells[:, 0], cells[:, 1]]])
+    assert np.array_equal(g[f"cols{last}"], want)
+
+
 @pytest.mark.skipif(not RefModel.available(), reason="compiled reference not built here")
-@pytest.mark.parametrize("law", ["wilcock", "einstein"])
-def test_oracle_erosion_run_vs_reference(tmp_path, law):
+@pytest.mark.parametrize("law,cover", [("wilcock", 1.0), ("einstein", 1.0), ("wilcock", 0.5)])
+def test_oracle_erosion_run_vs_reference(tmp_path, law, cover):
     """A whole flow + erosion run (48^2, 1 simulated hour, about 2600
     iterations): the loop body of src/main.cpp:139-204 with call_erosion and
     the three hillslope processes on their triggers (LCM.cpp:1968-1996:
@@ -425,8 +457,11 @@ def test_oracle_erosion_run_vs_reference(tmp_path, law):
     on the first iteration, where they move material: up to 2.4 m at the 8
     degree failure angle).  The compiled reference runs single-threaded (its
     erosion loops race); every step's outlet discharge, clock and flow
-    timestep and the final state are bitwise equal."""
-    w = inputs.erosion_run(str(tmp_path / "er"), law=law)
+    timestep and the final state are bitwise equal.  With cover 0.5 the grain
+    data file lists half the cells (shuffled): the cells without a row take
+    the other branches of slide_GS and the run makes rows as it goes; the
+    grain-row numbers agree too."""
+    w = inputs.erosion_run(str(tmp_path / "er"), law=law, grain_cover=cover)
     r = RefModel(w.path, w.params_file, threads=1)
     r.scalar("d1", inputs.EROSION_GRAIN_D[0])
     rt, rc, rd = r.iterate(10 ** 7)
@@ -439,6 +474,8 @@ def test_oracle_erosion_run_vs_reference(tmp_path, law):
         assert np.array_equal(o.field(f), s[f]), f
     assert np.array_equal(o.field("strata").reshape(s["strata"].shape), s["strata"])
     assert np.array_equal(o.field_int("hasgrain"), s["hasgrain"])
+    assert np.array_equal(o.field_int("index"), s["index"])
+    assert o.scalar("grain_array_tot") == s["grain_array_tot"]
     assert o.scalar("globalsediq") == s["globalsediq"]
     assert np.abs(s["elev"][1:-1, 1:-1] - w.dem).max() > 1.0  # the landslides moved material
 
