@@ -301,7 +301,9 @@ __device__ __forceinline__ double flux_block(int imax, int jmax, int P, double D
     T* hp = &h[static_cast<size_t>(x) * P + y];
     const double hv = static_cast<double>(*hp);
     if (hv > thr) {
-      acc += (hv - thr) * DX * DX / ts;
+      // the depth left behind is thr as stored: in FP32 float(thr), so the
+      // volume counted out is exactly the volume taken (thr itself in FP64)
+      acc += (hv - static_cast<double>(static_cast<T>(thr))) * DX * DX / ts;
       *hp = static_cast<T>(thr);
     }
   }

@@ -661,8 +661,8 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       if (drains && owned && xo >= 1 && xo <= a.imax && c >= 1 && c <= a.jmax &&
           (xo == 1 || xo == a.imax || c == 1 || c == a.jmax)) {
         const double hv = static_cast<double>(hn);
-        if (hv > drain.thr) {
-          dacc += (hv - drain.thr) * a.DX * a.DX / ts;
+        if (hv > drain.thr) {  // counted out down to the stored threshold
+          dacc += (hv - static_cast<double>(static_cast<T>(drain.thr))) * a.DX * a.DX / ts;
           hn = static_cast<T>(drain.thr);
         }
       }

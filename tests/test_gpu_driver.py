@@ -239,18 +239,54 @@ def test_driver_complex_rainfall_vs_oracle(tmp_path, rfnum):
     assert err["vol"] < 1e-6
 
 
+def _reach_ledger(w, **kw):
+    """Per-iteration volume ledger of a host-stepped reach-mode run: the
+    change of the stored volume sum(h) DX^2 against what entered (the reach
+    discharge of LCM.cpp:3420-3440 over the flow timestep, and the TOPMODEL
+    rain at the input points, waterinput over the flow timestep) and what
+    was drained (temptot over time_step).  Returns (sum of the residuals,
+    final volume)."""
+    m = Model(w.path, w.params_file, **kw)
+    dx, div, rdt = w.dx, float(w.params["divide_inputs_by"]), float(w.params["reach_input_data_timestep"])
+    rows = [np.loadtxt(os.path.join(w.path, w.params[f"reach{k}_input_file"])).astype(np.float32)
+            for k in (1, 2)]
+    v0 = m.download("water_depth")[1:-1, 1:-1].sum() * dx * dx
+    total = 0.0
+    while True:
+        tt, cy, dt = m.iterate(1)
+        if len(tt) == 0:
+            break
+        i = int(cy[0] / rdt)
+        prop = (((i + 1) * rdt) - cy[0]) / rdt
+        reach = sum(float(r[i, 1]) + (float(r[i + 1, 1]) - float(r[i, 1])) * (1 - prop) for r in rows)
+        vin = (reach / div) * dt[0] + m.scalar("waterinput") * dt[0]
+        v1 = m.download("water_depth")[1:-1, 1:-1].sum() * dx * dx
+        total += (v1 - v0) - vin + tt[0] * m.scalar("time_step")
+        v0 = v1
+        if m.finished:
+            break
+    m.close()
+    return total, v0
+
+
 def test_driver_reach_mode_vs_oracle(tmp_path):
     """reach_mode (reach_water_and_sediment_input, LSDCatchmentModel.cpp:
     3301-3452) through the driver, hydro-only: two reach points' discharge
     added on the device (hc_reach_inputs); FP64 within 1e-10 of the oracle
     (pinned bitwise to the compiled reference) over the whole run, and the
     FP32 form (fixed-point elevations) within 1e-4 of it or four times the
-    FP64 run's sensitivity to 1e-8 m of DEM noise.  Volume: the FP32 run
-    ends 8e-6 away from the FP64 run's volume here (the FP64 run moves by
-    7e-8 under the DEM noise), a systematic difference not yet isolated --
-    neither the rounding of the point adds (an FP64 carry on them changed it
-    by under 1%) nor the open edge next to the points (6.2e-6 with them 10
-    rows inland); the bar is 1e-5."""
+    FP64 run's sensitivity to 1e-8 m of DEM noise.
+
+    Volume, by a per-iteration ledger (inflow, drainage and the stored
+    volume): the FP64 run closes it to rounding (measured 3e-16 of the final
+    volume).  The FP32 run drifts by -9.5e-6 over the run, nearly all of it
+    the rounding of the small TOPMODEL baseflow amounts added at the input
+    points to FP32 depths: the amount is nearly the same each step and the
+    depth at a point changes slowly, so the same fraction of an ulp is lost
+    step after step (an FP64 carry on those adds, tried outside the product,
+    closes the ledger to 2.6e-7; the reach point adds themselves contribute
+    2e-7 m^3).  The bars: FP64 ledger 1e-12, FP32 ledger and final volume
+    against FP64 1.2e-5."""
     w = inputs.reach_inputs(os.path.join(str(tmp_path), "rh"))
     o = OracleModel.from_workload(w)
     ot, oc, od = o.iterate(10 ** 7)
@@ -274,7 +310,12 @@ def test_driver_reach_mode_vs_oracle(tmp_path):
     f32.iterate(10 ** 7)
     err = _errs(f32.download("water_depth"), o.field("water_depth"))
     print(f"reach fp32 depth error {err}, floor {floor}")
-    assert err["max"] <= max(1e-4, 4 * floor["max"]) and err["vol"] < 1e-5
+    assert err["max"] <= max(1e-4, 4 * floor["max"]) and err["vol"] < 1.2e-5
+    led64, vol64 = _reach_ledger(w)
+    led32, vol32 = _reach_ledger(w, precision=hc.HC_FP32, elev_fixed=True)
+    print(f"reach volume ledger: fp64 {led64 / vol64:.2e}, fp32 {led32 / vol32:.2e}")
+    assert abs(led64) <= 1e-12 * vol64
+    assert abs(led32) <= 1.2e-5 * vol32
 
 
 def test_driver_reach_mode_erosion_runs(tmp_path):
