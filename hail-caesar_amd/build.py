@@ -37,6 +37,12 @@ HOST_SOURCES = ["driver/hc_model.cpp"]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra"]
 BIN = os.path.join(HERE, "bin", "hail-caesar-mi355x")
+# The same driver with its host code (run loop, parsers, writers,
+# checkpoints) under AddressSanitizer and UndefinedBehaviorSanitizer, for
+# tests; the device code in libhc.so is the ordinary build.
+BIN_ASAN = os.path.join(HERE, "bin", "hail-caesar-mi355x-asan")
+ASAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-std=c++17", "-ffp-contract=off",
+              "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
 
 
 def compile_host(src: str) -> str:
@@ -67,6 +73,12 @@ def build(jobs: int = 4) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"driver link failed:\n{r.stdout}\n{r.stderr}")
+    cmd = [CXX, *ASAN_FLAGS, os.path.join(CSRC, "driver", "hc_main.cpp"),
+           os.path.join(CSRC, "driver", "hc_model.cpp"), "-o", BIN_ASAN, "-L" + HERE, "-lhc",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"sanitizer driver link failed:\n{r.stdout}\n{r.stderr}")
     return LIB
 
 

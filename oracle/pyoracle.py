@@ -15,7 +15,9 @@ from typing import Dict, Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# HC_ORACLE_SO selects another build of the same source (the sanitizer
+# builds of oracle/Makefile, driven by oracle/sanitize.py)
+ORACLE_SO = os.environ.get("HC_ORACLE_SO") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libhcref.so")
 
 # Reference defaults (LSDCatchmentModel.hpp:437-568) used when a key is absent.

@@ -221,3 +221,57 @@ def test_checkpoint_write_is_atomic(tmp_path):
     c = Model(w.path, w.params_file, **kw)
     c.restart(ck)
     assert c.scalar("counter") == a.scalar("counter")
+
+
+BIN_ASAN = BIN + "-asan"
+
+
+def _run_bin(binary, cwd, *args, env=None):
+    import subprocess
+    r = subprocess.run([binary, "./", *args], cwd=cwd, capture_output=True, text=True, timeout=300,
+                       env=env)
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.mark.parametrize("case", ["erosion-grainz-ckpt", "flow-graph"])
+def test_driver_host_code_under_asan(tmp_path, case):
+    """The command-line driver with its host code (parameter and raster
+    parsing, the run loop, TOPMODEL, input points, the series, raster and
+    Grainz writers, checkpoints and restarts) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer, on the device as usual: no report, and every
+    output file byte-identical to the ordinary build's.  Cases: a whole
+    erosion run with half the cells in its grain data file (rows made as it
+    goes), Grainz output and a checkpoint every 500 iterations, cut at 900
+    iterations and restarted; a flow-only run in device-resident graphs."""
+    import shutil
+    if case == "erosion-grainz-ckpt":
+        src = inputs.erosion_run(os.path.join(str(tmp_path), "src"), grain_cover=0.5,
+                                 extra_params={"write_grainsize_file": "yes",
+                                               "grainsize_file": "grainz",
+                                               "raster_output_interval": "30",
+                                               "dem_write_extension": "asc"})
+        runs = [("--checkpoint", "run.ckpt", "--checkpoint-every", "500", "--max-iterations", "900"),
+                ("--restart", "run.ckpt", "--checkpoint", "run.ckpt")]
+    else:
+        src = inputs.flow_only_256(os.path.join(str(tmp_path), "src"), size=64, hours=1)
+        runs = [("--graph",)]
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:verify_asan_link_order=0:halt_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    dirs = {}
+    for kind, binary in (("plain", BIN), ("asan", BIN_ASAN)):
+        d = os.path.join(str(tmp_path), kind)
+        shutil.copytree(src.path, d)
+        for args in runs:
+            print(f"{case} {kind}: {' '.join(args)}", flush=True)
+            rc, out, err = _run_bin(binary, d, src.params_file, *args, env=env)
+            assert rc == 0, (kind, args, out[-2000:], err[-4000:])
+            assert "AddressSanitizer" not in err and "runtime error" not in err, err[-4000:]
+        dirs[kind] = d
+    outs = sorted(os.listdir(os.path.join(dirs["plain"], "out")))
+    assert outs == sorted(os.listdir(os.path.join(dirs["asan"], "out"))) and outs
+    if case == "erosion-grainz-ckpt":
+        assert {"grainz0.asc", "grainz30.asc", "grainz60.asc"} <= set(outs)
+    for name in outs:
+        with open(os.path.join(dirs["plain"], "out", name), "rb") as fa, \
+                open(os.path.join(dirs["asan"], "out", name), "rb") as fb:
+            assert fa.read() == fb.read(), name

@@ -620,3 +620,24 @@ def test_in_channel_landsliding_vs_compiled_reference(tmp_path, angle):
     assert np.array_equal(r.field("elev"), o.field("elev"))
     assert np.array_equal(o.field("grain"), s["grain"])
     assert np.array_equal(o.field("strata").reshape(s["strata"].shape), s["strata"])
+
+
+def test_oracle_clean_under_sanitizers():
+    """The oracle under AddressSanitizer + UndefinedBehaviorSanitizer and
+    under ThreadSanitizer (its OpenMP loops on 4 threads, LLVM's OpenMP with
+    the Archer tool): four whole runs (flow-only, erosion with rows made as
+    it goes, reach mode with sediment, three-cell complex rain) report
+    nothing and end bit-identical to the plain build; planted mistakes (a
+    data race, a heap overrun) are reported (oracle/sanitize.py)."""
+    import subprocess
+    import sys
+    llvm = os.environ.get("LLVM", "/opt/rocm/llvm")
+    if not os.path.exists(os.path.join(llvm, "lib", "libarcher.so")):
+        pytest.skip("LLVM OpenMP with Archer not installed")
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([sys.executable, os.path.join(here, "..", "oracle", "sanitize.py")],
+                       capture_output=True, text=True, timeout=900)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr[-4000:]
+    assert r.stdout.count("clean, results identical") == 8
+    assert r.stdout.count("planted mistake reported") == 2
