@@ -66,56 +66,133 @@ struct GrainT {
   __device__ E& S(size_t c, int z, int n) const { return s[(z * 9 + n) * nc + c]; }
 };
 
-// LCM.cpp:3874-3947
+// A cell's grain row as the grain helpers use it: get(n), and set(n, x),
+// which keeps x as the storage type holds it.  MemRow works on the planes in
+// memory; RegRow holds the row in registers (values rounded to the storage
+// type at each set, so a run on registers and one through memory agree bit
+// for bit).  The strata stay in memory in both.
+// Both also answer whether the cell has a row (index != -9999) and record
+// that it now has one: MemRow through the flags in memory, RegRow from a
+// register copy it keeps in step with them.
 template <typename E>
-__device__ void sort_active(const GrainT<E>& gr, size_t c, const ErodeArgs& a);
-
-// LCM.cpp:3949-3983 (sort_active afterwards, as addGS does)
+struct MemRow {
+  E* g;
+  int32_t* hasp;
+  size_t nc, c;
+  __device__ double get(int n) const { return g[n * nc + c]; }
+  __device__ void set(int n, double x) const { g[n * nc + c] = static_cast<E>(x); }
+  __device__ bool has() const { return hasp[c] != 0; }
+  __device__ void set_has() const { hasp[c] = 1; }
+};
 template <typename E>
-__device__ void add_gs(const GrainT<E>& gr, size_t c, double elev, double bedrock, const ErodeArgs& a) {
-  gr.has[c] = 1;
-  gr.G(c, 0) = 0;
-  for (int n = 1; n <= kG - 1; ++n) gr.G(c, n) = a.active * a.dprop[n];
-  gr.G(c, kG) = 0;
-  for (int z = 0; z <= 9; ++z) {
-    const bool zero = elev - (a.active * (z + 1)) < (bedrock - a.active);
-    for (int n2 = 0; n2 <= kG - 2; ++n2) gr.S(c, z, n2) = zero ? 0.0 : (a.active) * a.dprop[n2 + 1];
+struct RegRow {
+  double v[kG + 1];
+  bool h;
+  int32_t* hasp;
+  size_t c;
+  __device__ double get(int n) const { return v[n]; }
+  __device__ void set(int n, double x) { v[n] = static_cast<double>(static_cast<E>(x)); }
+  __device__ bool has() const { return h; }
+  __device__ void set_has() {
+    h = true;
+    hasp[c] = 1;
   }
-  sort_active(gr, c, a);
+};
+template <typename E>
+__device__ __forceinline__ MemRow<E> mem_row(const GrainT<E>& gr, size_t c) {
+  return MemRow<E>{gr.g, gr.has, gr.nc, c};
+}
+// a cell's row and flag into registers
+template <typename E>
+__device__ __forceinline__ void reg_row_load(RegRow<E>& r, const GrainT<E>& gr, size_t c) {
+  r.hasp = gr.has;
+  r.c = c;
+  r.h = gr.has[c] != 0;
+#pragma unroll
+  for (int n = 0; n <= kG; ++n) r.v[n] = gr.G(c, n);
 }
 
-template <typename E>
-__device__ void sort_active(const GrainT<E>& gr, size_t c, const ErodeArgs& a) {
+// LCM.cpp:3874-3947
+template <typename E, typename R>
+__device__ __forceinline__ void sort_active_r(const GrainT<E>& gr, size_t c, R& row,
+                                              const ErodeArgs& a) {
   double total = 0.0;
-  for (int n = 0; n <= kG; ++n) total += gr.G(c, n);
+#pragma unroll
+  for (int n = 0; n <= kG; ++n) total += row.get(n);
+  // (the rare layer shifts stay loops: unrolled, their loads would all be
+  // live at once)
   if (total > (a.active * 1.5)) {
+#pragma unroll 1
     for (int z = 9; z >= 1; --z)
+#pragma unroll 1
       for (int n = 0; n <= kG - 2; ++n) gr.S(c, z, n) = gr.S(c, z - 1, n);
     const double coeff = a.active / total;
+#pragma unroll
     for (int n = 1; n <= kG - 1; ++n) {
-      const double gv = gr.G(c, n);
+      const double gv = row.get(n);
       if (gv > 0.0) {
         const double amount = coeff * (gv);
         gr.S(c, 0, n - 1) = amount;
-        gr.G(c, n) = gv - amount;
+        row.set(n, gv - amount);
       }
     }
   }
   if (total < (a.active / 4)) {
-    for (int n = 1; n <= kG - 1; ++n) gr.G(c, n) += gr.S(c, 0, n - 1);
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n)  // in the storage type, as G += S adds there
+      row.set(n, static_cast<E>(row.get(n)) + gr.S(c, 0, n - 1));
+#pragma unroll 1
     for (int z = 0; z <= 8; ++z)
+#pragma unroll 1
       for (int n = 0; n <= kG - 2; ++n) gr.S(c, z, n) = gr.S(c, z + 1, n);
     const double amount = a.active;
+#pragma unroll 1
     for (int n = 1; n <= kG - 1; ++n) gr.S(c, 9, n - 1) = amount * a.dprop[n];
   }
 }
 
+// LCM.cpp:3949-3983 (sort_active afterwards, as addGS does)
+template <typename E, typename R>
+__device__ __forceinline__ void add_gs_r(const GrainT<E>& gr, size_t c, R& row, double elev,
+                                         double bedrock, const ErodeArgs& a) {
+  row.set_has();
+  row.set(0, 0);
+#pragma unroll
+  for (int n = 1; n <= kG - 1; ++n) row.set(n, a.active * a.dprop[n]);
+  row.set(kG, 0);
+#pragma unroll 1
+  for (int z = 0; z <= 9; ++z) {
+    const bool zero = elev - (a.active * (z + 1)) < (bedrock - a.active);
+#pragma unroll 1
+    for (int n2 = 0; n2 <= kG - 2; ++n2) gr.S(c, z, n2) = zero ? 0.0 : (a.active) * a.dprop[n2 + 1];
+  }
+  sort_active_r(gr, c, row, a);
+}
+
 // sort_active as called from outside addGS: allocates first if needed
+template <typename E, typename R>
+__device__ __forceinline__ void sort_active_alloc_r(const GrainT<E>& gr, size_t c, R& row,
+                                                    const double* elev, const double* bedrock,
+                                                    const ErodeArgs& a) {
+  if (!row.has()) add_gs_r(gr, c, row, elev[c], bedrock[c], a);
+  sort_active_r(gr, c, row, a);
+}
+
+template <typename E>
+__device__ void sort_active(const GrainT<E>& gr, size_t c, const ErodeArgs& a) {
+  MemRow<E> r = mem_row(gr, c);
+  sort_active_r(gr, c, r, a);
+}
+template <typename E>
+__device__ void add_gs(const GrainT<E>& gr, size_t c, double elev, double bedrock, const ErodeArgs& a) {
+  MemRow<E> r = mem_row(gr, c);
+  add_gs_r(gr, c, r, elev, bedrock, a);
+}
 template <typename E>
 __device__ void sort_active_alloc(const GrainT<E>& gr, size_t c, const double* elev,
                                   const double* bedrock, const ErodeArgs& a) {
-  if (!gr.has[c]) add_gs(gr, c, elev[c], bedrock[c], a);
-  sort_active(gr, c, a);
+  MemRow<E> r = mem_row(gr, c);
+  sort_active_alloc_r(gr, c, r, elev, bedrock, a);
 }
 
 // LCM.cpp:3985-4009
@@ -150,55 +227,72 @@ __device__ double d50(const GrainT<E>& gr, size_t c, const ErodeArgs& a) {
   return D;
 }
 
-// LCM.cpp:4077-4171, material moves from `src` to `dst`
-template <typename E>
-__device__ void slide_gs(const GrainT<E>& gr, size_t src, size_t dst, double amount, const double* elev,
-                         const double* bedrock, const ErodeArgs& a, uint64_t dst_key) {
+// LCM.cpp:4077-4171, material moves from `src` to `dst`; a row made for
+// dst gets the creation key dst_key
+template <typename E, typename RS, typename RD>
+__device__ __forceinline__ void slide_gs_r(const GrainT<E>& gr, size_t src, RS& rs, size_t dst,
+                                           RD& rd, double amount, const double* elev,
+                                           const double* bedrock, const ErodeArgs& a,
+                                           uint64_t dst_key) {
   double total = 0;
-  if (gr.has[src] && gr.has[dst]) {
+  if (rs.has() && rd.has()) {
+#pragma unroll
     for (int n = 1; n <= kG - 1; ++n)
-      if (gr.G(src, n) > 0) total += gr.G(src, n);
+      if (rs.get(n) > 0) total += rs.get(n);
     if (amount > total) {
-      for (int n = 1; n <= kG - 1; ++n) gr.G(dst, n) += (amount - total) * a.dprop[n];
+#pragma unroll
+      for (int n = 1; n <= kG - 1; ++n) rd.set(n, rd.get(n) + (amount - total) * a.dprop[n]);
       amount = total;
     }
     if (total > 0) {
+#pragma unroll
       for (int n = 1; n <= kG - 1; ++n) {
-        const double tr = amount * (gr.G(src, n) / total);
-        gr.G(dst, n) += tr;
-        gr.G(src, n) -= tr;
-        if (gr.G(src, n) < 0) gr.G(src, n) = 0;
+        const double tr = amount * (rs.get(n) / total);
+        rd.set(n, rd.get(n) + tr);
+        rs.set(n, rs.get(n) - tr);
+        if (rs.get(n) < 0) rs.set(n, 0);
       }
     }
-    sort_active_alloc(gr, src, elev, bedrock, a);
-    sort_active_alloc(gr, dst, elev, bedrock, a);
+    sort_active_alloc_r(gr, src, rs, elev, bedrock, a);
+    sort_active_alloc_r(gr, dst, rd, elev, bedrock, a);
     return;
   }
-  if (!gr.has[src] && gr.has[dst]) {
-    for (int n = 1; n <= kG - 1; ++n) gr.G(dst, n) += (amount) * a.dprop[n];
-    sort_active_alloc(gr, dst, elev, bedrock, a);
+  if (!rs.has() && rd.has()) {
+#pragma unroll
+    for (int n = 1; n <= kG - 1; ++n) rd.set(n, rd.get(n) + (amount) * a.dprop[n]);
+    sort_active_alloc_r(gr, dst, rd, elev, bedrock, a);
     return;
   }
-  if (gr.has[src] && !gr.has[dst]) {
-    add_gs(gr, dst, elev[dst], bedrock[dst], a);
+  if (rs.has() && !rd.has()) {
+    add_gs_r(gr, dst, rd, elev[dst], bedrock[dst], a);
     gr.key[dst] = dst_key;
     if (amount > a.active) {
-      for (int n = 1; n <= kG - 1; ++n) gr.G(dst, n) += amount * a.dprop[n];
+#pragma unroll
+      for (int n = 1; n <= kG - 1; ++n) rd.set(n, rd.get(n) + amount * a.dprop[n]);
       amount = a.active;
     }
+#pragma unroll
     for (int n = 1; n <= kG - 1; ++n)
-      if (gr.G(src, n) > 0) total += gr.G(src, n);
+      if (rs.get(n) > 0) total += rs.get(n);
+#pragma unroll
     for (int n = 1; n <= kG - 1; ++n) {
       if (total > 0) {
-        gr.G(dst, n) += amount * (gr.G(src, n) / total);
-        if (gr.G(src, n) > 0.0001) gr.G(src, n) -= amount * (gr.G(src, n) / total);
-        if (gr.G(src, n) < 0) gr.G(src, n) = 0;
+        rd.set(n, rd.get(n) + amount * (rs.get(n) / total));
+        if (rs.get(n) > 0.0001) rs.set(n, rs.get(n) - amount * (rs.get(n) / total));
+        if (rs.get(n) < 0) rs.set(n, 0);
       }
     }
-    sort_active_alloc(gr, src, elev, bedrock, a);
-    sort_active_alloc(gr, dst, elev, bedrock, a);
+    sort_active_alloc_r(gr, src, rs, elev, bedrock, a);
+    sort_active_alloc_r(gr, dst, rd, elev, bedrock, a);
     return;
   }
+}
+
+template <typename E>
+__device__ void slide_gs(const GrainT<E>& gr, size_t src, size_t dst, double amount, const double* elev,
+                         const double* bedrock, const ErodeArgs& a, uint64_t dst_key) {
+  MemRow<E> rs = mem_row(gr, src), rd = mem_row(gr, dst);
+  slide_gs_r(gr, src, rs, dst, rd, amount, elev, bedrock, a, dst_key);
 }
 
 // direction p (1,3,5,7) -> (dx, dy) of LCM.hpp:445-446 and bedload plane
@@ -644,11 +738,15 @@ __device__ __forceinline__ void lateral_slide_w(const ErodeArgs& a, const GrainT
 
 // ------------------------------------------- deposition + x-lateral slide
 // One wavefront per column y, walking x = 2 .. imax-1 (LCM.cpp:4570-4682).
-// Lane l fetches one of the 64 bedload terms of the next cell (fraction
-// 2 + l/8; terms 0-3 its own su, sr, sd, sl, terms 4-7 the neighbours' su of
-// y+1, sd of y-1, sl of x+1, sr of x-1), summed in the reference's order by
-// lanes 8k and 8k+4.
-template <typename E>
+// Lane l fetches one of the 64 bedload terms of a cell (fraction 2 + l/8;
+// terms 0-3 its own su, sr, sd, sl, terms 4-7 the neighbours' su of y+1, sd
+// of y-1, sl of x+1, sr of x-1), summed in the reference's order by lanes 8k
+// and 8k+4.  The sweep is a chain of dependent steps, so each step's loads
+// are issued K steps ahead: a ring of K cells entering the window and K
+// cells' inputs, held in registers (the loop is unrolled by K so that every
+// ring slot is a fixed register set).  Cells ahead of the window are not
+// touched before they enter it.
+template <typename E, int K>
 __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, double mult,
                                                 const E* __restrict__ h,
                                                 double* __restrict__ elev,
@@ -683,20 +781,33 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
     in.term = sdir[(static_cast<size_t>(td) * 10 + tn) * nc + c + toff];
   };
   CellW wm, wc, wp;
-  CellRaw rn;
-  In in, inn;
-  cell_fetch(rn, gr, static_cast<size_t>(1) * P + y, h, elev, bedrock, lane);
-  cell_take(wm, rn);
-  cell_fetch(rn, gr, static_cast<size_t>(2) * P + y, h, elev, bedrock, lane);
-  cell_take(wc, rn);
-  cell_fetch(rn, gr, static_cast<size_t>(3) * P + y, h, elev, bedrock, lane);
-  cell_take(wp, rn);
-  fetch(2, in);
-  inn = in;
-  for (int x = 2; x < a.imax; ++x) {
-    // issued one step ahead: the next cell's inputs and the cell entering the window
-    if (x + 1 < a.imax) fetch(x + 1, inn);
-    if (x + 2 <= a.imax) cell_fetch(rn, gr, static_cast<size_t>(x + 2) * P + y, h, elev, bedrock, lane);
+  CellRaw ring[K];  // slot j: the cell entering the window at a step x = j (mod K)
+  In inr[K];        // slot j: the inputs of that step's cell
+  {
+    CellRaw r0;
+    cell_fetch(r0, gr, static_cast<size_t>(1) * P + y, h, elev, bedrock, lane);
+    cell_take(wm, r0);
+    cell_fetch(r0, gr, static_cast<size_t>(2) * P + y, h, elev, bedrock, lane);
+    cell_take(wc, r0);
+    cell_fetch(r0, gr, static_cast<size_t>(3) * P + y, h, elev, bedrock, lane);
+    cell_take(wp, r0);
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (2 + j < a.imax) fetch(2 + j, inr[j]);
+    if (4 + j <= a.imax) cell_fetch(ring[j], gr, static_cast<size_t>(4 + j) * P + y, h, elev, bedrock, lane);
+  }
+  for (int x0 = 2; x0 < a.imax; x0 += K) {
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int x = x0 + j;
+    if (x >= a.imax) break;  // uniform
+    const In in = inr[j];
+    const CellRaw rn = ring[j];  // cell x + 2
+    // refill the slots K steps ahead
+    if (x + K < a.imax) fetch(x + K, inr[j]);
+    if (x + 2 + K <= a.imax)
+      cell_fetch(ring[j], gr, static_cast<size_t>(x + 2 + K) * P + y, h, elev, bedrock, lane);
     if (in.act) {
       const size_t c = wc.c;
       if (!wc.has) add_gs_w(gr, wc, a, L);
@@ -760,7 +871,7 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
     wm = wc;
     wc = wp;
     if (x + 2 <= a.imax) cell_take(wp, rn);
-    in = inn;
+  }
   }
   // rows imax-1 and imax (the last two of the window)
   cell_store(wm, gr, elev, lane);
@@ -768,8 +879,9 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
 }
 
 // --------------------------------------------------------- y-lateral slide
-// One wavefront per row x, walking y = 2 .. jmax-1 (LCM.cpp:4684-4747).
-template <typename E>
+// One wavefront per row x, walking y = 2 .. jmax-1 (LCM.cpp:4684-4747), its
+// loads issued K steps ahead as in the deposition sweep.
+template <typename E, int K>
 __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step, double mult,
                                                   const E* __restrict__ h,
                                                   double* __restrict__ elev,
@@ -793,19 +905,31 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
     in.tau = Tau[c];
   };
   CellW wl, wc, wr;
-  CellRaw rn;
-  In in, inn;
-  cell_fetch(rn, gr, row + 1, h, elev, bedrock, lane);
-  cell_take(wl, rn);
-  cell_fetch(rn, gr, row + 2, h, elev, bedrock, lane);
-  cell_take(wc, rn);
-  cell_fetch(rn, gr, row + 3, h, elev, bedrock, lane);
-  cell_take(wr, rn);
-  fetch(2, in);
-  inn = in;
-  for (int y = 2; y < a.jmax; ++y) {
-    if (y + 1 < a.jmax) fetch(y + 1, inn);
-    if (y + 2 <= a.jmax) cell_fetch(rn, gr, row + y + 2, h, elev, bedrock, lane);
+  CellRaw ring[K];
+  In inr[K];
+  {
+    CellRaw r0;
+    cell_fetch(r0, gr, row + 1, h, elev, bedrock, lane);
+    cell_take(wl, r0);
+    cell_fetch(r0, gr, row + 2, h, elev, bedrock, lane);
+    cell_take(wc, r0);
+    cell_fetch(r0, gr, row + 3, h, elev, bedrock, lane);
+    cell_take(wr, r0);
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (2 + j < a.jmax) fetch(2 + j, inr[j]);
+    if (4 + j <= a.jmax) cell_fetch(ring[j], gr, row + 4 + j, h, elev, bedrock, lane);
+  }
+  for (int y0 = 2; y0 < a.jmax; y0 += K) {
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int y = y0 + j;
+    if (y >= a.jmax) break;  // uniform
+    const In in = inr[j];
+    const CellRaw rn = ring[j];  // column y + 2
+    if (y + K < a.jmax) fetch(y + K, inr[j]);
+    if (y + 2 + K <= a.jmax) cell_fetch(ring[j], gr, row + y + 2 + K, h, elev, bedrock, lane);
     if (in.act) {
       double elev_update = 0;
       if (wl.elev > wc.elev) {  // (by value: as in the deposition sweep)
@@ -826,78 +950,88 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
     wl = wc;
     wc = wr;
     if (y + 2 <= a.jmax) cell_take(wr, rn);
-    in = inn;
+  }
   }
   cell_store(wl, gr, elev, lane);
   if (a.jmax >= 3) cell_store(wc, gr, elev, lane);
 }
 
 // ------------------------------------------------------------ edge outputs
+// The reference's two loops visit the edge cells in a fixed order (entry
+// e: y or x = 2 + e/2, the far edge for even e, the near edge for odd e),
+// and each fraction's float total must add them in that order.  The two
+// loops' totals are independent (added together at the end): wavefront 0
+// sums the rows' entries and wavefront 1 the columns'.  Chunks of entries
+// are staged in LDS by the whole block (an entry's V, or 0 where the
+// reference skips it, and its eight bedload terms), then lane n of the
+// wavefront adds fraction n from LDS in entry order.  A skipped entry adds 0,
+// which leaves a float total unchanged (it starts at +0 and 0 + -0 = +0), so
+// the sum runs without branches.  The edge cells are distinct and each is
+// read before it is cleared, so the clearing follows in a last pass.
 template <typename E>
-__global__ __launch_bounds__(256) void k_edges(ErodeArgs a, const E* __restrict__ h,
-                                               E* __restrict__ V,
-                                               const E* __restrict__ sdir, size_t nc,
-                                               float* __restrict__ gtot) {
-  // The reference's two loops visit the edge cells in a fixed order (entry
-  // e: y or x = 2 + e/2, the far edge for even e, the near edge for odd e),
-  // and each fraction's float total must add them in that order.  Chunks of
-  // 256 entries are staged in LDS by all threads (one entry each: its
-  // condition, V, and the eight bedload terms), then lane n adds fraction n
-  // from LDS in entry order.  The edge cells are distinct and each is read
-  // before it is cleared, so the clearing follows in a last pass.
-  constexpr int C = 256;
-  __shared__ double s_val[kG - 1][C];  // fraction n at [n-1]: V for n = 1, bedload otherwise
-  __shared__ int s_cond[C];
+__global__ __launch_bounds__(512) void k_edges(ErodeArgs a, const E* __restrict__ h,
+                                                E* __restrict__ V,
+                                                const E* __restrict__ sdir, size_t nc,
+                                                float* __restrict__ gtot) {
+  constexpr int C = 256;  // entries of one loop per chunk (blockDim = 2C)
+  __shared__ double s_val[2][kG - 1][C];  // [loop][fraction n - 1][entry]
   const int t = threadIdx.x;
   const int I = a.imax, J = a.jmax;
   const size_t P = a.pitch;
-  const bool sums = (t >= 1 && t <= kG - 1);  // lane t adds fraction t
-  float part[2] = {0.0f, 0.0f};
-  for (int phase = 0; phase < 2; ++phase) {
-    const int ne = 2 * max((phase == 0 ? J : I) - 2, 0);
-    float acc = 0.0f;
-    for (int base = 0; base < ne; base += C) {
-      const int e = base + t;
-      if (e < ne) {
-        size_t c, src;
-        int d;
-        const int i = 2 + e / 2;
-        if (phase == 0) {  // rows imax (sr of the row above) and 1 (sl of the row below)
-          c = (e % 2 == 0) ? static_cast<size_t>(I) * P + i : P + i;
-          src = (e % 2 == 0) ? c - P : c + P;
-          d = (e % 2 == 0) ? 1 : 3;
-        } else {  // columns jmax (sd of the column left) and 1 (su of the column right)
-          c = (e % 2 == 0) ? static_cast<size_t>(i) * P + J : static_cast<size_t>(i) * P + 1;
-          src = (e % 2 == 0) ? c - 1 : c + 1;
-          d = (e % 2 == 0) ? 2 : 0;
-        }
-        const double v = V[c];
-        s_cond[t] = (h[c] > a.wdet || v > 0) ? 1 : 0;
-        s_val[0][t] = v;
+  const int ne[2] = {2 * max(J - 2, 0), 2 * max(I - 2, 0)};
+  const int wv = t >> 6, ln = t & 63;
+  const bool sums = wv < 2 && ln >= 1 && ln <= kG - 1;  // lane ln of wavefront wv adds fraction ln
+  float acc = 0.0f;
+  const int nchunk = (max(ne[0], ne[1]) + C - 1) / C;
+  for (int k = 0; k < nchunk; ++k) {
+    const int ph = t / C, e = k * C + (t % C);  // threads 0..C-1 stage loop 0, C..2C-1 loop 1
+    if (e < ne[ph]) {
+      size_t c, src;
+      int d;
+      const int i = 2 + e / 2;
+      if (ph == 0) {  // rows imax (sr of the row above) and 1 (sl of the row below)
+        c = (e % 2 == 0) ? static_cast<size_t>(I) * P + i : P + i;
+        src = (e % 2 == 0) ? c - P : c + P;
+        d = (e % 2 == 0) ? 1 : 3;
+      } else {  // columns jmax (sd of the column left) and 1 (su of the column right)
+        c = (e % 2 == 0) ? static_cast<size_t>(i) * P + J : static_cast<size_t>(i) * P + 1;
+        src = (e % 2 == 0) ? c - 1 : c + 1;
+        d = (e % 2 == 0) ? 2 : 0;
+      }
+      const double v = V[c];
+      const bool cond = h[c] > a.wdet || v > 0;
+      s_val[ph][0][t % C] = cond ? v : 0.0;
 #pragma unroll
-        for (int n = 2; n <= kG - 1; ++n) s_val[n - 1][t] = sdir[(d * 10 + n) * nc + src];
-      }
-      __syncthreads();
-      if (sums) {
-        const int cnt = min(C, ne - base);
-        for (int k = 0; k < cnt; ++k)
-          if (s_cond[k]) acc = static_cast<float>(static_cast<double>(acc) + s_val[t - 1][k]);
-      }
-      __syncthreads();  // the chunk's LDS is rewritten by the next one
+      for (int n = 2; n <= kG - 1; ++n)
+        s_val[ph][n - 1][t % C] = cond ? static_cast<double>(sdir[(d * 10 + n) * nc + src]) : 0.0;
     }
-    part[phase] = acc;
+    __syncthreads();
+    if (sums) {
+      const int cnt = min(C, ne[wv] - k * C);
+      const double* sv = s_val[wv][ln - 1];
+      int q = 0;
+      for (; q + 8 <= cnt; q += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = static_cast<float>(static_cast<double>(acc) + sv[q + u]);
+      }
+      for (; q < cnt; ++q) acc = static_cast<float>(static_cast<double>(acc) + sv[q]);
+    }
+    __syncthreads();  // the chunk's LDS is rewritten by the next one
   }
-  for (int y = 2 + t; y < J; y += C) {
+  __shared__ float s_part[2][kG];
+  if (sums) s_part[wv][ln] = acc;
+  for (int y = 2 + t; y < J; y += blockDim.x) {
     const size_t cA = static_cast<size_t>(I) * P + y, cB = P + y;
     if (h[cA] > a.wdet || V[cA] > 0) V[cA] = 0;
     if (h[cB] > a.wdet || V[cB] > 0) V[cB] = 0;
   }
-  for (int x = 2 + t; x < I; x += C) {
+  for (int x = 2 + t; x < I; x += blockDim.x) {
     const size_t cA = static_cast<size_t>(x) * P + J, cB = static_cast<size_t>(x) * P + 1;
     if (h[cA] > a.wdet || V[cA] > 0) V[cA] = 0;
     if (h[cB] > a.wdet || V[cB] > 0) V[cB] = 0;
   }
-  if (t < 20) gtot[t] = sums ? (part[0] + part[1]) : 0.0f;
+  __syncthreads();
+  if (t < 20) gtot[t] = (t >= 1 && t <= kG - 1) ? (s_part[0][t] + s_part[1][t]) : 0.0f;
 }
 
 ErodeArgs erode_args(const hc_ctx* c) {
@@ -1106,38 +1240,77 @@ __global__ __launch_bounds__(1024) void k_global_landslide(int imax, int jmax, i
 __device__ constexpr int kCreepOx[8] = {0, 1, 1, 1, 0, -1, -1, -1};
 __device__ constexpr int kCreepOy[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
 
+// en[k] for a k known only at run time, without indexing the array (which
+// would move it to scratch memory)
+__device__ __forceinline__ double en_at(const double (&en)[8], int k) {
+  double v = en[0];
+#pragma unroll
+  for (int q = 1; q < 8; ++q) v = (k == q) ? en[q] : v;
+  return v;
+}
+
 // The grain slides: the reference's row-major sweep as wavefronts t = 3x + y
 // (a slide touches the cell and one neighbour, so sources interact within
 // Chebyshev distance 2), one block, a barrier per wavefront.  Runs before
-// the elevations change (the slides' grain allocation reads them).
+// the elevations change (the slides' grain allocation reads them).  A cell's
+// moves are worked out first; its grain row and the row of the neighbour it
+// slides to are held in registers (RegRow), the next neighbour's row fetched
+// while the current slide runs; no other cell of the wavefront touches
+// these rows.
 template <typename E>
-__global__ __launch_bounds__(1024) void k_creep_grain(ErodeArgs a, int imax, int jmax, int P,
-                                                      double DX, double rate, double time,
-                                                      const double* __restrict__ elev,
-                                                      const double* __restrict__ bedrock,
-                                                      GrainT<E> gr, uint32_t ep) {
+__global__ __launch_bounds__(512) void k_creep_grain(ErodeArgs a, int imax, int jmax, int P,
+                                                     double DX, double rate, double time,
+                                                     const double* __restrict__ elev,
+                                                     const double* __restrict__ bedrock,
+                                                     GrainT<E> gr, uint32_t ep) {
   const int t0 = 3 * 2 + 2, t1 = 3 * (imax - 1) + (jmax - 1);
   for (int t = t0; t <= t1; ++t) {
     const int xlo = max(2, (t - (jmax - 1) + 2) / 3);
     const int xhi = min(imax - 1, (t - 2) / 3);
     for (int x = xlo + static_cast<int>(threadIdx.x); x <= xhi; x += blockDim.x) {
-      const size_t c = static_cast<size_t>(x) * P + (t - 3 * x);
-      if (!(elev[c] > bedrock[c])) continue;
-      for (int k = 0; k < 8; ++k) {
-        const long off = static_cast<long>(kCreepOx[k]) * P + kCreepOy[k];
+      const int y = t - 3 * x;
+      const size_t c = static_cast<size_t>(x) * P + y;
+      const double es = elev[c];
+      const double bc = bedrock[c];
+      // slide_GS is called only where the cell has a grain row (LCM.cpp:5315-5415)
+      if (!(es > bc) || !gr.has[c]) continue;
+      double en[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        en[k] = elev[static_cast<size_t>(static_cast<long>(c) + static_cast<long>(kCreepOx[k]) * P + kCreepOy[k])];
+      unsigned moves = 0;  // bit k: the slide towards neighbour k happens
+#pragma unroll
+      for (int k = 0; k < 8; ++k) moves |= (en[k] < es && en[k] > -9999) ? (1u << k) : 0u;
+      RegRow<E> own, cur, nxt;
+      reg_row_load(own, gr, c);
+      auto dst_of = [&](int k) {
+        return static_cast<size_t>(static_cast<long>(c) + static_cast<long>(kCreepOx[k]) * P + kCreepOy[k]);
+      };
+      // the moving directions in order; the next one's row is fetched while
+      // the current one slides
+      int k = moves ? __builtin_ctz(moves) : 8;
+      if (k < 8) reg_row_load(cur, gr, dst_of(k));
+#pragma unroll 1
+      while (k < 8) {
+        const unsigned rest = moves & ~((2u << k) - 1u);
+        const int kn = rest ? __builtin_ctz(rest) : 8;
+        if (kn < 8) reg_row_load(nxt, gr, dst_of(kn));
         // rate * time / D as the reference groups it: ((dz / D) * rate) * time / D
-        const double es = elev[c];
-        const double en = elev[static_cast<size_t>(static_cast<long>(c) + off)];
-        if (en < es && en > -9999) {
-          const double D = (kCreepOx[k] != 0 && kCreepOy[k] != 0) ? 7.07 : DX;
-          double temp = ((es - en) / D) * rate * time / D;
-          if ((es - temp) < bedrock[c]) temp = es - bedrock[c];
-          if (temp < 0) temp = 0;
-          if (gr.has[c])  // a row made here is numbered at (x, y), direction k
-            slide_gs(gr, c, static_cast<size_t>(static_cast<long>(c) + off), temp, elev, bedrock, a,
-                     hc::row_key(ep, x, t - 3 * x, k));
-        }
+        const bool diag = kCreepOx[k] != 0 && kCreepOy[k] != 0;
+        const double D = diag ? 7.07 : DX;
+        double temp = ((es - en_at(en, k)) / D) * rate * time / D;
+        if ((es - temp) < bc) temp = es - bc;
+        if (temp < 0) temp = 0;
+        const size_t d = dst_of(k);
+        // a row made here is numbered at (x, y), direction k
+        slide_gs_r(gr, c, own, d, cur, temp, elev, bedrock, a, hc::row_key(ep, x, y, k));
+#pragma unroll
+        for (int n = 0; n <= kG; ++n) gr.G(d, n) = static_cast<E>(cur.v[n]);
+        cur = nxt;
+        k = kn;
       }
+#pragma unroll
+      for (int n = 0; n <= kG; ++n) gr.G(c, n) = static_cast<E>(own.v[n]);
     }
     __syncthreads();
   }
@@ -1268,6 +1441,9 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
 
 namespace {
 
+// Steps ahead that the deposition and y-slide sweeps issue their loads.
+constexpr int kDepositAhead = 3, kLateralAhead = 3;
+
 // The epoch of the next launch that can make grain rows.
 int next_row_epoch(hc_ctx* c, uint32_t* ep) {
   if (c->row_epoch >= (1u << (64 - hc::kRowEpochShift)) - 1)
@@ -1335,18 +1511,18 @@ int erode_t(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* 
   // row left to right (the reference's in-place order): one wavefront per
   // column / row, its lanes sharing the per-cell loads and strata moves.
   if (ncol > 0) {
-    hipLaunchKernelGGL(k_deposit<E>, dim3(ncol), dim3(64), 0, c->stream, a, ts, mult, h, elev,
+    hipLaunchKernelGGL((k_deposit<E, kDepositAhead>), dim3(ncol), dim3(64), 0, c->stream, a, ts, mult, h, elev,
                        c->bedrock, c->mask, V, static_cast<const E*>(Tau),
                        static_cast<const E*>(ss), static_cast<const E*>(sdir), gr, etot, etot3);
     HC_HIP(c, hipGetLastError());
     if (int rc = stamp_rows(c)) return rc;
   }
-  hipLaunchKernelGGL(k_lateral_y<E>, dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
+  hipLaunchKernelGGL((k_lateral_y<E, kLateralAhead>), dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
                      c->bedrock, c->mask, static_cast<const E*>(Tau), gr,
                      static_cast<const E*>(etot3));
   HC_HIP(c, hipGetLastError());
   if (int rc = stamp_rows(c)) return rc;
-  hipLaunchKernelGGL(k_edges<E>, dim3(1), dim3(256), 0, c->stream, a, h, V,
+  hipLaunchKernelGGL(k_edges<E>, dim3(1), dim3(512), 0, c->stream, a, h, V,
                      static_cast<const E*>(sdir), c->ncell, c->gtot);
   HC_HIP(c, hipGetLastError());
   float g[20];
@@ -1419,10 +1595,10 @@ int hc_creep_impl(hc_ctx* c, double time, double creep_rate) {
     uint32_t ep;
     if (int rc = next_row_epoch(c, &ep)) return rc;
     if (c->sesz == 4)
-      hipLaunchKernelGGL(k_creep_grain<float>, dim3(1), dim3(1024), 0, c->stream, a, imax, jmax,
+      hipLaunchKernelGGL(k_creep_grain<float>, dim3(1), dim3(512), 0, c->stream, a, imax, jmax,
                          c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<float>(c), ep);
     else
-      hipLaunchKernelGGL(k_creep_grain<double>, dim3(1), dim3(1024), 0, c->stream, a, imax, jmax,
+      hipLaunchKernelGGL(k_creep_grain<double>, dim3(1), dim3(512), 0, c->stream, a, imax, jmax,
                          c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<double>(c), ep);
   }
   const dim3 blk(256), grd(static_cast<unsigned>((jmax + 255) / 256), static_cast<unsigned>(imax));
