@@ -257,7 +257,7 @@ def run_adjoint(size: int = 512, steps: int = 2000, dt: float = 0.5, every: int 
     return out
 
 
-def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50):
+def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50, cpu: bool = True):
     """Flow + erosion (host-stepped as the reference's loop): size^2 at 10 m,
     fractal relief 20 m on a 1 % slope with a carved channel, 9 grain
     fractions (d = 0.0005 .. 0.128 m) in seeded Dirichlet surface proportions
@@ -326,6 +326,59 @@ def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50):
                          "fp64_floor": errs(fields["floor"]),
                          "note": "grain: the cells' summed active-layer grain; fp64_floor: the FP64 "
                                  "form from the DEM plus 1e-8 m of noise, against the FP64 form"}
+    if cpu:
+        out["cpu_baseline"] = cpu_erosion_baseline(arrays, params, warmup)
+        print(f"[bench] erosion cpu baseline: {out['cpu_baseline']['value']:.4e} cell-steps/s",
+              file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_erosion_baseline(arrays, params, warmup: int, budget_s: float = 15.0):
+    """The oracle (FP64 OpenMP restatement, every core this process may use)
+    on the same flow + erosion workload: whole loop bodies after `warmup`
+    bodies, a bounded sample of about budget_s.  Beside it, the port's speed
+    ratio to the compiled reference on the erosion run where the reference can
+    be built (oracle/time_cpu.py --erosion, profiles/cpu_baseline_erosion_256.json)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from hail_caesar_amd import inputs
+    from pyoracle import OracleModel, pad, params_from_dict
+    threads = host_cores()
+    dem = arrays["dem"]
+    p = params_from_dict(params, dem.shape[0], dem.shape[1], float(arrays["DX"]), nthreads=threads,
+                         grain_d=inputs.EROSION_GRAIN_D)
+    o = OracleModel(p)
+    o.field("elev")[...] = pad(dem, -9999.0)
+    o.field("water_depth")[1:-1, 1:-1] = arrays["water_init"]
+    o.set_rain(np.asarray(arrays["rain"], dtype=np.float64))
+    o.load_grain_arrays(arrays["graindata"])
+    o.L.orc_initialise(o.h)
+    warmup = min(warmup, 3)  # the first body runs creep and global landsliding
+    t = time.perf_counter()
+    o.iterate(warmup)
+    warm = time.perf_counter() - t
+    t = time.perf_counter()
+    o.iterate(2)
+    two = time.perf_counter() - t
+    n = max(2, min(400, int(budget_s / max(two / 2, 1e-3))))
+    t = time.perf_counter()
+    o.iterate(n)
+    el = time.perf_counter() - t
+    cells = dem.shape[0] * dem.shape[1]
+    out = {"value": round(cells * n / el, 1), "unit": "cell-steps/s", "cores": threads, "kind": "port",
+           "sample": f"oracle (FP64 OpenMP restatement, {threads} threads) on the same "
+                     f"{dem.shape[0]}x{dem.shape[1]} flow + erosion state: {n} loop bodies after "
+                     f"{warmup + 2} warm-up bodies ({warm:.1f} s), {el:.1f} s"}
+    ref = os.path.join(ROOT, "profiles", "cpu_baseline_erosion_256.json")
+    if os.path.exists(ref):
+        with open(ref) as f:
+            r = json.load(f)
+        out["port_over_reference"] = round(r["port_over_reference"], 3)
+        out["reference_estimate"] = round(out["value"] / r["port_over_reference"], 1)
+        out["reference_timing"] = (
+            f"the compiled reference (-O3 -march=native -fopenmp) and this port on the same "
+            f"{r['workload']} with {r['host']['threads']} threads: "
+            f"{r['runs']['reference_o3']['cell_steps_per_s']:.3e} and "
+            f"{r['runs']['port']['cell_steps_per_s']:.3e} cell-steps/s")
     return out
 
 
@@ -472,7 +525,8 @@ def main():
         print(json.dumps({"ensemble": run_ensemble(args.steps, args.warmup)}))
         return
     if args.section == "erosion":
-        print(json.dumps({"erosion": run_erosion(steps=args.steps, warmup=args.warmup)}))
+        print(json.dumps({"erosion": run_erosion(steps=args.steps, warmup=args.warmup,
+                                                 cpu=not args.no_cpu)}))
         return
     if args.section == "adjoint":
         print(json.dumps({"adjoint": run_adjoint()}))
@@ -541,7 +595,7 @@ def main():
               f"{out['ensemble']['ms_per_step']:.4f} ms/step, fused {out['ensemble']['fused_kernel_us']} us",
               file=sys.stderr, flush=True)
     if not args.no_erosion and args.only is None:
-        out["erosion"] = run_erosion()
+        out["erosion"] = run_erosion(cpu=not args.no_cpu)
     if not args.no_adjoint and args.only is None:
         out["adjoint"] = run_adjoint()
         print(f"[bench] adjoint: forward {out['adjoint']['forward_ms_per_step']} ms/step, reverse "

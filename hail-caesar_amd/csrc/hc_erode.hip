@@ -736,12 +736,68 @@ __device__ __forceinline__ void lateral_slide_w(const ErodeArgs& a, const GrainT
   }
 }
 
+// ------------------------------------------------- deposition pre-pass
+// The parts of the deposition loop (LCM.cpp:4578-4617) that no other cell's
+// step changes, for every cell the sweep will visit (active, deeper than the
+// erosion threshold, 1 < x < imax, 1 < y < jmax), in parallel: the
+// suspended-sediment entrainment and settling (V is final here), each
+// bedload fraction's val2 - val1, and erodetot and erodetot3 as the
+// reference accumulates them.  The sweep then adds these to the grain rows
+// and elevations in order and runs the ordered part (sort_active, the
+// slides).  Planes of dp (FP64, the sweep's arithmetic): 0 erodetot,
+// 1 erodetot3, 2 the settled amount, 3 + k fraction 2 + k's val2 - val1.
+constexpr int kDepPlanes = 11;
+template <typename E>
+__global__ __launch_bounds__(256) void k_deposit_pre(ErodeArgs a, double time_step,
+                                                     const E* __restrict__ h,
+                                                     const uint64_t* __restrict__ mask,
+                                                     E* __restrict__ V, const E* __restrict__ ss,
+                                                     const E* __restrict__ sdir, size_t nc,
+                                                     E* __restrict__ etot, E* __restrict__ etot3,
+                                                     double* __restrict__ dp) {
+  const int y = blockIdx.x * 64 + threadIdx.x;
+  const int x = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x < 2 || x >= a.imax || y < 2 || y >= a.jmax) return;
+  const size_t P = a.pitch;
+  const size_t c = static_cast<size_t>(x) * P + y;
+  const double hc = h[c];
+  if (!mask_bit(mask, a.mwords, x, y) || !(hc > a.wdet)) return;
+  double et = 0, et3 = 0;
+  {  // fraction 1: suspended load
+    const int n = 1;
+    const double s1 = ss[c];
+    const double v = static_cast<double>(V[c]) + s1;
+    et -= s1;
+    double coeff = (a.fall[n] * time_step) / hc;
+    if (coeff > 1) coeff = 1;
+    double drop = coeff * v;
+    if (drop > 0.001) drop = 0.001;
+    et += drop;
+    V[c] = static_cast<E>(v - drop);
+    dp[2 * nc + c] = drop;
+  }
+  auto sd = [&](int d, int n, size_t cc) { return static_cast<double>(sdir[(static_cast<size_t>(d) * 10 + n) * nc + cc]); };
+#pragma unroll
+  for (int n = 2; n <= kG - 1; ++n) {
+    // su sr sd sl of the cell; su of y+1, sd of y-1, sl of x+1, sr of x-1
+    const double val1 = ((sd(0, n, c) + sd(1, n, c)) + sd(2, n, c)) + sd(3, n, c);
+    const double val2 = ((sd(0, n, c + 1) + sd(2, n, c - 1)) + sd(3, n, c + P)) + sd(1, n, c - P);
+    const double dv = val2 - val1;
+    et += dv;
+    et3 += val1;
+    dp[static_cast<size_t>(1 + n) * nc + c] = dv;
+  }
+  dp[c] = et;
+  dp[nc + c] = et3;
+  etot[c] = static_cast<E>(et);
+  etot3[c] = static_cast<E>(et3);
+}
+
 // ------------------------------------------- deposition + x-lateral slide
-// One wavefront per column y, walking x = 2 .. imax-1 (LCM.cpp:4570-4682).
-// Lane l fetches one of the 64 bedload terms of a cell (fraction 2 + l/8;
-// terms 0-3 its own su, sr, sd, sl, terms 4-7 the neighbours' su of y+1, sd
-// of y-1, sl of x+1, sr of x-1), summed in the reference's order by lanes 8k
-// and 8k+4.  The sweep is a chain of dependent steps, so each step's loads
+// One wavefront per column y, walking x = 2 .. imax-1 (LCM.cpp:4570-4682),
+// on the pre-pass's per-cell terms: lane n of a step holds fraction n's
+// addition (lane 1 the suspended fraction's entrainment and settling).  The
+// sweep is a chain of dependent steps, so each step's loads
 // are issued K steps ahead: a ring of K cells entering the window and K
 // cells' inputs, held in registers (the loop is unrolled by K so that every
 // ring slot is a fixed register set).  Cells ahead of the window are not
@@ -752,33 +808,30 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
                                                 double* __restrict__ elev,
                                                 const double* __restrict__ bedrock,
                                                 const uint64_t* __restrict__ mask,
-                                                E* __restrict__ V,
                                                 const E* __restrict__ Tau,
-                                                const E* __restrict__ ss,
-                                                const E* __restrict__ sdir, GrainT<E> gr,
-                                                E* __restrict__ etot,
-                                                E* __restrict__ etot3) {
+                                                const E* __restrict__ ss, GrainT<E> gr,
+                                                const double* __restrict__ dp) {
   const int y = 2 + blockIdx.x;
   if (y >= a.jmax || a.imax < 3) return;
   const int lane = threadIdx.x;
   const LaneG L = make_lane(a, lane);
   const size_t P = a.pitch, nc = gr.nc;
-  const int tn = 2 + (lane >> 3), tt = lane & 7;  // this lane's bedload term
-  const int td = (tt < 4) ? tt : (tt == 4 ? 0 : (tt == 5 ? 2 : (tt == 6 ? 3 : 1)));
-  const long toff = (tt < 4) ? 0 : (tt == 4 ? 1 : (tt == 5 ? -1 : (tt == 6 ? static_cast<long>(P)
-                                                                            : -static_cast<long>(P))));
+  // lane 1: the settled amount (plane 2); lanes 2..9: val2 - val1 (plane
+  // lane + 1); other lanes read plane 2 and do not use it
+  const size_t tplane = (lane >= 2 && lane <= kG - 1) ? static_cast<size_t>(lane + 1) : 2;
   // inputs of one cell that nothing in the sweep changes
   struct In {
-    double V, ss, tau, term;
+    double ss, tau, term, et, et3;
     uint64_t act;
   };
   auto fetch = [&](int x, In& in) {
     const size_t c = static_cast<size_t>(x) * P + y;
     in.act = mask_bit(mask, a.mwords, x, y) && (h[c] > a.wdet);
-    in.V = V[c];
     in.ss = ss[c];
     in.tau = Tau[c];
-    in.term = sdir[(static_cast<size_t>(td) * 10 + tn) * nc + c + toff];
+    in.et = dp[c];
+    in.et3 = dp[nc + c];
+    in.term = dp[tplane * nc + c];
   };
   CellW wm, wc, wp;
   CellRaw ring[K];  // slot j: the cell entering the window at a step x = j (mod K)
@@ -809,42 +862,11 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
     if (x + 2 + K <= a.imax)
       cell_fetch(ring[j], gr, static_cast<size_t>(x + 2 + K) * P + y, h, elev, bedrock, lane);
     if (in.act) {
-      const size_t c = wc.c;
       if (!wc.has) add_gs_w(gr, wc, a, L);
-      // bedload sums of fractions 2..9 in the reference's order (lanes 8k
-      // and 8k+4), then moved to lane n = 2 + k
-      const double t1 = in.term + __shfl_down(in.term, 1, 8);
-      const double t2 = t1 + __shfl_down(in.term, 2, 8);
-      const double sum4 = t2 + __shfl_down(in.term, 3, 8);
-      const bool bl = lane >= 2 && lane <= kG - 1;
-      const int src1 = bl ? 8 * (lane - 2) : 0;
-      const double val1 = __shfl(sum4, src1, 64);
-      const double val2 = __shfl(sum4, src1 + 4, 64);
-      const double dv = val2 - val1;
-      double et = 0, et3 = 0;
-      {  // fraction 1: suspended load
-        const int n = 1;
-        const double v = in.V + in.ss;
-        if (lane == n) wc.g -= in.ss;
-        et -= in.ss;
-        double coeff = (a.fall[n] * time_step) / wc.h;
-        if (coeff > 1) coeff = 1;
-        double drop = coeff * v;
-        if (drop > 0.001) drop = 0.001;
-        if (lane == n) wc.g += drop;
-        et += drop;
-        if (lane == 0) V[c] = v - drop;
-      }
-      if (bl) wc.g += dv;  // fractions 2..9 at once
-#pragma unroll
-      for (int n = 2; n <= kG - 1; ++n) {  // the running sums in fraction order
-        et += bcast(dv, n);
-        et3 += bcast(val1, n);
-      }
-      if (lane == 0) {
-        etot[c] = et;
-        etot3[c] = et3;
-      }
+      // fraction 1: entrained, then settled; fractions 2..9: val2 - val1
+      if (lane == 1) wc.g = (wc.g - in.ss) + in.term;
+      if (lane >= 2 && lane <= kG - 1) wc.g += in.term;
+      const double et = in.et, et3 = in.et3;
       wc.elev += et;
       if (et < 0) sort_active_alloc_w(gr, wc, a, L);
       if (et3 > 0) {
@@ -1506,14 +1528,20 @@ int erode_t(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* 
   }
   HC_HIP(c, hipMemsetAsync(etot, 0, c->ncell * sizeof(E), c->stream));
   HC_HIP(c, hipMemsetAsync(etot3, 0, c->ncell * sizeof(E), c->stream));
+  if (!c->d_dep)
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_dep), kDepPlanes * c->ncell * sizeof(double)));
+  hipLaunchKernelGGL(k_deposit_pre<E>, cells, blk, 0, c->stream, a, ts, h, c->mask, V,
+                     static_cast<const E*>(ss), static_cast<const E*>(sdir), c->ncell, etot, etot3,
+                     c->d_dep);
+  HC_HIP(c, hipGetLastError());
   const int ncol = std::max(0, c->g.jmax - 2);
   // The deposition sweep runs each column top to bottom and the y-slide each
   // row left to right (the reference's in-place order): one wavefront per
   // column / row, its lanes sharing the per-cell loads and strata moves.
   if (ncol > 0) {
     hipLaunchKernelGGL((k_deposit<E, kDepositAhead>), dim3(ncol), dim3(64), 0, c->stream, a, ts, mult, h, elev,
-                       c->bedrock, c->mask, V, static_cast<const E*>(Tau),
-                       static_cast<const E*>(ss), static_cast<const E*>(sdir), gr, etot, etot3);
+                       c->bedrock, c->mask, static_cast<const E*>(Tau),
+                       static_cast<const E*>(ss), gr, static_cast<const double*>(c->d_dep));
     HC_HIP(c, hipGetLastError());
     if (int rc = stamp_rows(c)) return rc;
   }

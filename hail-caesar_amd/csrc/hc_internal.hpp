@@ -256,6 +256,7 @@ struct hc_ctx {
   void* sdir = nullptr;       // 4 directions x 10 planes: su, sr, sd, sl
   void* etot = nullptr;       // erodetot
   void* etot3 = nullptr;      // erodetot3
+  double* d_dep = nullptr;    // deposition pre-pass terms (11 FP64 planes; scratch)
   float* gtot = nullptr;      // 20 floats
 
   // reductions and small device scalars

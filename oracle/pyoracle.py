@@ -341,6 +341,19 @@ class OracleModel:
             st[x1, y1] = row[14:104].reshape(10, 9)
         self.scalar("grain_array_tot", float(tot))
 
+    def load_grain_arrays(self, graindata: np.ndarray) -> None:
+        """The grain data file's values given per cell in memory ((imax, jmax,
+        101): grain[0..10] then strata[z][n]), every cell with a row numbered
+        in row-major order, as a file of one line per cell would number them
+        (LSDCatchmentModel.cpp:611-683)."""
+        imax, jmax = self.p.imax, self.p.jmax
+        gd = np.asarray(graindata, dtype=np.float64).reshape(imax, jmax, 101)
+        self.field("grain")[1:-1, 1:-1, :] = gd[:, :, :11]
+        self.field("strata").reshape(self.shape + (90,))[1:-1, 1:-1, :] = gd[:, :, 11:]
+        self.field_int("hasgrain")[1:-1, 1:-1] = 1
+        self.field_int("index")[1:-1, 1:-1] = np.arange(1, imax * jmax + 1, dtype=np.int32).reshape(imax, jmax)
+        self.scalar("grain_array_tot", float(imax * jmax))
+
     def load_snapshot(self, snap: dict) -> None:
         """Inverse of RefModel.snapshot for the oracle's dense layout."""
         names = {"spat_var_mannings": "mannings"}
