@@ -151,7 +151,10 @@ def fp32_error(r32, r64, arrays, params):
     threshold; the floor shows how far that alone carries.  meets_max_1e-4
     states whether the depth's max-norm error is within 1e-4 of the largest
     depth; within_fp64_floor whether every depth figure is within the
-    floor's."""
+    floor's.  A second floor starts the FP64 form from its inputs rounded to
+    float32 (DEM, initial depth, Manning n: the FP32 form stores n in FP32),
+    a sub-ulp perturbation of the FP32 grid; within_fp64_floor_f32_inputs
+    compares against it."""
     from hail_caesar_amd import hc
     from hail_caesar_amd.model import Model
     # both forms to the same number of loop bodies (their tile trials differ)
@@ -174,6 +177,19 @@ def fp32_error(r32, r64, arrays, params):
         ref = m64.download(f)
         out["fields"][f] = _field_errors(m32.download(f), ref)
         out["fp64_floor"][f] = _field_errors(mf.download(f), ref)
+    rounded = dict(arrays)
+    for k in ("dem", "water_init", "mannings"):
+        if rounded.get(k) is not None:
+            rounded[k] = np.asarray(arrays[k], dtype=np.float32).astype(np.float64)
+    mr = Model(None, None, precision=hc.HC_FP64, arrays=rounded, params=params, device_loop=True,
+               use_graph=True)
+    mr.iterate(nsteps)
+    mr.synchronize()
+    out["fp64_floor_f32_inputs"] = {f: _field_errors(mr.download(f), m64.download(f))
+                                    for f in ("water_depth", "qx", "qy")}
+    out["fp64_floor_f32_inputs"]["what"] = ("the FP64 form from the DEM, initial depth and "
+                                            "Manning n rounded to float32, against the FP64 form")
+    mr.close()
     vol32 = float(m32.download("water_depth")[1:-1, 1:-1].sum())
     vol64 = float(m64.download("water_depth")[1:-1, 1:-1].sum())
     out["volume_rel"] = abs(vol32 - vol64) / vol64
@@ -181,6 +197,8 @@ def fp32_error(r32, r64, arrays, params):
     out["fp64_floor"]["what"] = "the FP64 form from the DEM plus 1e-8 m of noise, against the FP64 form"
     out["meets_max_1e-4"] = bool(h["max_rel"] <= 1e-4)
     out["within_fp64_floor"] = bool(all(h[k] <= hf[k] for k in ("max_rel", "l2_rel", "frac_cells_over_1e-4")))
+    hr = out["fp64_floor_f32_inputs"]["water_depth"]
+    out["within_fp64_floor_f32_inputs"] = bool(all(h[k] <= hr[k] for k in ("max_rel", "l2_rel", "frac_cells_over_1e-4")))
     mf.close()
     return out
 
@@ -310,6 +328,17 @@ def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50, cpu: bool 
     m.iterate(warmup + steps)
     fields["floor"] = {f: m.download(f) for f in ("elev", "water_depth", "qx", "qy", "Vsusptot", "grain")}
     m.close()
+    # and from the inputs rounded to float32 (DEM, initial depth, grain data):
+    # erode_mult = trunc(ERODEFACTOR / tempbmax) (LCM.cpp:2277-2294) is
+    # discontinuous, and a sub-ulp perturbation of this size changes its
+    # schedule within ~100 bodies, as the FP32 form's rounding does
+    rounded = dict(arrays)
+    for k in ("dem", "water_init", "graindata"):
+        rounded[k] = np.asarray(arrays[k], dtype=np.float32).astype(np.float64)
+    m = Model(None, None, precision=hc.HC_FP64, arrays=rounded, params=params)
+    m.iterate(warmup + steps)
+    fields["floor32"] = {f: m.download(f) for f in ("elev", "water_depth", "qx", "qy", "Vsusptot", "grain")}
+    m.close()
     out["value"] = out["fp64"]["value"]
     out["dtype"] = "f64"
     b = fields["fp64"]
@@ -324,8 +353,11 @@ def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50, cpu: bool 
         return e
     out["fp32_error"] = {"after_loop_bodies": warmup + steps, "fields": errs(fields["fp32"]),
                          "fp64_floor": errs(fields["floor"]),
+                         "fp64_floor_f32_inputs": errs(fields["floor32"]),
                          "note": "grain: the cells' summed active-layer grain; fp64_floor: the FP64 "
-                                 "form from the DEM plus 1e-8 m of noise, against the FP64 form"}
+                                 "form from the DEM plus 1e-8 m of noise, against the FP64 form; "
+                                 "fp64_floor_f32_inputs: the FP64 form from the DEM, initial depth "
+                                 "and grain data rounded to float32"}
     if cpu:
         out["cpu_baseline"] = cpu_erosion_baseline(arrays, params, warmup)
         print(f"[bench] erosion cpu baseline: {out['cpu_baseline']['value']:.4e} cell-steps/s",
@@ -356,18 +388,21 @@ def cpu_erosion_baseline(arrays, params, warmup: int, budget_s: float = 15.0):
     t = time.perf_counter()
     o.iterate(warmup)
     warm = time.perf_counter() - t
+    # bodies with and without an erode call differ widely in cost: step
+    # until the budget is spent
+    n = 0
     t = time.perf_counter()
-    o.iterate(2)
-    two = time.perf_counter() - t
-    n = max(2, min(400, int(budget_s / max(two / 2, 1e-3))))
-    t = time.perf_counter()
-    o.iterate(n)
-    el = time.perf_counter() - t
+    while True:
+        o.iterate(1)
+        n += 1
+        el = time.perf_counter() - t
+        if el >= budget_s or n >= 2000:
+            break
     cells = dem.shape[0] * dem.shape[1]
     out = {"value": round(cells * n / el, 1), "unit": "cell-steps/s", "cores": threads, "kind": "port",
            "sample": f"oracle (FP64 OpenMP restatement, {threads} threads) on the same "
                      f"{dem.shape[0]}x{dem.shape[1]} flow + erosion state: {n} loop bodies after "
-                     f"{warmup + 2} warm-up bodies ({warm:.1f} s), {el:.1f} s"}
+                     f"{warmup} warm-up bodies ({warm:.1f} s), {el:.1f} s"}
     ref = os.path.join(ROOT, "profiles", "cpu_baseline_erosion_256.json")
     if os.path.exists(ref):
         with open(ref) as f:
