@@ -356,7 +356,7 @@ int hc_destroy(hc_ctx* c) {
                   c->hasgrain, c->needgrain, c->rowkey, c->bedrock, c->Tau, c->Vel, c->ss, c->sdir,
                   c->etot, c->etot3, c->gtot, c->d_red, c->d_partials, c->d_count, c->d_out,
                   c->d_pts, c->d_amt, c->d_ctl, c->d_rec, c->d_top, c->d_area, c->d_rfarea,
-                  c->d_pmask, c->d_dpart, c->d_wmax, c->d_slide_moves, c->d_runoff,
+                  c->d_pmask, c->d_dpart, c->d_wmax, c->d_wet, c->d_slide_moves, c->d_runoff,
                   c->d_plist, c->d_pamt, c->d_dep};
   for (auto& gexec : c->loop_graph)
     if (gexec) (void)hipGraphExecDestroy(gexec);

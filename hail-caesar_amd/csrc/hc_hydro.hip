@@ -272,6 +272,12 @@ struct FusedPtrs {
   // and one conditional atomic on red per block
   double* wmax;
   hc::InjectArgs inj;  // rain input at the input points (device loop only)
+  // device loop, aligned tiles: on a body that rebuilds the active mask
+  // (*wet_now set) every stored depth also leaves its wet bit (h > 0) in
+  // wet, one 64-column word per row and wavefront, for the scan kernel's
+  // 3x3 dilation; null: no bits
+  uint64_t* wet = nullptr;
+  const int* wet_now = nullptr;
 };
 
 // One tile: a strip of nsub x R output rows by four wavefronts side by side.
@@ -403,6 +409,17 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   // wait for any of them is a wait for all, and the vector loads would
   // otherwise queue behind that chain.
   double dtd = dt_arg;
+  bool wetb = false;  // this body leaves wet-cell bits (device loop, scan bodies)
+  uint32_t wet_lo = 0, wet_hi = 0;  // lane k: the wet bits of the sub-tile's row r0 + k
+  // the sub-tile's R words of wet bits in one store (word wu of each row)
+  uint64_t* const wetp = p.wet;
+  auto flush_wet = [&](int r0) {
+    const int xr = r0 + lane;
+    const uint32_t wo = static_cast<uint32_t>(xr * a.mwords + wu) * 8u;
+    if (lane < R && xr < NR)  // an ordinary store: the words of a row meet in L2
+      *reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(wetp) + wo) =
+          (static_cast<uint64_t>(wet_hi) << 32) | wet_lo;
+  };
   float dtDX = 0.0f;  // FP32 form's depth-update factor
   const float fthr = static_cast<float>(a.flat_dz);
   Face32K K;
@@ -672,6 +689,14 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
         st_nt(qx1, o, qxNp);
         st_nt(qy1, o, qyNp);
       }
+      if constexpr (AL) {
+        if (wetb) {  // row xo's word (the wavefront's 64 columns) waits in lane k-2
+          const uint64_t bits = __ballot(owned && hn > T(0));
+          const bool mine = lane == k - 2;
+          wet_lo = mine ? static_cast<uint32_t>(bits) : wet_lo;
+          wet_hi = mine ? static_cast<uint32_t>(bits >> 32) : wet_hi;
+        }
+      }
     }
     qxNp = qxN;
     qyNp = qyN;
@@ -694,6 +719,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   if (dtp) dtd = *dtp;
   if (drains) ts = *drain.ts;
   if (inject && p.inj.rf_uniform >= 0) amt_u = p.inj.amt[p.inj.rf_uniform];
+  wetb = AL && p.wet != nullptr && *p.wet_now != 0;  // uniform over the grid
   dt_setup();
   if (active_wave && inject) {
 #pragma unroll
@@ -705,6 +731,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     if constexpr (AL) east_faces();
 #pragma unroll
     for (int k = 1; k < R + 2; ++k) row(win, k, r0);
+    if (AL && wetb) flush_wet(r0);
   }
   for (int t = 1; t < nsub; ++t) {
     r0 += R;
@@ -733,6 +760,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       if constexpr (AL) east_faces();
 #pragma unroll
       for (int k = 2; k < R + 2; ++k) row(win, k, r0);
+      if (AL && wetb) flush_wet(r0);
     }
   }
   const double mine = owned && active_wave ? (sick ? INFINITY : static_cast<double>(md)) : 0.0;
@@ -1047,7 +1075,8 @@ void hc_fused_geometry(const hc_ctx* c, int R, int sw, int* nbx, int* nby) {
 
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* haltp, bool swap,
                          const DrainArgs* drain, const hc::InjectArgs* inject, double* wmax,
-                         hipEvent_t ev_start, hipEvent_t ev_stop) {
+                         hipEvent_t ev_start, hipEvent_t ev_stop, uint64_t* wet,
+                         const int* wet_now) {
   if (c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
   if (!hc::fused_offsets_fit(c->NR, c->P))
@@ -1089,6 +1118,8 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     p.red = c->d_red;
     p.wmax = wmax;
     p.inj = inject ? *inject : hc::InjectArgs{};
+    p.wet = M != 0 ? wet : nullptr;  // the aligned forms only
+    p.wet_now = wet_now;
     if (ev_start)  // the events time the kernel's own execution (dispatch packet timestamps)
       hipExtLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, ev_start, ev_stop, 0, a,
                             dt, nbx, nblocks, ns, p, dtp, haltp, dr);

@@ -64,6 +64,7 @@ struct LoopCtl {
   int input_counter[kRfMax], nActual[kRfMax];
   int counter, halt, err, npts, pts_dirty, first_new, last_first, iter;  // err: kLoopErr*
   int halt_pending;  // the end-of-run test passed: halt before the next body
+  int scan_now;      // this body rebuilds the active mask (counter % scan_interval == 0)
 };
 
 static_assert(kRfMax == HC_RF_MAX, "rain cell capacity");
@@ -270,6 +271,7 @@ struct hc_ctx {
   double* d_dpart = nullptr;            // fused-kernel edge drain: one slot per tile
   size_t dpart_cap = 0;
   double* d_wmax = nullptr;             // device loop: fused kernel's per-wavefront maxdepth
+  uint64_t* d_wet = nullptr;            // device loop (FP32 forms): wet-cell bits, NR x W words
   size_t wmax_cap = 0;
   double* d_out = nullptr;              // [0] temptot
   double* h_pinned = nullptr;           // pinned host mirror (8 doubles)
@@ -330,7 +332,8 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
                          const int* haltp = nullptr, bool swap = true,
                          const hc::DrainArgs* drain = nullptr,
                          const hc::InjectArgs* inject = nullptr, double* wmax = nullptr,
-                         hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+                         hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr,
+                         uint64_t* wet = nullptr, const int* wet_now = nullptr);
 // Geometry of the fused kernel's launch for the context's tile rows.
 // Columns per wavefront of this context's single-grid fused kernel.
 int hc_fused_stride(const hc_ctx* c);

@@ -16,7 +16,9 @@
 //                              from the control block; the edge tiles drain
 //                              and store one partial sum each)
 //   k_loop_scan                temptot from the edge partials (block 0, in
-//                              tile order), scan_area when counter % 5 == 0,
+//                              tile order), scan_area when counter % 5 == 0
+//                              (FP32 forms: from the wet-cell bits the fused
+//                              kernel stored on that body, not the depths),
 //                              and the input-point rebuild after a baseflow
 //                              trigger (get_catchment_input_points,
 //                              LCM.cpp:3741-3767)
@@ -179,6 +181,7 @@ __device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restr
       for (int n = 0; n < kRfMax; ++n) r.j_mean[n] = C.j_mean[n];
       // main.cpp:204 on this body's clock (nothing later in the body moves it)
       if (!(C.cycle / 60 < a.maxcycle)) C.halt_pending = 1;
+      C.scan_now = (C.counter % a.scan_interval) == 0 ? 1 : 0;
     }
   }
 }
@@ -244,6 +247,38 @@ __device__ __forceinline__ void rebuild_points(const LoopArgs& a, LoopCtl* __res
   }
 }
 
+// scan_area from the wet-cell bits the fused kernel left (bit = stored depth
+// > 0, a NaN counting as dry, rows 0..imax+1): one thread per mask word ORs
+// the words of the rows above and below, dilates the result one column either
+// way (the side words supply the bits past the word's ends) and keeps
+// columns 1..jmax; rows 0 and imax+1 are cleared.  The same mask as
+// scan_item's from the depths themselves, from 8 bytes per 64 cells.
+__device__ __forceinline__ void scan_from_wet(int imax, int jmax, int W,
+                                              const uint64_t* __restrict__ wet,
+                                              uint64_t* __restrict__ mask) {
+  const long n = static_cast<long>(imax + 2) * W;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(i / W), w = static_cast<int>(i % W);
+    uint64_t m = 0;
+    if (x >= 1 && x <= imax) {
+      const uint64_t* r = wet + static_cast<size_t>(x - 1) * W + w;  // rows x-1 .. x+1
+      const uint64_t v = r[0] | r[W] | r[2 * W];
+      const uint64_t west = w > 0 ? (r[-1] | r[W - 1] | r[2 * W - 1]) >> 63 : 0ull;
+      const uint64_t east = w + 1 < W ? (r[1] | r[W + 1] | r[2 * W + 1]) << 63 : 0ull;
+      m = v | (v << 1) | west | (v >> 1) | east;
+      const int lo = w * 64;
+      uint64_t keep = ~0ull;
+      if (lo == 0) keep &= ~1ull;
+      const int hi = jmax + 1 - lo;  // first column past jmax, relative to the word
+      if (hi <= 0) keep = 0;
+      else if (hi < 64) keep &= (1ull << hi) - 1;
+      m &= keep;
+    }
+    mask[i] = m;
+  }
+}
+
 // Partial sums of the edge tiles in slot order, then a fixed tree: the
 // water_flux_out total for the iteration.
 __device__ void finish_drain(LoopCtl* __restrict__ ctl, LoopRecord* __restrict__ rec,
@@ -285,7 +320,8 @@ __global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restri
                                                    int nby, int nbx, int R, int sw,
                                                    LoopRecord* __restrict__ rec,
                                                    const double* __restrict__ wmax, int nwmax,
-                                                   unsigned long long* __restrict__ red) {
+                                                   unsigned long long* __restrict__ red,
+                                                   const uint64_t* __restrict__ wet) {
   // the control kernel alone sets halt, so the fused kernel of this
   // iteration ran exactly when it is still clear here
   const bool live = !ctl->halt && !ctl->err;
@@ -300,7 +336,9 @@ __global__ __launch_bounds__(256) void k_loop_scan(LoopArgs a, LoopCtl* __restri
   if (live && blockIdx.x == 0)
     finish_drain(ctl, rec, partials, nby, nbx, R, a.imax, a.jmax, sw);
   if (!scan && !rebuild) return;  // uniform over the grid
-  if (scan) {
+  if (scan && wet) {
+    scan_from_wet(a.imax, a.jmax, W, wet, mask);
+  } else if (scan) {
     const int items = scan_items(a.imax, W, static_cast<int>(sizeof(T)));
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
     for (int it = blockIdx.x * 4 + wv; it < items; it += gridDim.x * 4)
@@ -398,7 +436,8 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   inj.rf_uniform = c->rf_uniform;
   int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt, true, &dr, &inj,
                                 c->d_wmax, prof ? c->prof_ev[2 * k_in_run] : nullptr,
-                                prof ? c->prof_ev[2 * k_in_run + 1] : nullptr);
+                                prof ? c->prof_ev[2 * k_in_run + 1] : nullptr, c->d_wet,
+                                &c->d_ctl->scan_now);
   if (rc) return rc;
   if (prof) c->prof_n = k_in_run + 1;
   const int o = c->cur;  // flow_depth flipped the generation
@@ -412,11 +451,13 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   if (c->prec == HC_FP64)
     hipLaunchKernelGGL(k_loop_scan<double>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
                        (const double*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
-                       c->d_dpart, nby, nbx, R, sw, c->d_rec, c->d_wmax, nwmax, c->d_red);
+                       c->d_dpart, nby, nbx, R, sw, c->d_rec, c->d_wmax, nwmax, c->d_red,
+                       nullptr);
   else
     hipLaunchKernelGGL(k_loop_scan<float>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
                        (const float*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
-                       c->d_dpart, nby, nbx, R, sw, c->d_rec, c->d_wmax, nwmax, c->d_red);
+                       c->d_dpart, nby, nbx, R, sw, c->d_rec, c->d_wmax, nwmax, c->d_red,
+                       c->d_wet);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
@@ -517,6 +558,9 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
   }
   const size_t nwords = static_cast<size_t>(c->NR) * c->W;
   if (!c->d_pmask) HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_pmask), nwords * sizeof(uint64_t)));
+  // FP32 forms (aligned tiles): the fused kernel's wet bits for the scan
+  if (c->prec != HC_FP64 && !c->d_wet)
+    HC_HIP(c, hipMalloc(reinterpret_cast<void**>(&c->d_wet), nwords * sizeof(uint64_t)));
   std::vector<uint64_t> pm(nwords, 0ull);
   for (int k = 0; k < in->npoints; ++k) {
     const int x = in->in_x[k], y = in->in_y[k];

@@ -85,6 +85,52 @@ def test_device_loop_equals_host_loop(tmp_path, use_graph, chunks):
     np.testing.assert_allclose(dev.series(), host.series(), rtol=1e-12, atol=1e-14)
 
 
+@pytest.mark.parametrize("fixed", [True, False])
+def test_fp32_device_loop_equals_host_loop(tmp_path, fixed):
+    """FP32 forms (fixed-point and FP64 elevations): the device loop, whose
+    fused kernel adds the rain and drains the edges and whose scan_area
+    dilates the wet-cell bits that kernel stores on scan bodies, replays the
+    host loop (point adds, edge drain and scan from the depths as separate
+    calls) bit for bit."""
+    w = _flow(tmp_path, 64)
+    kw = dict(precision=hc.HC_FP32, flat_dz=0.0 if fixed else -1.0, elev_fixed=fixed)
+    host = Model(w.path, w.params_file, **kw)
+    th, ch, dh = host.iterate(10 ** 7)
+    dev = Model(w.path, w.params_file, device_loop=True, use_graph=True, **kw)
+    parts = [dev.iterate(1500)]
+    while not dev.finished:
+        parts.append(dev.iterate(1500))
+    td = np.concatenate([p[0] for p in parts])
+    cd = np.concatenate([p[1] for p in parts])
+    dd = np.concatenate([p[2] for p in parts])
+    assert len(td) == len(th)
+    assert np.array_equal(cd, ch) and np.array_equal(dd, dh)
+    np.testing.assert_allclose(td, th, rtol=1e-13, atol=1e-15)
+    for f in ("water_depth", "qx", "qy"):
+        assert np.array_equal(dev.download(f), host.download(f)), f
+
+
+def test_wet_bit_scan_equals_depth_scan(tmp_path):
+    """After a device-loop body that rebuilds the active mask (every 5th),
+    the mask dilated from the fused kernel's wet bits equals scan_area run
+    on the depths that body stored; checked at several points of a
+    partially wet FP32 run."""
+    w = _flow(tmp_path, 96)
+    dev = Model(w.path, w.params_file, precision=hc.HC_FP32, elev_fixed=True, device_loop=True,
+                use_graph=True)
+    ctx = dev.context()
+    dev.iterate(1)
+    for n in (25, 100, 1000, 1400):  # the run ends after 2701 bodies
+        dev.iterate(n - int(dev.scalar("counter")) % 5)  # end on a scan body
+        assert int(dev.scalar("counter")) % 5 == 0 and not dev.finished
+        m_loop = ctx.download("mask")
+        ctx.scan_area()
+        m_scan = ctx.download("mask")
+        inner = m_scan[1:-1, 1:-1]
+        assert 0 < inner.sum() < inner.size, "the test needs a partially wet grid"
+        assert np.array_equal(m_loop, m_scan), n
+
+
 def _bos(tmp):
     src = os.path.join(GOLDEN, "boscastle")
     dst = os.path.join(str(tmp), "bos")
