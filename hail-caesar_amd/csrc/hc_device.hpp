@@ -53,6 +53,21 @@ __device__ __forceinline__ float pin_value(float v) {
   asm volatile("" : "+v"(v));
   return v;
 }
+__device__ __forceinline__ double pin_value(double v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Zero in a vector register, opaque to the compiler.  Added to the index of
+// a load at a wave-uniform address it makes a vector-memory load: those
+// complete in order, so waiting for one leaves younger loads in flight,
+// whereas scalar loads complete out of order and a wait for any of them is
+// a wait for all (which empties a prefetch queue).
+__device__ __forceinline__ uint32_t opaque_vzero() {
+  uint32_t z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
 
 // Lane shifts by one across the whole wavefront (DPP wave_shr:1 / wave_shl:1):
 // lane_up1 gives lane l the value of lane l-1 (lane 0 keeps its own),

@@ -512,13 +512,30 @@ __device__ __forceinline__ double bcast(double v, int l) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | lo);
 }
 
-// sum of lanes lo .. hi of v, added in lane order from 0.0 (as the
-// reference's loops over n add)
-__device__ __forceinline__ double lane_sum(double v, int lo, int hi) {
-  double t = 0.0;
-#pragma unroll
-  for (int n = lo; n <= hi; ++n) t += bcast(v, n);
-  return t;
+// Lane n's value in every lane of its 16-lane row (DPP row_newbcast:n, one
+// 64-bit move); for n < 16 row 0 receives lane n itself.
+template <int N>
+__device__ __forceinline__ double row_bcast(double v) {
+  static_assert(N >= 0 && N < 16, "row_newbcast selects a lane of the row");
+  return __longlong_as_double(
+      __builtin_amdgcn_mov_dpp(__double_as_longlong(v), 0x150 + N, 0xf, 0xf, false));
+}
+template <int N, int HI>
+__device__ __forceinline__ double row_sum_from(double t, double v) {
+  if constexpr (N > HI) {
+    return t;
+  } else {
+    return row_sum_from<N + 1, HI>(t + row_bcast<N>(v), v);
+  }
+}
+// Sum of lanes LO .. HI (< 16) of v, added in lane order from 0.0 (as the
+// reference's loops over n add), the same in every lane: row 0's sum is
+// taken through DPP broadcasts, one move and one add per lane, then made
+// uniform from lane 0.
+template <int LO, int HI>
+__device__ __forceinline__ double lane_sum(double v) {
+  const double t = row_sum_from<LO, HI>(0.0, v);
+  return bcast(t, 0);
 }
 
 // A cell's state as fetched; becomes a CellW when the cell enters the
@@ -528,15 +545,18 @@ struct CellRaw {
   size_t c;
   int64_t has;
 };
+// (vz: opaque_vzero(), so that every load is a vector-memory load)
 template <typename E>
 __device__ __forceinline__ void cell_fetch(CellRaw& r, const GrainT<E>& gr, size_t c, const E* h,
-                                           const double* elev, const double* bedrock, int lane) {
+                                           const double* elev, const double* bedrock, int lane,
+                                           uint32_t vz) {
   r.c = c;
+  const size_t cv = c + vz;
   r.gv = (lane <= kG) ? gr.g[static_cast<size_t>(lane) * gr.nc + c] : 0.0;
-  r.elev = elev[c];
-  r.bedrock = bedrock[c];
-  r.h = h[c];
-  r.has = gr.has[c];
+  r.elev = elev[cv];
+  r.bedrock = bedrock[cv];
+  r.h = h[cv];
+  r.has = gr.has[cv];
 }
 __device__ __forceinline__ void cell_take(CellW& w, const CellRaw& r) {
   w.c = r.c;
@@ -591,6 +611,8 @@ __device__ __forceinline__ void strata_move(double& s0, double& s1, int lane, in
 struct LaneG {
   double dprop;   // dprop[lane] for lanes 0..10, else 0
   double dprop9;  // dprop[lane - 16] for lanes 17..25, else 0
+  double dz0;     // dprop[lane % 9 + 1]: a new strata element lane's default (slot 0)
+  double dz1;     // dprop[(lane + 64) % 9 + 1] (slot 1)
   int lane;
   int frac;       // 1 <= lane <= 9
 };
@@ -598,22 +620,45 @@ __device__ __forceinline__ LaneG make_lane(const ErodeArgs& a, int lane) {
   LaneG L;
   L.lane = lane;
   L.frac = (lane >= 1 && lane <= kG - 1) ? 1 : 0;
-  double d = 0.0, d9 = 0.0;
+  double d = 0.0, d9 = 0.0, z0 = 0.0, z1 = 0.0;
+  const int f0 = lane % 9 + 1, f1 = (lane + 64) % 9 + 1;
 #pragma unroll
   for (int n = 0; n <= kG; ++n) {
     d = (lane == n) ? a.dprop[n] : d;
-    if (n >= 1 && n <= kG - 1) d9 = (lane == 16 + n) ? a.dprop[n] : d9;
+    if (n >= 1 && n <= kG - 1) {
+      d9 = (lane == 16 + n) ? a.dprop[n] : d9;
+      z0 = (f0 == n) ? a.dprop[n] : z0;
+      z1 = (f1 == n) ? a.dprop[n] : z1;
+    }
   }
-  L.dprop = d;
-  L.dprop9 = d9;
+  L.dprop = pin_value(d);
+  L.dprop9 = pin_value(d9);
+  L.dz0 = pin_value(z0);
+  L.dz1 = pin_value(z1);
   return L;
+}
+
+// The sweeps' copy of the erosion constants: the floating-point ones held in
+// vector registers for the whole sweep (scalar registers are scarce there,
+// and a constant the compiler reloads from the argument block inside the
+// chain costs a scalar-memory round trip per use).
+__device__ __forceinline__ ErodeArgs sweep_args(const ErodeArgs& in) {
+  ErodeArgs a = in;
+  a.DX = pin_value(a.DX);
+  a.wdet = pin_value(a.wdet);
+  a.ERODEFACTOR = pin_value(a.ERODEFACTOR);
+  a.active = pin_value(a.active);
+  a.lateral_constant = pin_value(a.lateral_constant);
+  a.chann_lateral = pin_value(a.chann_lateral);
+  a.veg_lat = pin_value(a.veg_lat);
+  return a;
 }
 
 // sort_active (LCM.cpp:3874-3947) on a register cell
 template <typename E>
 __device__ __forceinline__ void sort_active_w(const GrainT<E>& gr, CellW& w, const ErodeArgs& a,
                                               const LaneG& L) {
-  const double total = lane_sum(w.g, 0, kG);
+  const double total = lane_sum<0, kG>(w.g);
   if (total > (a.active * 1.5)) {
     double s0, s1;
     strata_load(gr, w.c, L.lane, s0, s1);
@@ -650,7 +695,7 @@ __device__ __forceinline__ void add_gs_w(const GrainT<E>& gr, CellW& w, const Er
   auto val = [&](int i) {
     const int z = i / 9;
     const bool zero = w.elev - (a.active * (z + 1)) < (w.bedrock - a.active);
-    return zero ? 0.0 : (a.active) * a.dprop[i % 9 + 1];
+    return zero ? 0.0 : (a.active) * (i < 64 ? L.dz0 : L.dz1);  // dprop[i % 9 + 1]
   };
   strata_store(gr, w.c, L.lane, val(L.lane), (L.lane + 64 < 90) ? val(L.lane + 64) : 0.0);
   sort_active_w(gr, w, a, L);
@@ -671,7 +716,7 @@ __device__ __forceinline__ void slide_gs_w(const GrainT<E>& gr, CellW& src, Cell
   const bool f = L.frac != 0;
   // sum over n of G(src, n) where positive, in n order (adding 0.0 elsewhere
   // leaves a non-negative running sum unchanged)
-  auto pos_total = [&]() { return lane_sum((f && src.g > 0) ? src.g : 0.0, 1, kG - 1); };
+  auto pos_total = [&]() { return lane_sum<1, kG - 1>((f && src.g > 0) ? src.g : 0.0); };
   if (src.has && dst.has) {
     const double total = pos_total();
     if (amount > total) {
@@ -803,7 +848,7 @@ __global__ __launch_bounds__(256) void k_deposit_pre(ErodeArgs a, double time_st
 // ring slot is a fixed register set).  Cells ahead of the window are not
 // touched before they enter it.
 template <typename E, int K>
-__global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, double mult,
+__global__ __launch_bounds__(64) void k_deposit(ErodeArgs a_in, double time_step, double mult,
                                                 const E* __restrict__ h,
                                                 double* __restrict__ elev,
                                                 const double* __restrict__ bedrock,
@@ -812,21 +857,27 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
                                                 const E* __restrict__ ss, GrainT<E> gr,
                                                 const double* __restrict__ dp) {
   const int y = 2 + blockIdx.x;
-  if (y >= a.jmax || a.imax < 3) return;
+  if (y >= a_in.jmax || a_in.imax < 3) return;
+  const ErodeArgs a = sweep_args(a_in);
+  time_step = pin_value(time_step);
+  mult = pin_value(mult);
   const int lane = threadIdx.x;
   const LaneG L = make_lane(a, lane);
   const size_t P = a.pitch, nc = gr.nc;
   // lane 1: the settled amount (plane 2); lanes 2..9: val2 - val1 (plane
   // lane + 1); other lanes read plane 2 and do not use it
   const size_t tplane = (lane >= 2 && lane <= kG - 1) ? static_cast<size_t>(lane + 1) : 2;
-  // inputs of one cell that nothing in the sweep changes
+  // inputs of one cell that nothing in the sweep changes, all fetched as
+  // vector loads with nothing tested until the step that uses them
+  const uint32_t vz = opaque_vzero();
   struct In {
-    double ss, tau, term, et, et3;
-    uint64_t act;
+    double ss, tau, term, et, et3, h;
+    uint64_t mw;  // the cell's active-mask word
   };
   auto fetch = [&](int x, In& in) {
-    const size_t c = static_cast<size_t>(x) * P + y;
-    in.act = mask_bit(mask, a.mwords, x, y) && (h[c] > a.wdet);
+    const size_t c = static_cast<size_t>(x) * P + y + vz;
+    in.mw = mask[static_cast<size_t>(x) * a.mwords + (y >> 6) + vz];
+    in.h = h[c];
     in.ss = ss[c];
     in.tau = Tau[c];
     in.et = dp[c];
@@ -838,17 +889,18 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
   In inr[K];        // slot j: the inputs of that step's cell
   {
     CellRaw r0;
-    cell_fetch(r0, gr, static_cast<size_t>(1) * P + y, h, elev, bedrock, lane);
+    cell_fetch(r0, gr, static_cast<size_t>(1) * P + y, h, elev, bedrock, lane, vz);
     cell_take(wm, r0);
-    cell_fetch(r0, gr, static_cast<size_t>(2) * P + y, h, elev, bedrock, lane);
+    cell_fetch(r0, gr, static_cast<size_t>(2) * P + y, h, elev, bedrock, lane, vz);
     cell_take(wc, r0);
-    cell_fetch(r0, gr, static_cast<size_t>(3) * P + y, h, elev, bedrock, lane);
+    cell_fetch(r0, gr, static_cast<size_t>(3) * P + y, h, elev, bedrock, lane, vz);
     cell_take(wp, r0);
   }
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     if (2 + j < a.imax) fetch(2 + j, inr[j]);
-    if (4 + j <= a.imax) cell_fetch(ring[j], gr, static_cast<size_t>(4 + j) * P + y, h, elev, bedrock, lane);
+    if (4 + j <= a.imax)
+      cell_fetch(ring[j], gr, static_cast<size_t>(4 + j) * P + y, h, elev, bedrock, lane, vz);
   }
   for (int x0 = 2; x0 < a.imax; x0 += K) {
 #pragma unroll
@@ -856,12 +908,7 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
     const int x = x0 + j;
     if (x >= a.imax) break;  // uniform
     const In in = inr[j];
-    const CellRaw rn = ring[j];  // cell x + 2
-    // refill the slots K steps ahead
-    if (x + K < a.imax) fetch(x + K, inr[j]);
-    if (x + 2 + K <= a.imax)
-      cell_fetch(ring[j], gr, static_cast<size_t>(x + 2 + K) * P + y, h, elev, bedrock, lane);
-    if (in.act) {
+    if (((in.mw >> (y & 63)) & 1ull) && in.h > a.wdet) {
       if (!wc.has) add_gs_w(gr, wc, a, L);
       // fraction 1: entrained, then settled; fractions 2..9: val2 - val1
       if (lane == 1) wc.g = (wc.g - in.ss) + in.term;
@@ -892,7 +939,15 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
     cell_store(wm, gr, elev, lane);  // row x-1 is final
     wm = wc;
     wc = wp;
-    if (x + 2 <= a.imax) cell_take(wp, rn);
+    if (x + 2 <= a.imax) cell_take(wp, ring[j]);  // cell x + 2
+    // refill the slots K steps ahead, once their contents are used (the new
+    // loads then land in the same registers, and nothing reads a register
+    // whose load is in flight)
+    // (past the sweep's end a slot reloads the last cell: every step issues
+    // the same loads, so the wait for a slot is a fixed count of later loads)
+    fetch(min(x + K, a.imax - 1), inr[j]);
+    cell_fetch(ring[j], gr, static_cast<size_t>(min(x + 2 + K, a.imax)) * P + y, h, elev, bedrock,
+               lane, vz);
   }
   }
   // rows imax-1 and imax (the last two of the window)
@@ -904,7 +959,7 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a, double time_step, d
 // One wavefront per row x, walking y = 2 .. jmax-1 (LCM.cpp:4684-4747), its
 // loads issued K steps ahead as in the deposition sweep.
 template <typename E, int K>
-__global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step, double mult,
+__global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a_in, double time_step, double mult,
                                                   const E* __restrict__ h,
                                                   double* __restrict__ elev,
                                                   const double* __restrict__ bedrock,
@@ -912,18 +967,22 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
                                                   const E* __restrict__ Tau, GrainT<E> gr,
                                                   const E* __restrict__ etot3) {
   const int x = 1 + blockIdx.x;
-  if (x > a.imax || a.jmax < 3) return;
+  if (x > a_in.imax || a_in.jmax < 3) return;
+  const ErodeArgs a = sweep_args(a_in);
+  time_step = pin_value(time_step);
+  mult = pin_value(mult);
   const int lane = threadIdx.x;
   const LaneG L = make_lane(a, lane);
   const size_t row = static_cast<size_t>(x) * a.pitch;
+  const uint32_t vz = opaque_vzero();  // vector loads, tested at the step (k_deposit)
   struct In {
     double e3, tau;
-    uint64_t act;
+    uint64_t mw;  // the cell's active-mask word
   };
   auto fetch = [&](int y, In& in) {
-    const size_t c = row + y;
+    const size_t c = row + y + vz;
     in.e3 = etot3[c];
-    in.act = (in.e3 > 0) && mask_bit(mask, a.mwords, x, y);
+    in.mw = mask[static_cast<size_t>(x) * a.mwords + (y >> 6) + vz];
     in.tau = Tau[c];
   };
   CellW wl, wc, wr;
@@ -931,17 +990,17 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
   In inr[K];
   {
     CellRaw r0;
-    cell_fetch(r0, gr, row + 1, h, elev, bedrock, lane);
+    cell_fetch(r0, gr, row + 1, h, elev, bedrock, lane, vz);
     cell_take(wl, r0);
-    cell_fetch(r0, gr, row + 2, h, elev, bedrock, lane);
+    cell_fetch(r0, gr, row + 2, h, elev, bedrock, lane, vz);
     cell_take(wc, r0);
-    cell_fetch(r0, gr, row + 3, h, elev, bedrock, lane);
+    cell_fetch(r0, gr, row + 3, h, elev, bedrock, lane, vz);
     cell_take(wr, r0);
   }
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     if (2 + j < a.jmax) fetch(2 + j, inr[j]);
-    if (4 + j <= a.jmax) cell_fetch(ring[j], gr, row + 4 + j, h, elev, bedrock, lane);
+    if (4 + j <= a.jmax) cell_fetch(ring[j], gr, row + 4 + j, h, elev, bedrock, lane, vz);
   }
   for (int y0 = 2; y0 < a.jmax; y0 += K) {
 #pragma unroll
@@ -949,10 +1008,7 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
     const int y = y0 + j;
     if (y >= a.jmax) break;  // uniform
     const In in = inr[j];
-    const CellRaw rn = ring[j];  // column y + 2
-    if (y + K < a.jmax) fetch(y + K, inr[j]);
-    if (y + 2 + K <= a.jmax) cell_fetch(ring[j], gr, row + y + 2 + K, h, elev, bedrock, lane);
-    if (in.act) {
+    if (in.e3 > 0 && ((in.mw >> (y & 63)) & 1ull)) {
       double elev_update = 0;
       if (wl.elev > wc.elev) {  // (by value: as in the deposition sweep)
         CellW nb = wl;
@@ -971,7 +1027,10 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a, double time_step,
     cell_store(wl, gr, elev, lane);  // column y-1 is final
     wl = wc;
     wc = wr;
-    if (y + 2 <= a.jmax) cell_take(wr, rn);
+    if (y + 2 <= a.jmax) cell_take(wr, ring[j]);  // column y + 2
+    // refill once used, the same loads every step (k_deposit)
+    fetch(min(y + K, a.jmax - 1), inr[j]);
+    cell_fetch(ring[j], gr, row + min(y + 2 + K, a.jmax), h, elev, bedrock, lane, vz);
   }
   }
   cell_store(wl, gr, elev, lane);
