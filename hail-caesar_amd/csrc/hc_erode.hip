@@ -838,16 +838,27 @@ __global__ __launch_bounds__(256) void k_deposit_pre(ErodeArgs a, double time_st
   etot3[c] = static_cast<E>(et3);
 }
 
+// ------------------------------------------------- sweep input staging
+// A sweep's per-step inputs are the records of the cells it walks, in walk
+// order (record i: the sweep's cell i + 1).  They come from memory 64 at a
+// time, one record per lane (64 independent loads per field in flight), and
+// go through LDS: a ring of two 64-record chunks.  The chunk that the walk
+// needs next is loaded into registers a whole chunk ahead and written to the
+// ring as the walk enters it, so the steps themselves only read LDS.  The
+// ring holds the records as they were when the sweep started; a cell's
+// record is taken into the register window before any step changes the
+// cell, and nothing outside this wavefront changes the cells of its walk.
+constexpr int kStageSlots = 128;           // ring slots (two chunks)
+constexpr int kStagePitch = kStageSlots + 1;  // LDS row pitch: rows read by lanes at once fall in
+                                              // different banks
+
 // ------------------------------------------- deposition + x-lateral slide
 // One wavefront per column y, walking x = 2 .. imax-1 (LCM.cpp:4570-4682),
 // on the pre-pass's per-cell terms: lane n of a step holds fraction n's
-// addition (lane 1 the suspended fraction's entrainment and settling).  The
-// sweep is a chain of dependent steps, so each step's loads
-// are issued K steps ahead: a ring of K cells entering the window and K
-// cells' inputs, held in registers (the loop is unrolled by K so that every
-// ring slot is a fixed register set).  Cells ahead of the window are not
-// touched before they enter it.
-template <typename E, int K>
+// addition (lane 1 the suspended fraction's entrainment and settling).
+// Record i is cell x = i + 1 of the column: its grain row, elevation,
+// bedrock, depth, row flag, active-mask bit, ss, Tau and pre-pass terms.
+template <typename E>
 __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a_in, double time_step, double mult,
                                                 const E* __restrict__ h,
                                                 double* __restrict__ elev,
@@ -864,51 +875,100 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a_in, double time_step
   const int lane = threadIdx.x;
   const LaneG L = make_lane(a, lane);
   const size_t P = a.pitch, nc = gr.nc;
-  // lane 1: the settled amount (plane 2); lanes 2..9: val2 - val1 (plane
-  // lane + 1); other lanes read plane 2 and do not use it
-  const size_t tplane = (lane >= 2 && lane <= kG - 1) ? static_cast<size_t>(lane + 1) : 2;
-  // inputs of one cell that nothing in the sweep changes, all fetched as
-  // vector loads with nothing tested until the step that uses them
-  const uint32_t vz = opaque_vzero();
+  constexpr int NV = 7;  // elevation, bedrock, depth, ss, Tau, erodetot, erodetot3
+  __shared__ double s_g[kG + 1][kStagePitch];  // fractions 0..10
+  __shared__ double s_t[kG - 1][kStagePitch];  // fraction n's pre-pass term (n = 1..9)
+  __shared__ double s_v[NV][kStagePitch];
+  __shared__ int s_f[kStagePitch];             // row flag | active bit << 1
+  struct Chunk {  // (the flag and mask word are combined as the chunk is stored, so
+                  // that nothing waits for a load the step after it is issued)
+    double g[kG + 1], t[kG - 1], v[NV];
+    uint64_t mw;
+    int has;
+  };
+  auto load_chunk = [&](int m, Chunk& ch) {  // lane r: record 64 m + r
+    const int x = min(64 * m + lane + 1, a.imax);
+    const size_t c = static_cast<size_t>(x) * P + y;
+#pragma unroll
+    for (int n = 0; n <= kG; ++n) ch.g[n] = gr.g[static_cast<size_t>(n) * nc + c];
+    // fraction 1: the settled amount (plane 2); fraction n >= 2: val2 - val1
+    // (plane n + 1)
+#pragma unroll
+    for (int k = 0; k < kG - 1; ++k) ch.t[k] = dp[static_cast<size_t>(k == 0 ? 2 : k + 2) * nc + c];
+    ch.v[0] = elev[c];
+    ch.v[1] = bedrock[c];
+    ch.v[2] = h[c];
+    ch.v[3] = ss[c];
+    ch.v[4] = Tau[c];
+    ch.v[5] = dp[c];
+    ch.v[6] = dp[nc + c];
+    ch.mw = mask[static_cast<size_t>(x) * a.mwords + (y >> 6)];
+    ch.has = gr.has[c];
+  };
+  auto store_chunk = [&](int m, const Chunk& ch) {
+    const int sl = (64 * m + lane) & (kStageSlots - 1);
+#pragma unroll
+    for (int n = 0; n <= kG; ++n) s_g[n][sl] = ch.g[n];
+#pragma unroll
+    for (int k = 0; k < kG - 1; ++k) s_t[k][sl] = ch.t[k];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s_v[k][sl] = ch.v[k];
+    s_f[sl] = (ch.has != 0 ? 1 : 0) | (static_cast<int>((ch.mw >> (y & 63)) & 1ull) << 1);
+    __syncthreads();  // (one wavefront: orders the lanes' writes before the reads)
+  };
+  // the window cell of record i
+  auto take = [&](CellW& w, int i) {
+    const int sl = i & (kStageSlots - 1);
+    const double gv = s_g[min(lane, kG)][sl];
+    w.g = lane <= kG ? gv : 0.0;
+    w.elev = s_v[0][sl];
+    w.bedrock = s_v[1][sl];
+    w.h = s_v[2][sl];
+    w.has = (s_f[sl] & 1) ? 1u : 0u;
+    w.c = static_cast<size_t>(i + 1) * P + y;
+  };
+  // a step's own inputs (record of cell x)
+  const int tl = min(max(lane - 1, 0), kG - 2);  // lane n's term row (fractions 1..9)
   struct In {
-    double ss, tau, term, et, et3, h;
-    uint64_t mw;  // the cell's active-mask word
+    double h, ss, tau, et, et3, term;
+    int f;
   };
-  auto fetch = [&](int x, In& in) {
-    const size_t c = static_cast<size_t>(x) * P + y + vz;
-    in.mw = mask[static_cast<size_t>(x) * a.mwords + (y >> 6) + vz];
-    in.h = h[c];
-    in.ss = ss[c];
-    in.tau = Tau[c];
-    in.et = dp[c];
-    in.et3 = dp[nc + c];
-    in.term = dp[tplane * nc + c];
+  auto read_in = [&](int i, In& in) {
+    const int sl = i & (kStageSlots - 1);
+    in.f = s_f[sl];
+    in.h = s_v[2][sl];
+    in.ss = s_v[3][sl];
+    in.tau = s_v[4][sl];
+    in.et = s_v[5][sl];
+    in.et3 = s_v[6][sl];
+    in.term = s_t[tl][sl];
   };
+  Chunk pend;
+  load_chunk(0, pend);
+  store_chunk(0, pend);
+  load_chunk(1, pend);
   CellW wm, wc, wp;
-  CellRaw ring[K];  // slot j: the cell entering the window at a step x = j (mod K)
-  In inr[K];        // slot j: the inputs of that step's cell
-  {
-    CellRaw r0;
-    cell_fetch(r0, gr, static_cast<size_t>(1) * P + y, h, elev, bedrock, lane, vz);
-    cell_take(wm, r0);
-    cell_fetch(r0, gr, static_cast<size_t>(2) * P + y, h, elev, bedrock, lane, vz);
-    cell_take(wc, r0);
-    cell_fetch(r0, gr, static_cast<size_t>(3) * P + y, h, elev, bedrock, lane, vz);
-    cell_take(wp, r0);
-  }
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    if (2 + j < a.imax) fetch(2 + j, inr[j]);
-    if (4 + j <= a.imax)
-      cell_fetch(ring[j], gr, static_cast<size_t>(4 + j) * P + y, h, elev, bedrock, lane, vz);
-  }
-  for (int x0 = 2; x0 < a.imax; x0 += K) {
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    const int x = x0 + j;
-    if (x >= a.imax) break;  // uniform
-    const In in = inr[j];
-    if (((in.mw >> (y & 63)) & 1ull) && in.h > a.wdet) {
+  take(wm, 0);
+  take(wc, 1);
+  take(wp, 2);
+  In in;
+  read_in(1, in);  // cell 2's
+  for (int x = 2; x < a.imax; ++x) {
+    // the far cell of this step (x + 2, record x + 1) opens chunk m: write
+    // the chunk loaded a chunk ago and start loading the next
+    if (((x + 1) & 63) == 0) {  // uniform
+      const int m = (x + 1) >> 6;
+      store_chunk(m, pend);
+      load_chunk(m + 1, pend);
+    }
+    // LDS reads for later, issued first so that the step's arithmetic
+    // covers their latency: the far cell (taken at the end of this step) and
+    // the next step's inputs
+    CellW far;
+    take(far, x + 1);
+    In nin;
+    read_in(x, nin);
+    if ((in.f & 2) && in.h > a.wdet) {
       if (!wc.has) add_gs_w(gr, wc, a, L);
       // fraction 1: entrained, then settled; fractions 2..9: val2 - val1
       if (lane == 1) wc.g = (wc.g - in.ss) + in.term;
@@ -917,20 +977,20 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a_in, double time_step
       wc.elev += et;
       if (et < 0) sort_active_alloc_w(gr, wc, a, L);
       if (et3 > 0) {
+        const double tau = in.tau;
         double elev_update = 0;
         // (the neighbours go through the slide by value: two calls on
         // references to different cells are merged by the compiler into one
         // call on a selected address, which puts the cells in scratch memory)
         if (wm.elev > wc.elev && x > 2) {
           CellW nb = wm;
-          lateral_slide_w(a, gr, mult, time_step, in.tau, et3, nb, wc, x - 1 == 1, elev_update,
-                          L);
+          lateral_slide_w(a, gr, mult, time_step, tau, et3, nb, wc, x - 1 == 1, elev_update, L);
           wm = nb;
         }
         if (wp.elev > wc.elev && x < a.imax - 1) {
           CellW nb = wp;
-          lateral_slide_w(a, gr, mult, time_step, in.tau, et3, nb, wc, x + 1 == a.imax,
-                          elev_update, L);
+          lateral_slide_w(a, gr, mult, time_step, tau, et3, nb, wc, x + 1 == a.imax, elev_update,
+                          L);
           wp = nb;
         }
         wc.elev += elev_update;
@@ -939,16 +999,8 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a_in, double time_step
     cell_store(wm, gr, elev, lane);  // row x-1 is final
     wm = wc;
     wc = wp;
-    if (x + 2 <= a.imax) cell_take(wp, ring[j]);  // cell x + 2
-    // refill the slots K steps ahead, once their contents are used (the new
-    // loads then land in the same registers, and nothing reads a register
-    // whose load is in flight)
-    // (past the sweep's end a slot reloads the last cell: every step issues
-    // the same loads, so the wait for a slot is a fixed count of later loads)
-    fetch(min(x + K, a.imax - 1), inr[j]);
-    cell_fetch(ring[j], gr, static_cast<size_t>(min(x + 2 + K, a.imax)) * P + y, h, elev, bedrock,
-               lane, vz);
-  }
+    if (x + 2 <= a.imax) wp = far;  // cell x + 2
+    in = nin;
   }
   // rows imax-1 and imax (the last two of the window)
   cell_store(wm, gr, elev, lane);
@@ -957,8 +1009,9 @@ __global__ __launch_bounds__(64) void k_deposit(ErodeArgs a_in, double time_step
 
 // --------------------------------------------------------- y-lateral slide
 // One wavefront per row x, walking y = 2 .. jmax-1 (LCM.cpp:4684-4747), its
-// loads issued K steps ahead as in the deposition sweep.
-template <typename E, int K>
+// records (cell y = i + 1: grain row, elevation, bedrock, depth, row flag,
+// active-mask bit, Tau, erodetot3) staged as in the deposition sweep.
+template <typename E>
 __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a_in, double time_step, double mult,
                                                   const E* __restrict__ h,
                                                   double* __restrict__ elev,
@@ -974,52 +1027,91 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a_in, double time_st
   const int lane = threadIdx.x;
   const LaneG L = make_lane(a, lane);
   const size_t row = static_cast<size_t>(x) * a.pitch;
-  const uint32_t vz = opaque_vzero();  // vector loads, tested at the step (k_deposit)
+  const size_t nc = gr.nc;
+  constexpr int NV = 5;  // elevation, bedrock, depth, Tau, erodetot3
+  __shared__ double s_g[kG + 1][kStagePitch];
+  __shared__ double s_v[NV][kStagePitch];
+  __shared__ int s_f[kStagePitch];  // row flag | active bit << 1
+  struct Chunk {  // (flag and mask word combined at the store, as in k_deposit)
+    double g[kG + 1], v[NV];
+    uint64_t mw;
+    int has;
+  };
+  auto load_chunk = [&](int m, Chunk& ch) {  // lane r: record 64 m + r
+    const int y = min(64 * m + lane + 1, a.jmax);
+    const size_t c = row + y;
+#pragma unroll
+    for (int n = 0; n <= kG; ++n) ch.g[n] = gr.g[static_cast<size_t>(n) * nc + c];
+    ch.v[0] = elev[c];
+    ch.v[1] = bedrock[c];
+    ch.v[2] = h[c];
+    ch.v[3] = Tau[c];
+    ch.v[4] = etot3[c];
+    ch.mw = mask[static_cast<size_t>(x) * a.mwords + (y >> 6)];
+    ch.has = gr.has[c];
+  };
+  auto store_chunk = [&](int m, const Chunk& ch) {
+    const int sl = (64 * m + lane) & (kStageSlots - 1);
+#pragma unroll
+    for (int n = 0; n <= kG; ++n) s_g[n][sl] = ch.g[n];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s_v[k][sl] = ch.v[k];
+    const int yc = min(64 * m + lane + 1, a.jmax);  // the record's column
+    s_f[sl] = (ch.has != 0 ? 1 : 0) | (static_cast<int>((ch.mw >> (yc & 63)) & 1ull) << 1);
+    __syncthreads();
+  };
+  auto take = [&](CellW& w, int i) {
+    const int sl = i & (kStageSlots - 1);
+    const double gv = s_g[min(lane, kG)][sl];
+    w.g = lane <= kG ? gv : 0.0;
+    w.elev = s_v[0][sl];
+    w.bedrock = s_v[1][sl];
+    w.h = s_v[2][sl];
+    w.has = (s_f[sl] & 1) ? 1u : 0u;
+    w.c = row + static_cast<size_t>(i + 1);
+  };
   struct In {
-    double e3, tau;
-    uint64_t mw;  // the cell's active-mask word
+    double tau, e3;
+    int f;
   };
-  auto fetch = [&](int y, In& in) {
-    const size_t c = row + y + vz;
-    in.e3 = etot3[c];
-    in.mw = mask[static_cast<size_t>(x) * a.mwords + (y >> 6) + vz];
-    in.tau = Tau[c];
+  auto read_in = [&](int i, In& in) {
+    const int sl = i & (kStageSlots - 1);
+    in.f = s_f[sl];
+    in.tau = s_v[3][sl];
+    in.e3 = s_v[4][sl];
   };
+  Chunk pend;
+  load_chunk(0, pend);
+  store_chunk(0, pend);
+  load_chunk(1, pend);
   CellW wl, wc, wr;
-  CellRaw ring[K];
-  In inr[K];
-  {
-    CellRaw r0;
-    cell_fetch(r0, gr, row + 1, h, elev, bedrock, lane, vz);
-    cell_take(wl, r0);
-    cell_fetch(r0, gr, row + 2, h, elev, bedrock, lane, vz);
-    cell_take(wc, r0);
-    cell_fetch(r0, gr, row + 3, h, elev, bedrock, lane, vz);
-    cell_take(wr, r0);
-  }
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    if (2 + j < a.jmax) fetch(2 + j, inr[j]);
-    if (4 + j <= a.jmax) cell_fetch(ring[j], gr, row + 4 + j, h, elev, bedrock, lane, vz);
-  }
-  for (int y0 = 2; y0 < a.jmax; y0 += K) {
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    const int y = y0 + j;
-    if (y >= a.jmax) break;  // uniform
-    const In in = inr[j];
-    if (in.e3 > 0 && ((in.mw >> (y & 63)) & 1ull)) {
+  take(wl, 0);
+  take(wc, 1);
+  take(wr, 2);
+  In in;
+  read_in(1, in);
+  for (int y = 2; y < a.jmax; ++y) {
+    if (((y + 1) & 63) == 0) {  // uniform: the far record opens chunk m
+      const int m = (y + 1) >> 6;
+      store_chunk(m, pend);
+      load_chunk(m + 1, pend);
+    }
+    CellW far;  // LDS reads for later first (k_deposit)
+    take(far, y + 1);
+    In nin;
+    read_in(y, nin);
+    const double e3 = in.e3;
+    if (e3 > 0 && (in.f & 2)) {
+      const double tau = in.tau;
       double elev_update = 0;
       if (wl.elev > wc.elev) {  // (by value: as in the deposition sweep)
         CellW nb = wl;
-        lateral_slide_w(a, gr, mult, time_step, in.tau, in.e3, nb, wc, y - 1 == 1, elev_update,
-                        L);
+        lateral_slide_w(a, gr, mult, time_step, tau, e3, nb, wc, y - 1 == 1, elev_update, L);
         wl = nb;
       }
       if (wr.elev > wc.elev) {
         CellW nb = wr;
-        lateral_slide_w(a, gr, mult, time_step, in.tau, in.e3, nb, wc, y + 1 == a.jmax,
-                        elev_update, L);
+        lateral_slide_w(a, gr, mult, time_step, tau, e3, nb, wc, y + 1 == a.jmax, elev_update, L);
         wr = nb;
       }
       wc.elev += elev_update;
@@ -1027,11 +1119,8 @@ __global__ __launch_bounds__(64) void k_lateral_y(ErodeArgs a_in, double time_st
     cell_store(wl, gr, elev, lane);  // column y-1 is final
     wl = wc;
     wc = wr;
-    if (y + 2 <= a.jmax) cell_take(wr, ring[j]);  // column y + 2
-    // refill once used, the same loads every step (k_deposit)
-    fetch(min(y + K, a.jmax - 1), inr[j]);
-    cell_fetch(ring[j], gr, row + min(y + 2 + K, a.jmax), h, elev, bedrock, lane, vz);
-  }
+    if (y + 2 <= a.jmax) wr = far;  // column y + 2
+    in = nin;
   }
   cell_store(wl, gr, elev, lane);
   if (a.jmax >= 3) cell_store(wc, gr, elev, lane);
@@ -1522,9 +1611,6 @@ __global__ __launch_bounds__(1024) void k_in_channel_landslide(ErodeArgs a, int 
 
 namespace {
 
-// Steps ahead that the deposition and y-slide sweeps issue their loads.
-constexpr int kDepositAhead = 3, kLateralAhead = 3;
-
 // The epoch of the next launch that can make grain rows.
 int next_row_epoch(hc_ctx* c, uint32_t* ep) {
   if (c->row_epoch >= (1u << (64 - hc::kRowEpochShift)) - 1)
@@ -1598,13 +1684,13 @@ int erode_t(hc_ctx* c, double mult, double* time_step, double* tempbmax, float* 
   // row left to right (the reference's in-place order): one wavefront per
   // column / row, its lanes sharing the per-cell loads and strata moves.
   if (ncol > 0) {
-    hipLaunchKernelGGL((k_deposit<E, kDepositAhead>), dim3(ncol), dim3(64), 0, c->stream, a, ts, mult, h, elev,
+    hipLaunchKernelGGL(k_deposit<E>, dim3(ncol), dim3(64), 0, c->stream, a, ts, mult, h, elev,
                        c->bedrock, c->mask, static_cast<const E*>(Tau),
                        static_cast<const E*>(ss), gr, static_cast<const double*>(c->d_dep));
     HC_HIP(c, hipGetLastError());
     if (int rc = stamp_rows(c)) return rc;
   }
-  hipLaunchKernelGGL((k_lateral_y<E, kLateralAhead>), dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
+  hipLaunchKernelGGL(k_lateral_y<E>, dim3(c->g.imax), dim3(64), 0, c->stream, a, ts, mult, h, elev,
                      c->bedrock, c->mask, static_cast<const E*>(Tau), gr,
                      static_cast<const E*>(etot3));
   HC_HIP(c, hipGetLastError());
