@@ -358,8 +358,9 @@ int hc_destroy(hc_ctx* c) {
                   c->d_pts, c->d_amt, c->d_ctl, c->d_rec, c->d_top, c->d_area, c->d_rfarea,
                   c->d_pmask, c->d_dpart, c->d_wmax, c->d_wet, c->d_slide_moves, c->d_runoff,
                   c->d_plist, c->d_pamt, c->d_dep};
-  for (auto& gexec : c->loop_graph)
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+  for (auto& row : c->loop_graph)
+    for (auto& gexec : row)
+      if (gexec) (void)hipGraphExecDestroy(gexec);
   for (auto e : c->prof_ev) (void)hipEventDestroy(e);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

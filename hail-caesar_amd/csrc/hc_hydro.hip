@@ -272,12 +272,11 @@ struct FusedPtrs {
   // and one conditional atomic on red per block
   double* wmax;
   hc::InjectArgs inj;  // rain input at the input points (device loop only)
-  // device loop, aligned tiles: on a body that rebuilds the active mask
-  // (*wet_now set) every stored depth also leaves its wet bit (h > 0) in
+  // device loop, aligned tiles, the kernel variant for a body that rebuilds
+  // the active mask: every stored depth also leaves its wet bit (h > 0) in
   // wet, one 64-column word per row and wavefront, for the scan kernel's
-  // 3x3 dilation; null: no bits
+  // 3x3 dilation
   uint64_t* wet = nullptr;
-  const int* wet_now = nullptr;
 };
 
 // One tile: a strip of nsub x R output rows by four wavefronts side by side.
@@ -312,7 +311,7 @@ struct FusedPtrs {
 // theirs is used or stored, and the loads need no branch.  Loads and stores
 // address a row by a 32-bit byte offset from the field's base.  Every thread
 // of the block calls this function (it has block barriers).
-template <int MODE, int R, bool AL>
+template <int MODE, int R, bool AL, bool WET = false>
 __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, int nbx, int nsub,
                                            double dt_arg, const double* dtp, const int* haltp,
                                            const FusedPtrs<MODE>& p, const DrainArgs& drain) {
@@ -409,7 +408,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   // wait for any of them is a wait for all, and the vector loads would
   // otherwise queue behind that chain.
   double dtd = dt_arg;
-  bool wetb = false;  // this body leaves wet-cell bits (device loop, scan bodies)
+  constexpr bool wetb = WET && AL;  // this launch leaves wet-cell bits
   uint32_t wet_lo = 0, wet_hi = 0;  // lane k: the wet bits of the sub-tile's row r0 + k
   // the sub-tile's R words of wet bits in one store (word wu of each row)
   uint64_t* const wetp = p.wet;
@@ -689,8 +688,8 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
         st_nt(qx1, o, qxNp);
         st_nt(qy1, o, qyNp);
       }
-      if constexpr (AL) {
-        if (wetb) {  // row xo's word (the wavefront's 64 columns) waits in lane k-2
+      if constexpr (wetb) {
+        {  // row xo's word (the wavefront's 64 columns) waits in lane k-2
           const uint64_t bits = __ballot(owned && hn > T(0));
           const bool mine = lane == k - 2;
           wet_lo = mine ? static_cast<uint32_t>(bits) : wet_lo;
@@ -719,7 +718,6 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   if (dtp) dtd = *dtp;
   if (drains) ts = *drain.ts;
   if (inject && p.inj.rf_uniform >= 0) amt_u = p.inj.amt[p.inj.rf_uniform];
-  wetb = AL && p.wet != nullptr && *p.wet_now != 0;  // uniform over the grid
   dt_setup();
   if (active_wave && inject) {
 #pragma unroll
@@ -731,7 +729,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
     if constexpr (AL) east_faces();
 #pragma unroll
     for (int k = 1; k < R + 2; ++k) row(win, k, r0);
-    if (AL && wetb) flush_wet(r0);
+    if constexpr (wetb) flush_wet(r0);
   }
   for (int t = 1; t < nsub; ++t) {
     r0 += R;
@@ -760,7 +758,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       if constexpr (AL) east_faces();
 #pragma unroll
       for (int k = 2; k < R + 2; ++k) row(win, k, r0);
-      if (AL && wetb) flush_wet(r0);
+      if constexpr (wetb) flush_wet(r0);
     }
   }
   const double mine = owned && active_wave ? (sick ? INFINITY : static_cast<double>(md)) : 0.0;
@@ -786,7 +784,7 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
 // (Orders that walk groups of tile columns, so that vertically adjacent
 // tiles run close together, measured slower: the row-major stream keeps the
 // DRAM accesses sequential.)
-template <int MODE, int R>
+template <int MODE, int R, bool WET = false>
 __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int nbx, int nblocks,
                                                int nsub, FusedPtrs<MODE> p,
                                                const double* __restrict__ dtp,
@@ -796,7 +794,8 @@ __global__ __launch_bounds__(256) void k_fused(HydroArgs a, double dt_arg, int n
   const int q8 = nblocks / 8, r8 = nblocks % 8;
   const int xcd = b % 8;
   const int bb = xcd * q8 + min(xcd, r8) + b / 8;
-  fused_tile<MODE, R, MODE != 0>(a, bb % nbx, bb / nbx, nbx, nsub, dt_arg, dtp, haltp, p, drain);
+  fused_tile<MODE, R, MODE != 0, WET>(a, bb % nbx, bb / nbx, nbx, nsub, dt_arg, dtp, haltp, p,
+                                      drain);
 }
 
 // Ensemble: a persistent grid walks the ensemble's tile list (member-major).
@@ -1075,8 +1074,7 @@ void hc_fused_geometry(const hc_ctx* c, int R, int sw, int* nbx, int* nby) {
 
 int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* haltp, bool swap,
                          const DrainArgs* drain, const hc::InjectArgs* inject, double* wmax,
-                         hipEvent_t ev_start, hipEvent_t ev_stop, uint64_t* wet,
-                         const int* wet_now) {
+                         hipEvent_t ev_start, hipEvent_t ev_stop, uint64_t* wet) {
   if (c->g.sediment)
     return hc_fail(c, HC_ERR_STATE, "flow_depth: fused kernel is for hydro-only contexts");
   if (!hc::fused_offsets_fit(c->NR, c->P))
@@ -1118,8 +1116,7 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     p.red = c->d_red;
     p.wmax = wmax;
     p.inj = inject ? *inject : hc::InjectArgs{};
-    p.wet = M != 0 ? wet : nullptr;  // the aligned forms only
-    p.wet_now = wet_now;
+    p.wet = wet;
     if (ev_start)  // the events time the kernel's own execution (dispatch packet timestamps)
       hipExtLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, c->stream, ev_start, ev_stop, 0, a,
                             dt, nbx, nblocks, ns, p, dtp, haltp, dr);
@@ -1134,6 +1131,20 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp, const int* hal
     if (R == 4) go(k_fused<0, 4>, td, td);
     else if (R == 16) go(k_fused<0, 16>, td, td);
     else go(k_fused<0, 8>, td, td);
+  } else if (wet) {  // FP32 forms on a body that rebuilds the active mask
+    if (c->zq) {
+      if (R == 4) go(k_fused<3, 4, true>, tf, tq);
+      else if (R == 16) go(k_fused<3, 16, true>, tf, tq);
+      else go(k_fused<3, 8, true>, tf, tq);
+    } else if (c->prm.flat_dz < 0) {
+      if (R == 4) go(k_fused<2, 4, true>, tf, td);
+      else if (R == 16) go(k_fused<2, 16, true>, tf, td);
+      else go(k_fused<2, 8, true>, tf, td);
+    } else {
+      if (R == 4) go(k_fused<1, 4, true>, tf, tf);
+      else if (R == 16) go(k_fused<1, 16, true>, tf, tf);
+      else go(k_fused<1, 8, true>, tf, tf);
+    }
   } else if (c->zq) {
     if (R == 4) go(k_fused<3, 4>, tf, tq);
     else if (R == 16) go(k_fused<3, 16>, tf, tq);

@@ -64,7 +64,6 @@ struct LoopCtl {
   int input_counter[kRfMax], nActual[kRfMax];
   int counter, halt, err, npts, pts_dirty, first_new, last_first, iter;  // err: kLoopErr*
   int halt_pending;  // the end-of-run test passed: halt before the next body
-  int scan_now;      // this body rebuilds the active mask (counter % scan_interval == 0)
 };
 
 static_assert(kRfMax == HC_RF_MAX, "rain cell capacity");
@@ -288,8 +287,11 @@ struct hc_ctx {
   bool loop_ready = false;
   int loop_cur0 = 0;                    // `cur` when the last run was enqueued
   int loop_enqueued = 0;                // iterations enqueued by the last run
-  int loop_counter0 = 0;
-  hipGraphExec_t loop_graph[2] = {nullptr, nullptr};  // per starting parity
+  // the loop's counter as the next enqueued body starts (the host's forecast:
+  // each body adds one unless the loop halts; hc_loop_sync resets it), which
+  // tells which bodies rebuild the active mask
+  long loop_counter = 0;
+  hipGraphExec_t loop_graph[2][5] = {};  // per starting parity and counter % 5
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev;      // start/stop pairs around fused launches
   int prof_n = 0;
@@ -308,10 +310,11 @@ struct hc_ctx {
 // any reallocation of such a buffer, or a change of launch shapes, must drop
 // them so the next run captures again.
 inline void hc_drop_loop_graphs(hc_ctx* c) {
-  for (auto& gexec : c->loop_graph) {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-  }
+  for (auto& row : c->loop_graph)
+    for (auto& gexec : row) {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+    }
 }
 
 
@@ -333,7 +336,7 @@ int hc_launch_flow_depth(hc_ctx* c, double dt, const double* dtp = nullptr,
                          const hc::DrainArgs* drain = nullptr,
                          const hc::InjectArgs* inject = nullptr, double* wmax = nullptr,
                          hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr,
-                         uint64_t* wet = nullptr, const int* wet_now = nullptr);
+                         uint64_t* wet = nullptr);
 // Geometry of the fused kernel's launch for the context's tile rows.
 // Columns per wavefront of this context's single-grid fused kernel.
 int hc_fused_stride(const hc_ctx* c);

@@ -181,7 +181,6 @@ __device__ __forceinline__ void loop_control(const LoopArgs& a, LoopCtl* __restr
       for (int n = 0; n < kRfMax; ++n) r.j_mean[n] = C.j_mean[n];
       // main.cpp:204 on this body's clock (nothing later in the body moves it)
       if (!(C.cycle / 60 < a.maxcycle)) C.halt_pending = 1;
-      C.scan_now = (C.counter % a.scan_interval) == 0 ? 1 : 0;
     }
   }
 }
@@ -418,9 +417,19 @@ __global__ void k_ens_reset(const EnsMember* __restrict__ mem, int n) {
   for (int m = threadIdx.x; m < n; m += blockDim.x) table_ptr<LoopCtl>(mem[m].ctl)->iter = 0;
 }
 
-// One iteration's launches on the context stream.
+// One iteration's launches on the context stream.  On a body that the
+// host's counter forecast says rebuilds the active mask (FP32 forms), the
+// fused kernel's variant that also stores the wet-cell bits runs, and the
+// scan kernel dilates those; the scan kernel still tests the counter itself
+// (and scans the depths if no bits were stored), so the forecast only
+// chooses the faster path.
+int a_scan_interval(const hc_ctx* c) { return c->largs.scan_interval; }
+
 int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   const LoopArgs& a = c->largs;
+  const bool scan_body = (c->loop_counter + 1) % a.scan_interval == 0;
+  ++c->loop_counter;
+  uint64_t* wet = (c->prec != HC_FP64 && scan_body) ? c->d_wet : nullptr;
   hipLaunchKernelGGL(k_loop_control, dim3(1), dim3(64), 0, c->stream, a, c->d_ctl, c->d_top,
                      c->d_pmask, c->d_red, c->d_rec);
   HC_HIP(c, hipGetLastError());
@@ -436,8 +445,7 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
   inj.rf_uniform = c->rf_uniform;
   int rc = hc_launch_flow_depth(c, 0.0, &c->d_ctl->dt_flow, &c->d_ctl->halt, true, &dr, &inj,
                                 c->d_wmax, prof ? c->prof_ev[2 * k_in_run] : nullptr,
-                                prof ? c->prof_ev[2 * k_in_run + 1] : nullptr, c->d_wet,
-                                &c->d_ctl->scan_now);
+                                prof ? c->prof_ev[2 * k_in_run + 1] : nullptr, wet);
   if (rc) return rc;
   if (prof) c->prof_n = k_in_run + 1;
   const int o = c->cur;  // flow_depth flipped the generation
@@ -457,7 +465,7 @@ int enqueue_iteration(hc_ctx* c, int k_in_run = -1) {
     hipLaunchKernelGGL(k_loop_scan<float>, dim3(nb), dim3(256), 0, c->stream, a, c->d_ctl, c->W,
                        (const float*)c->h[o], c->mask, c->d_area, c->d_rfarea, c->d_pmask,
                        c->d_dpart, nby, nbx, R, sw, c->d_rec, c->d_wmax, nwmax, c->d_red,
-                       c->d_wet);
+                       wet);
   HC_HIP(c, hipGetLastError());
   return HC_OK;
 }
@@ -579,6 +587,7 @@ int hc_loop_begin(hc_ctx* c, const hc_loop_init* in) {
   C.time_1 = in->time_1;
   C.baseflow = in->baseflow;
   C.counter = in->counter;
+  c->loop_counter = in->counter;
   for (int n = 0; n < kRfMax; ++n) {
     C.j_mean[n] = in->j_mean[n];
     C.old_j_mean[n] = in->old_j_mean[n];
@@ -619,22 +628,30 @@ int hc_loop_run(hc_ctx* c, int n, int use_graph) {
   constexpr int kBlock = 10;  // even (ping-pong) and a multiple of the scan interval
   int left = n;
   if (use_graph && n >= kBlock) {
+    // a graph per starting generation and counter phase (which of its
+    // bodies rebuild the active mask); kBlock keeps both fixed from one
+    // launch of the graph to the next
     const int parity = c->cur;
-    if (!c->loop_graph[parity]) {
+    const int phase = static_cast<int>(c->loop_counter % a_scan_interval(c));
+    hipGraphExec_t& gx = c->loop_graph[parity][phase];
+    if (!gx) {
       hipGraph_t graph;
+      const long counter = c->loop_counter;
       HC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       int rc = HC_OK;
       for (int k = 0; k < kBlock && rc == HC_OK; ++k) rc = enqueue_iteration(c);
       hipError_t e = hipStreamEndCapture(c->stream, &graph);
-      c->cur = parity;  // capture does not execute; restore the generation
+      c->cur = parity;  // capture does not execute; restore the generation and forecast
+      c->loop_counter = counter;
       if (rc) return rc;
       if (e != hipSuccess) return hc_fail(c, HC_ERR_HIP, "loop: graph capture failed");
-      e = hipGraphInstantiate(&c->loop_graph[parity], graph, nullptr, nullptr, 0);
+      e = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (e != hipSuccess) return hc_fail(c, HC_ERR_HIP, "loop: graph instantiate failed");
     }
     while (left >= kBlock) {
-      HC_HIP(c, hipGraphLaunch(c->loop_graph[parity], c->stream));
+      HC_HIP(c, hipGraphLaunch(gx, c->stream));
+      c->loop_counter += kBlock;
       left -= kBlock;
     }
   }
@@ -690,6 +707,7 @@ int hc_loop_sync(hc_ctx* c, hc_loop_state* st, hc_loop_record* rec, int max_reco
   HC_HIP(c, hipMemcpy(&C, c->d_ctl, sizeof(C), hipMemcpyDeviceToHost));
   // iterations that ran flipped the ping-pong generation once each
   c->cur = c->loop_cur0 ^ (C.iter & 1);
+  c->loop_counter = C.counter;
   c->loop_enqueued = 0;
   if (st) {
     st->time_step = C.time_step;
