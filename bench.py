@@ -275,6 +275,71 @@ def run_adjoint(size: int = 512, steps: int = 2000, dt: float = 0.5, every: int 
     return out
 
 
+def erosion_accuracy_run(size: int, hours: int):
+    """The flow + erosion configuration at its stated shape: from dry, the
+    6 h storm, `hours` simulated, host-stepped as the reference's loop, in the
+    FP64 form, the FP32 storage form and the FP64 form from its inputs (DEM,
+    grain data) rounded to float32; per-field errors at the end and the outlet
+    water and sediment series' errors over the run."""
+    from hail_caesar_amd import hc, inputs
+    from hail_caesar_amd.model import Model
+    arrays, params = inputs.erosion_arrays(size, wet_start=False, hours=hours)
+    rounded = dict(arrays)
+    for k in ("dem", "graindata"):
+        rounded[k] = np.asarray(arrays[k], dtype=np.float32).astype(np.float64)
+    res = {}
+    names = ("elev", "water_depth", "qx", "qy", "Vsusptot", "grain")
+    for prec, name, arr in ((hc.HC_FP64, "fp64", arrays), (hc.HC_FP32, "fp32", arrays),
+                            (hc.HC_FP64, "fp64_floor", rounded)):
+        m = Model(None, None, precision=prec, arrays=arr, params=params)
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            tt, cy, dt = m.iterate(5000)
+            n += len(tt)
+            print(f"[erosion accuracy] {name}: {n} bodies, {m.scalar('cycle') / 60:.2f} h",
+                  file=sys.stderr, flush=True)
+            if m.finished:
+                break
+        res[name] = {"fields": {f: m.download(f) for f in names}, "series": m.series(), "bodies": n,
+                     "wall_s": round(time.perf_counter() - t0, 1)}
+        m.close()
+    b = res["fp64"]["fields"]
+    dem = arrays["dem"]
+    change = float(np.abs(b["elev"][1:-1, 1:-1] - dem).max())
+
+    def errs(r):
+        a = r["fields"]
+        e = {"elev": {"max_abs": float(np.abs(a["elev"] - b["elev"])[1:-1, 1:-1].max()),
+                      "max_elevation_change": change}}
+        for f in names[1:]:
+            e[f] = _field_errors(a[f] if a[f].ndim == 2 else a[f].sum(axis=2),
+                                 b[f] if b[f].ndim == 2 else b[f].sum(axis=2))
+        return e
+
+    def series_err(x, y, col):
+        k = min(len(x), len(y))
+        d = x[:k, col] - y[:k, col]
+        ref = np.abs(y[:k, col]).max()
+        return {"rows": int(k), "max_rel": float(np.abs(d).max() / ref) if ref > 0 else 0.0,
+                "l2_rel": float(np.sqrt((d ** 2).sum() / (y[:k, col] ** 2).sum())) if ref > 0 else 0.0,
+                "total": float(x[:k, col].sum()), "total_fp64": float(y[:k, col].sum())}
+    sb = res["fp64"]["series"]
+    out = {"workload": f"{size}^2 at 10 m, 9 fractions (seeded Dirichlet), Wilcock-Crowe, 1 m "
+                       f"active layer, 6 h triangular storm peaking at 40 mm/h, dry start, {hours} h",
+           "bodies": {k: v["bodies"] for k, v in res.items()},
+           "wall_s": {k: v["wall_s"] for k, v in res.items()},
+           "fp32": {"fields": errs(res["fp32"]),
+                    "water_out": series_err(res["fp32"]["series"], sb, 1),
+                    "sediment_out": series_err(res["fp32"]["series"], sb, 4)},
+           "fp64_floor_f32_inputs": {"fields": errs(res["fp64_floor"]),
+                                     "water_out": series_err(res["fp64_floor"]["series"], sb, 1),
+                                     "sediment_out": series_err(res["fp64_floor"]["series"], sb, 4)},
+           "note": "grain: the cells' summed active-layer grain; series columns: the .dat file's "
+                   "water (2nd column) and sediment (5th) discharge per output interval"}
+    return out
+
+
 def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50, cpu: bool = True):
     """Flow + erosion (host-stepped as the reference's loop): size^2 at 10 m,
     fractal relief 20 m on a 1 % slope with a carved channel, 9 grain
@@ -288,19 +353,8 @@ def run_erosion(size: int = 1024, steps: int = 200, warmup: int = 50, cpu: bool 
     form after the same loop bodies."""
     from hail_caesar_amd import hc, inputs
     from hail_caesar_amd.model import Model
-    dem = inputs.planar_dem(size, size, 10.0, 0.01, 20.0, 0.6, 1002)
-    props = inputs.dirichlet_props(size, size, np.random.Generator(np.random.PCG64(1002)))
-    arrays = {"dem": dem, "water_init": np.where(dem > inputs.NODATA, 0.2, 0.0),
-              "rain": np.vstack([inputs.triangular_storm(6, 40.0, 15)[:, None],
-                                 np.zeros((200, 1))]).astype(np.float32), "DX": 10.0,
-              "graindata": inputs.graindata_array(props, 1.0)}
-    params = inputs.params_text_dict({
-        "hydro_model_only": "no", "active_layer_thickness": "1", "transport_law": "wilcock",
-        "water_init_from_raster_on": "yes", "water_init_raster_file": "water_init.asc",
-        "read_in_graindata_from_file": "yes", "grain_data_file": "grain.dat",
-        "hc_grain_sizes": ",".join(repr(d) for d in inputs.EROSION_GRAIN_D)},
-        rain_step_min=15, run_hours=24)
-    del props
+    arrays, params = inputs.erosion_arrays(size, wet_start=True, hours=24)
+    dem = arrays["dem"]
     cells = size * size
     out = {"workload": f"{size}^2 at 10 m, 9 fractions (seeded Dirichlet proportions), "
                        "Wilcock-Crowe, 1 m active layer, 40 mm/h storm, wet start (0.2 m)",
@@ -544,6 +598,11 @@ def main():
                     help="first run a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--section", choices=["ensemble", "erosion", "adjoint"], default=None,
                     help="run only this part (its JSON object alone; for profiler runs)")
+    ap.add_argument("--erosion-hours", type=int, default=0, metavar="H",
+                    help="instead of the benchmark, run the flow + erosion configuration from dry "
+                         "for H hours (size from --size, 1024 for the stated shape) in FP64, the "
+                         "FP32 storage form and FP64 from float32-rounded inputs, and report "
+                         "their differences")
     ap.add_argument("--accuracy-hours", type=int, default=0, metavar="H",
                     help="instead of the benchmark, run the large grid from dry for H hours in "
                          "FP32 and FP64 and report their differences")
@@ -555,6 +614,9 @@ def main():
     load_package()
     if args.accuracy_hours:
         print(json.dumps({"accuracy": accuracy_run(args.size, args.accuracy_hours)}))
+        return
+    if args.erosion_hours:
+        print(json.dumps({"erosion_accuracy": erosion_accuracy_run(args.size, args.erosion_hours)}))
         return
     if args.section == "ensemble":
         print(json.dumps({"ensemble": run_ensemble(args.steps, args.warmup)}))

@@ -446,6 +446,27 @@ def erosion_1024(path: str, size: int = 1024, hours: int = 24, seed: int = 1002)
     return w
 
 
+def erosion_arrays(size: int = 1024, wet_start: bool = False, hours: int = 24):
+    """The flow + erosion workload: size^2 at 10 m, fractal relief 20 m on a
+    1 % slope with a carved channel, 9 grain fractions (d = 0.0005 .. 0.128 m)
+    in seeded Dirichlet surface proportions per cell, Wilcock-Crowe, 1 m active
+    layer, a 6 h triangular storm peaking at 40 mm/h, `hours` simulated; from
+    dry, or from a uniform 0.2 m (wet_start)."""
+    dem = planar_dem(size, size, 10.0, 0.01, 20.0, 0.6, 1002)
+    props = dirichlet_props(size, size, np.random.Generator(np.random.PCG64(1002)))
+    arrays = {"dem": dem,
+              "rain": np.vstack([triangular_storm(6, 40.0, 15)[:, None],
+                                 np.zeros((4 * hours + 200, 1))]).astype(np.float32), "DX": 10.0,
+              "graindata": graindata_array(props, 1.0)}
+    over = {"hydro_model_only": "no", "active_layer_thickness": "1", "transport_law": "wilcock",
+            "read_in_graindata_from_file": "yes", "grain_data_file": "grain.dat",
+            "hc_grain_sizes": ",".join(repr(d) for d in EROSION_GRAIN_D)}
+    if wet_start:
+        arrays["water_init"] = np.where(dem > NODATA, 0.2, 0.0)
+        over.update({"water_init_from_raster_on": "yes", "water_init_raster_file": "water_init.asc"})
+    return arrays, params_text_dict(over, rain_step_min=15, run_hours=hours)
+
+
 def erosion_run(path: str, size: int = 48, hours: int = 1, seed: int = 1002,
                 failure_angle: float = 8.0, law: str = "wilcock",
                 dem_noise: float = 0.0, grain_cover: float = 1.0,

@@ -10,6 +10,11 @@
     is 1.6e12 cell-steps), to 1e-10; then the FP32 storage form against the
     FP64 form after 200 bodies, beside the FP64 form's own response to 1e-8 m
     of DEM noise;
+  * the flow + erosion configuration, 1024^2 at 10 m, 9 fractions, from dry:
+    the first 2 simulated hours (the storm's rise) in the FP64 form, the FP32
+    storage form and the FP64 form from its inputs rounded to float32, the
+    FP32 form's errors held to the rounded-input run's (the whole 24 h run is
+    bench.py --erosion-hours 24);
   * the batched ensemble, 64 catchments of 128..1024 cells a side, stepped
     together: its 4 smallest members against the oracle run on each alone,
     FP64 to 1e-10, FP32 (fixed-point elevations) within the depth bar.
@@ -139,6 +144,61 @@ def test_config3_fp32_against_fp64(grid4096):
     assert h["l2"] <= max(1e-5, hf["l2"])
     assert h["frac_1e-4"] <= max(1e-5, hf["frac_1e-4"])
     assert h["vol"] <= 1e-7
+
+
+# ------------------------------------------------ 1024^2 flow + erosion, 2 h
+def _series_l2(x, y, col):
+    k = min(len(x), len(y))
+    d = x[:k, col] - y[:k, col]
+    return float(np.sqrt((d ** 2).sum() / max((y[:k, col] ** 2).sum(), 1e-300)))
+
+
+def test_config2_fp32_against_fp64():
+    """The flow + erosion configuration at its stated grid (1024^2, 9
+    fractions, Wilcock-Crowe, 1 m active layer, the 40 mm/h storm from dry),
+    host-stepped for its first 2 simulated hours.  The erosion timestep
+    multiplier (trunc(ERODEFACTOR / tempbmax), LCM.cpp:2277-2294) is
+    discontinuous, so a sub-ulp perturbation of the inputs changes the erosion
+    schedule and the fields part; the bar is that perturbation's own effect:
+    the FP64 form from its DEM and grain data rounded to float32.  The FP32
+    form's errors (elevation max, depth and summed-grain L2, outlet water and
+    sediment series L2) stay within 4x that floor or a small absolute bar."""
+    arrays, params = inputs.erosion_arrays(1024, wet_start=False, hours=2)
+    rounded = dict(arrays)
+    for k in ("dem", "graindata"):
+        rounded[k] = np.asarray(arrays[k], dtype=np.float32).astype(np.float64)
+    runs = {}
+    for name, prec, arr in (("fp64", hc.HC_FP64, arrays), ("fp32", hc.HC_FP32, arrays),
+                            ("floor", hc.HC_FP64, rounded)):
+        m = Model(None, None, precision=prec, arrays=arr, params=params)
+        n = 0
+        while True:
+            tt, _, _ = m.iterate(5000)
+            n += len(tt)
+            if m.finished:
+                break
+        runs[name] = {"bodies": n, "series": m.series(),
+                      **{f: m.download(f) for f in ("elev", "water_depth", "grain")}}
+        m.close()
+    ref = runs["fp64"]
+
+    def metrics(r):
+        g, gr = r["grain"].sum(axis=2), ref["grain"].sum(axis=2)
+        return {"elev_max": float(np.abs(r["elev"] - ref["elev"])[1:-1, 1:-1].max()),
+                "depth_l2": _errs(r["water_depth"], ref["water_depth"])["l2"],
+                "grain_l2": _errs(g, gr)["l2"],
+                "water_out_l2": _series_l2(r["series"], ref["series"], 1),
+                "sediment_out_l2": _series_l2(r["series"], ref["series"], 4)}
+    got, floor = metrics(runs["fp32"]), metrics(runs["floor"])
+    print(f"bodies {[runs[k]['bodies'] for k in runs]}\nfp32 vs fp64: {got}\n"
+          f"fp64 from float32-rounded inputs vs fp64: {floor}")
+    # measured (1806 bodies each): FP32 elevation 1.8e-4 m, depth L2 8.6e-6, grain L2
+    # 4.9e-7, outlet water 6.3e-7 and sediment 5.8e-7; the rounded-input floor
+    # 5.2e-4 m, 7.4e-5, 2.5e-6, 1.4e-6 and 3.8e-5
+    small = {"elev_max": 1e-3, "depth_l2": 1e-4, "grain_l2": 1e-5, "water_out_l2": 1e-4,
+             "sediment_out_l2": 1e-3}
+    for k, v in got.items():
+        assert v <= max(small[k], 4 * floor[k]), (k, v, floor[k])
 
 
 # ------------------------------------------------------- 64-member ensemble
