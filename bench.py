@@ -223,12 +223,20 @@ def run_ensemble(steps: int, warmup: int, n: int = 64):
     wall = time.perf_counter() - t
     ens.collect(steps)
     kus = ens.bench_fused(10)
+    # the walking ("persistent") grids the tuner weighed, at the tuned tile rows:
+    # blocks that each walk a contiguous run of the tile list
+    walk = {}
+    for per_cu in (2, 4, 8):
+        ens.set_tiles(rows, 256 * per_cu)
+        walk[str(256 * per_cu)] = round(ens.bench_fused(10), 2)
+    ens.set_tiles(rows, blocks)
     sizes = sorted(a["dem"].shape[0] for a, _ in specs)
     out = {"members": n, "cells": cells, "side_min": sizes[0], "side_max": sizes[-1],
            "steps": steps, "value": round(cells * steps / wall, 1), "unit": "cell-steps/s",
            "ms_per_step": round(1000 * wall / steps, 5), "fused_kernel_us": round(kus, 2),
            "fused_gbs": round(bytes_per_cell_step("fp32", spatial_n=False) * cells / kus / 1e3, 1),
-           "tile_rows": rows, "persistent_blocks": blocks,
+           "tile_rows": rows, "fused_grid_blocks": blocks,
+           "walking_grid_us": walk,
            "sim_minutes_range": [round(min(m.scalar("cycle") for m in ms), 3),
                                  round(max(m.scalar("cycle") for m in ms), 3)]}
     ens.close()
