@@ -389,6 +389,11 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       return wsh ? ((m0 >> wsh) | (m1 << (64 - wsh))) : m0;
     }
   };
+  // whether any of the sub-tile's rows has an input point under the
+  // wavefront (lanes 32.. hold the point-mask words)
+  auto any_point = [&](uint64_t win) -> bool {
+    return __ballot(lane >= 32 && lane < 32 + R + 2 && win != 0) != 0;
+  };
   auto row_pred = [&](uint64_t win, int src) -> bool {
     const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(win), src);
     const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(win >> 32), src);
@@ -720,8 +725,10 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
   if (inject && p.inj.rf_uniform >= 0) amt_u = p.inj.amt[p.inj.rf_uniform];
   dt_setup();
   if (active_wave && inject) {
+    if (any_point(win)) {  // uniform: input points are sparse, most sub-tiles have none
 #pragma unroll
-    for (int k = 0; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), off(k, r0));
+      for (int k = 0; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), off(k, r0));
+    }
     if constexpr (AL) rain_outer(r0);
   }
   if constexpr (AL) exchange(std::integral_constant<int, 1>{}, 0);
@@ -748,8 +755,10 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       load_n(2, r0);
       if constexpr (AL) load_outer(r0);
       if (inject) {
+        if (any_point(win)) {
 #pragma unroll
-        for (int k = 2; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), off(k, r0));
+          for (int k = 2; k < R + 2; ++k) hh[k] = rain(hh[k], row_pred(win, 32 + k), off(k, r0));
+        }
         if constexpr (AL) rain_outer(r0);
       }
     }
