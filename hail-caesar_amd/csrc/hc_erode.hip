@@ -41,6 +41,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "hc_device.hpp"
 #include "hc_internal.hpp"
 
@@ -49,6 +51,8 @@ using hc::kGravity;
 using namespace hcdev;
 
 namespace {
+
+namespace cg = cooperative_groups;
 
 constexpr int kG = 10;  // G_MAX
 
@@ -87,14 +91,14 @@ struct MemRow {
 template <typename E>
 struct RegRow {
   double v[kG + 1];
-  bool h;
+  int32_t h;  // the flag as loaded (tested where a slide needs it)
   int32_t* hasp;
   size_t c;
   __device__ double get(int n) const { return v[n]; }
   __device__ void set(int n, double x) { v[n] = static_cast<double>(static_cast<E>(x)); }
-  __device__ bool has() const { return h; }
+  __device__ bool has() const { return h != 0; }
   __device__ void set_has() {
-    h = true;
+    h = 1;
     hasp[c] = 1;
   }
 };
@@ -107,7 +111,7 @@ template <typename E>
 __device__ __forceinline__ void reg_row_load(RegRow<E>& r, const GrainT<E>& gr, size_t c) {
   r.hasp = gr.has;
   r.c = c;
-  r.h = gr.has[c] != 0;
+  r.h = gr.has[c];
 #pragma unroll
   for (int n = 0; n <= kG; ++n) r.v[n] = gr.G(c, n);
 }
@@ -1409,80 +1413,100 @@ __global__ __launch_bounds__(1024) void k_global_landslide(int imax, int jmax, i
 // Neighbour order of the reference's moves:
 __device__ constexpr int kCreepOx[8] = {0, 1, 1, 1, 0, -1, -1, -1};
 __device__ constexpr int kCreepOy[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
-
-// en[k] for a k known only at run time, without indexing the array (which
-// would move it to scratch memory)
-__device__ __forceinline__ double en_at(const double (&en)[8], int k) {
-  double v = en[0];
-#pragma unroll
-  for (int q = 1; q < 8; ++q) v = (k == q) ? en[q] : v;
-  return v;
-}
+// the same offsets for a direction known only at run time, from arithmetic
+// rather than a table load: oy(k) = -ox(k + 2 mod 8)
+__device__ __forceinline__ int creep_ox(int k) { return (k & 3) == 0 ? 0 : (k < 4 ? 1 : -1); }
+__device__ __forceinline__ int creep_oy(int k) { return -creep_ox((k + 2) & 7); }
 
 // The grain slides: the reference's row-major sweep as wavefronts t = 3x + y
 // (a slide touches the cell and one neighbour, so sources interact within
-// Chebyshev distance 2), one block, a barrier per wavefront.  Runs before
-// the elevations change (the slides' grain allocation reads them).  A cell's
-// moves are worked out first; its grain row and the row of the neighbour it
-// slides to are held in registers (RegRow), the next neighbour's row fetched
-// while the current slide runs; no other cell of the wavefront touches
-// these rows.
+// Chebyshev distance 2), a grid-wide barrier per wavefront.  Runs before
+// the elevations change (the slides' grain allocation reads them); no other
+// cell of a wavefront touches a cell's 3x3 block.  A cell's row, its
+// neighbours' elevations and all eight neighbours' rows are fetched in one
+// batch (the neighbours' rows staged in LDS, in the thread's own column);
+// the moves then run from registers and LDS and store each neighbour's row
+// once.  A wavefront's cells are one long instruction stream each, so they
+// are spread one per lane over blocks of two wavefronts (one block per
+// compute unit: the staging takes most of its LDS), launched cooperatively
+// so that every block is resident for the barriers.
+constexpr int kCreepThreads = 128;
 template <typename E>
-__global__ __launch_bounds__(512) void k_creep_grain(ErodeArgs a, int imax, int jmax, int P,
-                                                     double DX, double rate, double time,
-                                                     const double* __restrict__ elev,
-                                                     const double* __restrict__ bedrock,
-                                                     GrainT<E> gr, uint32_t ep) {
+__global__ __launch_bounds__(kCreepThreads) void k_creep_grain(ErodeArgs a, int imax, int jmax, int P,
+                                                               double DX, double rate, double time,
+                                                               const double* __restrict__ elev,
+                                                               const double* __restrict__ bedrock,
+                                                               GrainT<E> gr, uint32_t ep) {
+  __shared__ E s_row[8 * (kG + 1)][kCreepThreads];
+  __shared__ int32_t s_has[8][kCreepThreads];
+  __shared__ double s_en[8][kCreepThreads];
+  cg::grid_group grid = cg::this_grid();
+  const int tl = static_cast<int>(threadIdx.x);
+  const int gtid = static_cast<int>(blockIdx.x) * kCreepThreads + tl;
+  const int nthreads = static_cast<int>(gridDim.x) * kCreepThreads;
   const int t0 = 3 * 2 + 2, t1 = 3 * (imax - 1) + (jmax - 1);
   for (int t = t0; t <= t1; ++t) {
     const int xlo = max(2, (t - (jmax - 1) + 2) / 3);
     const int xhi = min(imax - 1, (t - 2) / 3);
-    for (int x = xlo + static_cast<int>(threadIdx.x); x <= xhi; x += blockDim.x) {
+    for (int x = xlo + gtid; x <= xhi; x += nthreads) {
       const int y = t - 3 * x;
       const size_t c = static_cast<size_t>(x) * P + y;
+      auto nb = [&](int k) {
+        return static_cast<size_t>(static_cast<long>(c) + static_cast<long>(kCreepOx[k]) * P + kCreepOy[k]);
+      };
+      // one batch of loads (x in 2..imax-1, y in 2..jmax-1: every neighbour
+      // is inside the padded grid); a cell that does not slide discards it
       const double es = elev[c];
       const double bc = bedrock[c];
-      // slide_GS is called only where the cell has a grain row (LCM.cpp:5315-5415)
-      if (!(es > bc) || !gr.has[c]) continue;
+      RegRow<E> own;
+      reg_row_load(own, gr, c);
       double en[8];
+      int32_t nh[8];
+      E nv[8][kG + 1];
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        en[k] = elev[static_cast<size_t>(static_cast<long>(c) + static_cast<long>(kCreepOx[k]) * P + kCreepOy[k])];
+      for (int k = 0; k < 8; ++k) {
+        en[k] = elev[nb(k)];
+        nh[k] = gr.has[nb(k)];
+#pragma unroll
+        for (int n = 0; n <= kG; ++n) nv[k][n] = gr.G(nb(k), n);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        s_en[k][tl] = en[k];
+        s_has[k][tl] = nh[k];
+#pragma unroll
+        for (int n = 0; n <= kG; ++n) s_row[k * (kG + 1) + n][tl] = nv[k][n];
+      }
+      // slide_GS is called only where the cell has a grain row (LCM.cpp:5315-5415)
+      if (!(es > bc) || !own.h) continue;
       unsigned moves = 0;  // bit k: the slide towards neighbour k happens
 #pragma unroll
       for (int k = 0; k < 8; ++k) moves |= (en[k] < es && en[k] > -9999) ? (1u << k) : 0u;
-      RegRow<E> own, cur, nxt;
-      reg_row_load(own, gr, c);
-      auto dst_of = [&](int k) {
-        return static_cast<size_t>(static_cast<long>(c) + static_cast<long>(kCreepOx[k]) * P + kCreepOy[k]);
-      };
-      // the moving directions in order; the next one's row is fetched while
-      // the current one slides
-      int k = moves ? __builtin_ctz(moves) : 8;
-      if (k < 8) reg_row_load(cur, gr, dst_of(k));
 #pragma unroll 1
-      while (k < 8) {
-        const unsigned rest = moves & ~((2u << k) - 1u);
-        const int kn = rest ? __builtin_ctz(rest) : 8;
-        if (kn < 8) reg_row_load(nxt, gr, dst_of(kn));
+      while (moves) {
+        const int k = __builtin_ctz(moves);
+        moves &= moves - 1;
+        const size_t d = static_cast<size_t>(static_cast<long>(c) + static_cast<long>(creep_ox(k)) * P + creep_oy(k));
+        RegRow<E> cur;
+        cur.hasp = gr.has;
+        cur.c = d;
+        cur.h = s_has[k][tl];
+#pragma unroll
+        for (int n = 0; n <= kG; ++n) cur.v[n] = s_row[k * (kG + 1) + n][tl];
         // rate * time / D as the reference groups it: ((dz / D) * rate) * time / D
-        const bool diag = kCreepOx[k] != 0 && kCreepOy[k] != 0;
-        const double D = diag ? 7.07 : DX;
-        double temp = ((es - en_at(en, k)) / D) * rate * time / D;
+        const double D = (k & 1) ? 7.07 : DX;  // odd directions are the diagonals
+        double temp = ((es - s_en[k][tl]) / D) * rate * time / D;
         if ((es - temp) < bc) temp = es - bc;
         if (temp < 0) temp = 0;
-        const size_t d = dst_of(k);
         // a row made here is numbered at (x, y), direction k
         slide_gs_r(gr, c, own, d, cur, temp, elev, bedrock, a, hc::row_key(ep, x, y, k));
 #pragma unroll
         for (int n = 0; n <= kG; ++n) gr.G(d, n) = static_cast<E>(cur.v[n]);
-        cur = nxt;
-        k = kn;
       }
 #pragma unroll
       for (int n = 0; n <= kG; ++n) gr.G(c, n) = static_cast<E>(own.v[n]);
     }
-    __syncthreads();
+    grid.sync();
   }
 }
 
@@ -1767,12 +1791,25 @@ int hc_creep_impl(hc_ctx* c, double time, double creep_rate) {
   if (imax >= 3 && jmax >= 3) {
     uint32_t ep;
     if (int rc = next_row_epoch(c, &ep)) return rc;
-    if (c->sesz == 4)
-      hipLaunchKernelGGL(k_creep_grain<float>, dim3(1), dim3(512), 0, c->stream, a, imax, jmax,
-                         c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<float>(c), ep);
-    else
-      hipLaunchKernelGGL(k_creep_grain<double>, dim3(1), dim3(512), 0, c->stream, a, imax, jmax,
-                         c->P, c->g.DX, creep_rate, time, elev, c->bedrock, grain_of<double>(c), ep);
+    // one lane per cell of the longest wavefront, within what the device
+    // holds at once
+    const int cells = std::min(imax - 2, (jmax - 2 + 2) / 3);
+    int nb = std::max(1, (cells + kCreepThreads - 1) / kCreepThreads);
+    int dev = 0, ncu = 0, per_cu = 0;
+    HC_HIP(c, hipGetDevice(&dev));
+    HC_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const void* fn = c->sesz == 4 ? reinterpret_cast<const void*>(&k_creep_grain<float>)
+                                  : reinterpret_cast<const void*>(&k_creep_grain<double>);
+    HC_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kCreepThreads, 0));
+    nb = std::min(nb, std::max(1, per_cu) * ncu);
+    double DX = c->g.DX;
+    const double* bedrock = c->bedrock;
+    GrainT<float> g4 = grain_of<float>(c);
+    GrainT<double> g8 = grain_of<double>(c);
+    void* args[] = {const_cast<ErodeArgs*>(&a), const_cast<int*>(&imax), const_cast<int*>(&jmax),
+                    &c->P, &DX, &creep_rate, &time, &elev, const_cast<const double**>(&bedrock),
+                    c->sesz == 4 ? static_cast<void*>(&g4) : static_cast<void*>(&g8), &ep};
+    HC_HIP(c, hipLaunchCooperativeKernel(fn, dim3(nb), dim3(kCreepThreads), args, 0, c->stream));
   }
   const dim3 blk(256), grd(static_cast<unsigned>((jmax + 255) / 256), static_cast<unsigned>(imax));
   hipLaunchKernelGGL(k_creep_sum, grd, blk, 0, c->stream, imax, jmax, c->P, c->g.DX, creep_rate,
