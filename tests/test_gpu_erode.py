@@ -97,13 +97,17 @@ def test_global_landsliding_vs_oracle(angle, nrows, ncols):
     assert np.array_equal(ctx.download("elev"), o.field("elev"))
 
 
-@pytest.mark.parametrize("rate,nrows,ncols", [(2.5, 24, 24), (2.5, 40, 70), (0.0025, 140, 50)])
+@pytest.mark.parametrize("rate,nrows,ncols", [(2.5, 24, 24), (2.5, 40, 70), (0.0025, 140, 50),
+                                               (2.5, 420, 400)])
 def test_creep_vs_oracle(tmp_path, rate, nrows, ncols):
     """creep (LSDCatchmentModel.cpp:5293-5427) on the device, the grain
     slides as wavefronts t = 3x + y and tempcreep gathered per cell in the
     sweep's order, against the oracle (pinned bitwise to the compiled
     reference): elevations, grain rows, strata and grain flags bitwise, on a
-    grain-bearing steep DEM with cells at and below their bedrock."""
+    grain-bearing steep DEM with cells at and below their bedrock.  The
+    420 x 400 grid has wavefronts of up to 140 cells, more than one block of
+    the cooperative grid holds (128), so cells of one wavefront sit in
+    different blocks."""
     rng = np.random.Generator(np.random.PCG64(23))
     dem = inputs.planar_dem(nrows, ncols, 1.0, 0.05, 6.0, 0.6, 1005 + nrows, south_only=False)
     props = inputs.dirichlet_props(nrows, ncols, rng)
