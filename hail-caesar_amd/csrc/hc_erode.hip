@@ -41,8 +41,6 @@
 #include <cmath>
 #include <cstring>
 
-#include <hip/hip_cooperative_groups.h>
-
 #include "hc_device.hpp"
 #include "hc_internal.hpp"
 
@@ -51,8 +49,6 @@ using hc::kGravity;
 using namespace hcdev;
 
 namespace {
-
-namespace cg = cooperative_groups;
 
 constexpr int kG = 10;  // G_MAX
 
@@ -1428,19 +1424,50 @@ __device__ __forceinline__ int creep_oy(int k) { return -creep_ox((k + 2) & 7); 
 // the moves then run from registers and LDS and store each neighbour's row
 // once.  A wavefront's cells are one long instruction stream each, so they
 // are spread one per lane over blocks of two wavefronts (one block per
-// compute unit: the staging takes most of its LDS), launched cooperatively
-// so that every block is resident for the barriers.
+// compute unit: the staging takes most of its LDS), as many as the device
+// holds at once, so that every block is resident for the barriers.
 constexpr int kCreepThreads = 128;
+
+// A barrier over a grid whose blocks are all resident at once (the launch is
+// sized to what the device holds): each block adds one arrival per pass to
+// *arrive and waits for target = passes * blocks.  Every wavefront releases
+// its stores at device scope before it and acquires after it, so blocks on
+// different XCDs see each other's rows.  A wait that outlasts
+// kGridBarrierPolls polls (seconds) sets *err, and no barrier waits once it
+// is set, so a grid that could not be made resident ends with an error
+// rather than a hang.
+constexpr unsigned kGridBarrierPolls = 1u << 22;
+__device__ __forceinline__ void grid_barrier(unsigned long long* arrive, unsigned long long target,
+                                             unsigned long long* err) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned polls = 0;
+    while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
+           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      if (++polls > kGridBarrierPolls) {
+        __hip_atomic_fetch_or(err, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 template <typename E>
 __global__ __launch_bounds__(kCreepThreads) void k_creep_grain(ErodeArgs a, int imax, int jmax, int P,
                                                                double DX, double rate, double time,
                                                                const double* __restrict__ elev,
                                                                const double* __restrict__ bedrock,
-                                                               GrainT<E> gr, uint32_t ep) {
+                                                               GrainT<E> gr, uint32_t ep,
+                                                               unsigned long long* __restrict__ bar) {
   __shared__ E s_row[8 * (kG + 1)][kCreepThreads];
   __shared__ int32_t s_has[8][kCreepThreads];
   __shared__ double s_en[8][kCreepThreads];
-  cg::grid_group grid = cg::this_grid();
+  unsigned long long target = 0;
   const int tl = static_cast<int>(threadIdx.x);
   const int gtid = static_cast<int>(blockIdx.x) * kCreepThreads + tl;
   const int nthreads = static_cast<int>(gridDim.x) * kCreepThreads;
@@ -1506,7 +1533,8 @@ __global__ __launch_bounds__(kCreepThreads) void k_creep_grain(ErodeArgs a, int 
 #pragma unroll
       for (int n = 0; n <= kG; ++n) gr.G(c, n) = static_cast<E>(own.v[n]);
     }
-    grid.sync();
+    target += gridDim.x;
+    grid_barrier(bar, target, bar + 1);
   }
 }
 
@@ -1806,10 +1834,20 @@ int hc_creep_impl(hc_ctx* c, double time, double creep_rate) {
     const double* bedrock = c->bedrock;
     GrainT<float> g4 = grain_of<float>(c);
     GrainT<double> g8 = grain_of<double>(c);
+    // d_red[5]: the barrier's arrivals, d_red[6]: its time-out flag
+    unsigned long long* bar = c->d_red + 5;
+    HC_HIP(c, hipMemsetAsync(bar, 0, 2 * sizeof(unsigned long long), c->stream));
     void* args[] = {const_cast<ErodeArgs*>(&a), const_cast<int*>(&imax), const_cast<int*>(&jmax),
                     &c->P, &DX, &creep_rate, &time, &elev, const_cast<const double**>(&bedrock),
-                    c->sesz == 4 ? static_cast<void*>(&g4) : static_cast<void*>(&g8), &ep};
-    HC_HIP(c, hipLaunchCooperativeKernel(fn, dim3(nb), dim3(kCreepThreads), args, 0, c->stream));
+                    c->sesz == 4 ? static_cast<void*>(&g4) : static_cast<void*>(&g8), &ep, &bar};
+    HC_HIP(c, hipLaunchKernel(fn, dim3(nb), dim3(kCreepThreads), args, 0, c->stream));
+    HC_HIP(c, hipMemcpyAsync(c->h_pinned, bar + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    HC_HIP(c, hipStreamSynchronize(c->stream));
+    unsigned long long timed_out = 0;
+    std::memcpy(&timed_out, c->h_pinned, sizeof(timed_out));
+    if (timed_out != 0)
+      return hc_fail(c, HC_ERR_HIP, "creep: the blocks of the grain sweep were not all resident");
   }
   const dim3 blk(256), grd(static_cast<unsigned>((jmax + 255) / 256), static_cast<unsigned>(imax));
   hipLaunchKernelGGL(k_creep_sum, grd, blk, 0, c->stream, imax, jmax, c->P, c->g.DX, creep_rate,

@@ -262,7 +262,8 @@ struct hc_ctx {
   // reductions and small device scalars
   unsigned long long* d_red = nullptr;  // [0] maxdepth bits, [1] tempbmax bits, [2] health count,
                                         // [3] global_landsliding sweep count,
-                                        // [4] landsliding: some cell can move
+                                        // [4] landsliding: some cell can move,
+                                        // [5] [6] creep's grid barrier: arrivals, time-out
   double* d_slide_moves = nullptr;      // hillslope scratch: 8 planes (landslide moves, tempcreep)
   double* d_runoff = nullptr;           // spatially complex rainfall: old, new j_mean planes
   double* d_partials = nullptr;         // per-block partial sums

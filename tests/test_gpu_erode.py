@@ -105,9 +105,9 @@ def test_creep_vs_oracle(tmp_path, rate, nrows, ncols):
     sweep's order, against the oracle (pinned bitwise to the compiled
     reference): elevations, grain rows, strata and grain flags bitwise, on a
     grain-bearing steep DEM with cells at and below their bedrock.  The
-    420 x 400 grid has wavefronts of up to 140 cells, more than one block of
-    the cooperative grid holds (128), so cells of one wavefront sit in
-    different blocks."""
+    420 x 400 grid has wavefronts of up to 133 cells, more than one block of
+    the sweep's grid holds (128), so cells of one wavefront sit in
+    different blocks and meet at its grid-wide barrier."""
     rng = np.random.Generator(np.random.PCG64(23))
     dem = inputs.planar_dem(nrows, ncols, 1.0, 0.05, 6.0, 0.6, 1005 + nrows, south_only=False)
     props = inputs.dirichlet_props(nrows, ncols, rng)
