@@ -225,7 +225,11 @@ int hc_global_landsliding(hc_ctx* ctx, double failureangle, int* sweeps);
  * material to its lower neighbours in proportion to the slope and
  * `creep_rate` (the parameter-file key creep_rate), sliding the moved grain
  * fractions in the reference's sweep order.  Elevations, grain rows and
- * strata bit-identical to the reference.  FP64 sediment contexts. */
+ * strata bit-identical to the reference.  Sediment contexts (grain stored in
+ * FP64 or FP32).  Synchronous.  The grain sweep runs on a grid of blocks
+ * that must all be resident on the device at once (it is sized to what the
+ * device holds); should its grid-wide barrier wait for seconds, as when
+ * other work holds the device's compute units, it returns HC_ERR_HIP. */
 int hc_creep(hc_ctx* ctx, double time, double creep_rate);
 
 /* LSDCatchmentModel::in_channel_landsliding (LCM.cpp:5429-5528), which the
