@@ -593,7 +593,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--size", type=int, default=None,
+                    help="grid side (default 4096; 1024 for --erosion-hours, the flow + erosion "
+                         "configuration's stated shape)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel from the host")
@@ -621,10 +623,10 @@ def main():
         sys.exit(2)
     load_package()
     if args.accuracy_hours:
-        print(json.dumps({"accuracy": accuracy_run(args.size, args.accuracy_hours)}))
+        print(json.dumps({"accuracy": accuracy_run(args.size or 4096, args.accuracy_hours)}))
         return
     if args.erosion_hours:
-        print(json.dumps({"erosion_accuracy": erosion_accuracy_run(args.size, args.erosion_hours)}))
+        print(json.dumps({"erosion_accuracy": erosion_accuracy_run(args.size or 1024, args.erosion_hours)}))
         return
     if args.section == "ensemble":
         print(json.dumps({"ensemble": run_ensemble(args.steps, args.warmup)}))
@@ -641,6 +643,8 @@ def main():
     from hail_caesar_amd import inputs
     with open(BASELINE) as f:
         base = json.load(f)
+    if args.size is None:
+        args.size = 4096
     arrays, params = inputs.grid4096_arrays(size=args.size)
     cells = args.size * args.size
     res = {}
