@@ -508,10 +508,14 @@ __device__ __forceinline__ void fused_tile(const HydroArgs& a, int bx, int by, i
       qyo[k] = ld_nt(qy0, o);
     }
   };
+  // (Manning's n, like the elevations, is read every step and never written:
+  // ordinary loads leave it in the caches between steps, where the fields
+  // that stream through -- the fluxes in, every field out -- go nontemporal;
+  // 106.5 against 108.5 us per FP32 launch at 4096^2)
   auto load_n = [&](int k0, int r0) {
     if (a.spatial_n) {
 #pragma unroll
-      for (int k = k0; k < R + 2; ++k) nn[k] = ld_nt(nman, off(k, r0));
+      for (int k = k0; k < R + 2; ++k) nn[k] = ld_elem(nman, off(k, r0));
     } else {
 #pragma unroll
       for (int k = k0; k < R + 2; ++k) nn[k] = nu;
