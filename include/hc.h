@@ -229,7 +229,9 @@ int hc_global_landsliding(hc_ctx* ctx, double failureangle, int* sweeps);
  * FP64 or FP32).  Synchronous.  The grain sweep runs on a grid of blocks
  * that must all be resident on the device at once (it is sized to what the
  * device holds); should its grid-wide barrier wait for seconds, as when
- * other work holds the device's compute units, it returns HC_ERR_HIP. */
+ * other work holds the device's compute units, it returns HC_ERR_HIP before
+ * the elevations move, with the grain rows part-way through the sweep
+ * (restore a saved state before going on). */
 int hc_creep(hc_ctx* ctx, double time, double creep_rate);
 
 /* LSDCatchmentModel::in_channel_landsliding (LCM.cpp:5429-5528), which the
